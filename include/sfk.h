@@ -1,0 +1,273 @@
+/* sfk.h — C ABI of libsfk, the MI355X (gfx950) implementation of the
+ * splatfield 3DGS render path and B3-Seg evidence path.
+ *
+ * Every entry point enqueues on the context's single HIP stream (the ordering
+ * contract of src/lib.rs:20-27) unless marked SYNC.  Return codes: 0 = ok,
+ * negative = error; sfk_last_error() gives the message.  No exception or abort
+ * crosses this boundary: the reference's panics (undersized scratch,
+ * sort.rs:323-327, scan.rs:230-234; zero splats, render.rs:248) become
+ * SFK_E_CAPACITY / SFK_E_ARG.  Calls on one context must be externally
+ * serialized (the reference takes &mut for the same reason).
+ *
+ * Device pointers are plain pointers from sfk_alloc (or any hipMalloc'd
+ * memory of the same device).  Objects returned through out-parameters alias
+ * their scratch and stay valid until the next call on that scratch
+ * (src/render.rs:366-370; crates/sort/src/sort.rs:278-281).
+ */
+#ifndef SFK_H
+#define SFK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFK_OK 0
+#define SFK_E_ARG (-1)
+#define SFK_E_CAPACITY (-2)
+#define SFK_E_HIP (-3)
+#define SFK_E_NOMEM (-4)
+
+#define SFK_ATTR_PLANES 11   /* src/layout.rs:47-58 */
+#define SFK_PROJ_STRIDE 12   /* floats per projected row (see sfk_render_buffers) */
+
+/* ------------------------------------------------------------ context / L0 */
+
+typedef struct sfk_ctx sfk_ctx;
+
+/* One device + one HIP stream; replaces the cubecl client and its
+ * single-stream policy (src/lib.rs:20-27, crates/sort/src/tensor.rs:15-27). */
+int sfk_ctx_create(int device, sfk_ctx** out);
+void sfk_ctx_destroy(sfk_ctx* ctx);
+const char* sfk_last_error(const sfk_ctx* ctx);
+void* sfk_ctx_stream(sfk_ctx* ctx); /* the hipStream_t */
+int sfk_sync(sfk_ctx* ctx);         /* SYNC */
+int sfk_device_name(sfk_ctx* ctx, char* buf, int len);
+
+/* GpuTensor storage (crates/sort/src/tensor.rs:29-77): empty / from /
+ * read_vec / write.  Upload and download are SYNC. */
+int sfk_alloc(sfk_ctx* ctx, size_t bytes, void** out);
+int sfk_free(sfk_ctx* ctx, void* ptr);
+int sfk_upload(sfk_ctx* ctx, void* dst, const void* src, size_t bytes);
+int sfk_download(sfk_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Stream-ordered u32 fill by a kernel (graph-capture safe); zero_u32 of
+ * src/seg/mod.rs:87-103 is fill(buf, 0). */
+int sfk_fill_u32(sfk_ctx* ctx, uint32_t* buf, uint32_t value, size_t count);
+
+/* Stream-ordered event timing (hipEventRecord/ElapsedTime). */
+int sfk_timer_start(sfk_ctx* ctx);
+int sfk_timer_stop_ms(sfk_ctx* ctx, float* ms); /* SYNC */
+
+/* ------------------------------------------------------------------ camera */
+
+/* Camera (src/camera.rs:4-19): per-axis fov, position, rotation (x,y,z,w). */
+typedef struct {
+    float fov[2];
+    float position[3];
+    float rotation[4];
+} sfk_camera;
+
+/* Kernel view of a camera (CameraView, src/project.rs:121-157). */
+typedef struct {
+    float rot0[3], rot1[3], rot2[3]; /* rows of the world->camera rotation */
+    float trans[3];
+    float focal[2];
+    float world_pos[3];
+    float img[2];
+    float tile_bounds[2];
+} sfk_view;
+
+/* CameraViewLaunch::for_camera (src/project.rs:137-157). */
+int sfk_camera_view(const sfk_camera* cam, uint32_t w, uint32_t h, uint32_t tile_w,
+                    sfk_view* out);
+/* Camera::look_at rotation (src/camera.rs:57-63). */
+int sfk_look_at(const float pos[3], const float target[3], float quat_xyzw[4]);
+
+/* -------------------------------------------------------- crates/sort (L1) */
+
+/* bits_for (crates/sort/src/sort.rs:259-261). */
+uint32_t sfk_bits_for(uint32_t max_exclusive);
+
+typedef struct sfk_radix_scratch sfk_radix_scratch;
+
+/* RadixScratch::new (sort.rs:290-300).  key_bytes is 4 (u32 keys) or 8
+ * (u64 keys). */
+int sfk_radix_scratch_create(sfk_ctx* ctx, uint32_t max_elems, uint32_t key_bytes,
+                             sfk_radix_scratch** out);
+void sfk_radix_scratch_destroy(sfk_radix_scratch* s);
+/* Digit width per pass, 4..8 bits (default 8).  The reference fixes 6
+ * (sort.rs:11-17); here it is a tunable. */
+int sfk_radix_set_digit_bits(sfk_radix_scratch* s, uint32_t digit_bits);
+
+/* RadixScratch::argsort (sort.rs:302-381): stable argsort of keys[0..n) on
+ * their low `bits` bits (keys must be < 2^bits), carrying vals.  vals may be
+ * NULL: the identity 0..n-1 is used.  If d_n is non-NULL the element count is
+ * min(*d_n, n) read on the device (no host sync).  With write_keys = 0 the
+ * sorted keys of the last pass are not written.  Inputs are only read;
+ * outputs alias the scratch until its next use. */
+int sfk_argsort_u32(sfk_ctx* ctx, sfk_radix_scratch* s, const uint32_t* keys,
+                    const uint32_t* vals, uint32_t n, const uint32_t* d_n, uint32_t bits,
+                    int write_keys, uint32_t** out_keys, uint32_t** out_vals);
+int sfk_argsort_u64(sfk_ctx* ctx, sfk_radix_scratch* s, const uint64_t* keys,
+                    const uint32_t* vals, uint32_t n, const uint32_t* d_n, uint32_t bits,
+                    int write_keys, uint64_t** out_keys, uint32_t** out_vals);
+
+typedef struct sfk_scan_scratch sfk_scan_scratch;
+
+/* ScanScratch::new (crates/sort/src/scan.rs:53-60). */
+int sfk_scan_scratch_create(sfk_ctx* ctx, uint32_t max_elems, sfk_scan_scratch** out);
+void sfk_scan_scratch_destroy(sfk_scan_scratch* s);
+/* exclusive_scan_gather (scan.rs:215-255): offsets[i] = sum_{j<i}
+ * counts[gids[j]] for i < n; n = 0 leaves offsets untouched. */
+int sfk_exclusive_scan_gather(sfk_ctx* ctx, const uint32_t* gids, const uint32_t* counts,
+                              uint32_t* offsets, uint32_t n, sfk_scan_scratch* s);
+/* exclusive_scan_buf (scan.rs:182-213): in-place exclusive scan of buf[0..n). */
+int sfk_exclusive_scan(sfk_ctx* ctx, uint32_t* buf, uint32_t n, sfk_scan_scratch* s);
+
+/* ---------------------------------------------------------- render (L2) */
+
+typedef struct sfk_splats sfk_splats;
+
+/* Splats::new (src/render.rs:241-269): upload field-major attribute planes
+ * (11*n floats) and SH planes (3*sh_per_ch*n floats).  n = 0 is SFK_E_ARG. */
+int sfk_splats_create(sfk_ctx* ctx, const float* attrs, const float* sh, uint32_t n,
+                      uint32_t sh_per_ch, sfk_splats** out);
+void sfk_splats_destroy(sfk_splats* s);
+int sfk_splats_info(const sfk_splats* s, uint32_t* n, uint32_t* sh_per_ch, float bounds_min[3],
+                    float bounds_max[3], const float** d_attrs, const float** d_sh);
+
+typedef struct sfk_render_scratch sfk_render_scratch;
+
+/* RenderScratch::new (src/render.rs:143-167): per-(n, w x h) buffers reused
+ * across frames.  Isect capacity starts at next_pow2(min(tiles*n, 2^22)) and
+ * grows between frames from the previous frame's emission up to 67,107,840
+ * (render.rs:12-18, 169-197). */
+int sfk_render_scratch_create(sfk_ctx* ctx, uint32_t n, uint32_t w, uint32_t h,
+                              sfk_render_scratch** out);
+void sfk_render_scratch_destroy(sfk_render_scratch* s);
+/* Tile width 8, 16 (the reference's TILE_WIDTH, layout.rs:41) or 32. */
+int sfk_render_scratch_set_tile_width(sfk_render_scratch* s, uint32_t tile_w);
+/* Raise the isect capacity now (next_pow2, capped); a pre-sized scratch
+ * renders its first frame untruncated. */
+int sfk_render_scratch_reserve_isects(sfk_render_scratch* s, uint32_t isects);
+/* Radix digit width of the scratch's sorts (4..8). */
+int sfk_render_scratch_set_digit_bits(sfk_render_scratch* s, uint32_t digit_bits);
+
+/* TileIsects (src/render.rs:463-471) plus the device-side counters. */
+typedef struct {
+    const uint32_t* tile_ids;     /* sorted, tile-major / depth-minor */
+    const uint32_t* gaussian_ids; /* splat id per isect */
+    const uint32_t* tile_starts;  /* lower_bound(tile) for tile in [0, tiles] */
+    const uint32_t* d_counters;   /* device [isects_raw, visible, isects] */
+    uint32_t capacity;
+    uint32_t tiles_x, tiles_y, tile_w, row_stride;
+    uint32_t w, h;
+} sfk_tile_isects;
+
+/* Splats::prepare_isects (src/render.rs:271-364): project, depth argsort,
+ * gather scan, map, stable tile argsort, tile ranges.  No host sync: counts
+ * stay on the device (sfk_tile_isects_counts reads them). */
+int sfk_prepare_isects(sfk_ctx* ctx, const sfk_splats* splats, sfk_render_scratch* scratch,
+                       const sfk_camera* cam, uint32_t w, uint32_t h, sfk_tile_isects* out);
+/* The same with an explicit kernel view (tests and batched callers). */
+int sfk_prepare_isects_view(sfk_ctx* ctx, const sfk_splats* splats,
+                            sfk_render_scratch* scratch, const sfk_view* view, uint32_t w,
+                            uint32_t h, sfk_tile_isects* out);
+/* SYNC: [isects_raw, visible, isects]. */
+int sfk_tile_isects_counts(sfk_ctx* ctx, const sfk_tile_isects* isects, uint32_t counts[3]);
+
+/* Finalize (src/render.rs:220-239, 382-460) in its three modes. */
+int sfk_finalize_rgb(sfk_ctx* ctx, sfk_render_scratch* scratch, const sfk_tile_isects* isects);
+int sfk_finalize_total(sfk_ctx* ctx, sfk_render_scratch* scratch, const sfk_tile_isects* isects,
+                       float scale, uint32_t* resp);
+int sfk_finalize_evidence(sfk_ctx* ctx, sfk_render_scratch* scratch,
+                          const sfk_tile_isects* isects, float scale, const uint32_t* mask,
+                          uint32_t words_per_row, uint32_t* fg, uint32_t* bg);
+
+/* Splats::render_with (src/render.rs:366-380): prepare + finalize(Rgb).
+ * Outputs alias the scratch: bitmap (RGBA8, h x row_stride u32) and the
+ * f32 RGBA image (h x w x 4). */
+int sfk_render_with(sfk_ctx* ctx, const sfk_splats* splats, sfk_render_scratch* scratch,
+                    const sfk_camera* cam, uint32_t w, uint32_t h, const uint32_t** bitmap,
+                    const float** rgba);
+
+/* The scratch's device buffers, for inspection.  projected rows are
+ * SFK_PROJ_STRIDE floats: [mx, my, conic_a, conic_b, conic_c, opacity, r, g,
+ * b, 0, 0, 0] (the reference's 9-float row, src/layout.rs:60-61, reordered
+ * and padded to 16-byte loads).  depth_keys[i] is the f32 bits of camera z
+ * for visible splats and 0xffffffff otherwise (index i, not slot order);
+ * depth_order[r] is the splat of depth rank r. */
+typedef struct {
+    const uint32_t* depth_keys;
+    const uint32_t* depth_order;
+    const float* projected;
+    const uint32_t* tile_counts;
+    const uint32_t* bbox;
+    const uint32_t* offsets;
+    const uint32_t* counters;
+    const uint32_t* bitmap;
+    const float* rgba;
+    const float* final_t;
+    const uint32_t* n_contrib;
+    uint32_t n, w, h, row_stride, capacity;
+} sfk_render_buffers;
+int sfk_render_scratch_buffers(const sfk_render_scratch* s, sfk_render_buffers* out);
+
+/* --------------------------------------------------------------- seg (L3) */
+
+/* fractional_bits (src/seg/mod.rs:25-34). */
+uint32_t sfk_fractional_bits(uint32_t w, uint32_t h);
+
+/* Mask::from_bytes (src/seg/mod.rs:115-143), host side: one byte per pixel,
+ * row-major, nonzero = inside -> ceil(w/32)*h LSB-first words. */
+int sfk_mask_pack(const uint8_t* bytes, uint32_t w, uint32_t h, uint32_t* words,
+                  uint32_t* fg_pixels);
+
+/* Splats::render_responsibility (src/seg/mod.rs:146-159): prepare, clear
+ * resp, accumulate total epsilon at scale 2^fractional_bits(w, h). */
+int sfk_render_responsibility(sfk_ctx* ctx, const sfk_splats* splats,
+                              sfk_render_scratch* scratch, const sfk_camera* cam, uint32_t w,
+                              uint32_t h, uint32_t* resp, float* scale_out);
+/* accumulate_prepared (src/seg/mod.rs:161-210): over isects already prepared
+ * for this camera and resolution; mask NULL -> total into resp, else split
+ * into fg/bg.  Buffers are cleared first. */
+int sfk_accumulate_prepared(sfk_ctx* ctx, sfk_render_scratch* scratch,
+                            const sfk_tile_isects* isects, const uint32_t* mask,
+                            uint32_t words_per_row, uint32_t* resp, uint32_t* fg, uint32_t* bg,
+                            float* scale_out);
+
+/* BetaState::eig (src/seg/beta.rs:454-482) with the entropy in f64 on the
+ * device instead of the 445x445 table.  SYNC (one f64 readback). */
+int sfk_eig(sfk_ctx* ctx, const float* a, const float* b, const uint32_t* resp, uint32_t n,
+            float scale, double* out);
+/* The same, leaving the f64 result in device memory (no sync). */
+int sfk_eig_device(sfk_ctx* ctx, const float* a, const float* b, const uint32_t* resp,
+                   uint32_t n, float scale, double* d_out);
+/* BetaState::update (src/seg/beta.rs:484-500): a += fg/scale, b += bg/scale. */
+int sfk_beta_update(sfk_ctx* ctx, float* a, float* b, const uint32_t* fg, const uint32_t* bg,
+                    uint32_t n, float scale);
+
+/* ------------------------------------------------------ kernel-level calls */
+
+/* project_splats (src/project.rs:295-388) over all n splats.  Writes per
+ * splat i: depth_keys[i], projected row i, tile_counts[i], bbox[2i..2i+1];
+ * counters[0] += isects, counters[1] += visible (counters are not cleared). */
+int sfk_project(sfk_ctx* ctx, const sfk_view* view, const float* attrs, const float* sh,
+                uint32_t n, uint32_t sh_per_ch, uint32_t tile_w, uint32_t* depth_keys,
+                float* projected, uint32_t* tile_counts, uint32_t* bbox, uint32_t* counters);
+/* map_isects (src/raster.rs:12-48) over ranks [0, n) of depth_order. */
+int sfk_map_isects(sfk_ctx* ctx, const uint32_t* depth_order, const uint32_t* tile_counts,
+                   const uint32_t* bbox, const uint32_t* offsets, uint32_t n,
+                   uint32_t tiles_per_row, uint32_t max_isects, uint32_t* tile_ids,
+                   uint32_t* gaussian_ids);
+/* lower_bound tile starts (src/raster.rs:52-65, 111-115) for all tiles in
+ * [0, num_tiles]; n isects, or min(*d_n, n) when d_n is given. */
+int sfk_tile_starts(sfk_ctx* ctx, const uint32_t* tile_ids, uint32_t n, const uint32_t* d_n,
+                    uint32_t num_tiles, uint32_t* starts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,264 @@
+"""Oracle (test infrastructure only): numpy/ctypes front-end of the C
+restatement in ``oracle/sfo.c``.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import this module, and only as the checker.  The
+product path (``splatfield_amd``) never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "libsfo.so")
+
+MODE_RGB, MODE_TOTAL, MODE_EVIDENCE = 0, 1, 2
+PROJ_FLOATS = 9
+
+
+class View(C.Structure):
+    _fields_ = [
+        ("rot0", C.c_float * 3), ("rot1", C.c_float * 3), ("rot2", C.c_float * 3),
+        ("trans", C.c_float * 3), ("focal", C.c_float * 2), ("world_pos", C.c_float * 3),
+        ("img", C.c_float * 2), ("tile_bounds", C.c_float * 2),
+    ]
+
+    def as_array(self):
+        return np.frombuffer(bytes(self), dtype=np.float32).copy()
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _lib():
+    if not os.path.exists(_SO):
+        build()
+    lib = C.CDLL(_SO)
+    P = C.c_void_p
+    u32, f32, f64, i32 = C.c_uint32, C.c_float, C.c_double, C.c_int
+    sig = {
+        "sfo_camera_view": (None, [P, P, P, u32, u32, u32, P]),
+        "sfo_look_at": (None, [P, P, P]),
+        "sfo_bits_for": (u32, [u32]),
+        "sfo_fractional_bits": (u32, [u32, u32]),
+        "sfo_project": (None, [P, P, P, u32, u32, u32, i32, P, P, P, P, P, P]),
+        "sfo_depth_order": (u32, [P, P, u32, P]),
+        "sfo_argsort_u32": (None, [P, P, u32, u32, P, P]),
+        "sfo_argsort_u64": (None, [P, P, u32, u32, P, P]),
+        "sfo_scan_gather": (None, [P, P, u32, P]),
+        "sfo_map_isects": (None, [P, P, P, u32, u32, u32, P, P]),
+        "sfo_lower_bound": (u32, [P, u32, u32]),
+        "sfo_tile_ranges": (None, [P, u32, u32, P]),
+        "sfo_rasterize": (None, [i32, u32, u32, u32, u32, f32, P, P, P, P, u32, i32,
+                                 P, P, P, P, P, P, P]),
+        "sfo_lgamma": (f64, [f64]),
+        "sfo_digamma": (f64, [f64]),
+        "sfo_beta_entropy": (f64, [f64, f64]),
+        "sfo_eig": (f64, [P, P, P, u32, f32]),
+        "sfo_beta_update": (None, [f32, P, P, u32, P, P]),
+        "sfo_expf_export": (f32, [f32]),
+        "sfo_logf_export": (f32, [f32]),
+        "sfo_backward": (None, [P, P, P, u32, u32, u32, u32, u32, P, P, P, P, P, P, P, i32,
+                                P, P, P, P, P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        if not hasattr(lib, name):
+            continue
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+LIB = _lib()
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def expf(x):
+    return np.float32(LIB.sfo_expf_export(float(x)))
+
+
+def logf(x):
+    return np.float32(LIB.sfo_logf_export(float(x)))
+
+
+def camera_view(fov, pos, quat_xyzw, w, h, tile_w=16):
+    v = View()
+    fov = _f32(fov); pos = _f32(pos); q = _f32(quat_xyzw)
+    LIB.sfo_camera_view(_p(fov), _p(pos), _p(q), w, h, tile_w, C.byref(v))
+    return v
+
+
+def look_at(pos, target):
+    q = np.zeros(4, np.float32)
+    p = _f32(pos); t = _f32(target)
+    LIB.sfo_look_at(_p(p), _p(t), _p(q))
+    return q
+
+
+def bits_for(m):
+    return int(LIB.sfo_bits_for(m))
+
+
+def fractional_bits(w, h):
+    return int(LIB.sfo_fractional_bits(w, h))
+
+
+def project(view, attrs, sh, n, sh_per_ch, tile_w=16, nthreads=1):
+    """project_splats (src/project.rs:295-388) -> dict of per-splat outputs."""
+    attrs = _f32(attrs); sh = _f32(sh)
+    out = dict(
+        visible=np.zeros(n, np.uint8), depth_keys=np.zeros(n, np.uint32),
+        projected=np.zeros(n * PROJ_FLOATS, np.float32), tile_counts=np.zeros(n, np.uint32),
+        bbox=np.zeros(2 * n, np.uint32), counters=np.zeros(2, np.uint32))
+    LIB.sfo_project(C.byref(view), _p(attrs), _p(sh), n, sh_per_ch, tile_w, nthreads,
+                    _p(out["visible"]), _p(out["depth_keys"]), _p(out["projected"]),
+                    _p(out["tile_counts"]), _p(out["bbox"]), _p(out["counters"]))
+    return out
+
+
+def depth_order(visible, keys):
+    n = len(keys)
+    order = np.zeros(max(n, 1), np.uint32)
+    v = LIB.sfo_depth_order(_p(np.ascontiguousarray(visible, np.uint8)), _p(_u32(keys)), n,
+                            _p(order))
+    return order[:v].copy()
+
+
+def argsort_u32(keys, vals, bits):
+    keys = _u32(keys); vals = _u32(vals)
+    ok = np.zeros_like(keys); ov = np.zeros_like(vals)
+    LIB.sfo_argsort_u32(_p(keys), _p(vals), len(keys), bits, _p(ok), _p(ov))
+    return ok, ov
+
+
+def argsort_u64(keys, vals, bits):
+    keys = np.ascontiguousarray(keys, np.uint64); vals = _u32(vals)
+    ok = np.zeros_like(keys); ov = np.zeros_like(vals)
+    LIB.sfo_argsort_u64(_p(keys), _p(vals), len(keys), bits, _p(ok), _p(ov))
+    return ok, ov
+
+
+def scan_gather(gids, counts):
+    gids = _u32(gids); counts = _u32(counts)
+    out = np.zeros(len(gids), np.uint32)
+    LIB.sfo_scan_gather(_p(gids), _p(counts), len(gids), _p(out))
+    return out
+
+
+def map_isects(order, bbox, offsets, tiles_per_row, max_isects):
+    order = _u32(order); bbox = _u32(bbox); offsets = _u32(offsets)
+    tid = np.zeros(max(max_isects, 1), np.uint32); gid = np.zeros(max(max_isects, 1), np.uint32)
+    LIB.sfo_map_isects(_p(order), _p(bbox), _p(offsets), tiles_per_row, max_isects, len(order),
+                       _p(tid), _p(gid))
+    return tid[:max_isects], gid[:max_isects]
+
+
+def tile_ranges(tile_ids, num_tiles):
+    tile_ids = _u32(tile_ids)
+    r = np.zeros(2 * num_tiles, np.uint32)
+    LIB.sfo_tile_ranges(_p(tile_ids), len(tile_ids), num_tiles, _p(r))
+    return r
+
+
+def lower_bound(ids, target):
+    ids = _u32(ids)
+    return int(LIB.sfo_lower_bound(_p(ids), len(ids), target))
+
+
+def rasterize(mode, w, h, ranges, gaussian_ids, projected, n, tile_w=16, scale=0.0,
+              mask=None, words_per_row=0, nthreads=1, row_stride=None):
+    """rasterize_kernel (src/raster.rs:87-230) in one of its three modes."""
+    if row_stride is None:
+        row_stride = ((w * 4 + 255) // 256 * 256) // 4
+    ranges = _u32(ranges); gaussian_ids = _u32(gaussian_ids); projected = _f32(projected)
+    out = dict(final_t=np.zeros(w * h, np.float32), n_contrib=np.zeros(w * h, np.uint32))
+    if mode == MODE_RGB:
+        out["rgba"] = np.zeros(w * h * 4, np.float32)
+        out["bitmap"] = np.zeros(h * row_stride, np.uint32)
+    elif mode == MODE_TOTAL:
+        out["resp"] = np.zeros(n, np.uint32)
+    else:
+        out["fg"] = np.zeros(n, np.uint32)
+        out["bg"] = np.zeros(n, np.uint32)
+    mask_a = None if mask is None else _u32(mask)
+    LIB.sfo_rasterize(mode, w, h, tile_w, row_stride, scale, _p(ranges), _p(gaussian_ids),
+                      _p(projected), _p(mask_a), words_per_row, nthreads,
+                      _p(out.get("rgba")), _p(out.get("bitmap")), _p(out["final_t"]),
+                      _p(out["n_contrib"]), _p(out.get("resp")), _p(out.get("fg")),
+                      _p(out.get("bg")))
+    return out
+
+
+def prepare(view, attrs, sh, n, sh_per_ch, w, h, tile_w=16, max_isects=None, nthreads=1):
+    """prepare_isects (src/render.rs:274-364): project, depth order, scan, map,
+    stable tile sort, tile ranges."""
+    pr = project(view, attrs, sh, n, sh_per_ch, tile_w, nthreads)
+    order = depth_order(pr["visible"], pr["depth_keys"])
+    offsets = scan_gather(order, pr["tile_counts"])
+    raw = int(pr["counters"][0])
+    cap = raw if max_isects is None else max_isects
+    tx = (w + tile_w - 1) // tile_w
+    ty = (h + tile_w - 1) // tile_w
+    tid, gid = map_isects(order, pr["bbox"], offsets, tx, cap)
+    ni = min(raw, cap)
+    tid, gid = tid[:ni], gid[:ni]
+    bits = bits_for(tx * ty)
+    stid, sgid = argsort_u32(tid, gid, bits)
+    ranges = tile_ranges(stid, tx * ty)
+    pr.update(depth_order=order, offsets=offsets, tile_ids=stid, gaussian_ids=sgid,
+              num_isects=ni, num_isects_raw=raw, num_visible=len(order), ranges=ranges,
+              tiles_x=tx, tiles_y=ty, tile_bits=bits)
+    return pr
+
+
+def render(view, attrs, sh, n, sh_per_ch, w, h, tile_w=16, mode=MODE_RGB, scale=0.0,
+           mask=None, words_per_row=0, nthreads=1):
+    pr = prepare(view, attrs, sh, n, sh_per_ch, w, h, tile_w, nthreads=nthreads)
+    ras = rasterize(mode, w, h, pr["ranges"], pr["gaussian_ids"], pr["projected"], n, tile_w,
+                    scale, mask, words_per_row, nthreads)
+    pr.update(ras)
+    return pr
+
+
+def lgamma(x):
+    return LIB.sfo_lgamma(x)
+
+
+def digamma(x):
+    return LIB.sfo_digamma(x)
+
+
+def beta_entropy(a, b):
+    return LIB.sfo_beta_entropy(a, b)
+
+
+def eig(a, b, resp, scale):
+    a = _f32(a); b = _f32(b); resp = _u32(resp)
+    return LIB.sfo_eig(_p(a), _p(b), _p(resp), len(a), scale)
+
+
+def beta_update(a, b, fg, bg, scale):
+    a = _f32(a).copy(); b = _f32(b).copy()
+    LIB.sfo_beta_update(scale, _p(_u32(fg)), _p(_u32(bg)), len(a), _p(a), _p(b))
+    return a, b
+
+
+def tile_keys_u64(tile_ids, gaussian_ids, depth_keys):
+    """The sorted u64 ``tile << 32 | depth`` keys of the two-stage sort."""
+    return (np.asarray(tile_ids, np.uint64) << np.uint64(32)) | \
+        np.asarray(depth_keys, np.uint64)[np.asarray(gaussian_ids, np.int64)]

@@ -1,0 +1,380 @@
+// libsfk C ABI: context, storage, camera, sort/scan primitives and the
+// render / seg orchestration (the host side of src/render.rs, src/seg/mod.rs,
+// src/seg/beta.rs and crates/sort, over one HIP stream).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+
+#include "sfk_internal.h"
+
+using namespace sfk;
+
+namespace sfk {
+
+int fail(sfk_ctx* ctx, int code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int hip_fail(sfk_ctx* ctx, hipError_t e, const char* what) {
+    return fail(ctx, e == hipErrorOutOfMemory ? SFK_E_NOMEM : SFK_E_HIP, "%s: %s (%d)", what,
+                hipGetErrorString(e), (int)e);
+}
+
+}  // namespace sfk
+
+// ----------------------------------------------------------------- context
+
+extern "C" int sfk_ctx_create(int device, sfk_ctx** out) {
+    if (!out) return SFK_E_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        return SFK_E_HIP;
+    if (hipSetDevice(device) != hipSuccess) return SFK_E_HIP;
+    auto* c = new (std::nothrow) sfk_ctx();
+    if (!c) return SFK_E_NOMEM;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipMalloc(&c->eig_out, sizeof(double)) != hipSuccess) {
+        delete c;
+        return SFK_E_HIP;
+    }
+    *out = c;
+    return SFK_OK;
+}
+
+extern "C" void sfk_ctx_destroy(sfk_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    if (c->eig_partials) hipFree(c->eig_partials);
+    if (c->eig_out) hipFree(c->eig_out);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char* sfk_last_error(const sfk_ctx* c) { return c ? c->err.c_str() : ""; }
+extern "C" void* sfk_ctx_stream(sfk_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+extern "C" int sfk_sync(sfk_ctx* c) {
+    if (!c) return SFK_E_ARG;
+    SFK_HIP(c, hipStreamSynchronize(c->stream));
+    return SFK_OK;
+}
+
+extern "C" int sfk_device_name(sfk_ctx* c, char* buf, int len) {
+    if (!c || !buf || len <= 0) return SFK_E_ARG;
+    hipDeviceProp_t p;
+    SFK_HIP(c, hipGetDeviceProperties(&p, c->device));
+    snprintf(buf, (size_t)len, "%s %s cu=%d", p.name, p.gcnArchName, p.multiProcessorCount);
+    return SFK_OK;
+}
+
+extern "C" int sfk_alloc(sfk_ctx* c, size_t bytes, void** out) {
+    if (!c || !out) return SFK_E_ARG;
+    *out = nullptr;
+    SFK_HIP(c, hipMalloc(out, bytes ? bytes : 4));
+    return SFK_OK;
+}
+
+extern "C" int sfk_free(sfk_ctx* c, void* p) {
+    if (!c) return SFK_E_ARG;
+    if (p) SFK_HIP(c, hipFree(p));
+    return SFK_OK;
+}
+
+extern "C" int sfk_upload(sfk_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return SFK_E_ARG;
+    if (!bytes) return SFK_OK;
+    SFK_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    SFK_HIP(c, hipStreamSynchronize(c->stream));
+    return SFK_OK;
+}
+
+extern "C" int sfk_download(sfk_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return SFK_E_ARG;
+    if (!bytes) return SFK_OK;
+    SFK_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    SFK_HIP(c, hipStreamSynchronize(c->stream));
+    return SFK_OK;
+}
+
+extern "C" int sfk_fill_u32(sfk_ctx* c, uint32_t* buf, uint32_t v, size_t count) {
+    if (!c || (count && !buf)) return SFK_E_ARG;
+    SFK_HIP(c, launch_fill_u32(c->stream, buf, v, count));
+    return SFK_OK;
+}
+
+extern "C" int sfk_timer_start(sfk_ctx* c) {
+    if (!c) return SFK_E_ARG;
+    SFK_HIP(c, hipEventRecord(c->ev0, c->stream));
+    return SFK_OK;
+}
+
+extern "C" int sfk_timer_stop_ms(sfk_ctx* c, float* ms) {
+    if (!c || !ms) return SFK_E_ARG;
+    SFK_HIP(c, hipEventRecord(c->ev1, c->stream));
+    SFK_HIP(c, hipEventSynchronize(c->ev1));
+    SFK_HIP(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+    return SFK_OK;
+}
+
+// ------------------------------------------------------------------ camera
+
+// Rotation matrix columns of a unit quaternion (glam's Mat3::from_quat form).
+static void quat_cols(const float q[4], float c0[3], float c1[3], float c2[3]) {
+    float x = q[0], y = q[1], z = q[2], w = q[3];
+    float x2 = x + x, y2 = y + y, z2 = z + z;
+    float xx = x * x2, xy = x * y2, xz = x * z2;
+    float yy = y * y2, yz = y * z2, zz = z * z2;
+    float wx = w * x2, wy = w * y2, wz = w * z2;
+    c0[0] = 1.0f - (yy + zz); c0[1] = xy + wz;          c0[2] = xz - wy;
+    c1[0] = xy - wz;          c1[1] = 1.0f - (xx + zz); c1[2] = yz + wx;
+    c2[0] = xz + wy;          c2[1] = yz - wx;          c2[2] = 1.0f - (xx + yy);
+}
+
+static float hdot3(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// CameraViewLaunch::for_camera (src/project.rs:137-157); w2c = (R^T, -R^T p),
+// the exact rigid inverse of camera.rs:42-44.
+extern "C" int sfk_camera_view(const sfk_camera* cam, uint32_t w, uint32_t h, uint32_t tile_w,
+                               sfk_view* out) {
+    if (!cam || !out || w == 0 || h == 0 || !(tile_w == 8 || tile_w == 16 || tile_w == 32))
+        return SFK_E_ARG;
+    quat_cols(cam->rotation, out->rot0, out->rot1, out->rot2);
+    out->trans[0] = -hdot3(out->rot0, cam->position);
+    out->trans[1] = -hdot3(out->rot1, cam->position);
+    out->trans[2] = -hdot3(out->rot2, cam->position);
+    out->focal[0] = ((float)w * 0.5f) / tanf(cam->fov[0] * 0.5f);
+    out->focal[1] = ((float)h * 0.5f) / tanf(cam->fov[1] * 0.5f);
+    for (int k = 0; k < 3; ++k) out->world_pos[k] = cam->position[k];
+    out->img[0] = (float)w;
+    out->img[1] = (float)h;
+    out->tile_bounds[0] = (float)((w + tile_w - 1) / tile_w);
+    out->tile_bounds[1] = (float)((h + tile_w - 1) / tile_w);
+    return SFK_OK;
+}
+
+// Camera::look_at (src/camera.rs:57-63): minimal arc from +Z; antiparallel
+// turns pi about +Y.
+extern "C" int sfk_look_at(const float pos[3], const float target[3], float q[4]) {
+    if (!pos || !target || !q) return SFK_E_ARG;
+    float d[3] = {target[0] - pos[0], target[1] - pos[1], target[2] - pos[2]};
+    float inv = 1.0f / sqrtf(hdot3(d, d));
+    float dir[3] = {d[0] * inv, d[1] * inv, d[2] * inv};
+    const float eps = 1.1920929e-7f;
+    float dp = dir[2];
+    if (dp > 1.0f - 2.0f * eps) {
+        q[0] = 0.0f; q[1] = 0.0f; q[2] = 0.0f; q[3] = 1.0f;
+    } else if (dp < -1.0f + 2.0f * eps) {
+        float half = 3.14159265358979323846f * 0.5f;
+        q[0] = 0.0f; q[1] = sinf(half); q[2] = 0.0f; q[3] = cosf(half);
+    } else {
+        float c[4] = {-dir[1], dir[0], 0.0f, 1.0f + dp};
+        float len = sqrtf((c[0] * c[0] + c[1] * c[1]) + (c[2] * c[2] + c[3] * c[3]));
+        float r = 1.0f / len;
+        q[0] = c[0] * r; q[1] = c[1] * r; q[2] = c[2] * r; q[3] = c[3] * r;
+    }
+    return SFK_OK;
+}
+
+// --------------------------------------------------------------- sort/scan
+
+extern "C" uint32_t sfk_bits_for(uint32_t m) {
+    uint32_t v = m == 0 ? 0 : m - 1, b = 0;
+    while (v) { ++b; v >>= 1; }
+    return b;
+}
+
+struct sfk_radix_scratch {
+    sfk_ctx* ctx = nullptr;
+    uint32_t max_elems = 0, key_bytes = 4, digit_bits = 8, max_blocks = 0;
+    void* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* hist = nullptr;
+    uint32_t* totals = nullptr;
+};
+
+static void radix_release(sfk_radix_scratch* s) {
+    if (!s) return;
+    for (int i = 0; i < 2; ++i) {
+        if (s->keys[i]) hipFree(s->keys[i]);
+        if (s->vals[i]) hipFree(s->vals[i]);
+        s->keys[i] = nullptr;
+        s->vals[i] = nullptr;
+    }
+    if (s->hist) hipFree(s->hist);
+    if (s->totals) hipFree(s->totals);
+    s->hist = s->totals = nullptr;
+}
+
+static int radix_alloc(sfk_ctx* c, sfk_radix_scratch* s, uint32_t max_elems) {
+    size_t m = max_elems ? max_elems : 1;
+    s->max_elems = max_elems;
+    s->max_blocks = radix_max_blocks(max_elems);
+    for (int i = 0; i < 2; ++i) {
+        SFK_HIP(c, hipMalloc(&s->keys[i], m * s->key_bytes));
+        SFK_HIP(c, hipMalloc(&s->vals[i], m * 4));
+    }
+    SFK_HIP(c, hipMalloc(&s->hist, (size_t)s->max_blocks * 256 * 4));
+    SFK_HIP(c, hipMalloc(&s->totals, 256 * 4));
+    return SFK_OK;
+}
+
+extern "C" int sfk_radix_scratch_create(sfk_ctx* c, uint32_t max_elems, uint32_t key_bytes,
+                                        sfk_radix_scratch** out) {
+    if (!c || !out || (key_bytes != 4 && key_bytes != 8)) return fail(c, SFK_E_ARG, "radix scratch: bad args");
+    *out = nullptr;
+    auto* s = new (std::nothrow) sfk_radix_scratch();
+    if (!s) return SFK_E_NOMEM;
+    s->ctx = c;
+    s->key_bytes = key_bytes;
+    int rc = radix_alloc(c, s, max_elems);
+    if (rc) {
+        radix_release(s);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return SFK_OK;
+}
+
+extern "C" void sfk_radix_scratch_destroy(sfk_radix_scratch* s) {
+    if (!s) return;
+    radix_release(s);
+    delete s;
+}
+
+extern "C" int sfk_radix_set_digit_bits(sfk_radix_scratch* s, uint32_t bits) {
+    if (!s || bits < 4 || bits > 8) return SFK_E_ARG;
+    s->digit_bits = bits;
+    return SFK_OK;
+}
+
+template <typename K>
+static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const uint32_t* vals,
+                        uint32_t n, const uint32_t* d_n, uint32_t bits, int write_keys,
+                        K** out_keys, uint32_t** out_vals) {
+    if (!c || !s || !out_vals || (n && !keys)) return fail(c, SFK_E_ARG, "argsort: bad args");
+    if (sizeof(K) != s->key_bytes) return fail(c, SFK_E_ARG, "argsort: scratch key width %u", s->key_bytes);
+    if (bits > 8 * sizeof(K)) return fail(c, SFK_E_ARG, "argsort: %u bits on %zu-byte keys", bits, sizeof(K));
+    if (n > s->max_elems)
+        return fail(c, SFK_E_CAPACITY, "radix scratch undersized: sorted %u elements, capacity %u", n,
+                    s->max_elems);
+    // n <= 1 or bits == 0: nothing moves (sort.rs:319-321).
+    if ((n <= 1 || bits == 0) && vals) {
+        if (out_keys) *out_keys = const_cast<K*>(keys);
+        *out_vals = const_cast<uint32_t*>(vals);
+        return SFK_OK;
+    }
+    if (n == 0) {
+        if (out_keys) *out_keys = const_cast<K*>(keys);
+        *out_vals = s->vals[0];
+        return SFK_OK;
+    }
+    // Without values the identity permutation is wanted: a zero-width pass
+    // is a stable copy that materializes it.
+    if (n <= 1 || bits == 0) bits = 0;
+    const K* cur_k = keys;
+    const uint32_t* cur_v = vals;
+    int dst = 0;
+    const uint32_t db = s->digit_bits;
+    const uint32_t passes = bits == 0 ? 1u : (bits + db - 1) / db;
+    for (uint32_t p = 0; p < passes; ++p) {
+        uint32_t shift = p * db;
+        uint32_t width = bits == 0 ? 0u : std::min(db, bits - shift);
+        bool last = p + 1 == passes;
+        K* ok = (!last || write_keys) ? static_cast<K*>(s->keys[dst]) : nullptr;
+        uint32_t* ov = s->vals[dst];
+        hipError_t e;
+        if constexpr (sizeof(K) == 4)
+            e = launch_radix_pass_u32(c->stream, cur_k, cur_v, ok, ov, n, d_n, shift, width, db,
+                                      s->hist, s->max_blocks, s->totals);
+        else
+            e = launch_radix_pass_u64(c->stream, cur_k, cur_v, ok, ov, n, d_n, shift, width, db,
+                                      s->hist, s->max_blocks, s->totals);
+        if (e != hipSuccess) return hip_fail(c, e, "radix pass");
+        cur_k = static_cast<K*>(s->keys[dst]);
+        cur_v = ov;
+        dst ^= 1;
+    }
+    if (out_keys) *out_keys = const_cast<K*>(cur_k);
+    *out_vals = const_cast<uint32_t*>(cur_v);
+    return SFK_OK;
+}
+
+extern "C" int sfk_argsort_u32(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys,
+                               const uint32_t* vals, uint32_t n, const uint32_t* d_n,
+                               uint32_t bits, int write_keys, uint32_t** out_keys,
+                               uint32_t** out_vals) {
+    return argsort_impl<uint32_t>(c, s, keys, vals, n, d_n, bits, write_keys, out_keys, out_vals);
+}
+
+extern "C" int sfk_argsort_u64(sfk_ctx* c, sfk_radix_scratch* s, const uint64_t* keys,
+                               const uint32_t* vals, uint32_t n, const uint32_t* d_n,
+                               uint32_t bits, int write_keys, uint64_t** out_keys,
+                               uint32_t** out_vals) {
+    return argsort_impl<uint64_t>(c, s, keys, vals, n, d_n, bits, write_keys, out_keys, out_vals);
+}
+
+struct sfk_scan_scratch {
+    uint32_t max_elems = 0;
+    uint32_t* sums = nullptr;
+};
+
+extern "C" int sfk_scan_scratch_create(sfk_ctx* c, uint32_t max_elems, sfk_scan_scratch** out) {
+    if (!c || !out) return SFK_E_ARG;
+    *out = nullptr;
+    auto* s = new (std::nothrow) sfk_scan_scratch();
+    if (!s) return SFK_E_NOMEM;
+    s->max_elems = max_elems;
+    hipError_t e = hipMalloc(&s->sums, (size_t)div_up(max_elems ? max_elems : 1, kScanBlock) * 4);
+    if (e != hipSuccess) {
+        delete s;
+        return hip_fail(c, e, "scan scratch");
+    }
+    *out = s;
+    return SFK_OK;
+}
+
+extern "C" void sfk_scan_scratch_destroy(sfk_scan_scratch* s) {
+    if (!s) return;
+    if (s->sums) hipFree(s->sums);
+    delete s;
+}
+
+extern "C" int sfk_exclusive_scan_gather(sfk_ctx* c, const uint32_t* gids, const uint32_t* counts,
+                                         uint32_t* offsets, uint32_t n, sfk_scan_scratch* s) {
+    if (!c || !s) return SFK_E_ARG;
+    if (n == 0) return SFK_OK;  // scan.rs:226-228
+    if (!gids || !counts || !offsets) return fail(c, SFK_E_ARG, "scan_gather: null buffer");
+    if (n > s->max_elems)
+        return fail(c, SFK_E_CAPACITY, "offsets undersized: scanned %u elements, capacity %u", n,
+                    s->max_elems);
+    SFK_HIP(c, launch_scan(c->stream, gids, counts, offsets, n, s->sums));
+    return SFK_OK;
+}
+
+extern "C" int sfk_exclusive_scan(sfk_ctx* c, uint32_t* buf, uint32_t n, sfk_scan_scratch* s) {
+    if (!c || !s) return SFK_E_ARG;
+    if (n == 0) return SFK_OK;
+    if (!buf) return SFK_E_ARG;
+    if (n > s->max_elems) return fail(c, SFK_E_CAPACITY, "scan scratch undersized");
+    SFK_HIP(c, launch_scan(c->stream, nullptr, buf, buf, n, s->sums));
+    return SFK_OK;
+}

@@ -1,0 +1,81 @@
+// Wavefront (64-lane) and workgroup helpers for gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sfk {
+
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Number of set bits of `mask` below this lane.
+__device__ __forceinline__ uint32_t mask_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Inclusive prefix sum across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
+    uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += t;
+    }
+    return v;
+}
+
+// Exclusive prefix sum over a workgroup of NT threads (NT a multiple of 64,
+// at most 1024); `tmp` holds NT/64 + 1 words of LDS.  Returns the exclusive
+// prefix; *total receives the workgroup sum.  Contains barriers: every thread
+// of the group must call it.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* tmp,
+                                                        uint32_t* total) {
+    constexpr int NW = NT / 64;
+    uint32_t lane = lane_id();
+    uint32_t w = threadIdx.x / 64;
+    uint32_t inc = wave_incl_scan_u32(v);
+    if (lane == 63) tmp[w] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < NW; ++i) {
+            uint32_t c = tmp[i];
+            tmp[i] = run;
+            run += c;
+        }
+        tmp[NW] = run;
+    }
+    __syncthreads();
+    uint32_t r = tmp[w] + inc - v;
+    *total = tmp[NW];
+    __syncthreads();
+    return r;
+}
+
+// Spread consecutive workgroup ids of a 1D grid over the 8 XCDs of the chip
+// so that each XCD (and its L2) works on a contiguous range of logical ids:
+// the dispatcher assigns workgroup i to XCD i % 8.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
+    uint32_t xcd = bid & 7u, q = bid >> 3;
+    uint32_t per = nblocks >> 3, rem = nblocks & 7u;
+    uint32_t base = xcd * per + (xcd < rem ? xcd : rem);
+    return base + q;
+}
+
+}  // namespace sfk

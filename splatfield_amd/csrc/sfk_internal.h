@@ -1,0 +1,112 @@
+// Internal declarations shared by the libsfk translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/sfk.h"
+
+struct sfk_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // EIG reduction workspace (grown on demand).
+    double* eig_partials = nullptr;
+    uint32_t eig_partials_cap = 0;
+    double* eig_out = nullptr;
+};
+
+namespace sfk {
+
+constexpr int kWave = 64;
+
+// Set ctx->err and return code.
+int fail(sfk_ctx* ctx, int code, const char* fmt, ...);
+int hip_fail(sfk_ctx* ctx, hipError_t e, const char* what);
+
+#define SFK_HIP(ctx, expr)                                   \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return sfk::hip_fail(ctx, _e, #expr); \
+    } while (0)
+
+#define SFK_LAUNCHED(ctx, what)                                          \
+    do {                                                                 \
+        hipError_t _e = hipGetLastError();                               \
+        if (_e != hipSuccess) return sfk::hip_fail(ctx, _e, what);       \
+    } while (0)
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+inline uint32_t next_pow2(uint64_t v) {
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return (uint32_t)(p > 0xffffffffull ? 0x80000000u : p);
+}
+
+// ---- kernel launchers (each defined in the .hip file holding the kernel)
+
+hipError_t launch_fill_u32(hipStream_t s, uint32_t* buf, uint32_t v, size_t count);
+
+hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, const float* sh,
+                          uint32_t n, uint32_t sh_per_ch, uint32_t tile_w, uint32_t* depth_keys,
+                          float* proj, uint32_t* tile_counts, uint32_t* bbox, uint32_t* counters);
+
+// Radix sort: one pass = upsweep + digit scan + scatter.
+constexpr uint32_t kSortThreads = 256;
+constexpr uint32_t kSortEpt = 16;
+constexpr uint32_t kSortBlock = kSortThreads * kSortEpt;  // keys per block
+uint32_t radix_max_blocks(uint32_t max_elems);
+hipError_t launch_radix_pass_u32(hipStream_t s, const uint32_t* keys_in, const uint32_t* vals_in,
+                                 uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
+                                 const uint32_t* d_n, uint32_t shift, uint32_t width,
+                                 uint32_t digit_bits, uint32_t* hist, uint32_t max_blocks,
+                                 uint32_t* totals);
+hipError_t launch_radix_pass_u64(hipStream_t s, const uint64_t* keys_in, const uint32_t* vals_in,
+                                 uint64_t* keys_out, uint32_t* vals_out, uint32_t n,
+                                 const uint32_t* d_n, uint32_t shift, uint32_t width,
+                                 uint32_t digit_bits, uint32_t* hist, uint32_t max_blocks,
+                                 uint32_t* totals);
+
+// Scan.
+constexpr uint32_t kScanBlock = 4096;
+hipError_t launch_scan(hipStream_t s, const uint32_t* gids, const uint32_t* counts,
+                       uint32_t* offsets, uint32_t n, uint32_t* block_sums);
+
+// Raster stage.
+hipError_t launch_map_isects(hipStream_t s, const uint32_t* order, const uint32_t* tile_counts,
+                             const uint32_t* bbox, const uint32_t* offsets, uint32_t n,
+                             uint32_t tiles_per_row, uint32_t max_isects, uint32_t* tile_ids,
+                             uint32_t* gaussian_ids);
+hipError_t launch_tile_starts(hipStream_t s, const uint32_t* tile_ids, uint32_t n,
+                              const uint32_t* d_n, uint32_t num_tiles, uint32_t* starts);
+
+enum RasterMode { kRgb = 0, kTotal = 1, kEvidence = 2 };
+struct RasterArgs {
+    uint32_t w, h, tiles_x, tiles_y, tile_w, row_stride;
+    float scale;
+    const uint32_t* tile_starts;
+    const uint32_t* gids;
+    const float* proj;
+    const uint32_t* mask;
+    uint32_t words_per_row;
+    float* rgba;
+    uint32_t* bitmap;
+    float* final_t;
+    uint32_t* n_contrib;
+    uint32_t* resp;
+    uint32_t* fg;
+    uint32_t* bg;
+};
+hipError_t launch_rasterize(hipStream_t s, int mode, const RasterArgs& a);
+
+// Seg.
+hipError_t launch_eig(hipStream_t s, const float* a, const float* b, const uint32_t* resp,
+                      uint32_t n, float scale, double* partials, double* out);
+uint32_t eig_num_partials(uint32_t n);
+hipError_t launch_beta_update(hipStream_t s, float* a, float* b, const uint32_t* fg,
+                              const uint32_t* bg, uint32_t n, float scale);
+
+}  // namespace sfk

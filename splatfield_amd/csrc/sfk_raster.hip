@@ -1,0 +1,229 @@
+// Raster stage for gfx950: intersection emission, tile ranges and the
+// front-to-back compositor (src/raster.rs).
+#include "sfk_device.h"
+#include "sfk_internal.h"
+#include "sfk_math.h"
+
+namespace sfk {
+
+namespace {
+
+// map_isects (raster.rs:12-48): rank r emits its bbox tiles, row-major, at
+// offsets[r]; writes past max_isects are dropped while the walk continues.
+// Ranks whose splat emits nothing (culled or off-screen) exit at once.
+__global__ __launch_bounds__(256) void k_map_isects(const uint32_t* __restrict__ order,
+                                                    const uint32_t* __restrict__ tile_counts,
+                                                    const uint2* __restrict__ bbox,
+                                                    const uint32_t* __restrict__ offsets,
+                                                    uint32_t n, uint32_t tiles_per_row,
+                                                    uint32_t max_isects,
+                                                    uint32_t* __restrict__ tile_ids,
+                                                    uint32_t* __restrict__ gaussian_ids) {
+    uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r >= n) return;
+    uint32_t g = order[r];
+    if (tile_counts[g] == 0) return;
+    uint2 bb = bbox[g];
+    uint32_t min_x = bb.x & 0xffffu, min_y = bb.x >> 16, max_x = bb.y & 0xffffu,
+             max_y = bb.y >> 16;
+    uint32_t slot = offsets[r];
+    for (uint32_t ty = min_y; ty < max_y; ++ty) {
+        for (uint32_t tx = min_x; tx < max_x; ++tx) {
+            if (slot < max_isects) {
+                tile_ids[slot] = tx + ty * tiles_per_row;
+                gaussian_ids[slot] = g;
+            }
+            ++slot;
+        }
+    }
+}
+
+// starts[t] = lower_bound(tile_ids, t) for t in [0, num_tiles] (raster.rs:
+// 52-65): isect k writes every t in (tile_ids[k-1], tile_ids[k]], and k = n
+// closes (tile_ids[n-1], num_tiles].  Each t is written exactly once.
+__global__ __launch_bounds__(256) void k_tile_starts(const uint32_t* __restrict__ tile_ids,
+                                                     uint32_t n_max,
+                                                     const uint32_t* __restrict__ d_n,
+                                                     uint32_t num_tiles,
+                                                     uint32_t* __restrict__ starts) {
+    uint32_t n = n_max;
+    if (d_n) n = *d_n < n_max ? *d_n : n_max;
+    uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k > n) return;
+    uint32_t lo = k == 0 ? 0u : tile_ids[k - 1] + 1u;
+    uint32_t hi = k == n ? num_tiles : tile_ids[k];
+    for (uint32_t t = lo; t <= hi; ++t) starts[t] = k;
+}
+
+constexpr float kInv255 = 1.0f / 255.0f;
+// exp(-5.6) * 1.0 < 1/255: beyond this Gaussian power no splat (opacity <= 1)
+// reaches the contribution threshold, whatever the last ulp of exp.
+constexpr float kPowerSkip = 5.6f;
+
+// rasterize_kernel (raster.rs:87-230): one workgroup per TW x TW tile, one
+// thread per pixel.  Splat rows are staged through LDS in chunks of TW*TW.
+// The responsibility modes add each splat's per-pixel u32 weights across the
+// wavefront, then across the workgroup in LDS, and issue one global atomic
+// per (splat, tile) — the integer sum is order independent, so the result is
+// bit-identical to one atomic per pixel.
+template <int MODE, int TW>
+__global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
+    constexpr int NT = TW * TW;
+    __shared__ float4 s_geo[NT];  // mx, my, conic a, conic b
+    __shared__ float4 s_aux[NT];  // conic c, opacity, r, g
+    __shared__ float s_blue[MODE == kRgb ? NT : 1];
+    __shared__ uint32_t s_gid[MODE == kRgb ? 1 : NT];
+    __shared__ uint32_t s_q0[MODE == kRgb ? 1 : NT];
+    __shared__ uint32_t s_q1[MODE == kEvidence ? NT : 1];
+
+    const uint32_t ntiles = a.tiles_x * a.tiles_y;
+    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
+    const uint32_t tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const uint32_t t = threadIdx.x;
+    const uint32_t px = tx * TW + (t % TW), py = ty * TW + (t / TW);
+    const bool in_bounds = px < a.w && py < a.h;
+    const uint32_t start = a.tile_starts[tile], end = a.tile_starts[tile + 1];
+    const float pxf = (float)px + 0.5f, pyf = (float)py + 0.5f;
+
+    uint32_t inside = 0;
+    if (MODE == kEvidence && in_bounds)
+        inside = (a.mask[px / 32 + py * a.words_per_row] >> (px % 32)) & 1u;
+
+    float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    bool done = !in_bounds;
+    uint32_t last = start;
+    const float4* proj = reinterpret_cast<const float4*>(a.proj);
+
+    for (uint32_t chunk = start; chunk < end; chunk += NT) {
+        __syncthreads();  // previous chunk fully consumed
+        const uint32_t idx = chunk + t;
+        if (idx < end) {
+            uint32_t g = a.gids[idx];
+            const float4* row = proj + (size_t)g * 3;
+            s_geo[t] = row[0];
+            float4 aux = row[1];
+            s_aux[t] = aux;
+            if constexpr (MODE == kRgb) {
+                s_blue[t] = row[2].x;
+            } else {
+                s_gid[t] = g;
+                s_q0[t] = 0;
+                if constexpr (MODE == kEvidence) s_q1[t] = 0;
+            }
+        }
+        // Staging fence and whole-tile early exit (raster.rs:166-171).
+        if (__syncthreads_count(!done) == 0) break;
+        const uint32_t cnt = (end - chunk) < (uint32_t)NT ? (end - chunk) : (uint32_t)NT;
+        for (uint32_t j = 0; j < cnt; ++j) {
+            uint32_t q = 0;
+            if (!done) {
+                float4 geo = s_geo[j];
+                float dx = geo.x - pxf, dy = geo.y - pyf;
+                float4 aux = s_aux[j];
+                float power = 0.5f * (geo.z * dx * dx + aux.x * dy * dy) + geo.w * dx * dy;
+                if (!(power > kPowerSkip)) {
+                    float alpha = fminf(aux.y * sfk_expf(-power), 0.999f);
+                    if (alpha >= kInv255) {
+                        float vis = alpha * T;
+                        if constexpr (MODE == kRgb) {
+                            cr += aux.z * vis;
+                            cg += aux.w * vis;
+                            cb += s_blue[j] * vis;
+                        } else {
+                            q = sfk_f2u(vis * a.scale);
+                        }
+                        T *= 1.0f - alpha;
+                        last = chunk + j + 1;
+                        if (T < kInv255) done = true;
+                    }
+                }
+            }
+            if constexpr (MODE != kRgb) {
+                if (__ballot(q != 0)) {
+                    if constexpr (MODE == kTotal) {
+                        uint32_t s = wave_sum_u32(q);
+                        if (lane_id() == 0 && s) atomicAdd(&s_q0[j], s);
+                    } else {
+                        uint32_t sf = wave_sum_u32(inside ? q : 0u);
+                        uint32_t sb = wave_sum_u32(inside ? 0u : q);
+                        if (lane_id() == 0) {
+                            if (sf) atomicAdd(&s_q0[j], sf);
+                            if (sb) atomicAdd(&s_q1[j], sb);
+                        }
+                    }
+                }
+            }
+        }
+        if constexpr (MODE != kRgb) {
+            __syncthreads();
+            if (t < cnt) {
+                uint32_t g = s_gid[t];
+                if constexpr (MODE == kTotal) {
+                    if (s_q0[t]) atomicAdd(&a.resp[g], s_q0[t]);
+                } else {
+                    if (s_q0[t]) atomicAdd(&a.fg[g], s_q0[t]);
+                    if (s_q1[t]) atomicAdd(&a.bg[g], s_q1[t]);
+                }
+            }
+        }
+    }
+
+    if (in_bounds) {
+        const size_t p = (size_t)py * a.w + px;
+        if (a.final_t) a.final_t[p] = T;
+        if (a.n_contrib) a.n_contrib[p] = last;
+        if constexpr (MODE == kRgb) {
+            if (a.rgba) reinterpret_cast<float4*>(a.rgba)[p] = make_float4(cr, cg, cb, 1.0f - T);
+            if (a.bitmap) {
+                uint32_t r = sfk_f2u(sfk_clampf(cr * 255.0f, 0.0f, 255.0f));
+                uint32_t g = sfk_f2u(sfk_clampf(cg * 255.0f, 0.0f, 255.0f));
+                uint32_t b = sfk_f2u(sfk_clampf(cb * 255.0f, 0.0f, 255.0f));
+                uint32_t al = sfk_f2u(sfk_clampf((1.0f - T) * 255.0f, 0.0f, 255.0f));
+                a.bitmap[px + (size_t)py * a.row_stride] = r | (g << 8) | (b << 16) | (al << 24);
+            }
+        }
+    }
+}
+
+template <int MODE>
+hipError_t raster_tw(hipStream_t s, const RasterArgs& a) {
+    uint32_t nt = a.tiles_x * a.tiles_y;
+    if (nt == 0) return hipSuccess;
+    switch (a.tile_w) {
+        case 8: k_rasterize<MODE, 8><<<nt, 64, 0, s>>>(a); break;
+        case 16: k_rasterize<MODE, 16><<<nt, 256, 0, s>>>(a); break;
+        case 32: k_rasterize<MODE, 32><<<nt, 1024, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_map_isects(hipStream_t s, const uint32_t* order, const uint32_t* tile_counts,
+                             const uint32_t* bbox, const uint32_t* offsets, uint32_t n,
+                             uint32_t tiles_per_row, uint32_t max_isects, uint32_t* tile_ids,
+                             uint32_t* gaussian_ids) {
+    if (n == 0) return hipSuccess;
+    k_map_isects<<<div_up(n, 256), 256, 0, s>>>(order, tile_counts,
+                                                reinterpret_cast<const uint2*>(bbox), offsets, n,
+                                                tiles_per_row, max_isects, tile_ids, gaussian_ids);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_starts(hipStream_t s, const uint32_t* tile_ids, uint32_t n,
+                              const uint32_t* d_n, uint32_t num_tiles, uint32_t* starts) {
+    k_tile_starts<<<div_up((uint64_t)n + 1, 256), 256, 0, s>>>(tile_ids, n, d_n, num_tiles, starts);
+    return hipGetLastError();
+}
+
+hipError_t launch_rasterize(hipStream_t s, int mode, const RasterArgs& a) {
+    switch (mode) {
+        case kRgb: return raster_tw<kRgb>(s, a);
+        case kTotal: return raster_tw<kTotal>(s, a);
+        case kEvidence: return raster_tw<kEvidence>(s, a);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace sfk

@@ -1,0 +1,559 @@
+// Render and seg orchestration behind the C ABI: Splats, RenderScratch,
+// prepare_isects / finalize / render_with (src/render.rs:114-460) and the
+// responsibility, EIG and Beta-update entry points (src/seg/mod.rs:146-210,
+// src/seg/beta.rs:440-500).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+
+#include "sfk_internal.h"
+
+using namespace sfk;
+
+namespace {
+
+// render.rs:12-18
+constexpr uint32_t kIsectsUpperBound = 2u * 512u * 65535u;
+constexpr uint32_t kInitialIsectsCap = 1u << 22;
+
+}  // namespace
+
+struct sfk_splats {
+    sfk_ctx* ctx = nullptr;
+    uint32_t n = 0, sh_per_ch = 0;
+    float* attrs = nullptr;
+    float* sh = nullptr;
+    float bmin[3], bmax[3];
+};
+
+extern "C" int sfk_splats_create(sfk_ctx* c, const float* attrs, const float* sh, uint32_t n,
+                                 uint32_t sh_per_ch, sfk_splats** out) {
+    if (!c || !out) return SFK_E_ARG;
+    *out = nullptr;
+    if (n == 0) return fail(c, SFK_E_ARG, "Splats::new: zero splats");
+    if (!attrs || !sh || !(sh_per_ch == 1 || sh_per_ch == 4 || sh_per_ch == 9 || sh_per_ch == 16))
+        return fail(c, SFK_E_ARG, "splats: bad attributes/sh (sh_per_ch=%u)", sh_per_ch);
+    auto* s = new (std::nothrow) sfk_splats();
+    if (!s) return SFK_E_NOMEM;
+    s->ctx = c;
+    s->n = n;
+    s->sh_per_ch = sh_per_ch;
+    size_t ab = (size_t)n * SFK_ATTR_PLANES * 4, sb = (size_t)n * sh_per_ch * 3 * 4;
+    hipError_t e = hipMalloc(&s->attrs, ab);
+    if (e == hipSuccess) e = hipMalloc(&s->sh, sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->attrs, attrs, ab, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->sh, sh, sb, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        if (s->attrs) hipFree(s->attrs);
+        if (s->sh) hipFree(s->sh);
+        delete s;
+        return hip_fail(c, e, "splats upload");
+    }
+    // Scene AABB of the xyz planes (render.rs:253-259).
+    for (int k = 0; k < 3; ++k) {
+        s->bmin[k] = 3.40282347e38f;
+        s->bmax[k] = -3.40282347e38f;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            float v = attrs[(size_t)k * n + i];
+            s->bmin[k] = std::min(s->bmin[k], v);
+            s->bmax[k] = std::max(s->bmax[k], v);
+        }
+    *out = s;
+    return SFK_OK;
+}
+
+extern "C" void sfk_splats_destroy(sfk_splats* s) {
+    if (!s) return;
+    if (s->attrs) hipFree(s->attrs);
+    if (s->sh) hipFree(s->sh);
+    delete s;
+}
+
+extern "C" int sfk_splats_info(const sfk_splats* s, uint32_t* n, uint32_t* sh_per_ch,
+                               float bmin[3], float bmax[3], const float** d_attrs,
+                               const float** d_sh) {
+    if (!s) return SFK_E_ARG;
+    if (n) *n = s->n;
+    if (sh_per_ch) *sh_per_ch = s->sh_per_ch;
+    if (bmin) memcpy(bmin, s->bmin, 12);
+    if (bmax) memcpy(bmax, s->bmax, 12);
+    if (d_attrs) *d_attrs = s->attrs;
+    if (d_sh) *d_sh = s->sh;
+    return SFK_OK;
+}
+
+struct sfk_render_scratch {
+    sfk_ctx* ctx = nullptr;
+    uint32_t n = 0, w = 0, h = 0, tile_w = 16, tiles_x = 0, tiles_y = 0, row_stride = 0;
+    uint32_t capacity = 0, digit_bits = 8;
+    uint32_t* depth_keys = nullptr;
+    const uint32_t* depth_order = nullptr;  // aliases sort_depth
+    float* projected = nullptr;
+    uint32_t* tile_counts = nullptr;
+    uint32_t* bbox = nullptr;
+    uint32_t* offsets = nullptr;
+    uint32_t* counters = nullptr;  // [isects_raw, visible, pad, pad]
+    uint32_t* tile_ids = nullptr;
+    uint32_t* gaussian_ids = nullptr;
+    uint32_t* tile_starts = nullptr;
+    uint32_t* bitmap = nullptr;
+    float* rgba = nullptr;
+    float* final_t = nullptr;
+    uint32_t* n_contrib = nullptr;
+    sfk_radix_scratch* sort_depth = nullptr;
+    sfk_radix_scratch* sort_tile = nullptr;
+    sfk_scan_scratch* scan = nullptr;
+    // Previous frame's raw emission, read back asynchronously for growth.
+    uint32_t* host_counters = nullptr;
+    hipEvent_t counters_ready = nullptr;
+    bool counters_pending = false;
+    uint32_t last_isects_raw = 0;
+    // Current frame outputs (sorted isects alias sort_tile).
+    const uint32_t* sorted_tiles = nullptr;
+    const uint32_t* sorted_gids = nullptr;
+};
+
+static void scratch_free_frame(sfk_render_scratch* s) {
+    void* bufs[] = {s->depth_keys, s->projected, s->tile_counts, s->bbox, s->offsets,
+                    s->counters, s->tile_starts, s->bitmap, s->rgba, s->final_t, s->n_contrib,
+                    s->tile_ids, s->gaussian_ids};
+    for (void* p : bufs)
+        if (p) hipFree(p);
+    s->depth_keys = nullptr; s->projected = nullptr; s->tile_counts = nullptr; s->bbox = nullptr;
+    s->offsets = nullptr; s->counters = nullptr; s->tile_starts = nullptr; s->bitmap = nullptr;
+    s->rgba = nullptr; s->final_t = nullptr; s->n_contrib = nullptr; s->tile_ids = nullptr;
+    s->gaussian_ids = nullptr;
+    sfk_radix_scratch_destroy(s->sort_depth);
+    sfk_radix_scratch_destroy(s->sort_tile);
+    sfk_scan_scratch_destroy(s->scan);
+    s->sort_depth = s->sort_tile = nullptr;
+    s->scan = nullptr;
+    s->depth_order = s->sorted_tiles = s->sorted_gids = nullptr;
+}
+
+static int alloc_isects(sfk_render_scratch* s, uint32_t cap) {
+    sfk_ctx* c = s->ctx;
+    if (s->tile_ids) hipFree(s->tile_ids);
+    if (s->gaussian_ids) hipFree(s->gaussian_ids);
+    s->tile_ids = s->gaussian_ids = nullptr;
+    sfk_radix_scratch_destroy(s->sort_tile);
+    s->sort_tile = nullptr;
+    s->sorted_tiles = s->sorted_gids = nullptr;
+    SFK_HIP(c, hipMalloc(&s->tile_ids, (size_t)cap * 4));
+    SFK_HIP(c, hipMalloc(&s->gaussian_ids, (size_t)cap * 4));
+    int rc = sfk_radix_scratch_create(c, cap, 4, &s->sort_tile);
+    if (rc) return rc;
+    sfk_radix_set_digit_bits(s->sort_tile, s->digit_bits);
+    s->capacity = cap;
+    return SFK_OK;
+}
+
+// RenderScratch::new (render.rs:143-167).
+static int scratch_build(sfk_render_scratch* s, uint32_t n, uint32_t w, uint32_t h) {
+    sfk_ctx* c = s->ctx;
+    scratch_free_frame(s);
+    s->n = n; s->w = w; s->h = h;
+    s->tiles_x = (w + s->tile_w - 1) / s->tile_w;
+    s->tiles_y = (h + s->tile_w - 1) / s->tile_w;
+    s->row_stride = ((w * 4 + 255) / 256 * 256) / 4;
+    uint64_t tiles = (uint64_t)s->tiles_x * s->tiles_y;
+    size_t nn = n ? n : 1, px = (size_t)w * h;
+    SFK_HIP(c, hipMalloc(&s->depth_keys, nn * 4));
+    SFK_HIP(c, hipMalloc(&s->projected, nn * SFK_PROJ_STRIDE * 4));
+    SFK_HIP(c, hipMalloc(&s->tile_counts, nn * 4));
+    SFK_HIP(c, hipMalloc(&s->bbox, nn * 8));
+    SFK_HIP(c, hipMalloc(&s->offsets, nn * 4));
+    SFK_HIP(c, hipMalloc(&s->counters, 16));
+    SFK_HIP(c, hipMalloc(&s->tile_starts, (size_t)(tiles + 1) * 4));
+    SFK_HIP(c, hipMalloc(&s->bitmap, (size_t)h * s->row_stride * 4));
+    SFK_HIP(c, hipMalloc(&s->rgba, px * 16));
+    SFK_HIP(c, hipMalloc(&s->final_t, px * 4));
+    SFK_HIP(c, hipMalloc(&s->n_contrib, px * 4));
+    int rc = sfk_radix_scratch_create(c, n, 4, &s->sort_depth);
+    if (rc) return rc;
+    sfk_radix_set_digit_bits(s->sort_depth, s->digit_bits);
+    rc = sfk_scan_scratch_create(c, n, &s->scan);
+    if (rc) return rc;
+    uint64_t cap = std::min<uint64_t>(tiles * n, kInitialIsectsCap);
+    return alloc_isects(s, next_pow2(cap ? cap : 1));
+}
+
+extern "C" int sfk_render_scratch_create(sfk_ctx* c, uint32_t n, uint32_t w, uint32_t h,
+                                         sfk_render_scratch** out) {
+    if (!c || !out) return SFK_E_ARG;
+    *out = nullptr;
+    if (n == 0 || w == 0 || h == 0) return fail(c, SFK_E_ARG, "render scratch: empty shape");
+    auto* s = new (std::nothrow) sfk_render_scratch();
+    if (!s) return SFK_E_NOMEM;
+    s->ctx = c;
+    int rc = SFK_OK;
+    if (hipHostMalloc(&s->host_counters, 16, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&s->counters_ready, hipEventDisableTiming) != hipSuccess)
+        rc = fail(c, SFK_E_HIP, "render scratch: pinned counters");
+    if (!rc) rc = scratch_build(s, n, w, h);
+    if (rc) {
+        sfk_render_scratch_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return SFK_OK;
+}
+
+extern "C" void sfk_render_scratch_destroy(sfk_render_scratch* s) {
+    if (!s) return;
+    if (s->ctx) hipStreamSynchronize(s->ctx->stream);
+    scratch_free_frame(s);
+    if (s->host_counters) hipHostFree(s->host_counters);
+    if (s->counters_ready) hipEventDestroy(s->counters_ready);
+    delete s;
+}
+
+extern "C" int sfk_render_scratch_set_tile_width(sfk_render_scratch* s, uint32_t tw) {
+    if (!s || !(tw == 8 || tw == 16 || tw == 32)) return SFK_E_ARG;
+    if (tw == s->tile_w) return SFK_OK;
+    s->tile_w = tw;
+    uint32_t raw = s->last_isects_raw;
+    int rc = scratch_build(s, s->n, s->w, s->h);
+    s->last_isects_raw = raw;
+    return rc;
+}
+
+extern "C" int sfk_render_scratch_set_digit_bits(sfk_render_scratch* s, uint32_t bits) {
+    if (!s || bits < 4 || bits > 8) return SFK_E_ARG;
+    s->digit_bits = bits;
+    if (s->sort_depth) sfk_radix_set_digit_bits(s->sort_depth, bits);
+    if (s->sort_tile) sfk_radix_set_digit_bits(s->sort_tile, bits);
+    return SFK_OK;
+}
+
+static int grow_to(sfk_render_scratch* s, uint64_t want) {
+    uint32_t cap = std::min<uint64_t>(next_pow2(want ? want : 1), kIsectsUpperBound);
+    if (cap > s->capacity) return alloc_isects(s, cap);
+    return SFK_OK;
+}
+
+extern "C" int sfk_render_scratch_reserve_isects(sfk_render_scratch* s, uint32_t isects) {
+    if (!s) return SFK_E_ARG;
+    return grow_to(s, isects);
+}
+
+extern "C" int sfk_render_scratch_buffers(const sfk_render_scratch* s, sfk_render_buffers* o) {
+    if (!s || !o) return SFK_E_ARG;
+    o->depth_keys = s->depth_keys;
+    o->depth_order = s->depth_order;
+    o->projected = s->projected;
+    o->tile_counts = s->tile_counts;
+    o->bbox = s->bbox;
+    o->offsets = s->offsets;
+    o->counters = s->counters;
+    o->bitmap = s->bitmap;
+    o->rgba = s->rgba;
+    o->final_t = s->final_t;
+    o->n_contrib = s->n_contrib;
+    o->n = s->n; o->w = s->w; o->h = s->h; o->row_stride = s->row_stride;
+    o->capacity = s->capacity;
+    return SFK_OK;
+}
+
+// RenderScratch::prepare (render.rs:169-197): rebuild on a shape change, then
+// grow the isect buffers from the previous frame's raw emission.
+static int scratch_prepare(sfk_render_scratch* s, uint32_t n, uint32_t w, uint32_t h) {
+    sfk_ctx* c = s->ctx;
+    if (s->counters_pending) {
+        SFK_HIP(c, hipEventSynchronize(s->counters_ready));
+        s->last_isects_raw = s->host_counters[0];
+        s->counters_pending = false;
+    }
+    uint32_t raw = s->last_isects_raw;
+    if (n != s->n || w != s->w || h != s->h) {
+        int rc = scratch_build(s, n, w, h);
+        if (rc) return rc;
+    }
+    return grow_to(s, raw);
+}
+
+static void fill_isects(const sfk_render_scratch* s, sfk_tile_isects* o) {
+    o->tile_ids = s->sorted_tiles;
+    o->gaussian_ids = s->sorted_gids;
+    o->tile_starts = s->tile_starts;
+    o->d_counters = s->counters;
+    o->capacity = s->capacity;
+    o->tiles_x = s->tiles_x;
+    o->tiles_y = s->tiles_y;
+    o->tile_w = s->tile_w;
+    o->row_stride = s->row_stride;
+    o->w = s->w;
+    o->h = s->h;
+}
+
+// Splats::prepare_isects (render.rs:271-364) without the host readback: the
+// depth sort runs over all n splats (culled ones carry key 0xffffffff and
+// sort last, contributing no tiles), the gather scan over all n ranks, and
+// the tile sort over the device-side isect count.
+extern "C" int sfk_prepare_isects_view(sfk_ctx* c, const sfk_splats* sp,
+                                       sfk_render_scratch* s, const sfk_view* view, uint32_t w,
+                                       uint32_t h, sfk_tile_isects* out) {
+    if (!c || !sp || !s || !view || !out) return SFK_E_ARG;
+    if ((uint32_t)view->img[0] != w || (uint32_t)view->img[1] != h)
+        return fail(c, SFK_E_ARG, "view is for %gx%g, not %ux%u", view->img[0], view->img[1], w, h);
+    int rc = scratch_prepare(s, sp->n, w, h);
+    if (rc) return rc;
+    if ((uint32_t)view->tile_bounds[0] != s->tiles_x || (uint32_t)view->tile_bounds[1] != s->tiles_y)
+        return fail(c, SFK_E_ARG, "view tile bounds do not match the scratch tile width %u", s->tile_w);
+    const uint32_t n = sp->n;
+    const uint32_t tiles = s->tiles_x * s->tiles_y;
+
+    SFK_HIP(c, launch_fill_u32(c->stream, s->counters, 0u, 4));
+    SFK_HIP(c, launch_project(c->stream, *view, sp->attrs, sp->sh, n, sp->sh_per_ch, s->tile_w,
+                              s->depth_keys, s->projected, s->tile_counts, s->bbox, s->counters));
+    SFK_HIP(c, hipMemcpyAsync(s->host_counters, s->counters, 8, hipMemcpyDeviceToHost, c->stream));
+    SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));
+    s->counters_pending = true;
+
+    uint32_t* order = nullptr;
+    rc = sfk_argsort_u32(c, s->sort_depth, s->depth_keys, nullptr, n, nullptr, 32, 0, nullptr,
+                         &order);
+    if (rc) return rc;
+    s->depth_order = order;
+    rc = sfk_exclusive_scan_gather(c, order, s->tile_counts, s->offsets, n, s->scan);
+    if (rc) return rc;
+    SFK_HIP(c, launch_map_isects(c->stream, order, s->tile_counts, s->bbox, s->offsets, n,
+                                 s->tiles_x, s->capacity, s->tile_ids, s->gaussian_ids));
+    uint32_t* st = nullptr;
+    uint32_t* sg = nullptr;
+    rc = sfk_argsort_u32(c, s->sort_tile, s->tile_ids, s->gaussian_ids, s->capacity, s->counters,
+                         sfk_bits_for(tiles), 1, &st, &sg);
+    if (rc) return rc;
+    s->sorted_tiles = st;
+    s->sorted_gids = sg;
+    SFK_HIP(c, launch_tile_starts(c->stream, st, s->capacity, s->counters, tiles, s->tile_starts));
+    fill_isects(s, out);
+    return SFK_OK;
+}
+
+extern "C" int sfk_prepare_isects(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
+                                  const sfk_camera* cam, uint32_t w, uint32_t h,
+                                  sfk_tile_isects* out) {
+    if (!c || !s || !cam) return SFK_E_ARG;
+    sfk_view v;
+    int rc = sfk_camera_view(cam, w, h, s->tile_w, &v);
+    if (rc) return fail(c, rc, "camera view: bad arguments");
+    return sfk_prepare_isects_view(c, sp, s, &v, w, h, out);
+}
+
+extern "C" int sfk_tile_isects_counts(sfk_ctx* c, const sfk_tile_isects* t, uint32_t counts[3]) {
+    if (!c || !t || !counts) return SFK_E_ARG;
+    uint32_t h[2];
+    SFK_HIP(c, hipMemcpyAsync(h, t->d_counters, 8, hipMemcpyDeviceToHost, c->stream));
+    SFK_HIP(c, hipStreamSynchronize(c->stream));
+    counts[0] = h[0];
+    counts[1] = h[1];
+    counts[2] = std::min(h[0], t->capacity);
+    return SFK_OK;
+}
+
+static RasterArgs raster_args(sfk_render_scratch* s, const sfk_tile_isects* t) {
+    RasterArgs a;
+    memset(&a, 0, sizeof a);
+    a.w = t->w; a.h = t->h; a.tiles_x = t->tiles_x; a.tiles_y = t->tiles_y;
+    a.tile_w = t->tile_w; a.row_stride = t->row_stride;
+    a.tile_starts = t->tile_starts;
+    a.gids = t->gaussian_ids;
+    a.proj = s->projected;
+    return a;
+}
+
+static int check_isects(sfk_ctx* c, sfk_render_scratch* s, const sfk_tile_isects* t) {
+    if (!c || !s || !t) return SFK_E_ARG;
+    if (t->w != s->w || t->h != s->h || t->tile_w != s->tile_w || !t->gaussian_ids)
+        return fail(c, SFK_E_ARG, "isects were not prepared by this scratch");
+    return SFK_OK;
+}
+
+extern "C" int sfk_finalize_rgb(sfk_ctx* c, sfk_render_scratch* s, const sfk_tile_isects* t) {
+    int rc = check_isects(c, s, t);
+    if (rc) return rc;
+    RasterArgs a = raster_args(s, t);
+    a.rgba = s->rgba;
+    a.bitmap = s->bitmap;
+    a.final_t = s->final_t;
+    a.n_contrib = s->n_contrib;
+    SFK_HIP(c, launch_rasterize(c->stream, kRgb, a));
+    return SFK_OK;
+}
+
+extern "C" int sfk_finalize_total(sfk_ctx* c, sfk_render_scratch* s, const sfk_tile_isects* t,
+                                  float scale, uint32_t* resp) {
+    int rc = check_isects(c, s, t);
+    if (rc) return rc;
+    if (!resp) return fail(c, SFK_E_ARG, "finalize_total: null resp");
+    RasterArgs a = raster_args(s, t);
+    a.scale = scale;
+    a.resp = resp;
+    SFK_HIP(c, launch_rasterize(c->stream, kTotal, a));
+    return SFK_OK;
+}
+
+extern "C" int sfk_finalize_evidence(sfk_ctx* c, sfk_render_scratch* s, const sfk_tile_isects* t,
+                                     float scale, const uint32_t* mask, uint32_t words_per_row,
+                                     uint32_t* fg, uint32_t* bg) {
+    int rc = check_isects(c, s, t);
+    if (rc) return rc;
+    if (!mask || !fg || !bg || words_per_row < (t->w + 31) / 32)
+        return fail(c, SFK_E_ARG, "finalize_evidence: bad mask/fg/bg");
+    RasterArgs a = raster_args(s, t);
+    a.scale = scale;
+    a.mask = mask;
+    a.words_per_row = words_per_row;
+    a.fg = fg;
+    a.bg = bg;
+    SFK_HIP(c, launch_rasterize(c->stream, kEvidence, a));
+    return SFK_OK;
+}
+
+extern "C" int sfk_render_with(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
+                               const sfk_camera* cam, uint32_t w, uint32_t h,
+                               const uint32_t** bitmap, const float** rgba) {
+    sfk_tile_isects t;
+    int rc = sfk_prepare_isects(c, sp, s, cam, w, h, &t);
+    if (rc) return rc;
+    rc = sfk_finalize_rgb(c, s, &t);
+    if (rc) return rc;
+    if (bitmap) *bitmap = s->bitmap;
+    if (rgba) *rgba = s->rgba;
+    return SFK_OK;
+}
+
+// ---------------------------------------------------------------------- seg
+
+extern "C" uint32_t sfk_fractional_bits(uint32_t w, uint32_t h) {
+    uint64_t p = (uint64_t)w * h, np2 = 1;
+    uint32_t tz = 0;
+    while (np2 < p) { np2 <<= 1; ++tz; }
+    return tz >= 32 ? 0 : 32 - tz;
+}
+
+extern "C" int sfk_mask_pack(const uint8_t* bytes, uint32_t w, uint32_t h, uint32_t* words,
+                             uint32_t* fg_pixels) {
+    if (!bytes || !words || w == 0) return SFK_E_ARG;
+    uint32_t wpr = (w + 31) / 32;
+    memset(words, 0, (size_t)wpr * h * 4);
+    uint32_t fg = 0;
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x)
+            if (bytes[(size_t)y * w + x]) {
+                words[(size_t)y * wpr + x / 32] |= 1u << (x % 32);
+                ++fg;
+            }
+    if (fg_pixels) *fg_pixels = fg;
+    return SFK_OK;
+}
+
+static float scale_for(uint32_t w, uint32_t h) {
+    return (float)(1ull << sfk_fractional_bits(w, h));
+}
+
+extern "C" int sfk_accumulate_prepared(sfk_ctx* c, sfk_render_scratch* s,
+                                       const sfk_tile_isects* t, const uint32_t* mask,
+                                       uint32_t words_per_row, uint32_t* resp, uint32_t* fg,
+                                       uint32_t* bg, float* scale_out) {
+    int rc = check_isects(c, s, t);
+    if (rc) return rc;
+    float scale = scale_for(t->w, t->h);
+    if (scale_out) *scale_out = scale;
+    const uint32_t n = s->n;
+    if (!mask) {
+        if (!resp) return fail(c, SFK_E_ARG, "accumulate: null resp");
+        SFK_HIP(c, launch_fill_u32(c->stream, resp, 0u, n));
+        return sfk_finalize_total(c, s, t, scale, resp);
+    }
+    if (!fg || !bg) return fail(c, SFK_E_ARG, "accumulate: null fg/bg");
+    SFK_HIP(c, launch_fill_u32(c->stream, fg, 0u, n));
+    SFK_HIP(c, launch_fill_u32(c->stream, bg, 0u, n));
+    return sfk_finalize_evidence(c, s, t, scale, mask, words_per_row, fg, bg);
+}
+
+extern "C" int sfk_render_responsibility(sfk_ctx* c, const sfk_splats* sp,
+                                         sfk_render_scratch* s, const sfk_camera* cam, uint32_t w,
+                                         uint32_t h, uint32_t* resp, float* scale_out) {
+    sfk_tile_isects t;
+    int rc = sfk_prepare_isects(c, sp, s, cam, w, h, &t);
+    if (rc) return rc;
+    return sfk_accumulate_prepared(c, s, &t, nullptr, 0, resp, nullptr, nullptr, scale_out);
+}
+
+static int eig_workspace(sfk_ctx* c, uint32_t n) {
+    uint32_t np = eig_num_partials(n);
+    if (np > c->eig_partials_cap) {
+        if (c->eig_partials) hipFree(c->eig_partials);
+        c->eig_partials = nullptr;
+        c->eig_partials_cap = 0;
+        SFK_HIP(c, hipMalloc(&c->eig_partials, (size_t)np * 8));
+        c->eig_partials_cap = np;
+    }
+    return SFK_OK;
+}
+
+extern "C" int sfk_eig_device(sfk_ctx* c, const float* a, const float* b, const uint32_t* resp,
+                              uint32_t n, float scale, double* d_out) {
+    if (!c || !d_out || (n && (!a || !b || !resp))) return SFK_E_ARG;
+    int rc = eig_workspace(c, n);
+    if (rc) return rc;
+    SFK_HIP(c, launch_eig(c->stream, a, b, resp, n, scale, c->eig_partials, d_out));
+    return SFK_OK;
+}
+
+extern "C" int sfk_eig(sfk_ctx* c, const float* a, const float* b, const uint32_t* resp,
+                       uint32_t n, float scale, double* out) {
+    if (!c || !out) return SFK_E_ARG;
+    int rc = sfk_eig_device(c, a, b, resp, n, scale, c->eig_out);
+    if (rc) return rc;
+    SFK_HIP(c, hipMemcpyAsync(out, c->eig_out, 8, hipMemcpyDeviceToHost, c->stream));
+    SFK_HIP(c, hipStreamSynchronize(c->stream));
+    return SFK_OK;
+}
+
+extern "C" int sfk_beta_update(sfk_ctx* c, float* a, float* b, const uint32_t* fg,
+                               const uint32_t* bg, uint32_t n, float scale) {
+    if (!c || (n && (!a || !b || !fg || !bg))) return SFK_E_ARG;
+    SFK_HIP(c, launch_beta_update(c->stream, a, b, fg, bg, n, scale));
+    return SFK_OK;
+}
+
+// ----------------------------------------------------------- kernel-level
+
+extern "C" int sfk_project(sfk_ctx* c, const sfk_view* v, const float* attrs, const float* sh,
+                           uint32_t n, uint32_t sh_per_ch, uint32_t tile_w, uint32_t* depth_keys,
+                           float* projected, uint32_t* tile_counts, uint32_t* bbox,
+                           uint32_t* counters) {
+    if (!c || !v || (n && (!attrs || !depth_keys || !projected || !tile_counts || !bbox || !counters)))
+        return SFK_E_ARG;
+    if (sh_per_ch && !sh) return SFK_E_ARG;
+    if (!(tile_w == 8 || tile_w == 16 || tile_w == 32)) return fail(c, SFK_E_ARG, "tile width %u", tile_w);
+    SFK_HIP(c, launch_project(c->stream, *v, attrs, sh, n, sh_per_ch, tile_w, depth_keys, projected,
+                              tile_counts, bbox, counters));
+    return SFK_OK;
+}
+
+extern "C" int sfk_map_isects(sfk_ctx* c, const uint32_t* order, const uint32_t* tile_counts,
+                              const uint32_t* bbox, const uint32_t* offsets, uint32_t n,
+                              uint32_t tiles_per_row, uint32_t max_isects, uint32_t* tile_ids,
+                              uint32_t* gaussian_ids) {
+    if (!c || (n && (!order || !tile_counts || !bbox || !offsets || !tile_ids || !gaussian_ids)))
+        return SFK_E_ARG;
+    SFK_HIP(c, launch_map_isects(c->stream, order, tile_counts, bbox, offsets, n, tiles_per_row,
+                                 max_isects, tile_ids, gaussian_ids));
+    return SFK_OK;
+}
+
+extern "C" int sfk_tile_starts(sfk_ctx* c, const uint32_t* tile_ids, uint32_t n,
+                               const uint32_t* d_n, uint32_t num_tiles, uint32_t* starts) {
+    if (!c || !starts || (n && !tile_ids)) return SFK_E_ARG;
+    SFK_HIP(c, launch_tile_starts(c->stream, tile_ids, n, d_n, num_tiles, starts));
+    return SFK_OK;
+}

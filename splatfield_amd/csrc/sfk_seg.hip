@@ -1,0 +1,142 @@
+// B3-Seg kernels for gfx950 (src/seg/mod.rs, src/seg/beta.rs): buffer fill,
+// analytic EIG with the Beta entropy in f64 on the device (in place of the
+// 445x445 f32 table, beta.rs:100-144), and the conjugate Beta update.
+#include "sfk_device.h"
+#include "sfk_internal.h"
+
+namespace sfk {
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ buf, uint32_t v,
+                                                  size_t count) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * 256;
+    for (; i < count; i += stride) buf[i] = v;
+}
+
+// ln Gamma, Lanczos g = 7, 9 terms (beta.rs:55-70); x >= 1 here.
+__device__ double d_lgamma(double x) {
+    const double z = x - 1.0;
+    double series = 0.9999999999998099;
+    series += 676.5203681218851 / (z + 1.0);
+    series += -1259.1392167224028 / (z + 2.0);
+    series += 771.3234287776531 / (z + 3.0);
+    series += -176.6150291621406 / (z + 4.0);
+    series += 12.507343278686905 / (z + 5.0);
+    series += -0.13857109526572012 / (z + 6.0);
+    series += 9.984369578019572e-6 / (z + 7.0);
+    series += 1.5056327351493116e-7 / (z + 8.0);
+    const double t = z + 7.5;
+    return 0.91893853320467274178 + (z + 0.5) * log(t) - t + log(series);
+}
+
+// digamma: recurrence to x >= 6, then the asymptotic series (beta.rs:75-86).
+__device__ double d_digamma(double x) {
+    double r = 0.0;
+    while (x < 6.0) {
+        r -= 1.0 / x;
+        x += 1.0;
+    }
+    const double inv = 1.0 / x, inv2 = inv * inv;
+    return r + log(x) - 0.5 * inv - inv2 / 12.0 + inv2 * inv2 / 120.0 -
+           inv2 * inv2 * inv2 / 252.0;
+}
+
+// H(a, b) of Beta(a, b) (beta.rs:88-96).
+__device__ double d_beta_entropy(double a, double b) {
+    const double s = a + b;
+    return d_lgamma(a) + d_lgamma(b) - d_lgamma(s) - (a - 1.0) * d_digamma(a) -
+           (b - 1.0) * d_digamma(b) + (s - 2.0) * d_digamma(s);
+}
+
+constexpr uint32_t kEigThreads = 256;
+constexpr uint32_t kEigEpt = 4;
+constexpr uint32_t kEigBlock = kEigThreads * kEigEpt;
+
+// Per-splat dH = max(0, H(a,b) - H(a k, b k)), k = 1 + eps/(a+b),
+// eps = f32(q)/scale, skipped at eps = 0 (beta.rs:365-415); per-block f64 sums.
+__global__ __launch_bounds__(kEigThreads) void k_eig_partial(const float* __restrict__ a,
+                                                             const float* __restrict__ b,
+                                                             const uint32_t* __restrict__ resp,
+                                                             uint32_t n, float scale,
+                                                             double* __restrict__ partials) {
+    __shared__ double s_w[kEigThreads / 64];
+    double sum = 0.0;
+    const size_t base = (size_t)blockIdx.x * kEigBlock + threadIdx.x;
+#pragma unroll
+    for (uint32_t e = 0; e < kEigEpt; ++e) {
+        size_t i = base + (size_t)e * kEigThreads;
+        if (i < n) {
+            float eps = (float)resp[i] / scale;
+            if (eps > 0.0f) {
+                double ai = a[i], bi = b[i];
+                double k = 1.0 + (double)eps / (ai + bi);
+                double d = d_beta_entropy(ai, bi) - d_beta_entropy(ai * k, bi * k);
+                sum += d > 0.0 ? d : 0.0;
+            }
+        }
+    }
+    sum = wave_sum_f64(sum);
+    if (lane_id() == 0) s_w[threadIdx.x / 64] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (uint32_t w = 0; w < kEigThreads / 64; ++w) t += s_w[w];
+        partials[blockIdx.x] = t;
+    }
+}
+
+// Deterministic final sum of the block partials (fixed order).
+__global__ __launch_bounds__(256) void k_eig_final(const double* __restrict__ partials,
+                                                   uint32_t np, double* __restrict__ out) {
+    __shared__ double s_w[4];
+    double sum = 0.0;
+    for (uint32_t i = threadIdx.x; i < np; i += 256) sum += partials[i];
+    sum = wave_sum_f64(sum);
+    if (lane_id() == 0) s_w[threadIdx.x / 64] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+}
+
+// beta_update_kernel (beta.rs:417-426).
+__global__ __launch_bounds__(256) void k_beta_update(float* __restrict__ a, float* __restrict__ b,
+                                                     const uint32_t* __restrict__ fg,
+                                                     const uint32_t* __restrict__ bg, uint32_t n,
+                                                     float scale) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) {
+        a[i] += (float)fg[i] / scale;
+        b[i] += (float)bg[i] / scale;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_fill_u32(hipStream_t s, uint32_t* buf, uint32_t v, size_t count) {
+    if (count == 0) return hipSuccess;
+    size_t blocks = (count + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    k_fill_u32<<<(uint32_t)blocks, 256, 0, s>>>(buf, v, count);
+    return hipGetLastError();
+}
+
+uint32_t eig_num_partials(uint32_t n) { return div_up(n > 0 ? n : 1, kEigBlock); }
+
+hipError_t launch_eig(hipStream_t s, const float* a, const float* b, const uint32_t* resp,
+                      uint32_t n, float scale, double* partials, double* out) {
+    uint32_t np = eig_num_partials(n);
+    if (n > 0) k_eig_partial<<<np, kEigThreads, 0, s>>>(a, b, resp, n, scale, partials);
+    else np = 0;
+    k_eig_final<<<1, 256, 0, s>>>(partials, np, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_beta_update(hipStream_t s, float* a, float* b, const uint32_t* fg,
+                              const uint32_t* bg, uint32_t n, float scale) {
+    if (n == 0) return hipSuccess;
+    k_beta_update<<<div_up(n, 256), 256, 0, s>>>(a, b, fg, bg, n, scale);
+    return hipGetLastError();
+}
+
+}  // namespace sfk

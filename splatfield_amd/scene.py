@@ -1,0 +1,59 @@
+"""Synthetic splat scenes and the camera set-ups of the benchmark workloads.
+
+Layout contract (src/layout.rs:44-61): attribute plane ``k`` of splat ``i``
+lives at ``attrs[k * n + i]`` in the order x, y, z, qw, qx, qy, qz, sx, sy,
+sz, opacity (log-scales, opacity logit, normalised quaternion).  SH
+coefficient ``k`` of channel ``c`` lives at ``sh[(k * 3 + c) * n + i]``.
+"""
+import math
+
+import numpy as np
+
+ATTR_PLANES = 11
+PLANE_X, PLANE_Y, PLANE_Z = 0, 1, 2
+PLANE_QW, PLANE_QX, PLANE_QY, PLANE_QZ = 3, 4, 5, 6
+PLANE_SX, PLANE_SY, PLANE_SZ = 7, 8, 9
+PLANE_OPACITY = 10
+PROJ_FLOATS = 9
+
+
+def synthetic_scene(n, sh_degree, seed=0):
+    """Seeded synthetic splats: means U[-1,1]^3, log-scales N(-4, 0.5),
+    quaternions normalize(N(0,1)^4) (stored normalised, as the PLY loader
+    does, src/ply.rs:84-85), opacity logits N(0,1), SH band 0 N(0, 0.3) and
+    higher bands N(0, 0.05).  Draw order: means, log-scales, quaternions,
+    opacities, SH DC, SH rest.  Returns (attrs[11*n], sh[3*K*n], K)."""
+    rng = np.random.default_rng(seed)
+    k = (sh_degree + 1) ** 2
+    attrs = np.empty((ATTR_PLANES, n), np.float32)
+    attrs[0:3] = rng.uniform(-1.0, 1.0, size=(3, n)).astype(np.float32)
+    log_scales = rng.normal(-4.0, 0.5, size=(3, n)).astype(np.float32)
+    q = rng.normal(0.0, 1.0, size=(4, n)).astype(np.float32)
+    q /= np.sqrt((q * q).sum(axis=0, keepdims=True))
+    attrs[3:7] = q
+    attrs[7:10] = log_scales
+    attrs[10] = rng.normal(0.0, 1.0, size=n).astype(np.float32)
+    sh = np.empty((k * 3, n), np.float32)
+    sh[0:3] = rng.normal(0.0, 0.3, size=(3, n)).astype(np.float32)
+    if k > 1:
+        sh[3:] = rng.normal(0.0, 0.05, size=((k - 1) * 3, n)).astype(np.float32)
+    return attrs.reshape(-1), sh.reshape(-1), k
+
+
+def fov_for_aspect(w, h, fov_y):
+    """Horizontal fov that keeps square pixels for a vertical fov
+    (the aspect rule of src/camera.rs:27-36 applied to a fixed fov_y)."""
+    return 2.0 * math.atan((w / h) * math.tan(fov_y * 0.5))
+
+
+def fibonacci_directions(count):
+    """Unit directions on the Fibonacci sphere (src/seg/views.rs:81-91)."""
+    golden = np.float32((math.sqrt(5.0) - 1.0) / 2.0)
+    out = []
+    for k in range(count):
+        z = np.float32(1.0) - np.float32(2.0) * (np.float32(k) + np.float32(0.5)) / np.float32(count)
+        r = np.float32(math.sqrt(max(0.0, float(np.float32(1.0) - z * z))))
+        theta = np.float32(2.0 * math.pi) * golden * np.float32(k)
+        out.append((np.float32(r * np.float32(math.cos(theta))),
+                    np.float32(r * np.float32(math.sin(theta))), np.float32(z)))
+    return np.array(out, np.float32)
