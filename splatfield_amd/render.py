@@ -1,0 +1,223 @@
+"""Render-path mirror (src/camera.rs, src/render.rs): Camera, Splats,
+RenderScratch, prepare_isects / finalize / render_with over libsfk."""
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib
+from .runtime import GpuTensor
+from .scene import ATTR_PLANES
+
+BASE_FOV = 0.8  # camera.rs:25
+PROJ_STRIDE = 12
+
+
+@dataclass
+class Camera:
+    """Pinhole camera (camera.rs:4-19): per-axis fov (radians), position,
+    rotation quaternion (x, y, z, w) mapping camera to world; the camera
+    looks along its local +Z."""
+    fov: tuple = (BASE_FOV, BASE_FOV)
+    position: tuple = (0.0, 0.0, 0.0)
+    rotation: tuple = (0.0, 0.0, 0.0, 1.0)
+
+    def to_c(self):
+        c = _lib.Camera()
+        c.fov[:] = [float(v) for v in self.fov]
+        c.position[:] = [float(v) for v in self.position]
+        c.rotation[:] = [float(v) for v in self.rotation]
+        return c
+
+    def focal(self, w, h):
+        """camera.rs:38-40 (f32)."""
+        f = np.float32
+        return (f(f(w) * f(0.5)) / f(math.tan(f(self.fov[0]) * f(0.5))),
+                f(f(h) * f(0.5)) / f(math.tan(f(self.fov[1]) * f(0.5))))
+
+    @staticmethod
+    def look_at(position, target, fov=(BASE_FOV, BASE_FOV)):
+        """Camera::look_at (camera.rs:57-63)."""
+        p = np.ascontiguousarray(position, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        q = np.zeros(4, np.float32)
+        lib().sfk_look_at(p.ctypes.data, t.ctypes.data, q.ctypes.data)
+        return Camera(tuple(fov), tuple(p.tolist()), tuple(q.tolist()))
+
+    def view(self, w, h, tile_w=16):
+        """CameraViewLaunch::for_camera (project.rs:137-157)."""
+        v = _lib.View()
+        c = self.to_c()
+        rc = lib().sfk_camera_view(C.byref(c), w, h, tile_w, C.byref(v))
+        if rc != 0:
+            raise ValueError(f"camera view rejected ({rc})")
+        return v
+
+
+def view_array(v):
+    return np.frombuffer(bytes(v), dtype=np.float32).copy()
+
+
+class Splats:
+    """Uploaded field-major splats (render.rs:200-269)."""
+
+    def __init__(self, ctx, attributes, sh_coeffs):
+        attributes = np.ascontiguousarray(attributes, np.float32)
+        sh_coeffs = np.ascontiguousarray(sh_coeffs, np.float32)
+        n = attributes.size // ATTR_PLANES
+        if n == 0:
+            raise ValueError("Splats::new: zero splats")
+        per_ch = sh_coeffs.size // n // 3
+        self.ctx = ctx
+        out = C.c_void_p()
+        ctx.check(lib().sfk_splats_create(ctx.ptr, attributes.ctypes.data, sh_coeffs.ctypes.data,
+                                          n, per_ch, C.byref(out)), "Splats::new")
+        self.ptr = out.value
+        self.n = n
+        self.sh_per_ch = per_ch
+        xyz = attributes[:3 * n].reshape(3, n)
+        self.bounds = (xyz.min(axis=1), xyz.max(axis=1))
+        self.positions = xyz.copy()
+
+    def device_ptrs(self):
+        a, s = C.c_void_p(), C.c_void_p()
+        lib().sfk_splats_info(self.ptr, None, None, None, None, C.byref(a), C.byref(s))
+        return a.value, s.value
+
+    def __del__(self):
+        try:
+            lib().sfk_splats_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class TileIsects:
+    """Sorted per-tile intersections (render.rs:463-471); aliases the scratch."""
+
+    def __init__(self, scratch, c):
+        self.scratch = scratch
+        self.c = c
+
+    @property
+    def tile_bounds(self):
+        return (self.c.tiles_x, self.c.tiles_y)
+
+    def counts(self):
+        """SYNC: (isects_raw, visible, isects)."""
+        out = np.zeros(3, np.uint32)
+        ctx = self.scratch.ctx
+        ctx.check(lib().sfk_tile_isects_counts(ctx.ptr, C.byref(self.c), out.ctypes.data),
+                  "isect counts")
+        return int(out[0]), int(out[1]), int(out[2])
+
+    def tile_ids(self, n):
+        return GpuTensor(self.scratch.ctx, self.c.tile_ids, (n,), np.uint32, False,
+                         self.scratch).read()
+
+    def gaussian_ids(self, n):
+        return GpuTensor(self.scratch.ctx, self.c.gaussian_ids, (n,), np.uint32, False,
+                         self.scratch).read()
+
+    def tile_starts(self):
+        t = self.c.tiles_x * self.c.tiles_y
+        return GpuTensor(self.scratch.ctx, self.c.tile_starts, (t + 1,), np.uint32, False,
+                         self.scratch).read()
+
+
+class RenderScratch:
+    """Per-frame buffers reused across frames (render.rs:114-198)."""
+
+    def __init__(self, ctx, total, w, h, tile_w=16, digit_bits=8):
+        self.ctx = ctx
+        out = C.c_void_p()
+        ctx.check(lib().sfk_render_scratch_create(ctx.ptr, total, w, h, C.byref(out)),
+                  "RenderScratch::new")
+        self.ptr = out.value
+        if tile_w != 16:
+            self.set_tile_width(tile_w)
+        if digit_bits != 8:
+            self.set_digit_bits(digit_bits)
+
+    def set_tile_width(self, tw):
+        self.ctx.check(lib().sfk_render_scratch_set_tile_width(self.ptr, tw), "tile width")
+
+    def set_digit_bits(self, bits):
+        self.ctx.check(lib().sfk_render_scratch_set_digit_bits(self.ptr, bits), "digit bits")
+
+    def reserve_isects(self, n):
+        self.ctx.check(lib().sfk_render_scratch_reserve_isects(self.ptr, n), "reserve")
+
+    def buffers(self):
+        b = _lib.RenderBuffers()
+        self.ctx.check(lib().sfk_render_scratch_buffers(self.ptr, C.byref(b)), "buffers")
+        return b
+
+    def read(self, name):
+        """Download one named scratch buffer as numpy."""
+        b = self.buffers()
+        n, w, h = b.n, b.w, b.h
+        spec = {
+            "depth_keys": (n, np.uint32), "depth_order": (n, np.uint32),
+            "projected": (n * PROJ_STRIDE, np.float32), "tile_counts": (n, np.uint32),
+            "bbox": (2 * n, np.uint32), "offsets": (n, np.uint32), "counters": (2, np.uint32),
+            "bitmap": (h * b.row_stride, np.uint32), "rgba": (w * h * 4, np.float32),
+            "final_t": (w * h, np.float32), "n_contrib": (w * h, np.uint32),
+        }
+        count, dt = spec[name]
+        return GpuTensor(self.ctx, getattr(b, name), (count,), dt, False, self).read()
+
+    def device(self, name):
+        b = self.buffers()
+        return getattr(b, name)
+
+    def __del__(self):
+        try:
+            lib().sfk_render_scratch_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def prepare_isects(splats, scratch, camera=None, w=None, h=None, view=None):
+    """Splats::prepare_isects (render.rs:271-364)."""
+    out = _lib.TileIsects()
+    ctx = splats.ctx
+    if view is not None:
+        ctx.check(lib().sfk_prepare_isects_view(ctx.ptr, splats.ptr, scratch.ptr, C.byref(view),
+                                                w, h, C.byref(out)), "prepare_isects")
+    else:
+        c = camera.to_c()
+        ctx.check(lib().sfk_prepare_isects(ctx.ptr, splats.ptr, scratch.ptr, C.byref(c), w, h,
+                                           C.byref(out)), "prepare_isects")
+    return TileIsects(scratch, out)
+
+
+def finalize_rgb(scratch, isects):
+    ctx = scratch.ctx
+    ctx.check(lib().sfk_finalize_rgb(ctx.ptr, scratch.ptr, C.byref(isects.c)), "finalize(Rgb)")
+
+
+def finalize_total(scratch, isects, scale, resp):
+    ctx = scratch.ctx
+    ctx.check(lib().sfk_finalize_total(ctx.ptr, scratch.ptr, C.byref(isects.c), scale, resp.ptr),
+              "finalize(Total)")
+
+
+def finalize_evidence(scratch, isects, scale, mask, words_per_row, fg, bg):
+    ctx = scratch.ctx
+    ctx.check(lib().sfk_finalize_evidence(ctx.ptr, scratch.ptr, C.byref(isects.c), scale,
+                                          mask.ptr, words_per_row, fg.ptr, bg.ptr),
+              "finalize(Evidence)")
+
+
+def render_with(splats, scratch, camera, w, h):
+    """Splats::render_with (render.rs:366-380): returns the RGBA8 bitmap
+    (h x row_stride u32) as numpy."""
+    ctx = splats.ctx
+    c = camera.to_c()
+    bm, rgba = C.c_void_p(), C.c_void_p()
+    ctx.check(lib().sfk_render_with(ctx.ptr, splats.ptr, scratch.ptr, C.byref(c), w, h,
+                                    C.byref(bm), C.byref(rgba)), "render_with")
+    b = scratch.buffers()
+    return GpuTensor(ctx, bm.value, (h, b.row_stride), np.uint32, False, scratch).read()

@@ -1,0 +1,134 @@
+"""B3-Seg mirror (src/seg/mod.rs, beta.rs, views.rs) over libsfk:
+fixed-point accumulators, mask packing, responsibility renders, analytic EIG
+with f64 entropy on the device, and the Beta update."""
+import ctypes as C
+import math
+
+import numpy as np
+
+from ._lib import lib
+from .render import BASE_FOV, Camera
+from .runtime import GpuTensor
+
+
+def fractional_bits(w, h):
+    """seg/mod.rs:25-34."""
+    return int(lib().sfk_fractional_bits(w, h))
+
+
+class Accumulators:
+    """Fixed-point epsilon accumulators (seg/mod.rs:45-85)."""
+
+    def __init__(self, ctx, total):
+        self.bits = GpuTensor.empty(ctx, total)
+        self.fg = GpuTensor.empty(ctx, total)
+        self.bg = GpuTensor.empty(ctx, total)
+        self.scale = 1.0
+
+    def read_f32(self):
+        return self.bits.read().astype(np.float32) / np.float32(self.scale)
+
+    def read_evidence_f32(self):
+        s = np.float32(self.scale)
+        return (self.fg.read().astype(np.float32) / s, self.bg.read().astype(np.float32) / s)
+
+
+class Mask:
+    """Bit-packed 2D mask (seg/mod.rs:105-144)."""
+
+    def __init__(self, ctx, w, h, bytes_):
+        b = np.ascontiguousarray(bytes_, np.uint8)
+        if b.size != w * h:
+            raise ValueError("mask bytes must be one per pixel")
+        self.words_per_row = (w + 31) // 32
+        words = np.zeros(self.words_per_row * h, np.uint32)
+        fg = C.c_uint32()
+        lib().sfk_mask_pack(b.ctypes.data, w, h, words.ctypes.data, C.byref(fg))
+        self.fg_pixels = fg.value
+        self.words = words
+        self.bits = GpuTensor.from_numpy(ctx, words)
+
+
+def render_responsibility(splats, scratch, acc, camera, w, h):
+    """Splats::render_responsibility (seg/mod.rs:146-159)."""
+    ctx = splats.ctx
+    c = camera.to_c()
+    scale = C.c_float()
+    ctx.check(lib().sfk_render_responsibility(ctx.ptr, splats.ptr, scratch.ptr, C.byref(c), w, h,
+                                              acc.bits.ptr, C.byref(scale)),
+              "render_responsibility")
+    acc.scale = scale.value
+
+
+def accumulate_prepared(scratch, isects, acc, mask=None):
+    """accumulate_prepared (seg/mod.rs:161-210)."""
+    ctx = scratch.ctx
+    scale = C.c_float()
+    if mask is None:
+        rc = lib().sfk_accumulate_prepared(ctx.ptr, scratch.ptr, C.byref(isects.c), None, 0,
+                                           acc.bits.ptr, None, None, C.byref(scale))
+    else:
+        rc = lib().sfk_accumulate_prepared(ctx.ptr, scratch.ptr, C.byref(isects.c),
+                                           mask.bits.ptr, mask.words_per_row, None, acc.fg.ptr,
+                                           acc.bg.ptr, C.byref(scale))
+    ctx.check(rc, "accumulate_prepared")
+    acc.scale = scale.value
+
+
+class BetaState:
+    """Per-splat Beta posteriors (beta.rs:428-501)."""
+
+    def __init__(self, ctx, a, b):
+        self.ctx = ctx
+        self.a = GpuTensor.from_numpy(ctx, np.asarray(a, np.float32))
+        self.b = GpuTensor.from_numpy(ctx, np.asarray(b, np.float32))
+        self.n = self.a.shape[0]
+        self._out = GpuTensor.empty(ctx, 1, np.float64)
+
+    @classmethod
+    def new_uniform(cls, ctx, total):
+        return cls(ctx, np.ones(total, np.float32), np.ones(total, np.float32))
+
+    def eig(self, acc):
+        """Sum of per-splat entropy drops, f64 (SYNC)."""
+        out = C.c_double()
+        self.ctx.check(lib().sfk_eig(self.ctx.ptr, self.a.ptr, self.b.ptr, acc.bits.ptr, self.n,
+                                     acc.scale, C.byref(out)), "eig")
+        return out.value
+
+    def update(self, acc):
+        self.ctx.check(lib().sfk_beta_update(self.ctx.ptr, self.a.ptr, self.b.ptr, acc.fg.ptr,
+                                             acc.bg.ptr, self.n, acc.scale), "beta update")
+
+    def posteriors(self):
+        return self.a.read(), self.b.read()
+
+
+def map_labels(a, b):
+    """MAP foreground iff a > b, ties background (beta.rs:503-511)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.shape != b.shape:
+        raise ValueError("posterior length mismatch")
+    return a > b
+
+
+CANDIDATE_FOV = BASE_FOV  # views.rs:69-71
+
+
+def candidates(center, radius, count):
+    """Fibonacci-sphere candidate cameras (views.rs:73-96)."""
+    f = np.float32
+    golden = f((math.sqrt(5.0) - 1.0) / 2.0)
+    radius = max(f(radius), f(1e-3))
+    dist = f(1.5) * radius / f(math.tan(f(CANDIDATE_FOV) * f(0.5)))
+    cams = []
+    c = np.asarray(center, np.float32)
+    for k in range(count):
+        z = f(1.0) - f(2.0) * (f(k) + f(0.5)) / f(count)
+        r = f(math.sqrt(max(0.0, float(f(1.0) - z * z))))
+        theta = f(2.0 * math.pi) * golden * f(k)
+        d = np.array([r * f(math.cos(theta)), r * f(math.sin(theta)), z], np.float32)
+        pos = c + dist * d
+        cams.append(Camera.look_at(pos, c))
+    return cams

@@ -1,0 +1,408 @@
+"""CPU checks that pin the oracle (oracle/sfo.c) before it is trusted as the
+GPU parity reference: the reference's own known-answer tests, independent
+library oracles (libm, scipy, numpy), closed forms and brute force."""
+import math
+
+import numpy as np
+import pytest
+from scipy import special
+
+from oracle import oracle as O
+
+from .fixtures import (GOLDEN_EPS, jitter, overdraw_attributes, row_attributes,
+                       sample_opaque_attributes)
+
+
+def default_view(w, h, tile_w=16, pos=(0, 0, 0), q=(0, 0, 0, 1), fov=(0.8, 0.8)):
+    return O.camera_view(fov, pos, q, w, h, tile_w)
+
+
+def ulp_dist(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(a - b)
+
+
+# ------------------------------------------------------------------- math
+
+def test_expf_matches_libm():
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(-104, 89, 20000), rng.uniform(-6, 6, 20000),
+                         [0.0, -0.0, 1.0, -1.0, 88.0, -87.0, -100.0]]).astype(np.float32)
+    got = np.array([O.expf(x) for x in xs], np.float32)
+    want = np.exp(xs.astype(np.float64)).astype(np.float32)
+    normal = want > np.float32(1.2e-38)
+    assert ulp_dist(got[normal], want[normal]).max() <= 2
+    # subnormal results: absolute error below one subnormal step pair
+    assert np.all(np.abs(got[~normal].astype(np.float64) - np.exp(xs[~normal].astype(np.float64)))
+                  <= 2.8e-45 * 2)
+    assert O.expf(0.0) == 1.0
+    assert O.expf(100.0) == np.inf and O.expf(-200.0) == 0.0
+
+
+def test_logf_matches_libm():
+    rng = np.random.default_rng(2)
+    xs = np.concatenate([np.exp(rng.uniform(-80, 80, 20000)), rng.uniform(0.5, 2.0, 20000),
+                         [1.0, 2.0, 0.5, 1e-40, 3.4e38]]).astype(np.float32)
+    got = np.array([O.logf(x) for x in xs], np.float32)
+    want = np.log(xs.astype(np.float64)).astype(np.float32)
+    assert ulp_dist(got, want).max() <= 2
+    assert O.logf(1.0) == 0.0
+    assert math.isnan(O.logf(-1.0)) and O.logf(0.0) == -np.inf
+
+
+# ----------------------------------------------------------------- camera
+
+def test_camera_view_default_and_look_at():
+    v = default_view(64, 64)
+    a = v.as_array()
+    np.testing.assert_array_equal(a[0:9], [1, 0, 0, 0, 1, 0, 0, 0, 1])
+    np.testing.assert_array_equal(a[9:12], [0, 0, 0])
+    f = np.float32(np.float32(32.0) / np.float32(math.tan(np.float32(0.4))))
+    assert abs(a[12] - f) <= 1e-4 and a[19] == 4 and a[20] == 4
+    # look_at from +Z back to the origin is pi about +Y (glam's axis choice)
+    q = O.look_at((0, 0, 3), (0, 0, 0))
+    assert abs(q[1] - 1.0) < 1e-7 and abs(q[3]) < 1e-7
+    v = O.camera_view((1, 1), (0, 0, 3), q, 64, 64)
+    z = np.array(v.rot2) @ np.zeros(3) + v.trans[2]
+    assert abs(z - 3.0) < 1e-5  # the origin is 3 in front
+    # generic look_at: forward axis points at the target
+    for pos in [(1.0, 2.0, 3.0), (-2.0, 0.5, -1.0), (0.0, -3.0, 0.1)]:
+        q = O.look_at(pos, (0, 0, 0))
+        v = O.camera_view((1, 1), pos, q, 8, 8)
+        fwd = np.array(v.rot2, np.float64)
+        d = -np.array(pos, np.float64)
+        assert np.dot(fwd, d / np.linalg.norm(d)) > 1 - 1e-5
+
+
+def test_bits_for_table():
+    """sort.rs:420-431."""
+    for m, b in [(0, 0), (1, 0), (2, 1), (3, 2), (255, 8), (256, 8), (257, 9), (65536, 16),
+                 (2**32 - 1, 32)]:
+        assert O.bits_for(m) == b
+
+
+def test_fractional_bits():
+    """seg/mod.rs:31-34 closed form: 32 - log2(next_pow2(w*h))."""
+    for w, h, fb in [(70, 70, 19), (64, 64, 20), (512, 512, 14), (1920, 1080, 11),
+                     (256, 256, 16), (1024, 1024, 12)]:
+        assert O.fractional_bits(w, h) == fb
+
+
+# ---------------------------------------------------------------- project
+
+def test_project_kat():
+    """project.rs:460-507."""
+    attrs = np.array([0, 0, 1, 1, 0, 0, 0, 0, 0, 0, 5], np.float32)
+    pr = O.project(default_view(64, 64), attrs, np.zeros(3, np.float32), 1, 1)
+    np.testing.assert_array_equal(pr["counters"], [16, 1])
+    np.testing.assert_array_equal(pr["tile_counts"], [16])
+    np.testing.assert_array_equal(pr["bbox"], [0, 4 | 4 << 16])
+
+
+def test_project_closed_form_isotropic():
+    """docs/01_project.md: an isotropic splat of scale s at depth z on the axis
+    has cov2d = (f s / z)^2 + 0.3 on the diagonal, 0 off it; conic = 1/diag;
+    mean at the image centre; ext = sqrt(2 ln(o/cutoff) cov)."""
+    for s, z, o_logit in [(0.1, 2.0, 2.0), (0.02, 3.0, 0.0), (0.5, 5.0, 4.0)]:
+        a = np.array([0, 0, z, 1, 0, 0, 0, math.log(s), math.log(s), math.log(s), o_logit],
+                     np.float32)
+        v = default_view(256, 256)
+        pr = O.project(v, a, np.zeros(3, np.float32), 1, 1)
+        f = v.focal[0]
+        cov = (f * s / z) ** 2 + 0.3
+        P = pr["projected"]
+        assert P[0] == 128 and P[1] == 128
+        assert abs(P[2] - 1 / cov) <= 1e-5 / cov and abs(P[3]) < 1e-9 and abs(P[4] - 1 / cov) <= 1e-5 / cov
+        o = 1 / (1 + math.exp(-o_logit))
+        assert abs(P[8] - o) < 1e-6
+        ext = math.sqrt(2 * math.log(o / (10 / 255)) * cov)
+        lo = max(0, int((128 - ext) / 16))
+        hi = min(16, int((128 + ext) / 16 + 1))
+        assert pr["bbox"][0] == lo | lo << 16 and pr["bbox"][1] == hi | hi << 16
+
+
+def test_project_culls():
+    """Opacity below 10/255 and depth <= 0.1 are culled (project.rs:329-337)."""
+    a = sample_opaque_attributes(3, lambda i: [(0, 0, 1), (0, 0, 0.05), (0, 0, -2)][i])
+    a = a.reshape(11, 3)
+    a[10, 0] = -4.0  # sigmoid(-4) = 0.018 < 10/255
+    pr = O.project(default_view(32, 32), a.reshape(-1), np.zeros(9, np.float32), 3, 1)
+    np.testing.assert_array_equal(pr["visible"], [0, 0, 0])
+    np.testing.assert_array_equal(pr["counters"], [0, 0])
+
+
+def test_project_invariants_random_scene():
+    from splatfield_amd import scene as S
+    n = 4096
+    attrs, sh, k = S.synthetic_scene(n, 3, seed=5)
+    q = O.look_at((0, 0, 3), (0, 0, 0))
+    v = O.camera_view((1.0, 1.0), (0, 0, 3), q, 128, 96)
+    pr = O.project(v, attrs, sh, n, k)
+    vis = pr["visible"].astype(bool)
+    assert pr["counters"][1] == vis.sum()
+    assert pr["counters"][0] == pr["tile_counts"][vis].sum()
+    bb = pr["bbox"].reshape(n, 2)[vis]
+    mnx, mny, mxx, mxy = bb[:, 0] & 0xFFFF, bb[:, 0] >> 16, bb[:, 1] & 0xFFFF, bb[:, 1] >> 16
+    assert np.all(mnx <= mxx) and np.all(mxx <= 8) and np.all(mxy <= 6)
+    np.testing.assert_array_equal((mxx - mnx) * (mxy - mny), pr["tile_counts"][vis])
+    # depth key is the f32 bits of camera z: monotone in z
+    z = pr["depth_keys"][vis].view(np.float32)
+    assert np.all(z > 0.1)
+
+
+def _real_sh_basis(x, y, z):
+    """Textbook real SH basis (orthonormal, Condon-Shortley) in the INRIA
+    3DGS ordering for bands 0-2, with the sign conventions of
+    project.rs:412-428 (band 1: -y, z, -x)."""
+    c0 = 0.28209479177387814
+    c1 = 0.4886025119029199
+    c2 = [1.0925484305920792, -1.0925484305920792, 0.31539156525252005,
+          -1.0925484305920792, 0.5462742152960396]
+    return [c0, -c1 * y, c1 * z, -c1 * x,
+            c2[0] * x * y, c2[1] * y * z, c2[2] * (2 * z * z - x * x - y * y),
+            c2[3] * x * z, c2[4] * (x * x - y * y)]
+
+
+def test_sh_bands_0_to_2_match_textbook_basis():
+    """project.rs:403-428 vs the textbook real SH basis: each coefficient in
+    turn set to 1 must reproduce its basis function at the view direction."""
+    rng = np.random.default_rng(7)
+    for _ in range(5):
+        pos = rng.normal(size=3)
+        pos = (pos / np.linalg.norm(pos) * 0.5).astype(np.float32)
+        d = pos - np.array([0, 0, -3], np.float32)
+        d = d / np.linalg.norm(d)
+        basis = _real_sh_basis(*d)
+        for k in range(9):
+            a = sample_opaque_attributes(1, lambda i: pos)
+            sh = np.zeros(27, np.float32)
+            sh[k * 3 + 0] = 1.0  # red channel of coefficient k
+            q = np.array([0, 0, 0, 1], np.float32)
+            v = O.camera_view((1.0, 1.0), (0, 0, -3), q, 32, 32)
+            pr = O.project(v, a, sh, 1, 9)
+            assert abs((pr["projected"][5] - 0.5) - basis[k]) < 2e-6, k
+
+
+# ------------------------------------------------------------- sort / scan
+
+@pytest.mark.parametrize("n", [1000, 5000, 200000])
+def test_argsort_stable_vs_numpy(n):
+    """sort.rs:433-453: stable argsort of keys < 37 at 8 bits."""
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 37, n).astype(np.uint32)
+    ok, ov = O.argsort_u32(keys, np.arange(n, dtype=np.uint32), 8)
+    np.testing.assert_array_equal(ov, np.argsort(keys, kind="stable"))
+    np.testing.assert_array_equal(ok, np.sort(keys))
+
+
+def test_argsort_32bit_and_u64():
+    rng = np.random.default_rng(4)
+    keys = rng.integers(0, 2**32, 50000, dtype=np.uint64).astype(np.uint32)
+    ok, ov = O.argsort_u32(keys, np.arange(keys.size, dtype=np.uint32), 32)
+    np.testing.assert_array_equal(ov, np.argsort(keys, kind="stable"))
+    k64 = rng.integers(0, 2**63, 50000, dtype=np.uint64)
+    k64[::7] = k64[3]  # ties
+    ok, ov = O.argsort_u64(k64, np.arange(k64.size, dtype=np.uint32), 64)
+    np.testing.assert_array_equal(ov, np.argsort(k64, kind="stable"))
+
+
+@pytest.mark.parametrize("n", [1, 5, 1024, 1025, 70000, 100000])
+def test_scan_gather_closed_form(n):
+    """scan.rs:283-331: offsets = exclusive cumsum of counts[gids]."""
+    rng = np.random.default_rng(n)
+    counts = (np.arange(n) % 4).astype(np.uint32) if n in (1025, 70000) else \
+        rng.integers(0, 20, n).astype(np.uint32)
+    gids = rng.permutation(n).astype(np.uint32)
+    got = O.scan_gather(gids, counts)
+    want = np.concatenate([[0], np.cumsum(counts[gids])[:-1]]).astype(np.uint32)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_lower_bound_matches_partition_point():
+    """render.rs:824-850."""
+    ids = np.array([0, 0, 1, 2, 2, 2, 5, 5], np.uint32)
+    for t in range(8):
+        assert O.lower_bound(ids, t) == int(np.searchsorted(ids, t, side="left"))
+    r = O.tile_ranges(ids, 8)
+    for t in range(8):
+        assert r[2 * t + 1] - r[2 * t] == int(np.sum(ids == t))
+
+
+def test_map_isects_brute_force():
+    """map_isects emits exactly each visible splat's bbox tiles in (rank,
+    row-major tile) order; the stable tile sort yields tile-major /
+    depth-minor order (render.rs:342-352)."""
+    from splatfield_amd import scene as S
+    n = 3000
+    attrs, sh, k = S.synthetic_scene(n, 0, seed=9)
+    q = O.look_at((0, 0, 3), (0, 0, 0))
+    v = O.camera_view((1.0, 1.0), (0, 0, 3), q, 96, 80)
+    pr = O.prepare(v, attrs, sh, n, k, 96, 80)
+    rank = np.empty(n, np.int64)
+    rank[pr["depth_order"]] = np.arange(len(pr["depth_order"]))
+    want = []
+    bb = pr["bbox"].reshape(n, 2)
+    for r, g in enumerate(pr["depth_order"]):
+        lo, hi = bb[g]
+        for ty in range(lo >> 16, hi >> 16):
+            for tx in range(lo & 0xFFFF, hi & 0xFFFF):
+                want.append((tx + ty * pr["tiles_x"], r, g))
+    want.sort(key=lambda t: (t[0], t[1]))
+    np.testing.assert_array_equal(pr["tile_ids"], [t[0] for t in want])
+    np.testing.assert_array_equal(pr["gaussian_ids"], [t[2] for t in want])
+    # depth order: non-decreasing keys, ties by splat index
+    keys = pr["depth_keys"][pr["depth_order"]]
+    assert np.all(keys[1:] >= keys[:-1])
+    ties = keys[1:] == keys[:-1]
+    assert np.all(pr["depth_order"][1:][ties] > pr["depth_order"][:-1][ties])
+
+
+# ----------------------------------------------------------------- raster
+
+def _render_default(attrs, n, w, h, mode=O.MODE_RGB, scale=0.0, mask=None, wpr=0):
+    return O.render(default_view(w, h), attrs, np.zeros(3 * n, np.float32), n, 1, w, h, 16,
+                    mode, scale, mask, wpr)
+
+
+def test_golden_center():
+    """render.rs:671-710."""
+    n = 5
+    s2 = np.float32(np.sin(np.float32(np.pi / 4)))
+    c2 = np.float32(np.cos(np.float32(np.pi / 4)))
+    for attrs, pos, q in [
+        (sample_opaque_attributes(n, lambda i: (0, 0, 1 + 0.5 * i)), (0, 0, 0), (0, 0, 0, 1)),
+        (sample_opaque_attributes(n, lambda i: (0, 0, 0)), (-2, 0, 0), (0, s2, 0, c2)),
+    ]:
+        r = O.render(default_view(32, 32, 16, pos, q), attrs, np.zeros(15, np.float32), n, 1,
+                     32, 32)
+        bm = r["bitmap"].reshape(32, -1)
+        c = int(bm[16, 16])
+        assert c >> 24 >= 254 and int(bm[0, 0]) >> 24 == 0
+        for s in (0, 8, 16):
+            assert abs(((c >> s) & 255) - 127) <= 1
+
+
+def test_eps_golden_words():
+    """seg/tests.rs:186-203: GOLDEN from the reference's device exp; the
+    oracle's exp differs in the last ulp, so words match within 2e-7
+    relative (observed: <= 157 LSB on 4.1e8), and zeros match exactly."""
+    n = 40
+    r = _render_default(overdraw_attributes(), n, 70, 70, O.MODE_TOTAL, float(2 ** 19))
+    got = r["resp"].astype(np.int64)
+    gold = np.array(GOLDEN_EPS, np.int64)
+    assert np.all((got == 0) == (gold == 0))
+    assert np.all(np.abs(got - gold) <= np.maximum(8, gold * 5e-7))
+
+
+def test_responsibility_conserves_pixel_weights():
+    """seg/tests.rs:112-126."""
+    n = 5
+    a = sample_opaque_attributes(n, lambda i: (0, 0, 1 + 0.5 * i))
+    rgb = _render_default(a, n, 64, 64)
+    alpha_sum = (rgb["bitmap"].reshape(64, -1)[:, :64] >> 24).astype(np.float64).sum() / 255
+    eps = _render_default(a, n, 64, 64, O.MODE_TOTAL, float(2 ** 20))
+    eps_sum = eps["resp"].astype(np.float64).sum() / 2 ** 20
+    assert abs(alpha_sum - eps_sum) < 64 * 64 * 3 / 255
+
+
+def test_responsibility_orders_depth_and_culls():
+    """seg/tests.rs:128-150."""
+    n = 6
+    a = sample_opaque_attributes(n, lambda i: (0, 0, -3 if i == 5 else 1 + 0.5 * i))
+    eps = _render_default(a, n, 64, 64, O.MODE_TOTAL, float(2 ** 20))["resp"]
+    assert all(eps[k] > eps[k + 1] for k in range(n - 2))
+    assert eps[5] == 0
+
+
+def test_evidence_split_and_update():
+    """seg/tests.rs:205-227, 293-310: fg + bg == total bit for bit; the
+    update moves a/b by fg/scale, bg/scale."""
+    n = 2
+    a = row_attributes(n, 2.0, 1.0)
+    mask = np.zeros((64, 64), np.uint8)
+    mask[:, :32] = 1
+    words = np.zeros(2 * 64, np.uint32)
+    for y in range(64):
+        for x in range(32):
+            words[y * 2 + x // 32] |= 1 << (x % 32)
+    scale = float(2 ** 20)
+    tot = _render_default(a, n, 64, 64, O.MODE_TOTAL, scale)["resp"]
+    ev = _render_default(a, n, 64, 64, O.MODE_EVIDENCE, scale, words, 2)
+    np.testing.assert_array_equal(ev["fg"] + ev["bg"], tot)
+    assert ev["bg"][0] == 0 and ev["fg"][1] == 0
+    aa, bb = O.beta_update(np.ones(2), np.ones(2), ev["fg"], ev["bg"], scale)
+    assert aa[0] > bb[0] and bb[1] > aa[1]
+    assert abs(aa[0] - (1 + ev["fg"][0] / scale)) < 1e-5
+
+
+def test_tile_width_changes_only_low_alpha_tails():
+    """Tile size changes which splats a tile lists (the bbox uses the 10/255
+    cutoff, the compositor 1/255), so only alpha < 10/255 tails differ."""
+    n = 5
+    a = sample_opaque_attributes(n, lambda i: (0, 0, 1 + 0.5 * i))
+    imgs = []
+    for tw in (8, 16, 32):
+        r = O.render(default_view(64, 64, tw), a, np.zeros(15, np.float32), n, 1, 64, 64, tw)
+        imgs.append(r["rgba"])
+    assert np.max(np.abs(imgs[0] - imgs[1])) <= 10 / 255
+    assert np.max(np.abs(imgs[2] - imgs[1])) <= 10 / 255
+    assert np.abs(imgs[2] - imgs[1]).mean() < 1e-3
+
+
+# -------------------------------------------------------------------- seg
+
+def test_entropy_vs_scipy():
+    """beta.rs:55-96 vs scipy's betaln / digamma (an independent library)."""
+    rng = np.random.default_rng(3)
+    pairs = [(1.0, 1.0), (1.5, 1.5), (2.0, 5.0), (9.0, 1.0), (30.0, 30.0), (100.0, 3.0)]
+    pairs += [tuple(np.exp(rng.uniform(0, 14, 2))) for _ in range(256)]
+    for a, b in pairs:
+        want = (special.betaln(a, b) - (a - 1) * special.digamma(a) - (b - 1) * special.digamma(b)
+                + (a + b - 2) * special.digamma(a + b))
+        got = O.beta_entropy(a, b)
+        assert abs(got - want) <= 1e-7 + 1e-10 * abs(want), (a, b, got, want)
+        assert abs(O.lgamma(a) - special.gammaln(a)) <= 1e-10 * max(1, abs(special.gammaln(a)))
+        assert abs(O.digamma(a) - special.digamma(a)) <= 5e-9  # series truncated at x^-6, x >= 6
+    assert abs(O.beta_entropy(1.0, 1.0)) < 1e-12
+
+
+def test_eig_mirror_and_fresh_splat():
+    """beta.rs:714-756, 800-825: EIG equals the sum of per-splat entropy drops;
+    a fresh splat seen at eps = 0.5 gains H(1,1) - H(1.25,1.25)."""
+    rng = np.random.default_rng(6)
+    n = 1000
+    a = (1 + rng.random(n) * 20).astype(np.float32)
+    b = (1 + rng.random(n) * 20).astype(np.float32)
+    eps = np.where(np.arange(n) % 2 == 0, rng.random(n) * 0.5, 0).astype(np.float32)
+    scale = 4096.0
+    q = (eps * np.float32(scale)).astype(np.uint32)
+    got = O.eig(a, b, q, scale)
+    want = 0.0
+    for i in range(n):
+        e = np.float32(q[i]) / np.float32(scale)
+        if e > 0:
+            k = 1 + float(e) / (float(a[i]) + float(b[i]))
+            ai, bi = float(a[i]), float(b[i])
+            d = O.beta_entropy(ai, bi) - O.beta_entropy(ai * k, bi * k)
+            want += max(d, 0.0)
+    assert abs(got - want) < 1e-9
+    q2 = (2 * eps * np.float32(scale)).astype(np.uint32)
+    assert O.eig(a, b, q2, scale) > got > 0
+    fresh = O.eig([1.0], [1.0], [int(0.5 * 4096)], 4096.0)
+    assert abs(fresh - (O.beta_entropy(1, 1) - O.beta_entropy(1.25, 1.25))) < 1e-12
+    assert O.eig(np.ones(512), np.ones(512), np.zeros(512, np.uint32), 4096.0) == 0.0
+    d = [O.eig([aa], [1.0], [int(0.7 * 4096)], 4096.0) for aa in (1.0, 3.0, 50.0)]
+    assert d[0] > d[1] > d[2] > 0
+
+
+def test_candidate_sphere():
+    """views.rs:279-310: distance, facing, decreasing z."""
+    from splatfield_amd import scene as S
+    dirs = S.fibonacci_directions(20)
+    assert np.allclose(np.linalg.norm(dirs, axis=1), 1, atol=1e-6)
+    assert np.all(np.diff(dirs[:, 2]) < 0)
+
+
+def test_jitter_fixture():
+    assert np.all(np.abs(np.array([jitter(i) for i in range(100)])) <= 0.5)
