@@ -1,9 +1,11 @@
-"""Benchmark: 1080p frames/s at 1,048,576 synthetic Gaussians (SH degree 3).
+"""Benchmark: 1080p forward+backward frames/s at 1,048,576 Gaussians (SH3).
 
-One step = one full frame of the render path on one MI355X: project, depth
-argsort, gather scan, isect emission, tile argsort, tile ranges, RGB
-composite (f32 RGBA + RGBA8).  Inputs are resident in HBM before the timed
-region.  Prints one JSON line (see the repository's benchmark contract).
+One step = one differentiable frame on one MI355X: project, depth argsort,
+gather scan, isect emission, tile argsort, tile ranges, RGB composite
+(f32 RGBA + per-pixel final T / last contributor), L2 loss gradient against a
+target image, raster adjoint and projection adjoint (gradients w.r.t. means,
+quaternions, log-scales, opacity logits and SH).  Inputs and the target are
+resident in HBM before the timed region.  Prints one JSON line.
 """
 import argparse
 import json
@@ -30,7 +32,25 @@ def parse():
     p.add_argument("--height", type=int, default=1080)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--cpu-seconds", type=float, default=10.0)
     return p.parse_args()
+
+
+def kernel_bytes(n, k, vis, ni, px, tile_bits):
+    """Algorithmic HBM bytes per launch of each step of the frame, counting
+    the reference's two-stage pipeline (8-bit digits): reads + writes a
+    minimal implementation must make."""
+    return {
+        "project": n * (44 + 4 * 3 * k) + 56 * vis,
+        "radix_pass_u32": None,  # depth and tile passes differ; see below
+        "scan_gather": 16 * vis,
+        "map_isects": 16 * vis + 8 * ni,
+        "tile_starts": 4 * ni,
+        "raster_rgb": 40 * ni + 16 * px,
+        "loss_l2": 48 * px,
+        "raster_bwd": 40 * ni + 32 * px + 36 * vis,
+        "project_bwd": 2 * n * (44 + 4 * 3 * k) + 36 * vis,
+    }
 
 
 def main():
@@ -42,27 +62,50 @@ def main():
     from splatfield_amd import Context
     from splatfield_amd import render as R
     from splatfield_amd import scene as S
+    from splatfield_amd.runtime import GpuTensor
 
     n, w, h = args.n, args.width, args.height
+    px = w * h
     attrs, sh, k = S.synthetic_scene(n, 3, seed=0)
+    rng = np.random.default_rng(1)
+    pa = (attrs + rng.normal(0, 0.01, attrs.size)).astype(np.float32)
+    ps = (sh + rng.normal(0, 0.01, sh.size)).astype(np.float32)
     fy = np.float32(math.pi / 3)
     fx = np.float32(S.fov_for_aspect(w, h, fy))
     cam = R.Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), fov=(fx, fy))
 
     ctx = Context(0)
+    view = cam.view(w, h)
+    # Target image: the same scene with parameters perturbed by N(0, 0.01).
+    sp_t = R.Splats(ctx, pa, ps)
+    sc_t = R.RenderScratch(ctx, n, w, h)
+    sc_t.reserve_isects(16 << 20)
+    it = R.prepare_isects(sp_t, sc_t, view=view, w=w, h=h)
+    R.finalize_rgb(sc_t, it)
+    target = GpuTensor.from_numpy(ctx, sc_t.read("rgba"))
+    del it, sc_t, sp_t
+
     splats = R.Splats(ctx, attrs, sh)
     scratch = R.RenderScratch(ctx, n, w, h)
-    view = cam.view(w, h)
+    d_rgba = GpuTensor.empty(ctx, px * 4, np.float32)
+    loss = GpuTensor.empty(ctx, 1, np.float64)
+    g_attrs = GpuTensor.empty(ctx, n * 11, np.float32)
+    g_sh = GpuTensor.empty(ctx, n * 3 * k, np.float32)
+    rgba_ptr = None
 
     def step():
         isects = R.prepare_isects(splats, scratch, view=view, w=w, h=h)
         R.finalize_rgb(scratch, isects)
+        R.loss_l2(ctx, rgba_ptr, target.ptr, px, d_rgba.ptr, loss)
+        R.render_backward(splats, scratch, isects, d_rgba.ptr, g_attrs, g_sh)
         return isects
 
+    rgba_ptr = scratch.device("rgba")
     isects = step()
     ctx.sync()
     raw, vis, ni = isects.counts()
     scratch.reserve_isects(raw)
+    rgba_ptr = scratch.device("rgba")
     for _ in range(args.warmup):
         step()
     ctx.sync()
@@ -76,19 +119,45 @@ def main():
     gpu_ms = ctx.timer_stop_ms()
     wall = time.perf_counter() - t0
     ms_per_step = gpu_ms / args.steps
+    loss_v = float(loss.read()[0])
 
-    # Algorithmic bytes of the forward frame (two-stage sort at 8-bit digits).
-    tile_bits = int(math.ceil(math.log2(((w + 15) // 16) * ((h + 15) // 16))))
+    # Same steps again with per-launch events: the kernel breakdown.
+    ctx.prof_enable(True)
+    for _ in range(args.steps):
+        step()
+    prof = ctx.prof_report()
+    ctx.prof_enable(False)
+
+    tiles = ((w + 15) // 16) * ((h + 15) // 16)
+    tile_bits = int(math.ceil(math.log2(tiles)))
+    kb = kernel_bytes(n, k, vis, ni, px, tile_bits)
+    depth_passes, tile_passes = 4, math.ceil(tile_bits / 8)
+    # radix passes: depth passes move 16 B per visible key, tile passes 16 B per isect
+    kb["radix_pass_u32"] = (depth_passes * 16 * vis + tile_passes * 16 * ni) / (
+        depth_passes + tile_passes)
+    per_kernel = {}
+    for name, (cnt, ms) in prof.items():
+        per_launch = ms / max(cnt, 1)
+        entry = {"launches_per_step": cnt / args.steps, "ms_per_launch": round(per_launch, 5)}
+        if kb.get(name):
+            entry["gbs"] = round(kb[name] / (per_launch * 1e-3) / 1e9, 1)
+        per_kernel[name] = entry
+    dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
+    dom_ms = prof[dom][1] / prof[dom][0]
+    dom_gbs = kb[dom] / (dom_ms * 1e-3) / 1e9 if kb.get(dom) else None
+
     b_fwd = (n * (44 + 4 * 3 * k) + 56 * vis + 16 * vis + 8 * ni + 4 * 16 * vis +
-             math.ceil(tile_bits / 8) * 16 * ni + 40 * ni + 16 * w * h)
-    achieved = b_fwd / (ms_per_step * 1e-3) / 1e9
+             math.ceil(tile_bits / 8) * 16 * ni + 40 * ni + 16 * px)
+    b_bwd = 40 * ni + 32 * px + 36 * vis + n * (44 + 4 * 3 * k) + 36 * vis + n * (44 + 4 * 3 * k)
+    step_gbs = (b_fwd + b_bwd) / (ms_per_step * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(attrs, sh, n, k, view, w, h, args.cpu_threads)
+        cpu = cpu_baseline(attrs, sh, n, k, view, w, h, args.cpu_threads, args.cpu_seconds,
+                           sc_target=None)
 
     line = {
-        "metric": "frames/s at 1M Gaussians 1080p (forward render)",
+        "metric": "frames/s at 1M Gaussians 1080p fwd+bwd",
         "value": round(1000.0 / ms_per_step, 2),
         "unit": "frames/s",
         "n_gpus": 1,
@@ -99,12 +168,18 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded splats: means U[-1,1]^3, log-scales N(-4,0.5), SH3)",
-        "config": {"workload": "1080p forward frame, 1,048,576 Gaussians SH3",
-                   "gaussians": n, "width": w, "height": h, "isects": ni, "visible": vis},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "whole frame"},
+        "data": "synthetic (seeded splats: means U[-1,1]^3, log-scales N(-4,0.5), "
+                "opacity logits N(0,1), SH3; target = same scene perturbed by N(0,0.01))",
+        "config": {"workload": "1080p forward+backward frame, 1,048,576 Gaussians SH3, L2 loss",
+                   "gaussians": n, "width": w, "height": h, "isects": ni, "visible": vis,
+                   "loss": loss_v},
+        "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 1) if dom_gbs else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dom_gbs / HBM_PEAK_GBS, 4) if dom_gbs else None,
+                     "traffic": None, "kernel": dom,
+                     "frame_achieved": round(step_gbs, 1),
+                     "frame_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "kernels": per_kernel,
         "cpu_baseline": cpu,
         "wall_s": round(wall, 3),
     }
@@ -112,17 +187,29 @@ def main():
     ctx.close()
 
 
-def cpu_baseline(attrs, sh, n, k, view, w, h, threads):
-    """The oracle's C restatement of the same frame, timed on the host."""
+def cpu_baseline(attrs, sh, n, k, view, w, h, threads, seconds, sc_target=None):
+    """The oracle's C restatement of the same fwd+bwd frame, timed on the host
+    cores: f32 forward (project, sorts, map, composite) and the CPU reference
+    gradient, repeated for about `seconds`."""
     from oracle import oracle as O
     cores = threads or min(16, os.cpu_count() or 1)
     ov = O.View.from_buffer_copy(bytes(view))
+    target = None
+    frames = 0
     t0 = time.perf_counter()
-    O.render(ov, attrs, sh, n, k, w, h, nthreads=cores)
-    dt = time.perf_counter() - t0
-    return {"value": round(1.0 / dt, 4), "unit": "frames/s", "cores": cores, "kind": "port",
-            "sample": "one full 1080p forward frame of the same scene (project, sorts, map, "
-                      f"composite) by the oracle, {dt:.1f} s"}
+    while True:
+        r = O.render(ov, attrs, sh, n, k, w, h, nthreads=cores)
+        if target is None:
+            target = np.clip(r["rgba"] + np.float32(0.01), 0, None).astype(np.float32)
+        d = (2.0 * (r["rgba"] - target)).astype(np.float32)
+        O.backward(ov, attrs, sh, n, k, w, h, r, r, d, nthreads=cores)
+        frames += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    return {"value": round(frames / dt, 4), "unit": "frames/s", "cores": cores, "kind": "port",
+            "sample": f"{frames} full 1080p fwd+bwd frames of the same 1M-splat scene by the "
+                      f"oracle's C restatement (f32 forward + CPU reference gradient), {dt:.1f} s"}
 
 
 if __name__ == "__main__":

@@ -59,6 +59,11 @@ int sfk_fill_u32(sfk_ctx* ctx, uint32_t* buf, uint32_t value, size_t count);
 /* Stream-ordered event timing (hipEventRecord/ElapsedTime). */
 int sfk_timer_start(sfk_ctx* ctx);
 int sfk_timer_stop_ms(sfk_ctx* ctx, float* ms); /* SYNC */
+/* Per-launch timing: with profiling on, every kernel-launching step records
+ * an event pair on the stream; the report (SYNC) aggregates them by name as
+ * "name count total_ms" lines and clears the record. */
+int sfk_prof_enable(sfk_ctx* ctx, int on);
+int sfk_prof_report(sfk_ctx* ctx, char* buf, int len);
 
 /* ------------------------------------------------------------------ camera */
 
@@ -248,6 +253,22 @@ int sfk_eig_device(sfk_ctx* ctx, const float* a, const float* b, const uint32_t*
 /* BetaState::update (src/seg/beta.rs:484-500): a += fg/scale, b += bg/scale. */
 int sfk_beta_update(sfk_ctx* ctx, float* a, float* b, const uint32_t* fg, const uint32_t* bg,
                     uint32_t n, float scale);
+
+/* ------------------------------------------------ backward (no reference
+ * counterpart: the reference renders forward only, docs/00_pipeline.md:1) */
+
+/* L2 image loss: d_rgba = 2 (rgba - target) over pixels x 4 channels;
+ * d_loss (device f64, may be NULL) = sum of squared differences. */
+int sfk_loss_l2(sfk_ctx* ctx, const float* rgba, const float* target, uint32_t pixels,
+                float* d_rgba, double* d_loss);
+/* Adjoint of finalize(Rgb) and project_splats for the frame last prepared
+ * and composited in `scratch` (its final_t / n_contrib): given dL/d(rgba)
+ * (h x w x 4 f32), overwrite grad_attrs (11 planes x n, the attribute
+ * layout: means, raw quaternion, log-scales, opacity logit) and grad_sh
+ * (3*sh_per_ch planes x n). */
+int sfk_render_backward(sfk_ctx* ctx, const sfk_splats* splats, sfk_render_scratch* scratch,
+                        const sfk_tile_isects* isects, const float* d_rgba, float* grad_attrs,
+                        float* grad_sh);
 
 /* ------------------------------------------------------ kernel-level calls */
 

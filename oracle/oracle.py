@@ -262,3 +262,74 @@ def tile_keys_u64(tile_ids, gaussian_ids, depth_keys):
     """The sorted u64 ``tile << 32 | depth`` keys of the two-stage sort."""
     return (np.asarray(tile_ids, np.uint64) << np.uint64(32)) | \
         np.asarray(depth_keys, np.uint64)[np.asarray(gaussian_ids, np.int64)]
+
+
+# ------------------------------------------------------------- backward
+_bsig = {
+    "sfo_bwd_create": (C.c_void_p, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32]),
+    "sfo_bwd_destroy": (None, [C.c_void_p]),
+    "sfo_bwd_loss": (C.c_double, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                  C.c_void_p]),
+    "sfo_bwd_grad": (None, [C.c_void_p] * 6),
+}
+for _n, (_r, _a) in _bsig.items():
+    _f = getattr(LIB, _n)
+    _f.restype = _r
+    _f.argtypes = _a
+
+
+class Loss64:
+    """f64 L2 render loss with frozen branch decisions and its adjoint
+    (oracle/sfo_bwd.c).  ``prep`` is an f32 ``prepare`` result that fixes the
+    tile lists."""
+
+    def __init__(self, view, n, sh_per_ch, w, h, prep, target, tile_w=16):
+        self.n, self.k, self.w, self.h = n, sh_per_ch, w, h
+        self._ranges = _u32(prep["ranges"])
+        self._gids = _u32(prep["gaussian_ids"])
+        self.target = _f32(target)
+        self.ptr = LIB.sfo_bwd_create(C.byref(view), n, sh_per_ch, w, h, tile_w,
+                                      _p(self._ranges), _p(self._gids), len(self._gids))
+
+    def loss(self, attrs, sh, record=False, image=False):
+        a = np.ascontiguousarray(attrs, np.float64)
+        s = np.ascontiguousarray(sh, np.float64)
+        img = np.zeros(self.w * self.h * 4) if image else None
+        v = LIB.sfo_bwd_loss(self.ptr, _p(a), _p(s), _p(self.target), int(record), _p(img))
+        return (v, img) if image else v
+
+    def grad(self, attrs, sh):
+        a = np.ascontiguousarray(attrs, np.float64)
+        s = np.ascontiguousarray(sh, np.float64)
+        self.loss(a, s, record=True)
+        ga = np.zeros(self.n * 11)
+        gs = np.zeros(self.n * 3 * self.k)
+        LIB.sfo_bwd_grad(self.ptr, _p(a), _p(s), _p(self.target), _p(ga), _p(gs))
+        return ga, gs
+
+    def __del__(self):
+        try:
+            LIB.sfo_bwd_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+LIB.sfo_backward.restype = None
+LIB.sfo_backward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                             C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                             C.c_void_p, C.c_void_p]
+
+
+def backward(view, attrs, sh, n, sh_per_ch, w, h, prep, ras, d_rgba, tile_w=16, nthreads=1):
+    """CPU reference gradient of the f32 render (oracle/sfo_bwd.c
+    sfo_backward) from the f32 forward state ``prep`` (prepare) and ``ras``
+    (rasterize RGB: final_t, n_contrib)."""
+    ga = np.zeros(n * 11, np.float32)
+    gs = np.zeros(n * 3 * sh_per_ch, np.float32)
+    keep = [_f32(attrs), _f32(sh), _u32(prep["ranges"]), _u32(prep["gaussian_ids"]),
+            _f32(prep["projected"]), _f32(ras["final_t"]), _u32(ras["n_contrib"]), _f32(d_rgba)]
+    LIB.sfo_backward(C.byref(view), *[_p(a) for a in keep[:2]], n, sh_per_ch, w, h, tile_w,
+                     *[_p(a) for a in keep[2:]], nthreads, _p(ga), _p(gs))
+    return ga, gs

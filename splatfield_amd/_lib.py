@@ -57,6 +57,8 @@ SIGNATURES = {
     "sfk_fill_u32": (i32, [P, P, u32, sz]),
     "sfk_timer_start": (i32, [P]),
     "sfk_timer_stop_ms": (i32, [P, C.POINTER(f32)]),
+    "sfk_prof_enable": (i32, [P, i32]),
+    "sfk_prof_report": (i32, [P, C.c_char_p, i32]),
     "sfk_camera_view": (i32, [C.POINTER(Camera), u32, u32, u32, C.POINTER(View)]),
     "sfk_look_at": (i32, [P, P, P]),
     "sfk_bits_for": (u32, [u32]),
@@ -93,6 +95,8 @@ SIGNATURES = {
     "sfk_eig": (i32, [P, P, P, P, u32, f32, P]),
     "sfk_eig_device": (i32, [P, P, P, P, u32, f32, P]),
     "sfk_beta_update": (i32, [P, P, P, P, P, u32, f32]),
+    "sfk_loss_l2": (i32, [P, P, P, u32, P, P]),
+    "sfk_render_backward": (i32, [P, P, P, C.POINTER(TileIsects), P, P, P]),
     "sfk_project": (i32, [P, C.POINTER(View), P, P, u32, u32, u32, P, P, P, P, P]),
     "sfk_map_isects": (i32, [P, P, P, P, P, u32, u32, u32, P, P]),
     "sfk_tile_starts": (i32, [P, P, u32, P, u32, P]),

@@ -32,7 +32,57 @@ int hip_fail(sfk_ctx* ctx, hipError_t e, const char* what) {
                 hipGetErrorString(e), (int)e);
 }
 
+hipEvent_t prof_event(sfk_ctx* c) {
+    if (!c->prof_pool.empty()) {
+        hipEvent_t e = c->prof_pool.back();
+        c->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
 }  // namespace sfk
+
+extern "C" int sfk_prof_enable(sfk_ctx* c, int on) {
+    if (!c) return SFK_E_ARG;
+    c->prof = on != 0;
+    return SFK_OK;
+}
+
+// Aggregate the recorded launches by name: "name count total_ms\n" lines.
+extern "C" int sfk_prof_report(sfk_ctx* c, char* buf, int len) {
+    if (!c || !buf || len <= 0) return SFK_E_ARG;
+    SFK_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<std::string> names;
+    std::vector<double> ms;
+    std::vector<uint32_t> cnt;
+    for (auto& r : c->prof_recs) {
+        float t = 0.0f;
+        hipEventElapsedTime(&t, r.a, r.b);
+        size_t k = 0;
+        while (k < names.size() && names[k] != r.name) ++k;
+        if (k == names.size()) {
+            names.emplace_back(r.name);
+            ms.push_back(0.0);
+            cnt.push_back(0);
+        }
+        ms[k] += t;
+        cnt[k] += 1;
+        c->prof_pool.push_back(r.a);
+        c->prof_pool.push_back(r.b);
+    }
+    c->prof_recs.clear();
+    std::string out;
+    for (size_t k = 0; k < names.size(); ++k) {
+        char line[256];
+        snprintf(line, sizeof line, "%s %u %.6f\n", names[k].c_str(), cnt[k], ms[k]);
+        out += line;
+    }
+    snprintf(buf, (size_t)len, "%s", out.c_str());
+    return (int)out.size() < len ? SFK_OK : SFK_E_CAPACITY;
+}
 
 // ----------------------------------------------------------------- context
 
@@ -302,6 +352,7 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const u
         K* ok = (!last || write_keys) ? static_cast<K*>(s->keys[dst]) : nullptr;
         uint32_t* ov = s->vals[dst];
         hipError_t e;
+        SFK_PROF(c, sizeof(K) == 4 ? "radix_pass_u32" : "radix_pass_u64");
         if constexpr (sizeof(K) == 4)
             e = launch_radix_pass_u32(c->stream, cur_k, cur_v, ok, ov, n, d_n, shift, width, db,
                                       s->hist, s->max_blocks, s->totals);
@@ -366,6 +417,7 @@ extern "C" int sfk_exclusive_scan_gather(sfk_ctx* c, const uint32_t* gids, const
     if (n > s->max_elems)
         return fail(c, SFK_E_CAPACITY, "offsets undersized: scanned %u elements, capacity %u", n,
                     s->max_elems);
+    SFK_PROF(c, "scan_gather");
     SFK_HIP(c, launch_scan(c->stream, gids, counts, offsets, n, s->sums));
     return SFK_OK;
 }

@@ -1,0 +1,430 @@
+// Backward (adjoint) pass of the splat render for gfx950.  The reference is
+// forward-only (docs/00_pipeline.md:1); this differentiates its forward
+// function — project_splats (src/project.rs:236-388) and the front-to-back
+// compositor (src/raster.rs:138-229) — with respect to means, log-scales,
+// quaternions (raw, as the kernel uses them), opacity logits and SH.
+//
+// Raster adjoint: one workgroup per tile walks the tile's isects back to
+// front in LDS-staged chunks.  Per-pixel transmittance is recomputed from the
+// forward's final T (T_k = T_{k+1} / (1 - alpha_k)), never stored per isect;
+// the forward keeps only final T and the index past the last contributor per
+// pixel.  Per-splat screen-space gradients (mean2d, conic, opacity, rgb) are
+// summed across the wavefront, then across the workgroup in LDS, and added to
+// global memory once per (splat, tile).
+#include "sfk_device.h"
+#include "sfk_internal.h"
+#include "sfk_math.h"
+
+namespace sfk {
+
+namespace {
+
+constexpr float kInv255 = 1.0f / 255.0f;
+constexpr float kPowerSkip = 5.6f;
+
+__device__ __forceinline__ float wave_sum_f32(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_loss_l2(const float4* __restrict__ img,
+                                                 const float4* __restrict__ target, uint32_t np,
+                                                 float4* __restrict__ d_img,
+                                                 double* __restrict__ partials) {
+    __shared__ double s_w[4];
+    uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    double sq = 0.0;
+    if (p < np) {
+        float4 a = img[p], b = target[p];
+        float4 d = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+        d_img[p] = make_float4(2.0f * d.x, 2.0f * d.y, 2.0f * d.z, 2.0f * d.w);
+        sq = (double)d.x * d.x + (double)d.y * d.y + (double)d.z * d.z + (double)d.w * d.w;
+    }
+    sq = wave_sum_f64(sq);
+    if (lane_id() == 0) s_w[threadIdx.x / 64] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+}
+
+__global__ __launch_bounds__(256) void k_sum_f64(const double* __restrict__ partials, uint32_t np,
+                                                 double* __restrict__ out) {
+    __shared__ double s_w[4];
+    double sum = 0.0;
+    for (uint32_t i = threadIdx.x; i < np; i += 256) sum += partials[i];
+    sum = wave_sum_f64(sum);
+    if (lane_id() == 0) s_w[threadIdx.x / 64] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+}
+
+struct BwdArgs {
+    uint32_t w, h, tiles_x, tiles_y, n;
+    const uint32_t* tile_starts;
+    const uint32_t* gids;
+    const float* proj;
+    const float* final_t;
+    const uint32_t* n_contrib;
+    const float* d_img;
+    float* grad2d;  // 9 planes of n: mx, my, conic a, conic b, conic c, opacity, r, g, b
+};
+
+template <int TW>
+__global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
+    constexpr int NT = TW * TW;
+    __shared__ float4 s_geo[NT];
+    __shared__ float4 s_aux[NT];
+    __shared__ float s_blue[NT];
+    __shared__ uint32_t s_gid[NT];
+    __shared__ float s_acc[9][NT];
+    __shared__ uint32_t s_last;
+
+    const uint32_t ntiles = a.tiles_x * a.tiles_y;
+    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
+    const uint32_t tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const uint32_t t = threadIdx.x;
+    const uint32_t px = tx * TW + (t % TW), py = ty * TW + (t / TW);
+    const bool in_bounds = px < a.w && py < a.h;
+    const uint32_t start = a.tile_starts[tile];
+    const float pxf = (float)px + 0.5f, pyf = (float)py + 0.5f;
+
+    uint32_t last = start;
+    float T = 1.0f, Tfin = 1.0f;
+    float4 dC = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (in_bounds) {
+        size_t p = (size_t)py * a.w + px;
+        last = a.n_contrib[p];
+        Tfin = a.final_t[p];
+        T = Tfin;
+        dC = reinterpret_cast<const float4*>(a.d_img)[p];
+    }
+    if (t == 0) s_last = start;
+    __syncthreads();
+    atomicMax(&s_last, last);
+    __syncthreads();
+    const uint32_t block_last = s_last;
+    float acc_r = 0.f, acc_g = 0.f, acc_b = 0.f;
+    const float4* proj = reinterpret_cast<const float4*>(a.proj);
+
+    for (uint32_t cend = block_last; cend > start;) {
+        const uint32_t cbeg = cend - start > (uint32_t)NT ? cend - NT : start;
+        const uint32_t cnt = cend - cbeg;
+        __syncthreads();
+        if (t < cnt) {
+            uint32_t g = a.gids[cbeg + t];
+            const float4* row = proj + (size_t)g * 3;
+            s_geo[t] = row[0];
+            s_aux[t] = row[1];
+            s_blue[t] = row[2].x;
+            s_gid[t] = g;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) s_acc[k][t] = 0.0f;
+        }
+        __syncthreads();
+        for (uint32_t jj = cnt; jj > 0; --jj) {
+            const uint32_t j = jj - 1;
+            const uint32_t idx = cbeg + j;
+            float g[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) g[k] = 0.0f;
+            bool has = false;
+            if (idx < last) {
+                float4 geo = s_geo[j];
+                float4 aux = s_aux[j];
+                float dx = geo.x - pxf, dy = geo.y - pyf;
+                float power = 0.5f * (geo.z * dx * dx + aux.x * dy * dy) + geo.w * dx * dy;
+                if (!(power > kPowerSkip)) {
+                    float G = sfk_expf(-power);
+                    float araw = aux.y * G;
+                    float alpha = fminf(araw, 0.999f);
+                    if (alpha >= kInv255) {
+                        has = true;
+                        float om = 1.0f - alpha;
+                        T = T / om;
+                        float wgt = alpha * T;
+                        float cb = s_blue[j];
+                        g[6] = wgt * dC.x;
+                        g[7] = wgt * dC.y;
+                        g[8] = wgt * dC.z;
+                        float dal = T * ((aux.z - acc_r) * dC.x + (aux.w - acc_g) * dC.y +
+                                         (cb - acc_b) * dC.z) +
+                                    Tfin / om * dC.w;
+                        acc_r = alpha * aux.z + om * acc_r;
+                        acc_g = alpha * aux.w + om * acc_g;
+                        acc_b = alpha * cb + om * acc_b;
+                        if (araw < 0.999f) {
+                            float dpow = -dal * alpha;
+                            g[5] = dal * G;
+                            g[0] = dpow * (geo.z * dx + geo.w * dy);
+                            g[1] = dpow * (aux.x * dy + geo.w * dx);
+                            g[2] = dpow * 0.5f * dx * dx;
+                            g[3] = dpow * dx * dy;
+                            g[4] = dpow * 0.5f * dy * dy;
+                        }
+                    }
+                }
+            }
+            if (__ballot(has)) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    float s = wave_sum_f32(g[k]);
+                    if (lane_id() == 0 && s != 0.0f) atomicAdd(&s_acc[k][j], s);
+                }
+            }
+        }
+        __syncthreads();
+        if (t < cnt) {
+            uint32_t gid = s_gid[t];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                float v = s_acc[k][t];
+                if (v != 0.0f) unsafeAtomicAdd(&a.grad2d[(size_t)k * a.n + gid], v);
+            }
+        }
+        cend = cbeg;
+    }
+}
+
+constexpr float kShC0 = 0.2820948f;
+constexpr float kShC1 = 0.48860252f;
+constexpr float kShC20 = 0.9461747f;
+constexpr float kShC21 = 0.31539157f;
+constexpr float kShC22 = -1.0925485f;
+constexpr float kShC23 = 0.54627424f;
+constexpr float kK9 = -0.5900436f;
+
+// Projection adjoint, one thread per splat.  CHS = SH coefficients per channel.
+template <int CHS>
+__global__ __launch_bounds__(256) void k_project_bwd(sfk_view v, const float* __restrict__ A,
+                                                     const float* __restrict__ shs, uint32_t n,
+                                                     const float* __restrict__ g2,
+                                                     float* __restrict__ gA,
+                                                     float* __restrict__ gS) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const size_t N = n;
+    float mx = A[0 * N + i], my = A[1 * N + i], mz = A[2 * N + i];
+    float qw = A[3 * N + i], qx = A[4 * N + i], qy = A[5 * N + i], qz = A[6 * N + i];
+    float s[3] = {sfk_expf(A[7 * N + i]), sfk_expf(A[8 * N + i]), sfk_expf(A[9 * N + i])};
+    float op = 1.0f / (1.0f + sfk_expf(-A[10 * N + i]));
+    float cam[3];
+    cam[0] = v.rot0[0] * mx + v.rot0[1] * my + v.rot0[2] * mz + v.trans[0];
+    cam[1] = v.rot1[0] * mx + v.rot1[1] * my + v.rot1[2] * mz + v.trans[1];
+    cam[2] = v.rot2[0] * mx + v.rot2[1] * my + v.rot2[2] * mz + v.trans[2];
+    bool vis = !(op < 10.0f / 255.0f) && !(cam[2] <= 0.1f);
+    if (!vis) {
+        for (int k = 0; k < 11; ++k) gA[(size_t)k * N + i] = 0.0f;
+        for (int k = 0; k < 3 * CHS; ++k) gS[(size_t)k * N + i] = 0.0f;
+        return;
+    }
+    float G2[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) G2[k] = g2[(size_t)k * N + i];
+
+    const float iz = 1.0f / cam[2];
+    float x2 = qx * qx, y2 = qy * qy, z2 = qz * qz, xy = qx * qy, xz = qx * qz, yz = qy * qz;
+    float wx = qw * qx, wy = qw * qy, wz = qw * qz;
+    float R[3][3] = {{1.0f - 2.0f * (y2 + z2), 2.0f * (xy - wz), 2.0f * (xz + wy)},
+                     {2.0f * (xy + wz), 1.0f - 2.0f * (x2 + z2), 2.0f * (yz - wx)},
+                     {2.0f * (xz - wy), 2.0f * (yz + wx), 1.0f - 2.0f * (x2 + y2)}};
+    float M[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) M[r][k] = R[r][k] * s[k];
+    const float* r0 = v.rot0;
+    const float* r1 = v.rot1;
+    const float* r2 = v.rot2;
+    float limx = 1.3f * v.img[0] / (2.0f * v.focal[0]);
+    float limy = 1.3f * v.img[1] / (2.0f * v.focal[1]);
+    float u = cam[0] * iz, vv = cam[1] * iz;
+    bool uc = u < -limx || u > limx, vc = vv < -limy || vv > limy;
+    u = sfk_clampf(u, -limx, limx);
+    vv = sfk_clampf(vv, -limy, limy);
+    float fxz = v.focal[0] * iz, fyz = v.focal[1] * iz;
+    float t0[3], t1[3], j0[3], j1[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        t0[k] = fxz * r0[k] - fxz * u * r2[k];
+        t1[k] = fyz * r1[k] - fyz * vv * r2[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        j0[k] = t0[0] * M[0][k] + t0[1] * M[1][k] + t0[2] * M[2][k];
+        j1[k] = t1[0] * M[0][k] + t1[1] * M[1][k] + t1[2] * M[2][k];
+    }
+    float ca = j0[0] * j0[0] + j0[1] * j0[1] + j0[2] * j0[2] + 0.3f;
+    float cb = j0[0] * j1[0] + j0[1] * j1[1] + j0[2] * j1[2];
+    float cc = j1[0] * j1[0] + j1[1] * j1[1] + j1[2] * j1[2] + 0.3f;
+    float detr = ca * cc - cb * cb;
+    bool dclamp = detr < 1e-6f;
+    float det = dclamp ? 1e-6f : detr;
+
+    // colour -> SH, view direction
+    float d[3] = {mx - v.world_pos[0], my - v.world_pos[1], mz - v.world_pos[2]};
+    float vlen = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    float ivl = 1.0f / vlen;
+    float x = d[0] * ivl, y = d[1] * ivl, z = d[2] * ivl;
+    float B[16];
+    float D[16][3];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) { B[k] = 0.f; D[k][0] = D[k][1] = D[k][2] = 0.f; }
+    B[0] = kShC0;
+    if constexpr (CHS >= 4) {
+        B[1] = -kShC1 * y; B[2] = kShC1 * z; B[3] = -kShC1 * x;
+        D[1][1] = -kShC1; D[2][2] = kShC1; D[3][0] = -kShC1;
+    }
+    if constexpr (CHS >= 9) {
+        B[4] = kShC23 * 2.0f * x * y; B[5] = kShC22 * z * y; B[6] = kShC20 * z * z - kShC21;
+        B[7] = kShC22 * z * x; B[8] = kShC23 * (x * x - y * y);
+        D[4][0] = 2.f * kShC23 * y; D[4][1] = 2.f * kShC23 * x;
+        D[5][1] = kShC22 * z; D[5][2] = kShC22 * y;
+        D[6][2] = 2.f * kShC20 * z;
+        D[7][0] = kShC22 * z; D[7][2] = kShC22 * x;
+        D[8][0] = 2.f * kShC23 * x; D[8][1] = -2.f * kShC23 * y;
+    }
+    if constexpr (CHS >= 16) {
+        float c1x = x * x - y * y, c1y = 2.f * x * y;
+        float tt0 = -2.285229f * z * z + 0.4570458f, tt1 = 1.4453057f * z;
+        B[9] = kK9 * x * c1y + y * c1x;
+        B[10] = kK9 * x * c1x - y * c1y;
+        B[11] = tt0 * y; B[12] = tt0 * x;
+        B[13] = z * (1.8658817f * z * z - 1.119529f);
+        B[14] = tt1 * c1y; B[15] = tt1 * c1x;
+        D[9][0] = kK9 * c1y + kK9 * x * 2.f * y + y * 2.f * x;
+        D[9][1] = kK9 * x * 2.f * x + c1x - 2.f * y * y;
+        D[10][0] = kK9 * c1x + kK9 * x * 2.f * x - 2.f * y * y;
+        D[10][1] = -kK9 * x * 2.f * y - c1y - 2.f * x * y;
+        D[11][1] = tt0; D[11][2] = -4.570458f * z * y;
+        D[12][0] = tt0; D[12][2] = -4.570458f * z * x;
+        D[13][2] = 5.5976451f * z * z - 1.119529f;
+        D[14][0] = tt1 * 2.f * y; D[14][1] = tt1 * 2.f * x; D[14][2] = 1.4453057f * c1y;
+        D[15][0] = tt1 * 2.f * x; D[15][1] = -tt1 * 2.f * y; D[15][2] = 1.4453057f * c1x;
+    }
+    float gd0 = 0.f, gd1 = 0.f, gd2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < CHS; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float gc = G2[6 + c];
+            gS[(size_t)(3 * k + c) * N + i] = B[k] * gc;
+            float sv = shs[(size_t)(3 * k + c) * N + i] * gc;
+            gd0 += sv * D[k][0];
+            gd1 += sv * D[k][1];
+            gd2 += sv * D[k][2];
+        }
+    float dd = gd0 * x + gd1 * y + gd2 * z;
+    float gm[3] = {(gd0 - x * dd) * ivl, (gd1 - y * dd) * ivl, (gd2 - z * dd) * ivl};
+
+    gA[10 * N + i] = G2[5] * op * (1.0f - op);
+
+    float gx = G2[2], gy = G2[3], gz = G2[4];
+    float inv = 1.0f / det, inv2 = inv * inv;
+    float ga, gb, gc;
+    if (!dclamp) {
+        ga = gx * (-cc * cc * inv2) + gy * (cb * cc * inv2) + gz * (inv - ca * cc * inv2);
+        gb = gx * (2.f * cb * cc * inv2) + gy * (-inv - 2.f * cb * cb * inv2) + gz * (2.f * ca * cb * inv2);
+        gc = gx * (inv - ca * cc * inv2) + gy * (ca * cb * inv2) + gz * (-ca * ca * inv2);
+    } else {
+        ga = gz * inv; gb = -gy * inv; gc = gx * inv;
+    }
+    float gj0[3], gj1[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        gj0[k] = 2.f * ga * j0[k] + gb * j1[k];
+        gj1[k] = gb * j0[k] + 2.f * gc * j1[k];
+    }
+    float gt0[3] = {0.f, 0.f, 0.f}, gt1[3] = {0.f, 0.f, 0.f}, gR[3][3];
+    float gs[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float gM = t0[r] * gj0[k] + t1[r] * gj1[k];
+            gt0[r] += M[r][k] * gj0[k];
+            gt1[r] += M[r][k] * gj1[k];
+            gs[k] += R[r][k] * gM;
+            gR[r][k] = gM * s[k];
+        }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gA[(size_t)(7 + k) * N + i] = gs[k] * s[k];
+    float gqw = 0.f, gqx = 0.f, gqy = 0.f, gqz = 0.f;
+    gqy += gR[0][0] * (-4.f * qy); gqz += gR[0][0] * (-4.f * qz);
+    gqx += gR[0][1] * 2.f * qy; gqy += gR[0][1] * 2.f * qx; gqw -= gR[0][1] * 2.f * qz; gqz -= gR[0][1] * 2.f * qw;
+    gqx += gR[0][2] * 2.f * qz; gqz += gR[0][2] * 2.f * qx; gqw += gR[0][2] * 2.f * qy; gqy += gR[0][2] * 2.f * qw;
+    gqx += gR[1][0] * 2.f * qy; gqy += gR[1][0] * 2.f * qx; gqw += gR[1][0] * 2.f * qz; gqz += gR[1][0] * 2.f * qw;
+    gqx += gR[1][1] * (-4.f * qx); gqz += gR[1][1] * (-4.f * qz);
+    gqy += gR[1][2] * 2.f * qz; gqz += gR[1][2] * 2.f * qy; gqw -= gR[1][2] * 2.f * qx; gqx -= gR[1][2] * 2.f * qw;
+    gqx += gR[2][0] * 2.f * qz; gqz += gR[2][0] * 2.f * qx; gqw -= gR[2][0] * 2.f * qy; gqy -= gR[2][0] * 2.f * qw;
+    gqy += gR[2][1] * 2.f * qz; gqz += gR[2][1] * 2.f * qy; gqw += gR[2][1] * 2.f * qx; gqx += gR[2][1] * 2.f * qw;
+    gqx += gR[2][2] * (-4.f * qx); gqy += gR[2][2] * (-4.f * qy);
+    gA[3 * N + i] = gqw; gA[4 * N + i] = gqx; gA[5 * N + i] = gqy; gA[6 * N + i] = gqz;
+
+    float fx = v.focal[0], fy = v.focal[1];
+    float giz = 0.f, gu = 0.f, gv = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        giz += gt0[k] * fx * (r0[k] - u * r2[k]) + gt1[k] * fy * (r1[k] - vv * r2[k]);
+        gu += gt0[k] * (-fxz * r2[k]);
+        gv += gt1[k] * (-fyz * r2[k]);
+    }
+    float gcam[3] = {0.f, 0.f, 0.f};
+    if (!uc) { gcam[0] += gu * iz; giz += gu * cam[0]; }
+    if (!vc) { gcam[1] += gv * iz; giz += gv * cam[1]; }
+    gcam[0] += G2[0] * fx * iz; giz += G2[0] * fx * cam[0];
+    gcam[1] += G2[1] * fy * iz; giz += G2[1] * fy * cam[1];
+    gcam[2] += giz * (-iz * iz);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        gA[(size_t)k * N + i] = gm[k] + r0[k] * gcam[0] + r1[k] * gcam[1] + r2[k] * gcam[2];
+}
+
+}  // namespace
+
+hipError_t launch_loss_l2(hipStream_t s, const float* img, const float* target, uint32_t np,
+                          float* d_img, double* partials, double* out) {
+    uint32_t nb = div_up(np ? np : 1, 256);
+    if (np) k_loss_l2<<<nb, 256, 0, s>>>(reinterpret_cast<const float4*>(img),
+                                         reinterpret_cast<const float4*>(target), np,
+                                         reinterpret_cast<float4*>(d_img), partials);
+    else nb = 0;
+    if (out) k_sum_f64<<<1, 256, 0, s>>>(partials, nb, out);
+    return hipGetLastError();
+}
+
+uint32_t loss_num_partials(uint32_t np) { return div_up(np ? np : 1, 256); }
+
+hipError_t launch_raster_bwd(hipStream_t s, const RasterArgs& ra, const float* final_t,
+                             const uint32_t* n_contrib, const float* d_img, uint32_t n,
+                             float* grad2d) {
+    BwdArgs a;
+    a.w = ra.w; a.h = ra.h; a.tiles_x = ra.tiles_x; a.tiles_y = ra.tiles_y; a.n = n;
+    a.tile_starts = ra.tile_starts; a.gids = ra.gids; a.proj = ra.proj;
+    a.final_t = final_t; a.n_contrib = n_contrib; a.d_img = d_img; a.grad2d = grad2d;
+    uint32_t nt = ra.tiles_x * ra.tiles_y;
+    if (nt == 0) return hipSuccess;
+    switch (ra.tile_w) {
+        case 8: k_raster_bwd<8><<<nt, 64, 0, s>>>(a); break;
+        case 16: k_raster_bwd<16><<<nt, 256, 0, s>>>(a); break;
+        case 32: k_raster_bwd<32><<<nt, 1024, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_project_bwd(hipStream_t s, const sfk_view& v, const float* attrs,
+                              const float* sh, uint32_t n, uint32_t sh_per_ch,
+                              const float* grad2d, float* g_attrs, float* g_sh) {
+    if (n == 0) return hipSuccess;
+    dim3 grid(div_up(n, 256));
+    switch (sh_per_ch) {
+        case 1: k_project_bwd<1><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh); break;
+        case 4: k_project_bwd<4><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh); break;
+        case 9: k_project_bwd<9><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh); break;
+        case 16: k_project_bwd<16><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace sfk

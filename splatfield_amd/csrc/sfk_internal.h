@@ -5,8 +5,14 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/sfk.h"
+
+struct sfk_prof_rec {
+    const char* name;
+    hipEvent_t a, b;
+};
 
 struct sfk_ctx {
     int device = 0;
@@ -17,9 +23,35 @@ struct sfk_ctx {
     double* eig_partials = nullptr;
     uint32_t eig_partials_cap = 0;
     double* eig_out = nullptr;
+    // Per-launch event timing (sfk_prof_enable).
+    bool prof = false;
+    std::vector<sfk_prof_rec> prof_recs;
+    std::vector<hipEvent_t> prof_pool;
 };
 
 namespace sfk {
+
+// Event pair around one launch when profiling is on.
+hipEvent_t prof_event(sfk_ctx* c);
+struct ProfScope {
+    sfk_ctx* c;
+    const char* name;
+    hipEvent_t a = nullptr;
+    ProfScope(sfk_ctx* ctx, const char* nm) : c(ctx), name(nm) {
+        if (c && c->prof) {
+            a = prof_event(c);
+            hipEventRecord(a, c->stream);
+        }
+    }
+    ~ProfScope() {
+        if (a) {
+            hipEvent_t b = prof_event(c);
+            hipEventRecord(b, c->stream);
+            c->prof_recs.push_back({name, a, b});
+        }
+    }
+};
+#define SFK_PROF(ctx, name) sfk::ProfScope _sfk_prof_scope(ctx, name)
 
 constexpr int kWave = 64;
 
@@ -101,6 +133,17 @@ struct RasterArgs {
     uint32_t* bg;
 };
 hipError_t launch_rasterize(hipStream_t s, int mode, const RasterArgs& a);
+
+// Backward.
+uint32_t loss_num_partials(uint32_t np);
+hipError_t launch_loss_l2(hipStream_t s, const float* img, const float* target, uint32_t np,
+                          float* d_img, double* partials, double* out);
+hipError_t launch_raster_bwd(hipStream_t s, const RasterArgs& ra, const float* final_t,
+                             const uint32_t* n_contrib, const float* d_img, uint32_t n,
+                             float* grad2d);
+hipError_t launch_project_bwd(hipStream_t s, const sfk_view& v, const float* attrs,
+                              const float* sh, uint32_t n, uint32_t sh_per_ch,
+                              const float* grad2d, float* g_attrs, float* g_sh);
 
 // Seg.
 hipError_t launch_eig(hipStream_t s, const float* a, const float* b, const uint32_t* resp,

@@ -116,18 +116,25 @@ struct sfk_render_scratch {
     // Current frame outputs (sorted isects alias sort_tile).
     const uint32_t* sorted_tiles = nullptr;
     const uint32_t* sorted_gids = nullptr;
+    sfk_view view;  // view of the last prepare
+    bool has_view = false;
+    float* grad2d = nullptr;  // 9 planes x n screen-space gradients (backward)
+    double* loss_partials = nullptr;
+    uint32_t loss_partials_cap = 0;
 };
 
 static void scratch_free_frame(sfk_render_scratch* s) {
     void* bufs[] = {s->depth_keys, s->projected, s->tile_counts, s->bbox, s->offsets,
                     s->counters, s->tile_starts, s->bitmap, s->rgba, s->final_t, s->n_contrib,
-                    s->tile_ids, s->gaussian_ids};
+                    s->tile_ids, s->gaussian_ids, s->grad2d};
     for (void* p : bufs)
         if (p) hipFree(p);
     s->depth_keys = nullptr; s->projected = nullptr; s->tile_counts = nullptr; s->bbox = nullptr;
     s->offsets = nullptr; s->counters = nullptr; s->tile_starts = nullptr; s->bitmap = nullptr;
     s->rgba = nullptr; s->final_t = nullptr; s->n_contrib = nullptr; s->tile_ids = nullptr;
     s->gaussian_ids = nullptr;
+    s->grad2d = nullptr;
+    s->has_view = false;
     sfk_radix_scratch_destroy(s->sort_depth);
     sfk_radix_scratch_destroy(s->sort_tile);
     sfk_scan_scratch_destroy(s->scan);
@@ -209,6 +216,7 @@ extern "C" void sfk_render_scratch_destroy(sfk_render_scratch* s) {
     if (s->ctx) hipStreamSynchronize(s->ctx->stream);
     scratch_free_frame(s);
     if (s->host_counters) hipHostFree(s->host_counters);
+    if (s->loss_partials) hipFree(s->loss_partials);
     if (s->counters_ready) hipEventDestroy(s->counters_ready);
     delete s;
 }
@@ -309,8 +317,12 @@ extern "C" int sfk_prepare_isects_view(sfk_ctx* c, const sfk_splats* sp,
     const uint32_t tiles = s->tiles_x * s->tiles_y;
 
     SFK_HIP(c, launch_fill_u32(c->stream, s->counters, 0u, 4));
-    SFK_HIP(c, launch_project(c->stream, *view, sp->attrs, sp->sh, n, sp->sh_per_ch, s->tile_w,
-                              s->depth_keys, s->projected, s->tile_counts, s->bbox, s->counters));
+    {
+        SFK_PROF(c, "project");
+        SFK_HIP(c, launch_project(c->stream, *view, sp->attrs, sp->sh, n, sp->sh_per_ch,
+                                  s->tile_w, s->depth_keys, s->projected, s->tile_counts, s->bbox,
+                                  s->counters));
+    }
     SFK_HIP(c, hipMemcpyAsync(s->host_counters, s->counters, 8, hipMemcpyDeviceToHost, c->stream));
     SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));
     s->counters_pending = true;
@@ -322,8 +334,11 @@ extern "C" int sfk_prepare_isects_view(sfk_ctx* c, const sfk_splats* sp,
     s->depth_order = order;
     rc = sfk_exclusive_scan_gather(c, order, s->tile_counts, s->offsets, n, s->scan);
     if (rc) return rc;
-    SFK_HIP(c, launch_map_isects(c->stream, order, s->tile_counts, s->bbox, s->offsets, n,
-                                 s->tiles_x, s->capacity, s->tile_ids, s->gaussian_ids));
+    {
+        SFK_PROF(c, "map_isects");
+        SFK_HIP(c, launch_map_isects(c->stream, order, s->tile_counts, s->bbox, s->offsets, n,
+                                     s->tiles_x, s->capacity, s->tile_ids, s->gaussian_ids));
+    }
     uint32_t* st = nullptr;
     uint32_t* sg = nullptr;
     rc = sfk_argsort_u32(c, s->sort_tile, s->tile_ids, s->gaussian_ids, s->capacity, s->counters,
@@ -331,7 +346,13 @@ extern "C" int sfk_prepare_isects_view(sfk_ctx* c, const sfk_splats* sp,
     if (rc) return rc;
     s->sorted_tiles = st;
     s->sorted_gids = sg;
-    SFK_HIP(c, launch_tile_starts(c->stream, st, s->capacity, s->counters, tiles, s->tile_starts));
+    {
+        SFK_PROF(c, "tile_starts");
+        SFK_HIP(c, launch_tile_starts(c->stream, st, s->capacity, s->counters, tiles,
+                                      s->tile_starts));
+    }
+    s->view = *view;
+    s->has_view = true;
     fill_isects(s, out);
     return SFK_OK;
 }
@@ -383,6 +404,7 @@ extern "C" int sfk_finalize_rgb(sfk_ctx* c, sfk_render_scratch* s, const sfk_til
     a.bitmap = s->bitmap;
     a.final_t = s->final_t;
     a.n_contrib = s->n_contrib;
+    SFK_PROF(c, "raster_rgb");
     SFK_HIP(c, launch_rasterize(c->stream, kRgb, a));
     return SFK_OK;
 }
@@ -395,6 +417,7 @@ extern "C" int sfk_finalize_total(sfk_ctx* c, sfk_render_scratch* s, const sfk_t
     RasterArgs a = raster_args(s, t);
     a.scale = scale;
     a.resp = resp;
+    SFK_PROF(c, "raster_total");
     SFK_HIP(c, launch_rasterize(c->stream, kTotal, a));
     return SFK_OK;
 }
@@ -412,6 +435,7 @@ extern "C" int sfk_finalize_evidence(sfk_ctx* c, sfk_render_scratch* s, const sf
     a.words_per_row = words_per_row;
     a.fg = fg;
     a.bg = bg;
+    SFK_PROF(c, "raster_evidence");
     SFK_HIP(c, launch_rasterize(c->stream, kEvidence, a));
     return SFK_OK;
 }
@@ -504,6 +528,7 @@ extern "C" int sfk_eig_device(sfk_ctx* c, const float* a, const float* b, const 
     if (!c || !d_out || (n && (!a || !b || !resp))) return SFK_E_ARG;
     int rc = eig_workspace(c, n);
     if (rc) return rc;
+    SFK_PROF(c, "eig");
     SFK_HIP(c, launch_eig(c->stream, a, b, resp, n, scale, c->eig_partials, d_out));
     return SFK_OK;
 }
@@ -555,5 +580,37 @@ extern "C" int sfk_tile_starts(sfk_ctx* c, const uint32_t* tile_ids, uint32_t n,
                                const uint32_t* d_n, uint32_t num_tiles, uint32_t* starts) {
     if (!c || !starts || (n && !tile_ids)) return SFK_E_ARG;
     SFK_HIP(c, launch_tile_starts(c->stream, tile_ids, n, d_n, num_tiles, starts));
+    return SFK_OK;
+}
+
+// ----------------------------------------------------------------- backward
+
+extern "C" int sfk_loss_l2(sfk_ctx* c, const float* rgba, const float* target, uint32_t pixels,
+                           float* d_rgba, double* d_loss) {
+    if (!c || (pixels && (!rgba || !target || !d_rgba))) return SFK_E_ARG;
+    int rc = eig_workspace(c, loss_num_partials(pixels) * 1024u);
+    if (rc) return rc;
+    SFK_HIP(c, launch_loss_l2(c->stream, rgba, target, pixels, d_rgba, c->eig_partials, d_loss));
+    return SFK_OK;
+}
+
+extern "C" int sfk_render_backward(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
+                                   const sfk_tile_isects* t, const float* d_rgba,
+                                   float* grad_attrs, float* grad_sh) {
+    int rc = check_isects(c, s, t);
+    if (rc) return rc;
+    if (!sp || !d_rgba || !grad_attrs || !grad_sh) return fail(c, SFK_E_ARG, "backward: null buffer");
+    if (sp->n != s->n || !s->has_view) return fail(c, SFK_E_ARG, "backward: scratch was not prepared for these splats");
+    if (!s->grad2d) SFK_HIP(c, hipMalloc(&s->grad2d, (size_t)s->n * 9 * 4));
+    SFK_HIP(c, launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(s->grad2d), 0u, (size_t)s->n * 9));
+    RasterArgs a = raster_args(s, t);
+    {
+        SFK_PROF(c, "raster_bwd");
+        SFK_HIP(c, launch_raster_bwd(c->stream, a, s->final_t, s->n_contrib, d_rgba, s->n,
+                                     s->grad2d));
+    }
+    SFK_PROF(c, "project_bwd");
+    SFK_HIP(c, launch_project_bwd(c->stream, s->view, sp->attrs, sp->sh, sp->n, sp->sh_per_ch,
+                                  s->grad2d, grad_attrs, grad_sh));
     return SFK_OK;
 }

@@ -221,3 +221,16 @@ def render_with(splats, scratch, camera, w, h):
                                     C.byref(bm), C.byref(rgba)), "render_with")
     b = scratch.buffers()
     return GpuTensor(ctx, bm.value, (h, b.row_stride), np.uint32, False, scratch).read()
+
+
+def loss_l2(ctx, rgba, target, pixels, d_rgba, d_loss=None):
+    """d_rgba = 2 (rgba - target); d_loss (device f64) = sum of squares."""
+    ctx.check(lib().sfk_loss_l2(ctx.ptr, rgba, target, pixels, d_rgba,
+                                d_loss.ptr if d_loss is not None else None), "loss_l2")
+
+
+def render_backward(splats, scratch, isects, d_rgba, grad_attrs, grad_sh):
+    """Adjoint of finalize(Rgb) + project_splats (sfk_render_backward)."""
+    ctx = splats.ctx
+    ctx.check(lib().sfk_render_backward(ctx.ptr, splats.ptr, scratch.ptr, C.byref(isects.c),
+                                        d_rgba, grad_attrs.ptr, grad_sh.ptr), "render_backward")

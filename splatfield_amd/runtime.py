@@ -43,6 +43,20 @@ class Context:
         self.check(lib().sfk_timer_stop_ms(self.ptr, C.byref(ms)))
         return ms.value
 
+    def prof_enable(self, on=True):
+        self.check(lib().sfk_prof_enable(self.ptr, int(on)), "prof_enable")
+
+    def prof_report(self):
+        """{name: (launches, total_ms)} of the launches recorded since the
+        last report (SYNC)."""
+        buf = C.create_string_buffer(1 << 16)
+        self.check(lib().sfk_prof_report(self.ptr, buf, 1 << 16), "prof_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            out[name] = (int(cnt), float(ms))
+        return out
+
     def close(self):
         if self.ptr:
             lib().sfk_ctx_destroy(self.ptr)
