@@ -33,7 +33,79 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--no-extra", action="store_true",
+                   help="skip the EIG-round and standalone-sort measurements")
+    p.add_argument("--eig-views", type=int, default=20)
+    p.add_argument("--eig-res", type=int, default=512)
+    p.add_argument("--eig-reps", type=int, default=5)
     return p.parse_args()
+
+
+def bench_eig_round(ctx, args):
+    """EIG-round latency: 20 candidate views at 512^2 around a 1M-splat SH3
+    scene, plain (the reference's select_view loop: per view a
+    responsibility render and a synchronous EIG) vs the batched round
+    (one hipGraph replay), with the per-view u32 evidence compared."""
+    from splatfield_amd import render as R
+    from splatfield_amd import scene as S
+    from splatfield_amd import seg as G
+    from splatfield_amd.runtime import GpuTensor
+    n = args.n
+    attrs, sh, k = S.synthetic_scene(n, 3, seed=0)
+    splats = R.Splats(ctx, attrs, sh)
+    cams = G.sphere_cameras(args.eig_views, radius=3.0)
+    sizes = [(args.eig_res, args.eig_res)] * args.eig_views
+    rng = np.random.default_rng(2)
+    state = G.BetaState(ctx, (1 + rng.random(n) * 4).astype(np.float32),
+                        (1 + rng.random(n) * 4).astype(np.float32))
+    scratch = R.RenderScratch(ctx, n, args.eig_res, args.eig_res)
+    scratch.reserve_isects(1 << 23)
+    resp = GpuTensor.empty(ctx, n)
+    eig_p, best_p = G.eig_round_plain(splats, scratch, cams, sizes, state, resp)  # warm
+    t = []
+    for _ in range(args.eig_reps):
+        t0 = time.perf_counter()
+        eig_p, best_p = G.eig_round_plain(splats, scratch, cams, sizes, state, resp)
+        t.append(time.perf_counter() - t0)
+    plain_ms = 1e3 * float(np.median(t))
+    rnd = G.EigRound(splats, cams, sizes)
+    rnd.run(state, use_graph=True)
+    tb = []
+    for _ in range(args.eig_reps):
+        t0 = time.perf_counter()
+        eig_b, best_b = rnd.run(state, use_graph=True)
+        tb.append(time.perf_counter() - t0)
+    batch_ms = 1e3 * float(np.median(tb))
+    exact = bool(np.array_equal(rnd.resp(args.eig_views - 1)[0], resp.read()))
+    return {"views": args.eig_views, "res": args.eig_res, "plain_ms": round(plain_ms, 3),
+            "batched_ms": round(batch_ms, 3), "ratio": round(batch_ms / plain_ms, 4),
+            "views_per_s": round(args.eig_views / (batch_ms * 1e-3), 1),
+            "best_plain": best_p, "best_batched": best_b,
+            "eig_max_abs_diff": float(np.max(np.abs(eig_b - eig_p))),
+            "last_view_evidence_bit_exact": exact, "isects_total": int(rnd.counters()[-1])}
+
+
+def bench_sort(ctx, args):
+    """Standalone stable argsort of 16,777,216 uniform u64 keys + u32 index
+    values (HIP events around the sort launches only)."""
+    from splatfield_amd import sort as SO
+    from splatfield_amd.runtime import GpuTensor
+    n = 1 << 24
+    keys = np.random.default_rng(0).integers(0, 2**64 - 1, n, dtype=np.uint64)
+    kt = GpuTensor.from_numpy(ctx, keys)
+    vt = GpuTensor.from_numpy(ctx, np.arange(n, dtype=np.uint32))
+    s = SO.RadixScratch(ctx, n, 8, 8)
+    s.argsort(kt, vt, n, 64)
+    ctx.sync()
+    reps = 5
+    ctx.timer_start()
+    for _ in range(reps):
+        s.argsort(kt, vt, n, 64)
+    ms = ctx.timer_stop_ms() / reps
+    algo = n * 8 + 8 * 2 * (8 + 4) * n
+    return {"keys": n, "ms": round(ms, 4), "gkeys_per_s": round(n / (ms * 1e-3) / 1e9, 3),
+            "achieved_gbs": round(algo / (ms * 1e-3) / 1e9, 1),
+            "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 def kernel_bytes(n, k, vis, ni, px, tile_bits):
@@ -106,8 +178,8 @@ def main():
     raw, vis, ni = isects.counts()
     scratch.reserve_isects(raw)
     rgba_ptr = scratch.device("rgba")
-    for _ in range(args.warmup):
-        step()
+    for _ in range(max(args.warmup, 1)):
+        isects = step()
     ctx.sync()
     raw, vis, ni = isects.counts()
     assert ni == raw, "isect capacity must be steady before timing"
@@ -151,6 +223,11 @@ def main():
     b_bwd = 40 * ni + 32 * px + 36 * vis + n * (44 + 4 * 3 * k) + 36 * vis + n * (44 + 4 * 3 * k)
     step_gbs = (b_fwd + b_bwd) / (ms_per_step * 1e-3) / 1e9
 
+    extra = {}
+    if not args.no_extra:
+        extra["eig_round"] = bench_eig_round(ctx, args)
+        extra["sort_u64"] = bench_sort(ctx, args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(attrs, sh, n, k, view, w, h, args.cpu_threads, args.cpu_seconds,
@@ -180,6 +257,7 @@ def main():
                      "frame_achieved": round(step_gbs, 1),
                      "frame_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "kernels": per_kernel,
+        "extra": extra,
         "cpu_baseline": cpu,
         "wall_s": round(wall, 3),
     }

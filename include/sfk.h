@@ -119,6 +119,14 @@ int sfk_argsort_u64(sfk_ctx* ctx, sfk_radix_scratch* s, const uint64_t* keys,
                     const uint32_t* vals, uint32_t n, const uint32_t* d_n, uint32_t bits,
                     int write_keys, uint64_t** out_keys, uint32_t** out_vals);
 
+/* nseg independent stable argsorts of seg_len keys each, laid out back to
+ * back, in the same launches (no reference counterpart: the batched
+ * multi-view depth sort).  Identity values are indices within the segment. */
+int sfk_argsort_u32_segmented(sfk_ctx* ctx, sfk_radix_scratch* s, const uint32_t* keys,
+                              const uint32_t* vals, uint32_t nseg, uint32_t seg_len,
+                              uint32_t bits, int write_keys, uint32_t** out_keys,
+                              uint32_t** out_vals);
+
 typedef struct sfk_scan_scratch sfk_scan_scratch;
 
 /* ScanScratch::new (crates/sort/src/scan.rs:53-60). */
@@ -253,6 +261,38 @@ int sfk_eig_device(sfk_ctx* ctx, const float* a, const float* b, const uint32_t*
 /* BetaState::update (src/seg/beta.rs:484-500): a += fg/scale, b += bg/scale. */
 int sfk_beta_update(sfk_ctx* ctx, float* a, float* b, const uint32_t* fg, const uint32_t* bg,
                     uint32_t n, float scale);
+
+/* ------------------------------------- batched EIG round (no reference
+ * counterpart: the reference scores candidates one plain render at a time,
+ * src/seg/active.rs:113-129) */
+
+typedef struct sfk_eig_round sfk_eig_round;
+
+/* A round over nviews (<= 64) candidate cameras of mixed resolution, at one
+ * tile width (8, 16, 32) and radix digit width (4..8).  Buffers are sized
+ * once for (splats x views). */
+int sfk_eig_round_create(sfk_ctx* ctx, const sfk_splats* splats, const sfk_camera* cams,
+                         const uint32_t* widths, const uint32_t* heights, uint32_t nviews,
+                         uint32_t tile_w, uint32_t digit_bits, sfk_eig_round** out);
+void sfk_eig_round_destroy(sfk_eig_round* r);
+/* Score every view against the posterior (a, b): responsibility renders,
+ * f64 EIG per view and the argmax, as one sequence; use_graph != 0 captures
+ * it into a hipGraph on first use (re-captured if a or b move) and replays
+ * it.  eig_out (nviews doubles) and best_out, if non-NULL, are filled after
+ * one synchronization. */
+int sfk_eig_round_run(sfk_ctx* ctx, sfk_eig_round* r, const float* a, const float* b,
+                      int use_graph, double* eig_out, uint32_t* best_out);
+/* Device pointer of one view's u32 responsibility words and its scale. */
+int sfk_eig_round_info(const sfk_eig_round* r, uint32_t view, const uint32_t** resp,
+                       float* scale, uint32_t* capacity, uint32_t* total_tiles);
+/* SYNC: per view [isects, visible], then the total isect count. */
+int sfk_eig_round_counters(sfk_ctx* ctx, const sfk_eig_round* r, uint32_t* counters);
+/* The plain round (Segmenter::select_view, active.rs:113-129): per view a
+ * full render_responsibility and a synchronous sfk_eig, argmax on the host. */
+int sfk_eig_round_plain(sfk_ctx* ctx, const sfk_splats* splats, sfk_render_scratch* scratch,
+                        const sfk_camera* cams, const uint32_t* widths, const uint32_t* heights,
+                        uint32_t nviews, const float* a, const float* b, uint32_t* resp,
+                        double* eig_out, uint32_t* best_out);
 
 /* ------------------------------------------------ backward (no reference
  * counterpart: the reference renders forward only, docs/00_pipeline.md:1) */

@@ -67,6 +67,7 @@ SIGNATURES = {
     "sfk_radix_set_digit_bits": (i32, [P, u32]),
     "sfk_argsort_u32": (i32, [P, P, P, P, u32, P, u32, i32, PP, PP]),
     "sfk_argsort_u64": (i32, [P, P, P, P, u32, P, u32, i32, PP, PP]),
+    "sfk_argsort_u32_segmented": (i32, [P, P, P, P, u32, u32, u32, i32, PP, PP]),
     "sfk_scan_scratch_create": (i32, [P, u32, PP]),
     "sfk_scan_scratch_destroy": (None, [P]),
     "sfk_exclusive_scan_gather": (i32, [P, P, P, P, u32, P]),
@@ -95,6 +96,12 @@ SIGNATURES = {
     "sfk_eig": (i32, [P, P, P, P, u32, f32, P]),
     "sfk_eig_device": (i32, [P, P, P, P, u32, f32, P]),
     "sfk_beta_update": (i32, [P, P, P, P, P, u32, f32]),
+    "sfk_eig_round_create": (i32, [P, P, P, P, P, u32, u32, u32, PP]),
+    "sfk_eig_round_destroy": (None, [P]),
+    "sfk_eig_round_run": (i32, [P, P, P, P, i32, P, P]),
+    "sfk_eig_round_info": (i32, [P, u32, PP, P, P, P]),
+    "sfk_eig_round_counters": (i32, [P, P, P]),
+    "sfk_eig_round_plain": (i32, [P, P, P, P, P, P, u32, P, P, P, P, P]),
     "sfk_loss_l2": (i32, [P, P, P, u32, P, P]),
     "sfk_render_backward": (i32, [P, P, P, C.POINTER(TileIsects), P, P, P]),
     "sfk_project": (i32, [P, C.POINTER(View), P, P, u32, u32, u32, P, P, P, P, P]),

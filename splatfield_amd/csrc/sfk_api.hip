@@ -281,8 +281,9 @@ static int radix_alloc(sfk_ctx* c, sfk_radix_scratch* s, uint32_t max_elems) {
         SFK_HIP(c, hipMalloc(&s->keys[i], m * s->key_bytes));
         SFK_HIP(c, hipMalloc(&s->vals[i], m * 4));
     }
-    SFK_HIP(c, hipMalloc(&s->hist, (size_t)s->max_blocks * 256 * 4));
-    SFK_HIP(c, hipMalloc(&s->totals, 256 * 4));
+    // Room for up to 256 segments (each may round its last block up).
+    SFK_HIP(c, hipMalloc(&s->hist, ((size_t)s->max_blocks + 256) * 256 * 4));
+    SFK_HIP(c, hipMalloc(&s->totals, 256 * 256 * 4));
     return SFK_OK;
 }
 
@@ -319,13 +320,16 @@ extern "C" int sfk_radix_set_digit_bits(sfk_radix_scratch* s, uint32_t bits) {
 template <typename K>
 static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const uint32_t* vals,
                         uint32_t n, const uint32_t* d_n, uint32_t bits, int write_keys,
-                        K** out_keys, uint32_t** out_vals) {
+                        K** out_keys, uint32_t** out_vals, uint32_t nseg = 1) {
     if (!c || !s || !out_vals || (n && !keys)) return fail(c, SFK_E_ARG, "argsort: bad args");
     if (sizeof(K) != s->key_bytes) return fail(c, SFK_E_ARG, "argsort: scratch key width %u", s->key_bytes);
     if (bits > 8 * sizeof(K)) return fail(c, SFK_E_ARG, "argsort: %u bits on %zu-byte keys", bits, sizeof(K));
-    if (n > s->max_elems)
-        return fail(c, SFK_E_CAPACITY, "radix scratch undersized: sorted %u elements, capacity %u", n,
-                    s->max_elems);
+    if (nseg == 0 || nseg > 256 || (nseg > 1 && d_n))
+        return fail(c, SFK_E_ARG, "argsort: %u segments", nseg);
+    const uint64_t total = (uint64_t)n * nseg;  // n = keys per segment
+    if (total > s->max_elems)
+        return fail(c, SFK_E_CAPACITY, "radix scratch undersized: sorted %llu elements, capacity %u",
+                    (unsigned long long)total, s->max_elems);
     // n <= 1 or bits == 0: nothing moves (sort.rs:319-321).
     if ((n <= 1 || bits == 0) && vals) {
         if (out_keys) *out_keys = const_cast<K*>(keys);
@@ -345,23 +349,29 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const u
     int dst = 0;
     const uint32_t db = s->digit_bits;
     const uint32_t passes = bits == 0 ? 1u : (bits + db - 1) / db;
+    SortArgs a;
+    a.n_max = nseg > 1 ? 0u : n;
+    a.d_n = d_n;
+    a.nseg = nseg;
+    a.seg_len = nseg > 1 ? n : 0u;
+    a.hist = s->hist;
+    a.stride = nseg > 1 ? radix_max_blocks(n) : s->max_blocks;
+    a.totals = s->totals;
     for (uint32_t p = 0; p < passes; ++p) {
         uint32_t shift = p * db;
         uint32_t width = bits == 0 ? 0u : std::min(db, bits - shift);
         bool last = p + 1 == passes;
-        K* ok = (!last || write_keys) ? static_cast<K*>(s->keys[dst]) : nullptr;
-        uint32_t* ov = s->vals[dst];
-        hipError_t e;
+        a.keys_in = cur_k;
+        a.vals_in = cur_v;
+        a.keys_out = (!last || write_keys) ? s->keys[dst] : nullptr;
+        a.vals_out = s->vals[dst];
+        a.shift = shift;
+        a.width = width;
         SFK_PROF(c, sizeof(K) == 4 ? "radix_pass_u32" : "radix_pass_u64");
-        if constexpr (sizeof(K) == 4)
-            e = launch_radix_pass_u32(c->stream, cur_k, cur_v, ok, ov, n, d_n, shift, width, db,
-                                      s->hist, s->max_blocks, s->totals);
-        else
-            e = launch_radix_pass_u64(c->stream, cur_k, cur_v, ok, ov, n, d_n, shift, width, db,
-                                      s->hist, s->max_blocks, s->totals);
+        hipError_t e = launch_radix_pass(c->stream, a, sizeof(K), db);
         if (e != hipSuccess) return hip_fail(c, e, "radix pass");
         cur_k = static_cast<K*>(s->keys[dst]);
-        cur_v = ov;
+        cur_v = a.vals_out;
         dst ^= 1;
     }
     if (out_keys) *out_keys = const_cast<K*>(cur_k);
@@ -429,4 +439,12 @@ extern "C" int sfk_exclusive_scan(sfk_ctx* c, uint32_t* buf, uint32_t n, sfk_sca
     if (n > s->max_elems) return fail(c, SFK_E_CAPACITY, "scan scratch undersized");
     SFK_HIP(c, launch_scan(c->stream, nullptr, buf, buf, n, s->sums));
     return SFK_OK;
+}
+
+extern "C" int sfk_argsort_u32_segmented(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys,
+                                         const uint32_t* vals, uint32_t nseg, uint32_t seg_len,
+                                         uint32_t bits, int write_keys, uint32_t** out_keys,
+                                         uint32_t** out_vals) {
+    return argsort_impl<uint32_t>(c, s, keys, vals, seg_len, nullptr, bits, write_keys, out_keys,
+                                  out_vals, nseg);
 }

@@ -1,0 +1,513 @@
+// Batched EIG round for gfx950: every candidate view of one B3-Seg round
+// (src/seg/active.rs:113-129 scores them one at a time, each with two host
+// syncs) as one stream-ordered sequence, optionally captured once into a
+// hipGraph and replayed per round:
+//
+//   project all views (attributes read once per splat, geometry only: the
+//   responsibility does not depend on colour) -> one segmented depth argsort
+//   (one segment per view) -> one gather scan over all views -> isect
+//   emission with global tile ids (view tile base + local tile) -> one stable
+//   argsort of all isects by global tile id (= view|tile|depth order) ->
+//   tile ranges -> one responsibility raster over every view's tiles, each
+//   view at its own fixed-point scale -> per-view EIG with f64 entropy ->
+//   device argmax.
+//
+// The per-pixel and per-splat arithmetic is the single-view code's
+// (sfk_project_core.h, raster_tile), so each view's u32 evidence equals a
+// plain render of that view bit for bit.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "sfk_entropy.h"
+#include "sfk_project_core.h"
+
+using namespace sfk;
+
+namespace sfk {
+namespace {
+
+constexpr uint32_t kMaxViews = 64;
+
+// One thread per splat, looping over the views: the 44-byte attribute row is
+// read once for all views.
+__global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restrict__ views,
+                                                       uint32_t nviews,
+                                                       const float* __restrict__ A, uint32_t n,
+                                                       float inv_tile,
+                                                       uint32_t* __restrict__ depth_keys,
+                                                       float4* __restrict__ proj,
+                                                       uint32_t* __restrict__ tile_counts,
+                                                       uint2* __restrict__ bbox,
+                                                       uint32_t* __restrict__ counters) {
+    __shared__ sfk_view s_views[kMaxViews];
+    for (uint32_t k = threadIdx.x; k < nviews * (sizeof(sfk_view) / 4); k += 256)
+        reinterpret_cast<float*>(s_views)[k] = reinterpret_cast<const float*>(views)[k];
+    __syncthreads();
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const bool live = i < n;
+    SplatIn s;
+    if (live) s = load_splat(A, i, n);
+    uint32_t wave_total = 0;
+    for (uint32_t v = 0; v < nviews; ++v) {
+        bool vis = false;
+        uint32_t cnt = 0;
+        if (live) {
+            ProjGeom o;
+            const size_t gi = (size_t)v * n + i;
+            uint32_t key = 0xffffffffu;
+            if (project_geom(s_views[v], s, inv_tile, o)) {
+                vis = true;
+                key = __builtin_bit_cast(uint32_t, o.cz);
+                cnt = o.count;
+                proj[gi * 2] = make_float4(o.px, o.py, o.qa, o.qb);
+                proj[gi * 2 + 1] = make_float4(o.qc, s.opacity, 0.0f, 0.0f);
+                bbox[gi] = make_uint2(o.bb_lo, o.bb_hi);
+            }
+            depth_keys[gi] = key;
+            tile_counts[gi] = cnt;
+        }
+        uint32_t nvis = (uint32_t)__popcll(__ballot(vis));
+        uint32_t ncnt = wave_sum_u32(cnt);
+        wave_total += ncnt;
+        if (lane_id() == 0) {
+            if (ncnt) atomicAdd(&counters[2 * v], ncnt);
+            if (nvis) atomicAdd(&counters[2 * v + 1], nvis);
+        }
+    }
+    if (lane_id() == 0 && wave_total) atomicAdd(&counters[2 * nviews], wave_total);
+}
+
+// H(a_i, b_i) once per round, shared by every view.
+__global__ __launch_bounds__(256) void k_entropy_base(const float* __restrict__ a,
+                                                      const float* __restrict__ b, uint32_t n,
+                                                      double* __restrict__ h0) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) h0[i] = d_beta_entropy((double)a[i], (double)b[i]);
+}
+
+constexpr uint32_t kEigBlock = 1024;
+
+// grid (blocks per view, views): per-block f64 partial sums of the entropy
+// drops of one view.
+__global__ __launch_bounds__(256) void k_eig_batch(const float* __restrict__ a,
+                                                   const float* __restrict__ b,
+                                                   const double* __restrict__ h0,
+                                                   const uint32_t* __restrict__ resp, uint32_t n,
+                                                   const BatchView* __restrict__ views,
+                                                   double* __restrict__ partials) {
+    __shared__ double s_w[4];
+    const uint32_t v = blockIdx.y;
+    const float scale = views[v].scale;
+    const uint32_t* r = resp + (size_t)v * n;
+    double sum = 0.0;
+    const uint32_t base = blockIdx.x * kEigBlock + threadIdx.x;
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e) {
+        uint32_t i = base + e * 256;
+        if (i < n) {
+            uint32_t q = r[i];
+            if (q) sum += d_entropy_drop(a[i], b[i], h0[i], q, scale);
+        }
+    }
+    sum = wave_sum_f64(sum);
+    if (lane_id() == 0) s_w[threadIdx.x / 64] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partials[(size_t)v * gridDim.x + blockIdx.x] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+}
+
+// One workgroup per view: fixed-order sum of its partials.
+__global__ __launch_bounds__(256) void k_eig_batch_final(const double* __restrict__ partials,
+                                                         uint32_t per_view,
+                                                         double* __restrict__ eig) {
+    __shared__ double s_w[4];
+    const double* p = partials + (size_t)blockIdx.x * per_view;
+    double sum = 0.0;
+    for (uint32_t i = threadIdx.x; i < per_view; i += 256) sum += p[i];
+    sum = wave_sum_f64(sum);
+    if (lane_id() == 0) s_w[threadIdx.x / 64] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) eig[blockIdx.x] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+}
+
+// select_view's argmax (active.rs:116-127): strictly greater wins, so ties
+// keep the first candidate.
+__global__ void k_argmax(const double* __restrict__ eig, uint32_t nviews,
+                         uint32_t* __restrict__ best) {
+    if (threadIdx.x != 0) return;
+    uint32_t bi = 0;
+    double bv = -INFINITY;
+    for (uint32_t v = 0; v < nviews; ++v)
+        if (eig[v] > bv) {
+            bv = eig[v];
+            bi = v;
+        }
+    *best = bi;
+}
+
+}  // namespace
+}  // namespace sfk
+
+struct sfk_eig_round {
+    sfk_ctx* ctx = nullptr;
+    const sfk_splats* splats = nullptr;
+    uint32_t n = 0, nviews = 0, tile_w = 16, digit_bits = 8;
+    uint32_t total_tiles = 0, capacity = 0;
+    std::vector<sfk_view> views;
+    std::vector<BatchView> bviews;
+    sfk_view* d_views = nullptr;
+    BatchView* d_bviews = nullptr;
+    float4* proj = nullptr;
+    uint32_t* keys = nullptr;
+    uint32_t* counts = nullptr;
+    uint2* bbox = nullptr;
+    uint32_t* offsets = nullptr;
+    uint32_t* counters = nullptr;  // [2*v] isects, [2*v+1] visible, [2*V] total
+    uint32_t* tile_ids = nullptr;
+    uint32_t* gids = nullptr;
+    uint32_t* starts = nullptr;
+    uint32_t* resp = nullptr;
+    double* h0 = nullptr;
+    double* partials = nullptr;
+    double* eig = nullptr;
+    uint32_t* best = nullptr;
+    double* host_out = nullptr;  // pinned: eig[V], then best as a double slot
+    sfk_radix_scratch* sort_depth = nullptr;
+    sfk_radix_scratch* sort_tile = nullptr;
+    sfk_scan_scratch* scan = nullptr;
+    uint32_t* scan_sums = nullptr;
+    const uint32_t* sorted_tiles = nullptr;
+    const uint32_t* sorted_gids = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const float* graph_a = nullptr;
+    const float* graph_b = nullptr;
+};
+
+extern "C" void sfk_eig_round_destroy(sfk_eig_round* r);
+
+static void round_free_isects(sfk_eig_round* r) {
+    if (r->tile_ids) hipFree(r->tile_ids);
+    if (r->gids) hipFree(r->gids);
+    r->tile_ids = r->gids = nullptr;
+    sfk_radix_scratch_destroy(r->sort_tile);
+    r->sort_tile = nullptr;
+}
+
+static int round_alloc_isects(sfk_eig_round* r, uint32_t cap) {
+    sfk_ctx* c = r->ctx;
+    round_free_isects(r);
+    SFK_HIP(c, hipMalloc(&r->tile_ids, (size_t)cap * 4));
+    SFK_HIP(c, hipMalloc(&r->gids, (size_t)cap * 4));
+    int rc = sfk_radix_scratch_create(c, cap, 4, &r->sort_tile);
+    if (rc) return rc;
+    sfk_radix_set_digit_bits(r->sort_tile, r->digit_bits);
+    r->capacity = cap;
+    return SFK_OK;
+}
+
+// The whole round, enqueued on the context stream.
+static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
+    sfk_ctx* c = r->ctx;
+    const uint32_t n = r->n, V = r->nviews;
+    const size_t vn = (size_t)V * n;
+    const float* attrs = nullptr;
+    sfk_splats_info(r->splats, nullptr, nullptr, nullptr, nullptr, &attrs, nullptr);
+    SFK_HIP(c, launch_fill_u32(c->stream, r->counters, 0u, 2 * V + 2));
+    {
+        SFK_PROF(c, "batch_project");
+        k_project_batch<<<div_up(n, 256), 256, 0, c->stream>>>(
+            r->d_views, V, attrs, n, 1.0f / (float)r->tile_w, r->keys, r->proj, r->counts,
+            r->bbox, r->counters);
+        SFK_LAUNCHED(c, "batch project");
+    }
+    uint32_t* order = nullptr;
+    int rc = sfk_argsort_u32_segmented(c, r->sort_depth, r->keys, nullptr, V, n, 32, 0, nullptr,
+                                       &order);
+    if (rc) return rc;
+    {
+        SFK_PROF(c, "scan_gather");
+        SFK_HIP(c, launch_scan(c->stream, order, r->counts, r->offsets, (uint32_t)vn,
+                               r->scan_sums, n));
+    }
+    {
+        SFK_PROF(c, "map_isects");
+        SFK_HIP(c, launch_map_isects(c->stream, order, r->counts,
+                                     reinterpret_cast<const uint32_t*>(r->bbox), r->offsets,
+                                     (uint32_t)vn, 0, r->capacity, r->tile_ids, r->gids, n,
+                                     r->d_bviews));
+    }
+    uint32_t* st = nullptr;
+    uint32_t* sg = nullptr;
+    rc = sfk_argsort_u32(c, r->sort_tile, r->tile_ids, r->gids, r->capacity, r->counters + 2 * V,
+                         sfk_bits_for(r->total_tiles), 1, &st, &sg);
+    if (rc) return rc;
+    r->sorted_tiles = st;
+    r->sorted_gids = sg;
+    {
+        SFK_PROF(c, "tile_starts");
+        SFK_HIP(c, launch_tile_starts(c->stream, st, r->capacity, r->counters + 2 * V,
+                                      r->total_tiles, r->starts));
+    }
+    SFK_HIP(c, launch_fill_u32(c->stream, r->resp, 0u, vn));
+    {
+        SFK_PROF(c, "batch_raster_total");
+        RasterArgs ra;
+        memset(&ra, 0, sizeof ra);
+        ra.tile_w = r->tile_w;
+        ra.tile_starts = r->starts;
+        ra.gids = sg;
+        ra.proj = reinterpret_cast<const float*>(r->proj);
+        ra.resp = r->resp;
+        ra.views = r->d_bviews;
+        ra.nviews = V;
+        ra.batch_tiles = r->total_tiles;
+        ra.seg_len = n;
+        SFK_HIP(c, launch_rasterize_batch(c->stream, ra));
+    }
+    {
+        SFK_PROF(c, "batch_eig");
+        const uint32_t bpv = div_up(n, kEigBlock);
+        k_entropy_base<<<div_up(n, 256), 256, 0, c->stream>>>(a, b, n, r->h0);
+        k_eig_batch<<<dim3(bpv, V), 256, 0, c->stream>>>(a, b, r->h0, r->resp, n, r->d_bviews,
+                                                          r->partials);
+        k_eig_batch_final<<<V, 256, 0, c->stream>>>(r->partials, bpv, r->eig);
+        k_argmax<<<1, 64, 0, c->stream>>>(r->eig, V, r->best);
+        SFK_LAUNCHED(c, "batch eig");
+    }
+    SFK_HIP(c, hipMemcpyAsync(r->host_out, r->eig, (size_t)V * 8, hipMemcpyDeviceToHost,
+                              c->stream));
+    SFK_HIP(c, hipMemcpyAsync(r->host_out + V, r->best, 4, hipMemcpyDeviceToHost, c->stream));
+    return SFK_OK;
+}
+
+extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_camera* cams,
+                                    const uint32_t* widths, const uint32_t* heights,
+                                    uint32_t nviews, uint32_t tile_w, uint32_t digit_bits,
+                                    sfk_eig_round** out) {
+    if (!c || !sp || !cams || !widths || !heights || !out) return SFK_E_ARG;
+    *out = nullptr;
+    if (nviews == 0 || nviews > kMaxViews)
+        return fail(c, SFK_E_ARG, "eig round: %u views (1..%u)", nviews, kMaxViews);
+    if (!(tile_w == 8 || tile_w == 16 || tile_w == 32) || digit_bits < 4 || digit_bits > 8)
+        return fail(c, SFK_E_ARG, "eig round: tile width %u / digit bits %u", tile_w, digit_bits);
+    uint32_t n = 0;
+    sfk_splats_info(sp, &n, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if ((uint64_t)n * nviews > 0xffffffffull)
+        return fail(c, SFK_E_CAPACITY, "eig round: %u views x %u splats exceeds u32", nviews, n);
+    auto* r = new (std::nothrow) sfk_eig_round();
+    if (!r) return SFK_E_NOMEM;
+    r->ctx = c;
+    r->splats = sp;
+    r->n = n;
+    r->nviews = nviews;
+    r->tile_w = tile_w;
+    r->digit_bits = digit_bits;
+    uint64_t tiles = 0, cap0 = 0;
+    for (uint32_t v = 0; v < nviews; ++v) {
+        sfk_view vw;
+        int rc = sfk_camera_view(&cams[v], widths[v], heights[v], tile_w, &vw);
+        if (rc) {
+            delete r;
+            return fail(c, SFK_E_ARG, "eig round: view %u rejected", v);
+        }
+        r->views.push_back(vw);
+        BatchView bv;
+        memset(&bv, 0, sizeof bv);
+        bv.w = widths[v];
+        bv.h = heights[v];
+        bv.tiles_x = (widths[v] + tile_w - 1) / tile_w;
+        bv.tiles_y = (heights[v] + tile_w - 1) / tile_w;
+        bv.tile_base = (uint32_t)tiles;
+        bv.scale = (float)(1ull << sfk_fractional_bits(widths[v], heights[v]));
+        tiles += (uint64_t)bv.tiles_x * bv.tiles_y;
+        cap0 += std::min<uint64_t>((uint64_t)bv.tiles_x * bv.tiles_y * n, 1u << 22);
+        r->bviews.push_back(bv);
+    }
+    r->total_tiles = (uint32_t)tiles;
+    const size_t vn = (size_t)nviews * n;
+    auto fail_out = [&](int rc) {
+        sfk_eig_round_destroy(r);
+        return rc;
+    };
+#define RALLOC(ptr, bytes)                                                       \
+    do {                                                                         \
+        hipError_t _e = hipMalloc(&(ptr), (bytes));                             \
+        if (_e != hipSuccess) return fail_out(hip_fail(c, _e, "eig round alloc")); \
+    } while (0)
+    RALLOC(r->d_views, nviews * sizeof(sfk_view));
+    RALLOC(r->d_bviews, nviews * sizeof(BatchView));
+    RALLOC(r->proj, vn * 32);
+    RALLOC(r->keys, vn * 4);
+    RALLOC(r->counts, vn * 4);
+    RALLOC(r->bbox, vn * 8);
+    RALLOC(r->offsets, vn * 4);
+    RALLOC(r->counters, (2 * nviews + 2) * 4);
+    RALLOC(r->starts, ((size_t)tiles + 1) * 4);
+    RALLOC(r->resp, vn * 4);
+    RALLOC(r->h0, (size_t)n * 8);
+    RALLOC(r->partials, (size_t)nviews * div_up(n, kEigBlock) * 8);
+    RALLOC(r->eig, nviews * 8);
+    RALLOC(r->best, 4);
+    RALLOC(r->scan_sums, (size_t)div_up(vn, kScanBlock) * 4);
+#undef RALLOC
+    if (hipHostMalloc(&r->host_out, (nviews + 1) * 8, hipHostMallocDefault) != hipSuccess)
+        return fail_out(fail(c, SFK_E_HIP, "eig round: pinned output"));
+    hipError_t e = hipMemcpyAsync(r->d_views, r->views.data(), nviews * sizeof(sfk_view),
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(r->d_bviews, r->bviews.data(), nviews * sizeof(BatchView),
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail_out(hip_fail(c, e, "eig round upload"));
+    int rc = sfk_radix_scratch_create(c, (uint32_t)vn, 4, &r->sort_depth);
+    if (rc) return fail_out(rc);
+    sfk_radix_set_digit_bits(r->sort_depth, digit_bits);
+    rc = round_alloc_isects(r, next_pow2(cap0 ? cap0 : 1));
+    if (rc) return fail_out(rc);
+    *out = r;
+    return SFK_OK;
+}
+
+extern "C" void sfk_eig_round_destroy(sfk_eig_round* r) {
+    if (!r) return;
+    if (r->ctx) hipStreamSynchronize(r->ctx->stream);
+    if (r->exec) hipGraphExecDestroy(r->exec);
+    if (r->graph) hipGraphDestroy(r->graph);
+    void* bufs[] = {r->d_views, r->d_bviews, r->proj, r->keys, r->counts, r->bbox, r->offsets,
+                    r->counters, r->starts, r->resp, r->h0, r->partials, r->eig, r->best,
+                    r->scan_sums};
+    for (void* p : bufs)
+        if (p) hipFree(p);
+    if (r->host_out) hipHostFree(r->host_out);
+    round_free_isects(r);
+    sfk_radix_scratch_destroy(r->sort_depth);
+    delete r;
+}
+
+// Size the isect buffers from an eager run (the count is known only on the
+// device), so that the captured sequence never truncates.
+static int round_fit_capacity(sfk_eig_round* r, const float* a, const float* b) {
+    sfk_ctx* c = r->ctx;
+    int rc = round_enqueue(r, a, b);
+    if (rc) return rc;
+    uint32_t total = 0;
+    SFK_HIP(c, hipMemcpyAsync(&total, r->counters + 2 * r->nviews, 4, hipMemcpyDeviceToHost,
+                              c->stream));
+    SFK_HIP(c, hipStreamSynchronize(c->stream));
+    if (total > r->capacity) {
+        rc = round_alloc_isects(r, next_pow2(total));
+        if (rc) return rc;
+        if (r->exec) {
+            hipGraphExecDestroy(r->exec);
+            r->exec = nullptr;
+        }
+        if (r->graph) {
+            hipGraphDestroy(r->graph);
+            r->graph = nullptr;
+        }
+        rc = round_enqueue(r, a, b);
+        if (rc) return rc;
+        SFK_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return SFK_OK;
+}
+
+extern "C" int sfk_eig_round_run(sfk_ctx* c, sfk_eig_round* r, const float* a, const float* b,
+                                 int use_graph, double* eig_out, uint32_t* best_out) {
+    if (!c || !r || !a || !b) return SFK_E_ARG;
+    int rc;
+    if (!use_graph) {
+        rc = round_enqueue(r, a, b);
+        if (rc) return rc;
+    } else {
+        if (!r->exec || r->graph_a != a || r->graph_b != b) {
+            if (!r->exec) {
+                rc = round_fit_capacity(r, a, b);
+                if (rc) return rc;
+            }
+            if (r->exec) hipGraphExecDestroy(r->exec);
+            if (r->graph) hipGraphDestroy(r->graph);
+            r->exec = nullptr;
+            r->graph = nullptr;
+            bool prof = c->prof;
+            c->prof = false;  // no event records inside the capture
+            SFK_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            rc = round_enqueue(r, a, b);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            c->prof = prof;
+            if (rc) {
+                if (g) hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return hip_fail(c, e, "eig round capture");
+            r->graph = g;
+            SFK_HIP(c, hipGraphInstantiate(&r->exec, g, nullptr, nullptr, 0));
+            r->graph_a = a;
+            r->graph_b = b;
+        }
+        SFK_PROF(c, "eig_round_graph");
+        SFK_HIP(c, hipGraphLaunch(r->exec, c->stream));
+    }
+    if (eig_out || best_out) {
+        SFK_HIP(c, hipStreamSynchronize(c->stream));
+        if (eig_out) memcpy(eig_out, r->host_out, (size_t)r->nviews * 8);
+        if (best_out) memcpy(best_out, r->host_out + r->nviews, 4);
+    }
+    return SFK_OK;
+}
+
+extern "C" int sfk_eig_round_info(const sfk_eig_round* r, uint32_t view, const uint32_t** resp,
+                                  float* scale, uint32_t* capacity, uint32_t* total_tiles) {
+    if (!r || view >= r->nviews) return SFK_E_ARG;
+    if (resp) *resp = r->resp + (size_t)view * r->n;
+    if (scale) *scale = r->bviews[view].scale;
+    if (capacity) *capacity = r->capacity;
+    if (total_tiles) *total_tiles = r->total_tiles;
+    return SFK_OK;
+}
+
+extern "C" int sfk_eig_round_counters(sfk_ctx* c, const sfk_eig_round* r, uint32_t* counters) {
+    if (!c || !r || !counters) return SFK_E_ARG;
+    SFK_HIP(c, hipMemcpyAsync(counters, r->counters, (2 * r->nviews + 1) * 4,
+                              hipMemcpyDeviceToHost, c->stream));
+    SFK_HIP(c, hipStreamSynchronize(c->stream));
+    return SFK_OK;
+}
+
+// The plain round of the reference (Segmenter::select_view, active.rs:113-129):
+// per candidate, a full responsibility render then a synchronous EIG, argmax
+// on the host.
+extern "C" int sfk_eig_round_plain(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* scratch,
+                                   const sfk_camera* cams, const uint32_t* widths,
+                                   const uint32_t* heights, uint32_t nviews, const float* a,
+                                   const float* b, uint32_t* resp, double* eig_out,
+                                   uint32_t* best_out) {
+    if (!c || !sp || !scratch || !cams || !widths || !heights || !a || !b || !resp)
+        return SFK_E_ARG;
+    uint32_t n = 0;
+    sfk_splats_info(sp, &n, nullptr, nullptr, nullptr, nullptr, nullptr);
+    uint32_t best = 0;
+    double bv = -INFINITY;
+    for (uint32_t v = 0; v < nviews; ++v) {
+        float scale = 1.0f;
+        int rc = sfk_render_responsibility(c, sp, scratch, &cams[v], widths[v], heights[v], resp,
+                                           &scale);
+        if (rc) return rc;
+        double e = 0.0;
+        rc = sfk_eig(c, a, b, resp, n, scale, &e);
+        if (rc) return rc;
+        if (eig_out) eig_out[v] = e;
+        if (e > bv) {
+            bv = e;
+            best = v;
+        }
+    }
+    if (best_out) *best_out = best;
+    return SFK_OK;
+}

@@ -91,27 +91,41 @@ constexpr uint32_t kSortThreads = 256;
 constexpr uint32_t kSortEpt = 16;
 constexpr uint32_t kSortBlock = kSortThreads * kSortEpt;  // keys per block
 uint32_t radix_max_blocks(uint32_t max_elems);
-hipError_t launch_radix_pass_u32(hipStream_t s, const uint32_t* keys_in, const uint32_t* vals_in,
-                                 uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
-                                 const uint32_t* d_n, uint32_t shift, uint32_t width,
-                                 uint32_t digit_bits, uint32_t* hist, uint32_t max_blocks,
-                                 uint32_t* totals);
-hipError_t launch_radix_pass_u64(hipStream_t s, const uint64_t* keys_in, const uint32_t* vals_in,
-                                 uint64_t* keys_out, uint32_t* vals_out, uint32_t n,
-                                 const uint32_t* d_n, uint32_t shift, uint32_t width,
-                                 uint32_t digit_bits, uint32_t* hist, uint32_t max_blocks,
-                                 uint32_t* totals);
+struct SortArgs {
+    const void* keys_in;
+    const uint32_t* vals_in;  // nullptr: identity (index within the segment)
+    void* keys_out;           // nullptr: keys not written
+    uint32_t* vals_out;
+    uint32_t n_max;           // one segment: element count (or its cap with d_n)
+    const uint32_t* d_n;      // one segment: device-side count, may be nullptr
+    uint32_t nseg, seg_len;   // nseg > 1: segments of seg_len keys each
+    uint32_t shift, width;
+    uint32_t* hist;           // nseg * RADIX rows of `stride` blocks
+    uint32_t stride;
+    uint32_t* totals;         // nseg * RADIX
+};
+hipError_t launch_radix_pass(hipStream_t s, const SortArgs& a, uint32_t key_bytes,
+                             uint32_t digit_bits);
 
 // Scan.
 constexpr uint32_t kScanBlock = 4096;
 hipError_t launch_scan(hipStream_t s, const uint32_t* gids, const uint32_t* counts,
-                       uint32_t* offsets, uint32_t n, uint32_t* block_sums);
+                       uint32_t* offsets, uint32_t n, uint32_t* block_sums, uint32_t seg = 0);
+
+// Per-view constants of a batched round (device array, one per view).
+struct BatchView {
+    uint32_t w, h, tiles_x, tiles_y;
+    uint32_t tile_base;  // first global tile id of this view
+    float scale;         // fixed-point scale 2^fractional_bits(w, h)
+    uint32_t pad[2];
+};
 
 // Raster stage.
 hipError_t launch_map_isects(hipStream_t s, const uint32_t* order, const uint32_t* tile_counts,
                              const uint32_t* bbox, const uint32_t* offsets, uint32_t n,
                              uint32_t tiles_per_row, uint32_t max_isects, uint32_t* tile_ids,
-                             uint32_t* gaussian_ids);
+                             uint32_t* gaussian_ids, uint32_t seg_len = 0,
+                             const BatchView* views = nullptr);
 hipError_t launch_tile_starts(hipStream_t s, const uint32_t* tile_ids, uint32_t n,
                               const uint32_t* d_n, uint32_t num_tiles, uint32_t* starts);
 
@@ -131,8 +145,12 @@ struct RasterArgs {
     uint32_t* resp;
     uint32_t* fg;
     uint32_t* bg;
+    // batched responsibility (launch_rasterize_batch)
+    const BatchView* views;
+    uint32_t nviews, batch_tiles, seg_len;
 };
 hipError_t launch_rasterize(hipStream_t s, int mode, const RasterArgs& a);
+hipError_t launch_rasterize_batch(hipStream_t s, const RasterArgs& a);
 
 // Backward.
 uint32_t loss_num_partials(uint32_t np);

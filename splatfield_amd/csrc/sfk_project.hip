@@ -9,25 +9,18 @@
 //    wave instead of two per splat);
 //  * the projected row is 12 floats (three 16-byte stores/loads):
 //    [mx, my, conic_a, conic_b, conic_c, opacity, r, g, b, 0, 0, 0].
-#include "sfk_device.h"
-#include "sfk_internal.h"
-#include "sfk_math.h"
+#include "sfk_project_core.h"
 
 namespace sfk {
 
 namespace {
 
-constexpr float kAlphaCutoff = 10.0f / 255.0f;  // project.rs:74
-constexpr float kShC0 = 0.2820948f;              // project.rs:390-401
+constexpr float kShC0 = 0.2820948f;  // project.rs:390-401
 constexpr float kShC1 = 0.48860252f;
 constexpr float kShC20 = 0.9461747f;
 constexpr float kShC21 = 0.31539157f;
 constexpr float kShC22 = -1.0925485f;
 constexpr float kShC23 = 0.54627424f;
-
-__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
-    return ax * bx + ay * by + az * bz;
-}
 
 // sh_to_rgb (project.rs:403-449) with its band-3 basis as written.
 template <int CHS>
@@ -90,85 +83,25 @@ __global__ __launch_bounds__(256) void k_project(sfk_view v, const float* __rest
     bool vis = false;
     uint32_t cnt = 0;
     if (i < n) {
-        const size_t N = n;
-        float mx = A[0 * N + i], my = A[1 * N + i], mz = A[2 * N + i];
-        float qw = A[3 * N + i], qx = A[4 * N + i], qy = A[5 * N + i], qz = A[6 * N + i];
-        float sx = sfk_expf(A[7 * N + i]), sy = sfk_expf(A[8 * N + i]),
-              sz = sfk_expf(A[9 * N + i]);
-        float opacity = 1.0f / (1.0f + sfk_expf(-A[10 * N + i]));
+        SplatIn s = load_splat(A, i, n);
+        ProjGeom o;
         uint32_t key = 0xffffffffu;
-        if (!(opacity < kAlphaCutoff)) {
-            float cx = dot3(v.rot0[0], v.rot0[1], v.rot0[2], mx, my, mz) + v.trans[0];
-            float cy = dot3(v.rot1[0], v.rot1[1], v.rot1[2], mx, my, mz) + v.trans[1];
-            float cz = dot3(v.rot2[0], v.rot2[1], v.rot2[2], mx, my, mz) + v.trans[2];
-            if (!(cz <= 0.1f)) {
-                float iz = 1.0f / cz;
-                // compute_cov2d (project.rs:236-282)
-                float x2 = qx * qx, y2 = qy * qy, z2 = qz * qz;
-                float xy = qx * qy, xz = qx * qz, yz = qy * qz;
-                float wx = qw * qx, wy = qw * qy, wz = qw * qz;
-                float r00 = 1.0f - 2.0f * (y2 + z2), r01 = 2.0f * (xy - wz), r02 = 2.0f * (xz + wy);
-                float r10 = 2.0f * (xy + wz), r11 = 1.0f - 2.0f * (x2 + z2), r12 = 2.0f * (yz - wx);
-                float r20 = 2.0f * (xz - wy), r21 = 2.0f * (yz + wx), r22 = 1.0f - 2.0f * (x2 + y2);
-                float m00 = r00 * sx, m01 = r01 * sy, m02 = r02 * sz;
-                float m10 = r10 * sx, m11 = r11 * sy, m12 = r12 * sz;
-                float m20 = r20 * sx, m21 = r21 * sy, m22 = r22 * sz;
-                float lim_x = 1.3f * v.img[0] / (2.0f * v.focal[0]);
-                float lim_y = 1.3f * v.img[1] / (2.0f * v.focal[1]);
-                float u = sfk_clampf(cx * iz, -lim_x, lim_x);
-                float vv = sfk_clampf(cy * iz, -lim_y, lim_y);
-                float fx_iz = v.focal[0] * iz;
-                float fu_iz = fx_iz * u;
-                float t00 = fx_iz * v.rot0[0] - fu_iz * v.rot2[0];
-                float t01 = fx_iz * v.rot0[1] - fu_iz * v.rot2[1];
-                float t02 = fx_iz * v.rot0[2] - fu_iz * v.rot2[2];
-                float fy_iz = v.focal[1] * iz;
-                float fv_iz = fy_iz * vv;
-                float t10 = fy_iz * v.rot1[0] - fv_iz * v.rot2[0];
-                float t11 = fy_iz * v.rot1[1] - fv_iz * v.rot2[1];
-                float t12 = fy_iz * v.rot1[2] - fv_iz * v.rot2[2];
-                float j00 = t00 * m00 + t01 * m10 + t02 * m20;
-                float j01 = t00 * m01 + t01 * m11 + t02 * m21;
-                float j02 = t00 * m02 + t01 * m12 + t02 * m22;
-                float j10 = t10 * m00 + t11 * m10 + t12 * m20;
-                float j11 = t10 * m01 + t11 * m11 + t12 * m21;
-                float j12 = t10 * m02 + t11 * m12 + t12 * m22;
-                float ca = dot3(j00, j01, j02, j00, j01, j02) + 0.3f;
-                float cb = dot3(j00, j01, j02, j10, j11, j12);
-                float cc = dot3(j10, j11, j12, j10, j11, j12) + 0.3f;
-                // compute_conic (project.rs:284-293)
-                float det = fmaxf(ca * cc - cb * cb, 1e-6f);
-                float inv_det = 1.0f / det;
-
-                vis = true;
-                key = __builtin_bit_cast(uint32_t, cz);
-
-                float cr = 0.0f, cg = 0.0f, cbl = 0.0f;
-                if constexpr (CHS > 0) {
-                    float dx = mx - v.world_pos[0], dy = my - v.world_pos[1],
-                          dz = mz - v.world_pos[2];
-                    float inv = 1.0f / sqrtf(dot3(dx, dy, dz, dx, dy, dz));
-                    sh_to_rgb<CHS>(dx * inv, dy * inv, dz * inv, i, n, shs, cr, cg, cbl);
-                }
-                float px = v.focal[0] * cx * iz + v.img[0] * 0.5f;
-                float py = v.focal[1] * cy * iz + v.img[1] * 0.5f;
-                float4* row = proj + (size_t)i * 3;
-                row[0] = make_float4(px, py, cc * inv_det, -cb * inv_det);
-                row[1] = make_float4(ca * inv_det, opacity, cr + 0.5f, cg + 0.5f);
-                row[2] = make_float4(cbl + 0.5f, 0.0f, 0.0f, 0.0f);
-
-                float cutoff = sfk_logf(opacity / kAlphaCutoff);
-                float ex = sqrtf(2.0f * cutoff * ca);
-                float ey = sqrtf(2.0f * cutoff * cc);
-                // tile_bbox (project.rs:98-107)
-                float bx = v.tile_bounds[0], by = v.tile_bounds[1];
-                uint32_t min_x = sfk_f2u(sfk_clampf((px - ex) * inv_tile, 0.0f, bx));
-                uint32_t min_y = sfk_f2u(sfk_clampf((py - ey) * inv_tile, 0.0f, by));
-                uint32_t max_x = sfk_f2u(sfk_clampf((px + ex) * inv_tile + 1.0f, 0.0f, bx));
-                uint32_t max_y = sfk_f2u(sfk_clampf((py + ey) * inv_tile + 1.0f, 0.0f, by));
-                cnt = (max_x - min_x) * (max_y - min_y);
-                bbox[i] = make_uint2(min_x | (min_y << 16), max_x | (max_y << 16));
+        if (project_geom(v, s, inv_tile, o)) {
+            vis = true;
+            key = __builtin_bit_cast(uint32_t, o.cz);
+            cnt = o.count;
+            float cr = 0.0f, cg = 0.0f, cbl = 0.0f;
+            if constexpr (CHS > 0) {
+                float dx = s.mx - v.world_pos[0], dy = s.my - v.world_pos[1],
+                      dz = s.mz - v.world_pos[2];
+                float inv = 1.0f / sqrtf(dot3f(dx, dy, dz, dx, dy, dz));
+                sh_to_rgb<CHS>(dx * inv, dy * inv, dz * inv, i, n, shs, cr, cg, cbl);
             }
+            float4* row = proj + (size_t)i * 3;
+            row[0] = make_float4(o.px, o.py, o.qa, o.qb);
+            row[1] = make_float4(o.qc, s.opacity, cr + 0.5f, cg + 0.5f);
+            row[2] = make_float4(cbl + 0.5f, 0.0f, 0.0f, 0.0f);
+            bbox[i] = make_uint2(o.bb_lo, o.bb_hi);
         }
         depth_keys[i] = key;
         tile_counts[i] = cnt;

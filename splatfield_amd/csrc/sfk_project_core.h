@@ -1,0 +1,112 @@
+// Per-splat projection math of project_splats (src/project.rs:236-388),
+// shared by the single-view and the batched multi-view kernels so that both
+// produce the same bits.  Operation order follows the reference; the library
+// is compiled with -ffp-contract=off.
+#pragma once
+
+#include "sfk_device.h"
+#include "sfk_internal.h"
+#include "sfk_math.h"
+
+namespace sfk {
+
+constexpr float kAlphaCutoff = 10.0f / 255.0f;  // project.rs:74
+
+struct SplatIn {
+    float mx, my, mz, qw, qx, qy, qz, sx, sy, sz, opacity;
+};
+
+struct ProjGeom {
+    float px, py;          // mean2d
+    float qa, qb, qc;      // conic
+    float cz;              // camera depth
+    uint32_t bb_lo, bb_hi; // packed tile bbox
+    uint32_t count;        // tiles covered
+};
+
+__device__ __forceinline__ float dot3f(float ax, float ay, float az, float bx, float by, float bz) {
+    return ax * bx + ay * by + az * bz;
+}
+
+// Loads splat i of the field-major planes, applying exp/sigmoid as the
+// reference does before its culls (project.rs:313-333).
+__device__ __forceinline__ SplatIn load_splat(const float* __restrict__ A, uint32_t i, uint32_t n) {
+    const size_t N = n;
+    SplatIn s;
+    s.mx = A[0 * N + i]; s.my = A[1 * N + i]; s.mz = A[2 * N + i];
+    s.qw = A[3 * N + i]; s.qx = A[4 * N + i]; s.qy = A[5 * N + i]; s.qz = A[6 * N + i];
+    s.sx = sfk_expf(A[7 * N + i]);
+    s.sy = sfk_expf(A[8 * N + i]);
+    s.sz = sfk_expf(A[9 * N + i]);
+    s.opacity = 1.0f / (1.0f + sfk_expf(-A[10 * N + i]));
+    return s;
+}
+
+// Culls, EWA covariance, conic, mean2d, extent and tile bbox for one view.
+// Returns false when the splat is culled.
+__device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s, float inv_tile,
+                                             ProjGeom& o) {
+    if (s.opacity < kAlphaCutoff) return false;
+    float cx = dot3f(v.rot0[0], v.rot0[1], v.rot0[2], s.mx, s.my, s.mz) + v.trans[0];
+    float cy = dot3f(v.rot1[0], v.rot1[1], v.rot1[2], s.mx, s.my, s.mz) + v.trans[1];
+    float cz = dot3f(v.rot2[0], v.rot2[1], v.rot2[2], s.mx, s.my, s.mz) + v.trans[2];
+    if (cz <= 0.1f) return false;
+    float iz = 1.0f / cz;
+    // compute_cov2d (project.rs:236-282)
+    float x2 = s.qx * s.qx, y2 = s.qy * s.qy, z2 = s.qz * s.qz;
+    float xy = s.qx * s.qy, xz = s.qx * s.qz, yz = s.qy * s.qz;
+    float wx = s.qw * s.qx, wy = s.qw * s.qy, wz = s.qw * s.qz;
+    float r00 = 1.0f - 2.0f * (y2 + z2), r01 = 2.0f * (xy - wz), r02 = 2.0f * (xz + wy);
+    float r10 = 2.0f * (xy + wz), r11 = 1.0f - 2.0f * (x2 + z2), r12 = 2.0f * (yz - wx);
+    float r20 = 2.0f * (xz - wy), r21 = 2.0f * (yz + wx), r22 = 1.0f - 2.0f * (x2 + y2);
+    float m00 = r00 * s.sx, m01 = r01 * s.sy, m02 = r02 * s.sz;
+    float m10 = r10 * s.sx, m11 = r11 * s.sy, m12 = r12 * s.sz;
+    float m20 = r20 * s.sx, m21 = r21 * s.sy, m22 = r22 * s.sz;
+    float lim_x = 1.3f * v.img[0] / (2.0f * v.focal[0]);
+    float lim_y = 1.3f * v.img[1] / (2.0f * v.focal[1]);
+    float u = sfk_clampf(cx * iz, -lim_x, lim_x);
+    float vv = sfk_clampf(cy * iz, -lim_y, lim_y);
+    float fx_iz = v.focal[0] * iz;
+    float fu_iz = fx_iz * u;
+    float t00 = fx_iz * v.rot0[0] - fu_iz * v.rot2[0];
+    float t01 = fx_iz * v.rot0[1] - fu_iz * v.rot2[1];
+    float t02 = fx_iz * v.rot0[2] - fu_iz * v.rot2[2];
+    float fy_iz = v.focal[1] * iz;
+    float fv_iz = fy_iz * vv;
+    float t10 = fy_iz * v.rot1[0] - fv_iz * v.rot2[0];
+    float t11 = fy_iz * v.rot1[1] - fv_iz * v.rot2[1];
+    float t12 = fy_iz * v.rot1[2] - fv_iz * v.rot2[2];
+    float j00 = t00 * m00 + t01 * m10 + t02 * m20;
+    float j01 = t00 * m01 + t01 * m11 + t02 * m21;
+    float j02 = t00 * m02 + t01 * m12 + t02 * m22;
+    float j10 = t10 * m00 + t11 * m10 + t12 * m20;
+    float j11 = t10 * m01 + t11 * m11 + t12 * m21;
+    float j12 = t10 * m02 + t11 * m12 + t12 * m22;
+    float ca = dot3f(j00, j01, j02, j00, j01, j02) + 0.3f;
+    float cb = dot3f(j00, j01, j02, j10, j11, j12);
+    float cc = dot3f(j10, j11, j12, j10, j11, j12) + 0.3f;
+    // compute_conic (project.rs:284-293)
+    float det = fmaxf(ca * cc - cb * cb, 1e-6f);
+    float inv_det = 1.0f / det;
+    o.qa = cc * inv_det;
+    o.qb = -cb * inv_det;
+    o.qc = ca * inv_det;
+    o.cz = cz;
+    o.px = v.focal[0] * cx * iz + v.img[0] * 0.5f;
+    o.py = v.focal[1] * cy * iz + v.img[1] * 0.5f;
+    float cutoff = sfk_logf(s.opacity / kAlphaCutoff);
+    float ex = sqrtf(2.0f * cutoff * ca);
+    float ey = sqrtf(2.0f * cutoff * cc);
+    // tile_bbox (project.rs:98-107)
+    float bx = v.tile_bounds[0], by = v.tile_bounds[1];
+    uint32_t min_x = sfk_f2u(sfk_clampf((o.px - ex) * inv_tile, 0.0f, bx));
+    uint32_t min_y = sfk_f2u(sfk_clampf((o.py - ey) * inv_tile, 0.0f, by));
+    uint32_t max_x = sfk_f2u(sfk_clampf((o.px + ex) * inv_tile + 1.0f, 0.0f, bx));
+    uint32_t max_y = sfk_f2u(sfk_clampf((o.py + ey) * inv_tile + 1.0f, 0.0f, by));
+    o.count = (max_x - min_x) * (max_y - min_y);
+    o.bb_lo = min_x | (min_y << 16);
+    o.bb_hi = max_x | (max_y << 16);
+    return true;
+}
+
+}  // namespace sfk

@@ -1,5 +1,7 @@
 // Raster stage for gfx950: intersection emission, tile ranges and the
-// front-to-back compositor (src/raster.rs).
+// front-to-back compositor (src/raster.rs), for one view or for a batch of
+// views sharing one sorted isect list (global tile id = view tile base +
+// local tile id).
 #include "sfk_device.h"
 #include "sfk_internal.h"
 #include "sfk_math.h"
@@ -11,6 +13,9 @@ namespace {
 // map_isects (raster.rs:12-48): rank r emits its bbox tiles, row-major, at
 // offsets[r]; writes past max_isects are dropped while the walk continues.
 // Ranks whose splat emits nothing (culled or off-screen) exit at once.
+// Batched: rank r belongs to view r / seg_len; order[] holds in-view splat
+// indices, per-splat arrays are view-major, and tile ids get the view's
+// tile base.
 __global__ __launch_bounds__(256) void k_map_isects(const uint32_t* __restrict__ order,
                                                     const uint32_t* __restrict__ tile_counts,
                                                     const uint2* __restrict__ bbox,
@@ -18,19 +23,29 @@ __global__ __launch_bounds__(256) void k_map_isects(const uint32_t* __restrict__
                                                     uint32_t n, uint32_t tiles_per_row,
                                                     uint32_t max_isects,
                                                     uint32_t* __restrict__ tile_ids,
-                                                    uint32_t* __restrict__ gaussian_ids) {
+                                                    uint32_t* __restrict__ gaussian_ids,
+                                                    uint32_t seg_len,
+                                                    const BatchView* __restrict__ views) {
     uint32_t r = blockIdx.x * 256u + threadIdx.x;
     if (r >= n) return;
     uint32_t g = order[r];
-    if (tile_counts[g] == 0) return;
-    uint2 bb = bbox[g];
+    size_t gi = g;
+    uint32_t tpr = tiles_per_row, tbase = 0;
+    if (seg_len) {
+        uint32_t v = r / seg_len;
+        gi = (size_t)v * seg_len + g;
+        tpr = views[v].tiles_x;
+        tbase = views[v].tile_base;
+    }
+    if (tile_counts[gi] == 0) return;
+    uint2 bb = bbox[gi];
     uint32_t min_x = bb.x & 0xffffu, min_y = bb.x >> 16, max_x = bb.y & 0xffffu,
              max_y = bb.y >> 16;
     uint32_t slot = offsets[r];
     for (uint32_t ty = min_y; ty < max_y; ++ty) {
         for (uint32_t tx = min_x; tx < max_x; ++tx) {
             if (slot < max_isects) {
-                tile_ids[slot] = tx + ty * tiles_per_row;
+                tile_ids[slot] = tbase + tx + ty * tpr;
                 gaussian_ids[slot] = g;
             }
             ++slot;
@@ -60,14 +75,25 @@ constexpr float kInv255 = 1.0f / 255.0f;
 // reaches the contribution threshold, whatever the last ulp of exp.
 constexpr float kPowerSkip = 5.6f;
 
-// rasterize_kernel (raster.rs:87-230): one workgroup per TW x TW tile, one
-// thread per pixel.  Splat rows are staged through LDS in chunks of TW*TW.
-// The responsibility modes add each splat's per-pixel u32 weights across the
+// One tile of one view.
+struct TileJob {
+    uint32_t w, h, tx, ty, start, end, row_stride;
+    float scale;
+    const float4* proj;  // row base of this view
+    uint32_t rows4;      // float4s per projected row (3 with colour, 2 without)
+    uint32_t* resp;
+    uint32_t* fg;
+    uint32_t* bg;
+};
+
+// rasterize_kernel (raster.rs:87-230) for one tile: TW x TW threads, one per
+// pixel; splat rows staged through LDS in chunks of TW*TW.  The
+// responsibility modes add each splat's per-pixel u32 weights across the
 // wavefront, then across the workgroup in LDS, and issue one global atomic
-// per (splat, tile) — the integer sum is order independent, so the result is
+// per (splat, tile): the integer sum is order independent, so the result is
 // bit-identical to one atomic per pixel.
 template <int MODE, int TW>
-__global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
+__device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& a) {
     constexpr int NT = TW * TW;
     __shared__ float4 s_geo[NT];  // mx, my, conic a, conic b
     __shared__ float4 s_aux[NT];  // conic c, opacity, r, g
@@ -76,13 +102,10 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
     __shared__ uint32_t s_q0[MODE == kRgb ? 1 : NT];
     __shared__ uint32_t s_q1[MODE == kEvidence ? NT : 1];
 
-    const uint32_t ntiles = a.tiles_x * a.tiles_y;
-    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
-    const uint32_t tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const uint32_t t = threadIdx.x;
-    const uint32_t px = tx * TW + (t % TW), py = ty * TW + (t / TW);
-    const bool in_bounds = px < a.w && py < a.h;
-    const uint32_t start = a.tile_starts[tile], end = a.tile_starts[tile + 1];
+    const uint32_t px = J.tx * TW + (t % TW), py = J.ty * TW + (t / TW);
+    const bool in_bounds = px < J.w && py < J.h;
+    const uint32_t start = J.start, end = J.end;
     const float pxf = (float)px + 0.5f, pyf = (float)py + 0.5f;
 
     uint32_t inside = 0;
@@ -92,17 +115,15 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     bool done = !in_bounds;
     uint32_t last = start;
-    const float4* proj = reinterpret_cast<const float4*>(a.proj);
 
     for (uint32_t chunk = start; chunk < end; chunk += NT) {
         __syncthreads();  // previous chunk fully consumed
         const uint32_t idx = chunk + t;
         if (idx < end) {
             uint32_t g = a.gids[idx];
-            const float4* row = proj + (size_t)g * 3;
+            const float4* row = J.proj + (size_t)g * J.rows4;
             s_geo[t] = row[0];
-            float4 aux = row[1];
-            s_aux[t] = aux;
+            s_aux[t] = row[1];
             if constexpr (MODE == kRgb) {
                 s_blue[t] = row[2].x;
             } else {
@@ -130,7 +151,7 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
                             cg += aux.w * vis;
                             cb += s_blue[j] * vis;
                         } else {
-                            q = sfk_f2u(vis * a.scale);
+                            q = sfk_f2u(vis * J.scale);
                         }
                         T *= 1.0f - alpha;
                         last = chunk + j + 1;
@@ -159,17 +180,17 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
             if (t < cnt) {
                 uint32_t g = s_gid[t];
                 if constexpr (MODE == kTotal) {
-                    if (s_q0[t]) atomicAdd(&a.resp[g], s_q0[t]);
+                    if (s_q0[t]) atomicAdd(&J.resp[g], s_q0[t]);
                 } else {
-                    if (s_q0[t]) atomicAdd(&a.fg[g], s_q0[t]);
-                    if (s_q1[t]) atomicAdd(&a.bg[g], s_q1[t]);
+                    if (s_q0[t]) atomicAdd(&J.fg[g], s_q0[t]);
+                    if (s_q1[t]) atomicAdd(&J.bg[g], s_q1[t]);
                 }
             }
         }
     }
 
     if (in_bounds) {
-        const size_t p = (size_t)py * a.w + px;
+        const size_t p = (size_t)py * J.w + px;
         if (a.final_t) a.final_t[p] = T;
         if (a.n_contrib) a.n_contrib[p] = last;
         if constexpr (MODE == kRgb) {
@@ -179,10 +200,60 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
                 uint32_t g = sfk_f2u(sfk_clampf(cg * 255.0f, 0.0f, 255.0f));
                 uint32_t b = sfk_f2u(sfk_clampf(cb * 255.0f, 0.0f, 255.0f));
                 uint32_t al = sfk_f2u(sfk_clampf((1.0f - T) * 255.0f, 0.0f, 255.0f));
-                a.bitmap[px + (size_t)py * a.row_stride] = r | (g << 8) | (b << 16) | (al << 24);
+                a.bitmap[px + (size_t)py * J.row_stride] = r | (g << 8) | (b << 16) | (al << 24);
             }
         }
     }
+}
+
+template <int MODE, int TW>
+__global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
+    const uint32_t ntiles = a.tiles_x * a.tiles_y;
+    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
+    TileJob J;
+    J.w = a.w; J.h = a.h;
+    J.tx = tile % a.tiles_x;
+    J.ty = tile / a.tiles_x;
+    J.start = a.tile_starts[tile];
+    J.end = a.tile_starts[tile + 1];
+    J.row_stride = a.row_stride;
+    J.scale = a.scale;
+    J.proj = reinterpret_cast<const float4*>(a.proj);
+    J.rows4 = 3;
+    J.resp = a.resp; J.fg = a.fg; J.bg = a.bg;
+    raster_tile<MODE, TW>(J, a);
+}
+
+// Batched responsibility: one launch over the tiles of every view.
+template <int TW>
+__global__ __launch_bounds__(TW * TW) void k_rasterize_batch(RasterArgs a) {
+    const uint32_t total = a.batch_tiles;
+    const uint32_t tile = xcd_remap(blockIdx.x, total);
+    // view of this tile: last view whose tile base <= tile
+    uint32_t lo = 0, hi = a.nviews;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) / 2;
+        if (a.views[mid].tile_base <= tile) lo = mid;
+        else hi = mid;
+    }
+    const BatchView bv = a.views[lo];
+    const uint32_t local = tile - bv.tile_base;
+    TileJob J;
+    J.w = bv.w; J.h = bv.h;
+    J.tx = local % bv.tiles_x;
+    J.ty = local / bv.tiles_x;
+    J.start = a.tile_starts[tile];
+    J.end = a.tile_starts[tile + 1];
+    J.row_stride = 0;
+    J.scale = bv.scale;
+    J.proj = reinterpret_cast<const float4*>(a.proj) + (size_t)lo * a.seg_len * 2;
+    J.rows4 = 2;
+    J.resp = a.resp + (size_t)lo * a.seg_len;
+    J.fg = nullptr; J.bg = nullptr;
+    RasterArgs b = a;
+    b.final_t = nullptr;
+    b.n_contrib = nullptr;
+    raster_tile<kTotal, TW>(J, b);
 }
 
 template <int MODE>
@@ -203,11 +274,12 @@ hipError_t raster_tw(hipStream_t s, const RasterArgs& a) {
 hipError_t launch_map_isects(hipStream_t s, const uint32_t* order, const uint32_t* tile_counts,
                              const uint32_t* bbox, const uint32_t* offsets, uint32_t n,
                              uint32_t tiles_per_row, uint32_t max_isects, uint32_t* tile_ids,
-                             uint32_t* gaussian_ids) {
+                             uint32_t* gaussian_ids, uint32_t seg_len, const BatchView* views) {
     if (n == 0) return hipSuccess;
     k_map_isects<<<div_up(n, 256), 256, 0, s>>>(order, tile_counts,
                                                 reinterpret_cast<const uint2*>(bbox), offsets, n,
-                                                tiles_per_row, max_isects, tile_ids, gaussian_ids);
+                                                tiles_per_row, max_isects, tile_ids, gaussian_ids,
+                                                seg_len, views);
     return hipGetLastError();
 }
 
@@ -224,6 +296,17 @@ hipError_t launch_rasterize(hipStream_t s, int mode, const RasterArgs& a) {
         case kEvidence: return raster_tw<kEvidence>(s, a);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_rasterize_batch(hipStream_t s, const RasterArgs& a) {
+    if (a.batch_tiles == 0) return hipSuccess;
+    switch (a.tile_w) {
+        case 8: k_rasterize_batch<8><<<a.batch_tiles, 64, 0, s>>>(a); break;
+        case 16: k_rasterize_batch<16><<<a.batch_tiles, 256, 0, s>>>(a); break;
+        case 32: k_rasterize_batch<32><<<a.batch_tiles, 1024, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 }  // namespace sfk

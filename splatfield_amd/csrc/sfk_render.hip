@@ -118,6 +118,7 @@ struct sfk_render_scratch {
     const uint32_t* sorted_gids = nullptr;
     sfk_view view;  // view of the last prepare
     bool has_view = false;
+    bool has_colors = false;  // the last prepare projected SH colours
     float* grad2d = nullptr;  // 9 planes x n screen-space gradients (backward)
     double* loss_partials = nullptr;
     uint32_t loss_partials_cap = 0;
@@ -303,9 +304,21 @@ static void fill_isects(const sfk_render_scratch* s, sfk_tile_isects* o) {
 // depth sort runs over all n splats (culled ones carry key 0xffffffff and
 // sort last, contributing no tiles), the gather scan over all n ranks, and
 // the tile sort over the device-side isect count.
+static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
+                        const sfk_view* view, uint32_t w, uint32_t h, sfk_tile_isects* out,
+                        bool colors);
+
 extern "C" int sfk_prepare_isects_view(sfk_ctx* c, const sfk_splats* sp,
                                        sfk_render_scratch* s, const sfk_view* view, uint32_t w,
                                        uint32_t h, sfk_tile_isects* out) {
+    return prepare_impl(c, sp, s, view, w, h, out, true);
+}
+
+// colors = false projects geometry only (no SH): the responsibility modes
+// never read the colour, and skipping it leaves every epsilon word unchanged.
+static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
+                        const sfk_view* view, uint32_t w, uint32_t h, sfk_tile_isects* out,
+                        bool colors) {
     if (!c || !sp || !s || !view || !out) return SFK_E_ARG;
     if ((uint32_t)view->img[0] != w || (uint32_t)view->img[1] != h)
         return fail(c, SFK_E_ARG, "view is for %gx%g, not %ux%u", view->img[0], view->img[1], w, h);
@@ -319,9 +332,10 @@ extern "C" int sfk_prepare_isects_view(sfk_ctx* c, const sfk_splats* sp,
     SFK_HIP(c, launch_fill_u32(c->stream, s->counters, 0u, 4));
     {
         SFK_PROF(c, "project");
-        SFK_HIP(c, launch_project(c->stream, *view, sp->attrs, sp->sh, n, sp->sh_per_ch,
-                                  s->tile_w, s->depth_keys, s->projected, s->tile_counts, s->bbox,
-                                  s->counters));
+        SFK_HIP(c, launch_project(c->stream, *view, sp->attrs, sp->sh, n,
+                                  colors ? sp->sh_per_ch : 0u, s->tile_w, s->depth_keys,
+                                  s->projected, s->tile_counts, s->bbox, s->counters));
+        s->has_colors = colors;
     }
     SFK_HIP(c, hipMemcpyAsync(s->host_counters, s->counters, 8, hipMemcpyDeviceToHost, c->stream));
     SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));
@@ -399,6 +413,8 @@ static int check_isects(sfk_ctx* c, sfk_render_scratch* s, const sfk_tile_isects
 extern "C" int sfk_finalize_rgb(sfk_ctx* c, sfk_render_scratch* s, const sfk_tile_isects* t) {
     int rc = check_isects(c, s, t);
     if (rc) return rc;
+    if (!s->has_colors)
+        return fail(c, SFK_E_ARG, "finalize(Rgb) after a responsibility-only prepare");
     RasterArgs a = raster_args(s, t);
     a.rgba = s->rgba;
     a.bitmap = s->bitmap;
@@ -505,8 +521,12 @@ extern "C" int sfk_accumulate_prepared(sfk_ctx* c, sfk_render_scratch* s,
 extern "C" int sfk_render_responsibility(sfk_ctx* c, const sfk_splats* sp,
                                          sfk_render_scratch* s, const sfk_camera* cam, uint32_t w,
                                          uint32_t h, uint32_t* resp, float* scale_out) {
+    if (!c || !s || !cam) return SFK_E_ARG;
+    sfk_view v;
     sfk_tile_isects t;
-    int rc = sfk_prepare_isects(c, sp, s, cam, w, h, &t);
+    int rc = sfk_camera_view(cam, w, h, s->tile_w, &v);
+    if (rc) return fail(c, rc, "camera view: bad arguments");
+    rc = prepare_impl(c, sp, s, &v, w, h, &t, false);
     if (rc) return rc;
     return sfk_accumulate_prepared(c, s, &t, nullptr, 0, resp, nullptr, nullptr, scale_out);
 }

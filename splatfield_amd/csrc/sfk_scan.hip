@@ -17,16 +17,24 @@ constexpr uint32_t EPT = 16;
 constexpr uint32_t BLK = NT * EPT;
 static_assert(BLK == kScanBlock, "scan block");
 
+// Gathered cell i: counts[gids[i]]; with segments of `seg` cells, gids hold
+// in-segment indices into the segment's own counts.
+__device__ __forceinline__ size_t gather_index(const uint32_t* gids, size_t i, uint32_t seg) {
+    if (!gids) return i;
+    return seg ? (i / seg) * seg + gids[i] : gids[i];
+}
+
 __global__ __launch_bounds__(NT) void k_scan_reduce(const uint32_t* __restrict__ gids,
                                                     const uint32_t* __restrict__ counts,
-                                                    uint32_t n, uint32_t* __restrict__ sums) {
+                                                    uint32_t n, uint32_t* __restrict__ sums,
+                                                    uint32_t seg) {
     __shared__ uint32_t tmp[NT / 64 + 1];
     const size_t base = (size_t)blockIdx.x * BLK + threadIdx.x;
     uint32_t s = 0;
 #pragma unroll
     for (uint32_t e = 0; e < EPT; ++e) {
         size_t i = base + (size_t)e * NT;
-        if (i < n) s += counts[gids ? gids[i] : i];
+        if (i < n) s += counts[gather_index(gids, i, seg)];
     }
     uint32_t total;
     block_excl_scan_u32<NT>(s, tmp, &total);
@@ -59,7 +67,7 @@ __global__ __launch_bounds__(NT) void k_scan_sums(uint32_t* __restrict__ sums, u
 __global__ __launch_bounds__(NT) void k_scan_apply(const uint32_t* __restrict__ gids,
                                                    const uint32_t* counts, uint32_t n,
                                                    const uint32_t* __restrict__ sums,
-                                                   uint32_t* offsets) {
+                                                   uint32_t* offsets, uint32_t seg) {
     __shared__ uint32_t tmp[NT / 64 + 1];
     const size_t first = (size_t)blockIdx.x * BLK + (size_t)threadIdx.x * EPT;
     uint32_t v[EPT];
@@ -67,7 +75,7 @@ __global__ __launch_bounds__(NT) void k_scan_apply(const uint32_t* __restrict__ 
 #pragma unroll
     for (uint32_t e = 0; e < EPT; ++e) {
         size_t i = first + e;
-        v[e] = i < n ? counts[gids ? gids[i] : i] : 0u;
+        v[e] = i < n ? counts[gather_index(gids, i, seg)] : 0u;
         s += v[e];
     }
     uint32_t total;
@@ -83,12 +91,12 @@ __global__ __launch_bounds__(NT) void k_scan_apply(const uint32_t* __restrict__ 
 }  // namespace
 
 hipError_t launch_scan(hipStream_t s, const uint32_t* gids, const uint32_t* counts,
-                       uint32_t* offsets, uint32_t n, uint32_t* block_sums) {
+                       uint32_t* offsets, uint32_t n, uint32_t* block_sums, uint32_t seg) {
     if (n == 0) return hipSuccess;
     uint32_t nb = div_up(n, BLK);
-    k_scan_reduce<<<nb, NT, 0, s>>>(gids, counts, n, block_sums);
+    k_scan_reduce<<<nb, NT, 0, s>>>(gids, counts, n, block_sums, seg);
     k_scan_sums<<<1, NT, 0, s>>>(block_sums, nb);
-    k_scan_apply<<<nb, NT, 0, s>>>(gids, counts, n, block_sums, offsets);
+    k_scan_apply<<<nb, NT, 0, s>>>(gids, counts, n, block_sums, offsets, seg);
     return hipGetLastError();
 }
 

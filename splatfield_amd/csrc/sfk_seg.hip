@@ -2,6 +2,7 @@
 // analytic EIG with the Beta entropy in f64 on the device (in place of the
 // 445x445 f32 table, beta.rs:100-144), and the conjugate Beta update.
 #include "sfk_device.h"
+#include "sfk_entropy.h"
 #include "sfk_internal.h"
 
 namespace sfk {
@@ -13,41 +14,6 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ buf, ui
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     size_t stride = (size_t)gridDim.x * 256;
     for (; i < count; i += stride) buf[i] = v;
-}
-
-// ln Gamma, Lanczos g = 7, 9 terms (beta.rs:55-70); x >= 1 here.
-__device__ double d_lgamma(double x) {
-    const double z = x - 1.0;
-    double series = 0.9999999999998099;
-    series += 676.5203681218851 / (z + 1.0);
-    series += -1259.1392167224028 / (z + 2.0);
-    series += 771.3234287776531 / (z + 3.0);
-    series += -176.6150291621406 / (z + 4.0);
-    series += 12.507343278686905 / (z + 5.0);
-    series += -0.13857109526572012 / (z + 6.0);
-    series += 9.984369578019572e-6 / (z + 7.0);
-    series += 1.5056327351493116e-7 / (z + 8.0);
-    const double t = z + 7.5;
-    return 0.91893853320467274178 + (z + 0.5) * log(t) - t + log(series);
-}
-
-// digamma: recurrence to x >= 6, then the asymptotic series (beta.rs:75-86).
-__device__ double d_digamma(double x) {
-    double r = 0.0;
-    while (x < 6.0) {
-        r -= 1.0 / x;
-        x += 1.0;
-    }
-    const double inv = 1.0 / x, inv2 = inv * inv;
-    return r + log(x) - 0.5 * inv - inv2 / 12.0 + inv2 * inv2 / 120.0 -
-           inv2 * inv2 * inv2 / 252.0;
-}
-
-// H(a, b) of Beta(a, b) (beta.rs:88-96).
-__device__ double d_beta_entropy(double a, double b) {
-    const double s = a + b;
-    return d_lgamma(a) + d_lgamma(b) - d_lgamma(s) - (a - 1.0) * d_digamma(a) -
-           (b - 1.0) * d_digamma(b) + (s - 2.0) * d_digamma(s);
 }
 
 constexpr uint32_t kEigThreads = 256;

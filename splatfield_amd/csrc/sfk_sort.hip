@@ -3,18 +3,23 @@
 //
 // One pass over a `width`-bit digit (<= 8) is three launches:
 //   upsweep   per-block digit histogram (LDS, per-wave sub-histograms),
-//             written digit-major hist[d * max_blocks + b];
-//   digscan   one workgroup per digit: exclusive scan of its row over the
-//             blocks, row total into totals[d];
+//             written digit-major per segment: hist[(s*RADIX + d)*stride + b];
+//   digscan   one workgroup per (segment, digit): exclusive scan of its row
+//             over the segment's blocks, row total into totals[s*RADIX + d];
 //   scatter   per block: stable rank of every key inside the block with
-//             64-lane ballot match-any (one ballot per digit bit — a 64-bit
-//             ballot word, no 32-lane assumption as in sort.rs:57-87), LDS
-//             staging in sorted order, then coalesced runs to the global
-//             offset totals-prefix + hist-prefix.
-// Block = 256 threads x 16 keys; wave w owns keys [w*1024, (w+1)*1024) of
-// its block, item i lane l -> w*1024 + i*64 + l, so ranks follow input order.
-// The element count may come from device memory (min(*d_n, n)): blocks past
-// the live count exit at once, so a capacity-sized grid needs no host sync.
+//             64-lane ballot match-any (one 64-bit ballot per digit bit; no
+//             32-lane assumption as in sort.rs:57-87), LDS staging in sorted
+//             order, then coalesced runs to segment base + digit prefix +
+//             block prefix.
+// Block = 256 threads x 16 keys; wave w owns keys [w*1024, (w+1)*1024) of its
+// block, item i lane l -> w*1024 + i*64 + l, so ranks follow input order.
+//
+// Segments: nseg independent sorts of seg_len keys each, laid out back to
+// back (segment s at [s*seg_len, (s+1)*seg_len)), sorted in the same launches
+// (the batched multi-view depth sort).  With one segment the element count
+// may come from device memory (min(*d_n, n)): blocks past the live count exit
+// at once, so a capacity-sized grid needs no host sync.  Identity values
+// (vals_in == nullptr) are indices within the segment.
 #include "sfk_device.h"
 #include "sfk_internal.h"
 
@@ -27,70 +32,98 @@ constexpr uint32_t EPT = kSortEpt;
 constexpr uint32_t KPB = kSortBlock;
 constexpr uint32_t NW = NT / 64;
 
-__device__ __forceinline__ uint32_t live_count(uint32_t n, const uint32_t* d_n) {
-    if (d_n) {
-        uint32_t m = *d_n;
-        return m < n ? m : n;
-    }
-    return n;
-}
-
 template <typename K>
 __device__ __forceinline__ uint32_t digit_of(K key, uint32_t shift, uint32_t mask) {
     return (uint32_t)(key >> shift) & mask;
 }
 
+// Which segment and block-within-segment this workgroup handles, and the
+// segment's key range; false if the block has no live keys.
+struct SegPos {
+    uint32_t s, bl, nb, seg_n;
+    size_t seg_base;
+};
+
+__device__ __forceinline__ bool seg_pos(const SortArgs& a, uint32_t b, SegPos& p) {
+    if (a.nseg <= 1) {
+        uint32_t n = a.n_max;
+        if (a.d_n) {
+            uint32_t m = *a.d_n;
+            n = m < n ? m : n;
+        }
+        p.s = 0;
+        p.bl = b;
+        p.nb = (n + KPB - 1) / KPB;
+        p.seg_n = n;
+        p.seg_base = 0;
+    } else {
+        uint32_t bps = (a.seg_len + KPB - 1) / KPB;
+        p.s = b / bps;
+        p.bl = b % bps;
+        p.nb = bps;
+        p.seg_n = a.seg_len;
+        p.seg_base = (size_t)p.s * a.seg_len;
+        if (p.s >= a.nseg) return false;
+    }
+    return p.bl < p.nb;
+}
+
 template <typename K, int RADIX>
-__global__ __launch_bounds__(NT) void k_upsweep(const K* __restrict__ keys, uint32_t n_max,
-                                                const uint32_t* __restrict__ d_n, uint32_t shift,
-                                                uint32_t mask, uint32_t* __restrict__ hist,
-                                                uint32_t max_blocks) {
+__global__ __launch_bounds__(NT) void k_upsweep(SortArgs a) {
     __shared__ uint32_t h[NW][RADIX];
-    const uint32_t n = live_count(n_max, d_n);
-    const uint32_t nb = (n + KPB - 1) / KPB;
-    const uint32_t b = blockIdx.x;
-    if (b >= nb) return;
+    SegPos p;
+    if (!seg_pos(a, blockIdx.x, p)) return;
+    const K* keys = static_cast<const K*>(a.keys_in);
     const uint32_t w = threadIdx.x / 64;
+    const uint32_t mask = (1u << a.width) - 1u;
     for (uint32_t i = threadIdx.x; i < NW * RADIX; i += NT) (&h[0][0])[i] = 0;
     __syncthreads();
-    const size_t base = (size_t)b * KPB + threadIdx.x;
+    const uint32_t loc0 = p.bl * KPB + threadIdx.x;
 #pragma unroll
     for (uint32_t e = 0; e < EPT; ++e) {
-        size_t idx = base + (size_t)e * NT;
-        if (idx < n) atomicAdd(&h[w][digit_of(keys[idx], shift, mask)], 1u);
+        uint32_t loc = loc0 + e * NT;
+        if (loc < p.seg_n) atomicAdd(&h[w][digit_of(keys[p.seg_base + loc], a.shift, mask)], 1u);
     }
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < RADIX; d += NT) {
         uint32_t t = 0;
 #pragma unroll
         for (uint32_t k = 0; k < NW; ++k) t += h[k][d];
-        hist[(size_t)d * max_blocks + b] = t;
+        a.hist[((size_t)p.s * RADIX + d) * a.stride + p.bl] = t;
     }
 }
 
-// One workgroup per digit: exclusive scan of hist[d][0..nb) in place.
+// One workgroup per (segment, digit): exclusive scan of its row in place.
 template <int RADIX>
-__global__ __launch_bounds__(NT) void k_digscan(uint32_t* __restrict__ hist, uint32_t max_blocks,
-                                                uint32_t n_max, const uint32_t* __restrict__ d_n,
-                                                uint32_t* __restrict__ totals) {
+__global__ __launch_bounds__(NT) void k_digscan(SortArgs a) {
     __shared__ uint32_t tmp[NW + 1];
-    const uint32_t n = live_count(n_max, d_n);
-    const uint32_t nb = (n + KPB - 1) / KPB;
-    const uint32_t d = blockIdx.x;
-    uint32_t* row = hist + (size_t)d * max_blocks;
+    const uint32_t nbins = 1u << a.width;
+    const uint32_t s = blockIdx.x / nbins, d = blockIdx.x % nbins;
+    uint32_t nb;
+    if (a.nseg <= 1) {
+        uint32_t n = a.n_max;
+        if (a.d_n) {
+            uint32_t m = *a.d_n;
+            n = m < n ? m : n;
+        }
+        nb = (n + KPB - 1) / KPB;
+    } else {
+        nb = (a.seg_len + KPB - 1) / KPB;
+    }
+    uint32_t* row = a.hist + ((size_t)s * RADIX + d) * a.stride;
     uint32_t carry = 0;
     for (uint32_t c0 = 0; c0 < nb; c0 += NT * EPT) {
         uint32_t v[EPT];
-        uint32_t s = 0;
+        uint32_t sum = 0;
         uint32_t first = c0 + threadIdx.x * EPT;
 #pragma unroll
         for (uint32_t e = 0; e < EPT; ++e) {
             uint32_t idx = first + e;
             v[e] = idx < nb ? row[idx] : 0u;
-            s += v[e];
+            sum += v[e];
         }
         uint32_t total;
-        uint32_t ex = block_excl_scan_u32<NT>(s, tmp, &total);
+        uint32_t ex = block_excl_scan_u32<NT>(sum, tmp, &total);
         uint32_t run = carry + ex;
 #pragma unroll
         for (uint32_t e = 0; e < EPT; ++e) {
@@ -100,61 +133,57 @@ __global__ __launch_bounds__(NT) void k_digscan(uint32_t* __restrict__ hist, uin
         }
         carry += total;
     }
-    if (threadIdx.x == 0) totals[d] = carry;
+    if (threadIdx.x == 0) a.totals[(size_t)s * RADIX + d] = carry;
 }
 
 template <typename K, int RADIX>
-__global__ __launch_bounds__(NT) void k_scatter(const K* __restrict__ keys_in,
-                                                const uint32_t* __restrict__ vals_in,
-                                                K* __restrict__ keys_out,
-                                                uint32_t* __restrict__ vals_out, uint32_t n_max,
-                                                const uint32_t* __restrict__ d_n, uint32_t shift,
-                                                uint32_t width, const uint32_t* __restrict__ hist,
-                                                uint32_t max_blocks,
-                                                const uint32_t* __restrict__ totals) {
+__global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
     __shared__ K s_keys[KPB];
     __shared__ uint32_t s_vals[KPB];
     __shared__ uint32_t s_wh[NW][RADIX];  // per-wave digit counters, then bases
     __shared__ uint32_t s_gofs[RADIX];    // global position minus local position
     __shared__ uint32_t s_tmp[NW + 1];
 
-    const uint32_t n = live_count(n_max, d_n);
-    const uint32_t nb = (n + KPB - 1) / KPB;
-    const uint32_t b = blockIdx.x;
-    if (b >= nb) return;
+    SegPos p;
+    if (!seg_pos(a, blockIdx.x, p)) return;
+    const K* keys_in = static_cast<const K*>(a.keys_in);
+    K* keys_out = static_cast<K*>(a.keys_out);
     const uint32_t t = threadIdx.x, w = t / 64, lane = lane_id();
+    const uint32_t width = a.width, shift = a.shift;
     const uint32_t mask = (1u << width) - 1u;
     const uint32_t nbins = 1u << width;
-    const size_t blk0 = (size_t)b * KPB;
-    const uint32_t live = (uint32_t)((n - blk0) < KPB ? (n - blk0) : KPB);
+    const uint32_t blk0 = p.bl * KPB;  // within the segment
+    const uint32_t live = (p.seg_n - blk0) < KPB ? (p.seg_n - blk0) : KPB;
 
     for (uint32_t i = t; i < NW * RADIX; i += NT) (&s_wh[0][0])[i] = 0;
 
-    // Global digit base: exclusive scan of the digit totals (RADIX <= NT).
+    // Segment-global digit base: exclusive scan of the segment's digit totals.
     {
-        uint32_t tot = t < nbins ? totals[t] : 0u;
+        uint32_t tot = t < nbins ? a.totals[(size_t)p.s * RADIX + t] : 0u;
         uint32_t all;
         uint32_t ex = block_excl_scan_u32<NT>(tot, s_tmp, &all);
-        if (t < (uint32_t)RADIX) s_gofs[t] = t < nbins ? ex + hist[(size_t)t * max_blocks + b] : 0u;
+        if (t < (uint32_t)RADIX)
+            s_gofs[t] = t < nbins ? (uint32_t)p.seg_base + ex +
+                                        a.hist[((size_t)p.s * RADIX + t) * a.stride + p.bl]
+                                  : 0u;
     }
     // (the scan's barriers also order the s_wh clear)
 
     K key[EPT];
     uint32_t val[EPT];
     uint32_t loc[EPT];
-    const size_t wbase = blk0 + (size_t)w * 64 * EPT + lane;
+    const uint32_t wbase = blk0 + w * 64 * EPT + lane;
 #pragma unroll
     for (uint32_t e = 0; e < EPT; ++e) {
-        size_t idx = wbase + (size_t)e * 64;
-        bool valid = idx < n;
-        key[e] = valid ? keys_in[idx] : K(0);
-        val[e] = valid ? (vals_in ? vals_in[idx] : (uint32_t)idx) : 0u;
+        uint32_t li = wbase + e * 64;
+        bool valid = li < p.seg_n;
+        key[e] = valid ? keys_in[p.seg_base + li] : K(0);
+        val[e] = valid ? (a.vals_in ? a.vals_in[p.seg_base + li] : li) : 0u;
     }
 
 #pragma unroll
     for (uint32_t e = 0; e < EPT; ++e) {
-        size_t idx = wbase + (size_t)e * 64;
-        bool valid = idx < n;
+        bool valid = wbase + e * 64 < p.seg_n;
         uint32_t d = digit_of(key[e], shift, mask);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -197,8 +226,7 @@ __global__ __launch_bounds__(NT) void k_scatter(const K* __restrict__ keys_in,
 
 #pragma unroll
     for (uint32_t e = 0; e < EPT; ++e) {
-        size_t idx = wbase + (size_t)e * 64;
-        if (idx < n) {
+        if (wbase + e * 64 < p.seg_n) {
             uint32_t d = digit_of(key[e], shift, mask);
             uint32_t pos = s_wh[w][d] + loc[e];
             s_keys[pos] = key[e];
@@ -215,62 +243,35 @@ __global__ __launch_bounds__(NT) void k_scatter(const K* __restrict__ keys_in,
             uint32_t d = digit_of(k, shift, mask);
             size_t dst = (size_t)(s_gofs[d] + s);
             if (keys_out) keys_out[dst] = k;
-            vals_out[dst] = s_vals[s];
+            a.vals_out[dst] = s_vals[s];
         }
     }
 }
 
 template <typename K, int RADIX>
-hipError_t pass_impl(hipStream_t s, const K* keys_in, const uint32_t* vals_in, K* keys_out,
-                     uint32_t* vals_out, uint32_t n, const uint32_t* d_n, uint32_t shift,
-                     uint32_t width, uint32_t* hist, uint32_t max_blocks, uint32_t* totals) {
-    uint32_t grid = div_up(n, KPB);
+hipError_t pass_impl(hipStream_t st, const SortArgs& a) {
+    uint32_t grid = a.nseg <= 1 ? div_up(a.n_max, KPB) : a.nseg * div_up(a.seg_len, KPB);
     if (grid == 0) return hipSuccess;
-    uint32_t mask = (1u << width) - 1u;
-    k_upsweep<K, RADIX><<<grid, NT, 0, s>>>(keys_in, n, d_n, shift, mask, hist, max_blocks);
-    k_digscan<RADIX><<<1u << width, NT, 0, s>>>(hist, max_blocks, n, d_n, totals);
-    k_scatter<K, RADIX><<<grid, NT, 0, s>>>(keys_in, vals_in, keys_out, vals_out, n, d_n, shift,
-                                            width, hist, max_blocks, totals);
+    k_upsweep<K, RADIX><<<grid, NT, 0, st>>>(a);
+    k_digscan<RADIX><<<(a.nseg <= 1 ? 1u : a.nseg) << a.width, NT, 0, st>>>(a);
+    k_scatter<K, RADIX><<<grid, NT, 0, st>>>(a);
     return hipGetLastError();
-}
-
-template <typename K>
-hipError_t pass_dispatch(hipStream_t s, const K* keys_in, const uint32_t* vals_in, K* keys_out,
-                         uint32_t* vals_out, uint32_t n, const uint32_t* d_n, uint32_t shift,
-                         uint32_t width, uint32_t digit_bits, uint32_t* hist, uint32_t max_blocks,
-                         uint32_t* totals) {
-    // The LDS histogram is sized by the configured digit width; a narrower
-    // final digit runs in the same instantiation.
-    if (digit_bits <= 4)
-        return pass_impl<K, 16>(s, keys_in, vals_in, keys_out, vals_out, n, d_n, shift, width,
-                                hist, max_blocks, totals);
-    if (digit_bits <= 6)
-        return pass_impl<K, 64>(s, keys_in, vals_in, keys_out, vals_out, n, d_n, shift, width,
-                                hist, max_blocks, totals);
-    return pass_impl<K, 256>(s, keys_in, vals_in, keys_out, vals_out, n, d_n, shift, width, hist,
-                             max_blocks, totals);
 }
 
 }  // namespace
 
 uint32_t radix_max_blocks(uint32_t max_elems) { return div_up(max_elems > 0 ? max_elems : 1, KPB); }
 
-hipError_t launch_radix_pass_u32(hipStream_t s, const uint32_t* keys_in, const uint32_t* vals_in,
-                                 uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
-                                 const uint32_t* d_n, uint32_t shift, uint32_t width,
-                                 uint32_t digit_bits, uint32_t* hist, uint32_t max_blocks,
-                                 uint32_t* totals) {
-    return pass_dispatch<uint32_t>(s, keys_in, vals_in, keys_out, vals_out, n, d_n, shift, width,
-                                   digit_bits, hist, max_blocks, totals);
-}
-
-hipError_t launch_radix_pass_u64(hipStream_t s, const uint64_t* keys_in, const uint32_t* vals_in,
-                                 uint64_t* keys_out, uint32_t* vals_out, uint32_t n,
-                                 const uint32_t* d_n, uint32_t shift, uint32_t width,
-                                 uint32_t digit_bits, uint32_t* hist, uint32_t max_blocks,
-                                 uint32_t* totals) {
-    return pass_dispatch<uint64_t>(s, keys_in, vals_in, keys_out, vals_out, n, d_n, shift, width,
-                                   digit_bits, hist, max_blocks, totals);
+hipError_t launch_radix_pass(hipStream_t st, const SortArgs& a, uint32_t key_bytes,
+                             uint32_t digit_bits) {
+    if (key_bytes == 4) {
+        if (digit_bits <= 4) return pass_impl<uint32_t, 16>(st, a);
+        if (digit_bits <= 6) return pass_impl<uint32_t, 64>(st, a);
+        return pass_impl<uint32_t, 256>(st, a);
+    }
+    if (digit_bits <= 4) return pass_impl<uint64_t, 16>(st, a);
+    if (digit_bits <= 6) return pass_impl<uint64_t, 64>(st, a);
+    return pass_impl<uint64_t, 256>(st, a);
 }
 
 }  // namespace sfk

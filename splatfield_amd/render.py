@@ -88,7 +88,8 @@ class Splats:
 
     def __del__(self):
         try:
-            lib().sfk_splats_destroy(self.ptr)
+            if self.ctx.ptr:  # the context must outlive the objects it made
+                lib().sfk_splats_destroy(self.ptr)
         except Exception:
             pass
 
@@ -174,7 +175,8 @@ class RenderScratch:
 
     def __del__(self):
         try:
-            lib().sfk_render_scratch_destroy(self.ptr)
+            if self.ctx.ptr:  # the context must outlive the objects it made
+                lib().sfk_render_scratch_destroy(self.ptr)
         except Exception:
             pass
 

@@ -6,6 +6,7 @@ import math
 
 import numpy as np
 
+from . import _lib
 from ._lib import lib
 from .render import BASE_FOV, Camera
 from .runtime import GpuTensor
@@ -132,3 +133,79 @@ def candidates(center, radius, count):
         pos = c + dist * d
         cams.append(Camera.look_at(pos, c))
     return cams
+
+
+class EigRound:
+    """Batched EIG round over candidate cameras of mixed resolution
+    (sfk_eig_round_*): one stream-ordered sequence, optionally replayed as a
+    captured hipGraph."""
+
+    def __init__(self, splats, cameras, sizes, tile_w=16, digit_bits=8):
+        self.ctx = splats.ctx
+        self.splats = splats
+        self.nviews = len(cameras)
+        cams = (_lib.Camera * self.nviews)(*[c.to_c() for c in cameras])
+        self._w = np.ascontiguousarray([s[0] for s in sizes], np.uint32)
+        self._h = np.ascontiguousarray([s[1] for s in sizes], np.uint32)
+        out = C.c_void_p()
+        self.ctx.check(lib().sfk_eig_round_create(self.ctx.ptr, splats.ptr, cams,
+                                                  self._w.ctypes.data, self._h.ctypes.data,
+                                                  self.nviews, tile_w, digit_bits, C.byref(out)),
+                       "eig round create")
+        self.ptr = out.value
+
+    def run(self, state, use_graph=True, read=True):
+        """Returns (eig per view, best view) when read is set."""
+        eig = np.zeros(self.nviews, np.float64)
+        best = C.c_uint32()
+        self.ctx.check(lib().sfk_eig_round_run(self.ctx.ptr, self.ptr, state.a.ptr, state.b.ptr,
+                                               int(use_graph),
+                                               eig.ctypes.data if read else None,
+                                               C.byref(best) if read else None), "eig round run")
+        return (eig, best.value) if read else None
+
+    def resp(self, view):
+        """(u32 epsilon words of one view, its scale)."""
+        p = C.c_void_p()
+        sc = C.c_float()
+        lib().sfk_eig_round_info(self.ptr, view, C.byref(p), C.byref(sc), None, None)
+        t = GpuTensor(self.ctx, p.value, (self.splats.n,), np.uint32, False, self)
+        return t.read(), sc.value
+
+    def counters(self):
+        out = np.zeros(2 * self.nviews + 1, np.uint32)
+        self.ctx.check(lib().sfk_eig_round_counters(self.ctx.ptr, self.ptr, out.ctypes.data),
+                       "eig round counters")
+        return out
+
+    def __del__(self):
+        try:
+            if self.ctx.ptr:  # the context must outlive the objects it made
+                lib().sfk_eig_round_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def eig_round_plain(splats, scratch, cameras, sizes, state, resp):
+    """Segmenter::select_view (active.rs:113-129) in C: per candidate a
+    responsibility render and a synchronous EIG; returns (eig, best)."""
+    n = len(cameras)
+    cams = (_lib.Camera * n)(*[c.to_c() for c in cameras])
+    w = np.ascontiguousarray([s[0] for s in sizes], np.uint32)
+    h = np.ascontiguousarray([s[1] for s in sizes], np.uint32)
+    eig = np.zeros(n, np.float64)
+    best = C.c_uint32()
+    ctx = splats.ctx
+    ctx.check(lib().sfk_eig_round_plain(ctx.ptr, splats.ptr, scratch.ptr, cams, w.ctypes.data,
+                                        h.ctypes.data, n, state.a.ptr, state.b.ptr, resp.ptr,
+                                        eig.ctypes.data, C.byref(best)), "plain eig round")
+    return eig, best.value
+
+
+def sphere_cameras(count, radius=3.0, center=(0.0, 0.0, 0.0), fov=CANDIDATE_FOV):
+    """Fibonacci-sphere cameras at a fixed distance looking at the centre
+    (views.rs:77-96 with dist fixed)."""
+    from .scene import fibonacci_directions
+    c = np.asarray(center, np.float32)
+    dirs = fibonacci_directions(count)
+    return [Camera.look_at(c + np.float32(radius) * d, c, fov=(fov, fov)) for d in dirs]

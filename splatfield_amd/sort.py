@@ -45,7 +45,8 @@ class RadixScratch:
 
     def __del__(self):
         try:
-            lib().sfk_radix_scratch_destroy(self.ptr)
+            if self.ctx.ptr:  # the context must outlive the objects it made
+                lib().sfk_radix_scratch_destroy(self.ptr)
         except Exception:
             pass
 
@@ -61,7 +62,8 @@ class ScanScratch:
 
     def __del__(self):
         try:
-            lib().sfk_scan_scratch_destroy(self.ptr)
+            if self.ctx.ptr:  # the context must outlive the objects it made
+                lib().sfk_scan_scratch_destroy(self.ptr)
         except Exception:
             pass
 
