@@ -38,7 +38,25 @@ def parse():
     p.add_argument("--eig-views", type=int, default=20)
     p.add_argument("--eig-res", type=int, default=512)
     p.add_argument("--eig-reps", type=int, default=5)
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run a 256 MiB device copy (known traffic, for counter scaling)")
     return p.parse_args()
+
+
+def calibrate_copy(ctx):
+    from splatfield_amd.runtime import GpuTensor
+    from splatfield_amd._lib import lib
+    n = 64 << 20  # u32 -> 256 MiB
+    src = GpuTensor.empty(ctx, n)
+    dst = GpuTensor.empty(ctx, n)
+    src.fill_u32(7)
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(5):
+        ctx.check(lib().sfk_copy_u32(ctx.ptr, dst.ptr, src.ptr, n), "copy")
+    ms = ctx.timer_stop_ms() / 5
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4,
+                      "ms": round(ms, 4), "gbs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}))
 
 
 def bench_eig_round(ctx, args):
@@ -136,6 +154,11 @@ def main():
     from splatfield_amd import scene as S
     from splatfield_amd.runtime import GpuTensor
 
+    if args.calibrate_copy:
+        ctx = Context(0)
+        calibrate_copy(ctx)
+        ctx.close()
+        return
     n, w, h = args.n, args.width, args.height
     px = w * h
     attrs, sh, k = S.synthetic_scene(n, 3, seed=0)

@@ -55,6 +55,8 @@ int sfk_download(sfk_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Stream-ordered u32 fill by a kernel (graph-capture safe); zero_u32 of
  * src/seg/mod.rs:87-103 is fill(buf, 0). */
 int sfk_fill_u32(sfk_ctx* ctx, uint32_t* buf, uint32_t value, size_t count);
+/* Stream-ordered device copy of count u32 (count a multiple of 4). */
+int sfk_copy_u32(sfk_ctx* ctx, uint32_t* dst, const uint32_t* src, size_t count);
 
 /* Stream-ordered event timing (hipEventRecord/ElapsedTime). */
 int sfk_timer_start(sfk_ctx* ctx);
@@ -287,6 +289,15 @@ int sfk_eig_round_info(const sfk_eig_round* r, uint32_t view, const uint32_t** r
                        float* scale, uint32_t* capacity, uint32_t* total_tiles);
 /* SYNC: per view [isects, visible], then the total isect count. */
 int sfk_eig_round_counters(sfk_ctx* ctx, const sfk_eig_round* r, uint32_t* counters);
+/* Autotuner: time the graph-replayed round at every tile width {8,16,32} x
+ * digit width {4,8} (median of `reps` replays, HIP events) and return the
+ * fastest; ms_out (6 floats, may be NULL) gets each configuration's time,
+ * tile-major.  Evidence depends on the tile width (the bbox cutoff is 10/255,
+ * the compositor's 1/255), so compare against plain renders at the same one. */
+int sfk_eig_round_autotune(sfk_ctx* ctx, const sfk_splats* splats, const sfk_camera* cams,
+                           const uint32_t* widths, const uint32_t* heights, uint32_t nviews,
+                           const float* a, const float* b, uint32_t reps, uint32_t* best_tile_w,
+                           uint32_t* best_digit_bits, float* ms_out);
 /* The plain round (Segmenter::select_view, active.rs:113-129): per view a
  * full render_responsibility and a synchronous sfk_eig, argmax on the host. */
 int sfk_eig_round_plain(sfk_ctx* ctx, const sfk_splats* splats, sfk_render_scratch* scratch,

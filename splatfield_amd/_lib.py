@@ -55,6 +55,7 @@ SIGNATURES = {
     "sfk_upload": (i32, [P, P, P, sz]),
     "sfk_download": (i32, [P, P, P, sz]),
     "sfk_fill_u32": (i32, [P, P, u32, sz]),
+    "sfk_copy_u32": (i32, [P, P, P, sz]),
     "sfk_timer_start": (i32, [P]),
     "sfk_timer_stop_ms": (i32, [P, C.POINTER(f32)]),
     "sfk_prof_enable": (i32, [P, i32]),
@@ -102,6 +103,7 @@ SIGNATURES = {
     "sfk_eig_round_info": (i32, [P, u32, PP, P, P, P]),
     "sfk_eig_round_counters": (i32, [P, P, P]),
     "sfk_eig_round_plain": (i32, [P, P, P, P, P, P, u32, P, P, P, P, P]),
+    "sfk_eig_round_autotune": (i32, [P, P, P, P, P, u32, P, P, u32, P, P, P]),
     "sfk_loss_l2": (i32, [P, P, P, u32, P, P]),
     "sfk_render_backward": (i32, [P, P, P, C.POINTER(TileIsects), P, P, P]),
     "sfk_project": (i32, [P, C.POINTER(View), P, P, u32, u32, u32, P, P, P, P, P]),

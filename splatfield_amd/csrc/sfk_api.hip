@@ -170,6 +170,13 @@ extern "C" int sfk_fill_u32(sfk_ctx* c, uint32_t* buf, uint32_t v, size_t count)
     return SFK_OK;
 }
 
+extern "C" int sfk_copy_u32(sfk_ctx* c, uint32_t* dst, const uint32_t* src, size_t count) {
+    if (!c || (count && (!dst || !src)) || count % 4) return SFK_E_ARG;
+    SFK_PROF(c, "copy_u32");
+    SFK_HIP(c, launch_copy_u32(c->stream, dst, src, count));
+    return SFK_OK;
+}
+
 extern "C" int sfk_timer_start(sfk_ctx* c) {
     if (!c) return SFK_E_ARG;
     SFK_HIP(c, hipEventRecord(c->ev0, c->stream));

@@ -511,3 +511,46 @@ extern "C" int sfk_eig_round_plain(sfk_ctx* c, const sfk_splats* sp, sfk_render_
     if (best_out) *best_out = best;
     return SFK_OK;
 }
+
+// Autotuner: time the batched round (graph replays, HIP events) for every
+// tile width in {8, 16, 32} and radix digit width in {4, 8}; report the
+// fastest.  ms_out (6 entries, optional) receives the median replay time of
+// each configuration in that order.
+extern "C" int sfk_eig_round_autotune(sfk_ctx* c, const sfk_splats* sp, const sfk_camera* cams,
+                                      const uint32_t* widths, const uint32_t* heights,
+                                      uint32_t nviews, const float* a, const float* b,
+                                      uint32_t reps, uint32_t* best_tile_w,
+                                      uint32_t* best_digit_bits, float* ms_out) {
+    if (!c || !sp || !cams || !widths || !heights || !a || !b || !best_tile_w || !best_digit_bits)
+        return SFK_E_ARG;
+    const uint32_t tws[3] = {8, 16, 32}, dbs[2] = {4, 8};
+    float best = INFINITY;
+    int k = 0;
+    if (reps == 0) reps = 3;
+    for (uint32_t ti = 0; ti < 3; ++ti)
+        for (uint32_t di = 0; di < 2; ++di, ++k) {
+            sfk_eig_round* r = nullptr;
+            int rc = sfk_eig_round_create(c, sp, cams, widths, heights, nviews, tws[ti], dbs[di], &r);
+            if (rc) return rc;
+            rc = sfk_eig_round_run(c, r, a, b, 1, nullptr, nullptr);  // capture + warm
+            std::vector<float> t;
+            for (uint32_t i = 0; i < reps && !rc; ++i) {
+                float ms = 0.0f;
+                rc = sfk_timer_start(c);
+                if (!rc) rc = sfk_eig_round_run(c, r, a, b, 1, nullptr, nullptr);
+                if (!rc) rc = sfk_timer_stop_ms(c, &ms);
+                t.push_back(ms);
+            }
+            sfk_eig_round_destroy(r);
+            if (rc) return rc;
+            std::sort(t.begin(), t.end());
+            float med = t[t.size() / 2];
+            if (ms_out) ms_out[k] = med;
+            if (med < best) {
+                best = med;
+                *best_tile_w = tws[ti];
+                *best_digit_bits = dbs[di];
+            }
+        }
+    return SFK_OK;
+}

@@ -81,6 +81,7 @@ inline uint32_t next_pow2(uint64_t v) {
 // ---- kernel launchers (each defined in the .hip file holding the kernel)
 
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* buf, uint32_t v, size_t count);
+hipError_t launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, size_t count);
 
 hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, const float* sh,
                           uint32_t n, uint32_t sh_per_ch, uint32_t tile_w, uint32_t* depth_keys,

@@ -16,6 +16,14 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ buf, ui
     for (; i < count; i += stride) buf[i] = v;
 }
 
+// Plain streaming copy (a known-traffic kernel for counter calibration).
+__global__ __launch_bounds__(256) void k_copy_u32(uint4* __restrict__ dst,
+                                                  const uint4* __restrict__ src, size_t count4) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * 256;
+    for (; i < count4; i += stride) dst[i] = src[i];
+}
+
 constexpr uint32_t kEigThreads = 256;
 constexpr uint32_t kEigEpt = 4;
 constexpr uint32_t kEigBlock = kEigThreads * kEigEpt;
@@ -84,6 +92,16 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t* buf, uint32_t v, size_t coun
     size_t blocks = (count + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     k_fill_u32<<<(uint32_t)blocks, 256, 0, s>>>(buf, v, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, size_t count) {
+    size_t c4 = count / 4;
+    if (c4 == 0) return hipSuccess;
+    size_t blocks = (c4 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    k_copy_u32<<<(uint32_t)blocks, 256, 0, s>>>(reinterpret_cast<uint4*>(dst),
+                                                reinterpret_cast<const uint4*>(src), c4);
     return hipGetLastError();
 }
 

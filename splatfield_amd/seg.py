@@ -202,6 +202,24 @@ def eig_round_plain(splats, scratch, cameras, sizes, state, resp):
     return eig, best.value
 
 
+def autotune_round(splats, cameras, sizes, state, reps=3):
+    """Search tile width {8,16,32} x digit width {4,8} for the batched round;
+    returns ((tile_w, digit_bits), {(tile_w, digit_bits): ms})."""
+    n = len(cameras)
+    cams = (_lib.Camera * n)(*[c.to_c() for c in cameras])
+    w = np.ascontiguousarray([s[0] for s in sizes], np.uint32)
+    h = np.ascontiguousarray([s[1] for s in sizes], np.uint32)
+    tw, db = C.c_uint32(), C.c_uint32()
+    ms = np.zeros(6, np.float32)
+    ctx = splats.ctx
+    ctx.check(lib().sfk_eig_round_autotune(ctx.ptr, splats.ptr, cams, w.ctypes.data,
+                                           h.ctypes.data, n, state.a.ptr, state.b.ptr, reps,
+                                           C.byref(tw), C.byref(db), ms.ctypes.data),
+              "eig round autotune")
+    keys = [(t, d) for t in (8, 16, 32) for d in (4, 8)]
+    return (tw.value, db.value), dict(zip(keys, ms.tolist()))
+
+
 def sphere_cameras(count, radius=3.0, center=(0.0, 0.0, 0.0), fov=CANDIDATE_FOV):
     """Fibonacci-sphere cameras at a fixed distance looking at the centre
     (views.rs:77-96 with dist fixed)."""
