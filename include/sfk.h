@@ -325,7 +325,9 @@ int sfk_render_backward(sfk_ctx* ctx, const sfk_splats* splats, sfk_render_scrat
 
 /* project_splats (src/project.rs:295-388) over all n splats.  Writes per
  * splat i: depth_keys[i], projected row i, tile_counts[i], bbox[2i..2i+1];
- * counters[0] += isects, counters[1] += visible (counters are not cleared). */
+ * adds (isects, visible) of each workgroup to one of 64 slot pairs:
+ * counters[2s] += isects, counters[2s+1] += visible (128 u32, not cleared;
+ * the totals are the sums over s). */
 int sfk_project(sfk_ctx* ctx, const sfk_view* view, const float* attrs, const float* sh,
                 uint32_t n, uint32_t sh_per_ch, uint32_t tile_w, uint32_t* depth_keys,
                 float* projected, uint32_t* tile_counts, uint32_t* bbox, uint32_t* counters);

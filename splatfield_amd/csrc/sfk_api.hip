@@ -420,6 +420,8 @@ extern "C" int sfk_scan_scratch_create(sfk_ctx* c, uint32_t max_elems, sfk_scan_
     return SFK_OK;
 }
 
+uint32_t* sfk::scan_scratch_sums(sfk_scan_scratch* s) { return s ? s->sums : nullptr; }
+
 extern "C" void sfk_scan_scratch_destroy(sfk_scan_scratch* s) {
     if (!s) return;
     if (s->sums) hipFree(s->sums);

@@ -22,11 +22,6 @@ namespace {
 constexpr float kInv255 = 1.0f / 255.0f;
 constexpr float kPowerSkip = 5.6f;
 
-__device__ __forceinline__ float wave_sum_f32(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 __global__ __launch_bounds__(256) void k_loss_l2(const float4* __restrict__ img,
                                                  const float4* __restrict__ target, uint32_t np,
@@ -167,8 +162,8 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
             if (__ballot(has)) {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
-                    float s = wave_sum_f32(g[k]);
-                    if (lane_id() == 0 && s != 0.0f) atomicAdd(&s_acc[k][j], s);
+                    float s = wave_sum_last_f32(g[k]);
+                    if (lane_id() == 63 && s != 0.0f) atomicAdd(&s_acc[k][j], s);
                 }
             }
         }

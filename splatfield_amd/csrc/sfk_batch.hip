@@ -51,7 +51,6 @@ __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restric
     const bool live = i < n;
     SplatIn s;
     if (live) s = load_splat(A, i, n);
-    uint32_t wave_total = 0;
     for (uint32_t v = 0; v < nviews; ++v) {
         bool vis = false;
         uint32_t cnt = 0;
@@ -70,15 +69,8 @@ __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restric
             depth_keys[gi] = key;
             tile_counts[gi] = cnt;
         }
-        uint32_t nvis = (uint32_t)__popcll(__ballot(vis));
-        uint32_t ncnt = wave_sum_u32(cnt);
-        wave_total += ncnt;
-        if (lane_id() == 0) {
-            if (ncnt) atomicAdd(&counters[2 * v], ncnt);
-            if (nvis) atomicAdd(&counters[2 * v + 1], nvis);
-        }
+        block_count_pair(cnt, vis, counters + 2 + (size_t)v * 2 * kCountSlots);
     }
-    if (lane_id() == 0 && wave_total) atomicAdd(&counters[2 * nviews], wave_total);
 }
 
 // H(a_i, b_i) once per round, shared by every view.
@@ -166,7 +158,7 @@ struct sfk_eig_round {
     uint32_t* counts = nullptr;
     uint2* bbox = nullptr;
     uint32_t* offsets = nullptr;
-    uint32_t* counters = nullptr;  // [2*v] isects, [2*v+1] visible, [2*V] total
+    uint32_t* counters = nullptr;  // [0] total isects (scan); 2 + v*2*kCountSlots: slots
     uint32_t* tile_ids = nullptr;
     uint32_t* gids = nullptr;
     uint32_t* starts = nullptr;
@@ -217,7 +209,7 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
     const size_t vn = (size_t)V * n;
     const float* attrs = nullptr;
     sfk_splats_info(r->splats, nullptr, nullptr, nullptr, nullptr, &attrs, nullptr);
-    SFK_HIP(c, launch_fill_u32(c->stream, r->counters, 0u, 2 * V + 2));
+    SFK_HIP(c, launch_fill_u32(c->stream, r->counters, 0u, 2 + (size_t)V * 2 * kCountSlots));
     {
         SFK_PROF(c, "batch_project");
         k_project_batch<<<div_up(n, 256), 256, 0, c->stream>>>(
@@ -232,7 +224,7 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
     {
         SFK_PROF(c, "scan_gather");
         SFK_HIP(c, launch_scan(c->stream, order, r->counts, r->offsets, (uint32_t)vn,
-                               r->scan_sums, n));
+                               r->scan_sums, n, r->counters));
     }
     {
         SFK_PROF(c, "map_isects");
@@ -243,14 +235,14 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
     }
     uint32_t* st = nullptr;
     uint32_t* sg = nullptr;
-    rc = sfk_argsort_u32(c, r->sort_tile, r->tile_ids, r->gids, r->capacity, r->counters + 2 * V,
+    rc = sfk_argsort_u32(c, r->sort_tile, r->tile_ids, r->gids, r->capacity, r->counters,
                          sfk_bits_for(r->total_tiles), 1, &st, &sg);
     if (rc) return rc;
     r->sorted_tiles = st;
     r->sorted_gids = sg;
     {
         SFK_PROF(c, "tile_starts");
-        SFK_HIP(c, launch_tile_starts(c->stream, st, r->capacity, r->counters + 2 * V,
+        SFK_HIP(c, launch_tile_starts(c->stream, st, r->capacity, r->counters,
                                       r->total_tiles, r->starts));
     }
     SFK_HIP(c, launch_fill_u32(c->stream, r->resp, 0u, vn));
@@ -346,7 +338,7 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     RALLOC(r->counts, vn * 4);
     RALLOC(r->bbox, vn * 8);
     RALLOC(r->offsets, vn * 4);
-    RALLOC(r->counters, (2 * nviews + 2) * 4);
+    RALLOC(r->counters, (2 + (size_t)nviews * 2 * kCountSlots) * 4);
     RALLOC(r->starts, ((size_t)tiles + 1) * 4);
     RALLOC(r->resp, vn * 4);
     RALLOC(r->h0, (size_t)n * 8);
@@ -396,7 +388,7 @@ static int round_fit_capacity(sfk_eig_round* r, const float* a, const float* b) 
     int rc = round_enqueue(r, a, b);
     if (rc) return rc;
     uint32_t total = 0;
-    SFK_HIP(c, hipMemcpyAsync(&total, r->counters + 2 * r->nviews, 4, hipMemcpyDeviceToHost,
+    SFK_HIP(c, hipMemcpyAsync(&total, r->counters, 4, hipMemcpyDeviceToHost,
                               c->stream));
     SFK_HIP(c, hipStreamSynchronize(c->stream));
     if (total > r->capacity) {
@@ -474,9 +466,21 @@ extern "C" int sfk_eig_round_info(const sfk_eig_round* r, uint32_t view, const u
 
 extern "C" int sfk_eig_round_counters(sfk_ctx* c, const sfk_eig_round* r, uint32_t* counters) {
     if (!c || !r || !counters) return SFK_E_ARG;
-    SFK_HIP(c, hipMemcpyAsync(counters, r->counters, (2 * r->nviews + 1) * 4,
-                              hipMemcpyDeviceToHost, c->stream));
+    const size_t words = 2 + (size_t)r->nviews * 2 * kCountSlots;
+    std::vector<uint32_t> h(words);
+    SFK_HIP(c, hipMemcpyAsync(h.data(), r->counters, words * 4, hipMemcpyDeviceToHost,
+                              c->stream));
     SFK_HIP(c, hipStreamSynchronize(c->stream));
+    for (uint32_t v = 0; v < r->nviews; ++v) {
+        uint32_t is = 0, vis = 0;
+        for (uint32_t s = 0; s < kCountSlots; ++s) {
+            is += h[2 + ((size_t)v * kCountSlots + s) * 2];
+            vis += h[2 + ((size_t)v * kCountSlots + s) * 2 + 1];
+        }
+        counters[2 * v] = is;
+        counters[2 * v + 1] = vis;
+    }
+    counters[2 * r->nviews] = h[0];
     return SFK_OK;
 }
 

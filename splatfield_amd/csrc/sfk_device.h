@@ -28,6 +28,37 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
     return v;
 }
 
+// Sum across the 64 lanes with DPP moves, valid in lane 63 only: butterflies
+// inside each 16-lane row (quad_perm xor 1, xor 2, row_half_mirror,
+// row_mirror leave every lane of a row with the row sum), then row_bcast15
+// into rows 1 and 3 and row_bcast31 into rows 2 and 3.  Six VALU adds with a
+// DPP source operand instead of six LDS-crossbar shuffles.
+#define SFK_DPP_I(x, ctrl, rmask) __builtin_amdgcn_update_dpp(0, (x), (ctrl), (rmask), 0xf, false)
+
+__device__ __forceinline__ uint32_t wave_sum_last_u32(uint32_t v) {
+    int x = (int)v;
+    x += SFK_DPP_I(x, 0xb1, 0xf);   // quad_perm [1,0,3,2]
+    x += SFK_DPP_I(x, 0x4e, 0xf);   // quad_perm [2,3,0,1]
+    x += SFK_DPP_I(x, 0x141, 0xf);  // row_half_mirror
+    x += SFK_DPP_I(x, 0x140, 0xf);  // row_mirror
+    x += SFK_DPP_I(x, 0x142, 0xa);  // row_bcast15 -> rows 1, 3
+    x += SFK_DPP_I(x, 0x143, 0xc);  // row_bcast31 -> rows 2, 3
+    return (uint32_t)x;
+}
+
+__device__ __forceinline__ float wave_sum_last_f32(float v) {
+#define SFK_DPP_F(x, ctrl, rmask) \
+    __builtin_bit_cast(float, SFK_DPP_I(__builtin_bit_cast(int, (x)), (ctrl), (rmask)))
+    v += SFK_DPP_F(v, 0xb1, 0xf);
+    v += SFK_DPP_F(v, 0x4e, 0xf);
+    v += SFK_DPP_F(v, 0x141, 0xf);
+    v += SFK_DPP_F(v, 0x140, 0xf);
+    v += SFK_DPP_F(v, 0x142, 0xa);
+    v += SFK_DPP_F(v, 0x143, 0xc);
+#undef SFK_DPP_F
+    return v;
+}
+
 // Inclusive prefix sum across the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
     uint32_t lane = lane_id();

@@ -72,6 +72,14 @@ int hip_fail(sfk_ctx* ctx, hipError_t e, const char* what);
     } while (0)
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Slot pairs of the informational (isects, visible) counts of a projection.
+constexpr uint32_t kCountSlots = 64;
+// Render counters: [0] total isects (written by the gather scan), [1] unused,
+// then kCountSlots (isects, visible) pairs added by the projection.
+constexpr uint32_t kCounterWords = 2 + 2 * kCountSlots;
+
+uint32_t* scan_scratch_sums(sfk_scan_scratch* s);
 inline uint32_t next_pow2(uint64_t v) {
     uint64_t p = 1;
     while (p < v) p <<= 1;
@@ -111,7 +119,8 @@ hipError_t launch_radix_pass(hipStream_t s, const SortArgs& a, uint32_t key_byte
 // Scan.
 constexpr uint32_t kScanBlock = 4096;
 hipError_t launch_scan(hipStream_t s, const uint32_t* gids, const uint32_t* counts,
-                       uint32_t* offsets, uint32_t n, uint32_t* block_sums, uint32_t seg = 0);
+                       uint32_t* offsets, uint32_t n, uint32_t* block_sums, uint32_t seg = 0,
+                       uint32_t* total_out = nullptr);
 
 // Per-view constants of a batched round (device array, one per view).
 struct BatchView {

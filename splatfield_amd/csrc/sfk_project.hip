@@ -106,12 +106,7 @@ __global__ __launch_bounds__(256) void k_project(sfk_view v, const float* __rest
         depth_keys[i] = key;
         tile_counts[i] = cnt;
     }
-    uint32_t nvis = (uint32_t)__popcll(__ballot(vis));
-    uint32_t ncnt = wave_sum_u32(cnt);
-    if (lane_id() == 0) {
-        if (ncnt) atomicAdd(&counters[0], ncnt);
-        if (nvis) atomicAdd(&counters[1], nvis);
-    }
+    block_count_pair(cnt, vis, counters);
 }
 
 }  // namespace

@@ -109,4 +109,30 @@ __device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s
     return true;
 }
 
+// Informational per-frame counts (isects, visible): reduced across the
+// 256-thread workgroup, then added to one of kCountSlots slot pairs chosen
+// by workgroup id.  One hot address for every workgroup serializes in L2
+// (about 8K atomics per 1M splats); spread over 64 slot pairs it does not.
+// slots[2*s] += isects, slots[2*s+1] += visible; readers sum the slots.
+__device__ __forceinline__ void block_count_pair(uint32_t cnt, bool vis, uint32_t* slots) {
+    __shared__ uint32_t s_c[2];
+    if (threadIdx.x == 0) {
+        s_c[0] = 0;
+        s_c[1] = 0;
+    }
+    __syncthreads();
+    uint32_t nvis = (uint32_t)__popcll(__ballot(vis));
+    uint32_t ncnt = wave_sum_u32(cnt);
+    if (lane_id() == 0) {
+        if (ncnt) atomicAdd(&s_c[0], ncnt);
+        if (nvis) atomicAdd(&s_c[1], nvis);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t* p = slots + 2 * (blockIdx.x % kCountSlots);
+        if (s_c[0]) atomicAdd(&p[0], s_c[0]);
+        if (s_c[1]) atomicAdd(&p[1], s_c[1]);
+    }
+}
+
 }  // namespace sfk

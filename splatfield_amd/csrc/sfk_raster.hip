@@ -162,12 +162,12 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
             if constexpr (MODE != kRgb) {
                 if (__ballot(q != 0)) {
                     if constexpr (MODE == kTotal) {
-                        uint32_t s = wave_sum_u32(q);
-                        if (lane_id() == 0 && s) atomicAdd(&s_q0[j], s);
+                        uint32_t s = wave_sum_last_u32(q);
+                        if (lane_id() == 63 && s) atomicAdd(&s_q0[j], s);
                     } else {
-                        uint32_t sf = wave_sum_u32(inside ? q : 0u);
-                        uint32_t sb = wave_sum_u32(inside ? 0u : q);
-                        if (lane_id() == 0) {
+                        uint32_t sf = wave_sum_last_u32(inside ? q : 0u);
+                        uint32_t sb = wave_sum_last_u32(inside ? 0u : q);
+                        if (lane_id() == 63) {
                             if (sf) atomicAdd(&s_q0[j], sf);
                             if (sb) atomicAdd(&s_q1[j], sb);
                         }

@@ -176,7 +176,7 @@ static int scratch_build(sfk_render_scratch* s, uint32_t n, uint32_t w, uint32_t
     SFK_HIP(c, hipMalloc(&s->tile_counts, nn * 4));
     SFK_HIP(c, hipMalloc(&s->bbox, nn * 8));
     SFK_HIP(c, hipMalloc(&s->offsets, nn * 4));
-    SFK_HIP(c, hipMalloc(&s->counters, 16));
+    SFK_HIP(c, hipMalloc(&s->counters, kCounterWords * 4));
     SFK_HIP(c, hipMalloc(&s->tile_starts, (size_t)(tiles + 1) * 4));
     SFK_HIP(c, hipMalloc(&s->bitmap, (size_t)h * s->row_stride * 4));
     SFK_HIP(c, hipMalloc(&s->rgba, px * 16));
@@ -329,25 +329,30 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
     const uint32_t n = sp->n;
     const uint32_t tiles = s->tiles_x * s->tiles_y;
 
-    SFK_HIP(c, launch_fill_u32(c->stream, s->counters, 0u, 4));
+    SFK_HIP(c, launch_fill_u32(c->stream, s->counters, 0u, kCounterWords));
     {
         SFK_PROF(c, "project");
         SFK_HIP(c, launch_project(c->stream, *view, sp->attrs, sp->sh, n,
                                   colors ? sp->sh_per_ch : 0u, s->tile_w, s->depth_keys,
-                                  s->projected, s->tile_counts, s->bbox, s->counters));
+                                  s->projected, s->tile_counts, s->bbox, s->counters + 2));
         s->has_colors = colors;
     }
-    SFK_HIP(c, hipMemcpyAsync(s->host_counters, s->counters, 8, hipMemcpyDeviceToHost, c->stream));
-    SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));
-    s->counters_pending = true;
 
     uint32_t* order = nullptr;
     rc = sfk_argsort_u32(c, s->sort_depth, s->depth_keys, nullptr, n, nullptr, 32, 0, nullptr,
                          &order);
     if (rc) return rc;
     s->depth_order = order;
-    rc = sfk_exclusive_scan_gather(c, order, s->tile_counts, s->offsets, n, s->scan);
-    if (rc) return rc;
+    {
+        // Emission offsets in depth order; the scan's grand total is the raw
+        // isect count (counters[0]).
+        SFK_PROF(c, "scan_gather");
+        SFK_HIP(c, launch_scan(c->stream, order, s->tile_counts, s->offsets, n,
+                               scan_scratch_sums(s->scan), 0, s->counters));
+    }
+    SFK_HIP(c, hipMemcpyAsync(s->host_counters, s->counters, 4, hipMemcpyDeviceToHost, c->stream));
+    SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));
+    s->counters_pending = true;
     {
         SFK_PROF(c, "map_isects");
         SFK_HIP(c, launch_map_isects(c->stream, order, s->tile_counts, s->bbox, s->offsets, n,
@@ -383,11 +388,13 @@ extern "C" int sfk_prepare_isects(sfk_ctx* c, const sfk_splats* sp, sfk_render_s
 
 extern "C" int sfk_tile_isects_counts(sfk_ctx* c, const sfk_tile_isects* t, uint32_t counts[3]) {
     if (!c || !t || !counts) return SFK_E_ARG;
-    uint32_t h[2];
-    SFK_HIP(c, hipMemcpyAsync(h, t->d_counters, 8, hipMemcpyDeviceToHost, c->stream));
+    uint32_t h[kCounterWords];
+    SFK_HIP(c, hipMemcpyAsync(h, t->d_counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
     SFK_HIP(c, hipStreamSynchronize(c->stream));
+    uint32_t vis = 0;
+    for (uint32_t k = 0; k < kCountSlots; ++k) vis += h[2 + 2 * k + 1];
     counts[0] = h[0];
-    counts[1] = h[1];
+    counts[1] = vis;
     counts[2] = std::min(h[0], t->capacity);
     return SFK_OK;
 }

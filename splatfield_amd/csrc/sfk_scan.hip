@@ -41,7 +41,8 @@ __global__ __launch_bounds__(NT) void k_scan_reduce(const uint32_t* __restrict__
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(NT) void k_scan_sums(uint32_t* __restrict__ sums, uint32_t nb) {
+__global__ __launch_bounds__(NT) void k_scan_sums(uint32_t* __restrict__ sums, uint32_t nb,
+                                                  uint32_t* __restrict__ total_out) {
     __shared__ uint32_t tmp[NT / 64 + 1];
     uint32_t carry = 0;
     for (uint32_t c0 = 0; c0 < nb; c0 += BLK) {
@@ -62,6 +63,7 @@ __global__ __launch_bounds__(NT) void k_scan_sums(uint32_t* __restrict__ sums, u
         }
         carry += total;
     }
+    if (total_out && threadIdx.x == 0) *total_out = carry;
 }
 
 __global__ __launch_bounds__(NT) void k_scan_apply(const uint32_t* __restrict__ gids,
@@ -91,11 +93,12 @@ __global__ __launch_bounds__(NT) void k_scan_apply(const uint32_t* __restrict__ 
 }  // namespace
 
 hipError_t launch_scan(hipStream_t s, const uint32_t* gids, const uint32_t* counts,
-                       uint32_t* offsets, uint32_t n, uint32_t* block_sums, uint32_t seg) {
+                       uint32_t* offsets, uint32_t n, uint32_t* block_sums, uint32_t seg,
+                       uint32_t* total_out) {
     if (n == 0) return hipSuccess;
     uint32_t nb = div_up(n, BLK);
     k_scan_reduce<<<nb, NT, 0, s>>>(gids, counts, n, block_sums, seg);
-    k_scan_sums<<<1, NT, 0, s>>>(block_sums, nb);
+    k_scan_sums<<<1, NT, 0, s>>>(block_sums, nb, total_out);
     k_scan_apply<<<nb, NT, 0, s>>>(gids, counts, n, block_sums, offsets, seg);
     return hipGetLastError();
 }
