@@ -116,54 +116,64 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
             for (int k = 0; k < 9; ++k) s_acc[k][t] = 0.0f;
         }
         __syncthreads();
-        for (uint32_t jj = cnt; jj > 0; --jj) {
-            const uint32_t j = jj - 1;
-            const uint32_t idx = cbeg + j;
-            float g[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) g[k] = 0.0f;
+        // Splats in groups of 8, back to front: member s of the group whose
+        // top is `jtop` is staged row jtop - s.  The 9 x 8 per-pixel terms
+        // are reduced over the wave in one transposed pass; lane l ends with
+        // the sums of member l >> 3.
+        for (uint32_t jtop = cnt; jtop > 0; jtop = jtop > 8 ? jtop - 8 : 0) {
+            float g[9][8];
             bool has = false;
-            if (idx < last) {
-                float4 geo = s_geo[j];
-                float4 aux = s_aux[j];
-                float dx = geo.x - pxf, dy = geo.y - pyf;
-                float power = 0.5f * (geo.z * dx * dx + aux.x * dy * dy) + geo.w * dx * dy;
-                if (!(power > kPowerSkip)) {
-                    float G = sfk_expf(-power);
-                    float araw = aux.y * G;
-                    float alpha = fminf(araw, 0.999f);
-                    if (alpha >= kInv255) {
-                        has = true;
-                        float om = 1.0f - alpha;
-                        T = T / om;
-                        float wgt = alpha * T;
-                        float cb = s_blue[j];
-                        g[6] = wgt * dC.x;
-                        g[7] = wgt * dC.y;
-                        g[8] = wgt * dC.z;
-                        float dal = T * ((aux.z - acc_r) * dC.x + (aux.w - acc_g) * dC.y +
-                                         (cb - acc_b) * dC.z) +
-                                    Tfin / om * dC.w;
-                        acc_r = alpha * aux.z + om * acc_r;
-                        acc_g = alpha * aux.w + om * acc_g;
-                        acc_b = alpha * cb + om * acc_b;
-                        if (araw < 0.999f) {
-                            float dpow = -dal * alpha;
-                            g[5] = dal * G;
-                            g[0] = dpow * (geo.z * dx + geo.w * dy);
-                            g[1] = dpow * (aux.x * dy + geo.w * dx);
-                            g[2] = dpow * 0.5f * dx * dx;
-                            g[3] = dpow * dx * dy;
-                            g[4] = dpow * 0.5f * dy * dy;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) g[k][s] = 0.0f;
+                const uint32_t j = jtop - 1 - s;  // wraps when s >= jtop
+                if (s < (int)jtop && cbeg + j < last) {
+                    float4 geo = s_geo[j];
+                    float4 aux = s_aux[j];
+                    float dx = geo.x - pxf, dy = geo.y - pyf;
+                    float power = 0.5f * (geo.z * dx * dx + aux.x * dy * dy) + geo.w * dx * dy;
+                    if (!(power > kPowerSkip)) {
+                        float G = sfk_expf(-power);
+                        float araw = aux.y * G;
+                        float alpha = fminf(araw, 0.999f);
+                        if (alpha >= kInv255) {
+                            has = true;
+                            float om = 1.0f - alpha;
+                            T = T / om;
+                            float wgt = alpha * T;
+                            float cb = s_blue[j];
+                            g[6][s] = wgt * dC.x;
+                            g[7][s] = wgt * dC.y;
+                            g[8][s] = wgt * dC.z;
+                            float dal = T * ((aux.z - acc_r) * dC.x + (aux.w - acc_g) * dC.y +
+                                             (cb - acc_b) * dC.z) +
+                                        Tfin / om * dC.w;
+                            acc_r = alpha * aux.z + om * acc_r;
+                            acc_g = alpha * aux.w + om * acc_g;
+                            acc_b = alpha * cb + om * acc_b;
+                            if (araw < 0.999f) {
+                                float dpow = -dal * alpha;
+                                g[5][s] = dal * G;
+                                g[0][s] = dpow * (geo.z * dx + geo.w * dy);
+                                g[1][s] = dpow * (aux.x * dy + geo.w * dx);
+                                g[2][s] = dpow * 0.5f * dx * dx;
+                                g[3][s] = dpow * dx * dy;
+                                g[4][s] = dpow * 0.5f * dy * dy;
+                            }
                         }
                     }
                 }
             }
             if (__ballot(has)) {
+                const uint32_t lane = lane_id();
+                const uint32_t ms = lane >> 3;
+                const bool writer = (lane & 7u) == 0 && ms < jtop;
+                const uint32_t j = jtop - 1 - ms;
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
-                    float s = wave_sum_last_f32(g[k]);
-                    if (lane_id() == 63 && s != 0.0f) atomicAdd(&s_acc[k][j], s);
+                    float r = wave_sum8_transposed(g[k]);
+                    if (writer && r != 0.0f) atomicAdd(&s_acc[k][j], r);
                 }
             }
         }

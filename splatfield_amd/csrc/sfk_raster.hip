@@ -135,42 +135,68 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
         // Staging fence and whole-tile early exit (raster.rs:166-171).
         if (__syncthreads_count(!done) == 0) break;
         const uint32_t cnt = (end - chunk) < (uint32_t)NT ? (end - chunk) : (uint32_t)NT;
-        for (uint32_t j = 0; j < cnt; ++j) {
-            uint32_t q = 0;
-            if (!done) {
-                float4 geo = s_geo[j];
-                float dx = geo.x - pxf, dy = geo.y - pyf;
-                float4 aux = s_aux[j];
-                float power = 0.5f * (geo.z * dx * dx + aux.x * dy * dy) + geo.w * dx * dy;
-                if (!(power > kPowerSkip)) {
-                    float alpha = fminf(aux.y * sfk_expf(-power), 0.999f);
-                    if (alpha >= kInv255) {
-                        float vis = alpha * T;
-                        if constexpr (MODE == kRgb) {
-                            cr += aux.z * vis;
-                            cg += aux.w * vis;
-                            cb += s_blue[j] * vis;
-                        } else {
-                            q = sfk_f2u(vis * J.scale);
-                        }
-                        T *= 1.0f - alpha;
-                        last = chunk + j + 1;
-                        if (T < kInv255) done = true;
+        // One splat against this pixel: returns the contribution weight
+        // vis = alpha * T (0 when the splat does not contribute) and advances
+        // T, `last` and `done` exactly as raster.rs:183-217.
+        auto step = [&](uint32_t j, float& vis) -> bool {
+            float4 geo = s_geo[j];
+            float dx = geo.x - pxf, dy = geo.y - pyf;
+            float4 aux = s_aux[j];
+            float power = 0.5f * (geo.z * dx * dx + aux.x * dy * dy) + geo.w * dx * dy;
+            if (power > kPowerSkip) return false;
+            float alpha = fminf(aux.y * sfk_expf(-power), 0.999f);
+            if (!(alpha >= kInv255)) return false;
+            vis = alpha * T;
+            if constexpr (MODE == kRgb) {
+                cr += aux.z * vis;
+                cg += aux.w * vis;
+                cb += s_blue[j] * vis;
+            }
+            T *= 1.0f - alpha;
+            last = chunk + j + 1;
+            if (T < kInv255) done = true;
+            return true;
+        };
+        if constexpr (MODE == kRgb) {
+            for (uint32_t j = 0; j < cnt; ++j) {
+                float vis;
+                if (!done) step(j, vis);
+            }
+        } else {
+            // Groups of 8 splats: member s is staged row j0 + s; the u32
+            // weights of the 8 members are summed over the wave in one
+            // transposed pass, lane l ending with member l >> 3.
+            const uint32_t lane = lane_id();
+            const uint32_t ms = lane >> 3;
+            for (uint32_t j0 = 0; j0 < cnt; j0 += 8) {
+                uint32_t q[8];
+                bool any = false;
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    float vis;
+                    q[s] = 0;
+                    if (!done && j0 + s < cnt && step(j0 + s, vis)) {
+                        q[s] = sfk_f2u(vis * J.scale);
+                        any |= q[s] != 0;
                     }
                 }
-            }
-            if constexpr (MODE != kRgb) {
-                if (__ballot(q != 0)) {
-                    if constexpr (MODE == kTotal) {
-                        uint32_t s = wave_sum_last_u32(q);
-                        if (lane_id() == 63 && s) atomicAdd(&s_q0[j], s);
-                    } else {
-                        uint32_t sf = wave_sum_last_u32(inside ? q : 0u);
-                        uint32_t sb = wave_sum_last_u32(inside ? 0u : q);
-                        if (lane_id() == 63) {
-                            if (sf) atomicAdd(&s_q0[j], sf);
-                            if (sb) atomicAdd(&s_q1[j], sb);
-                        }
+                if (!__ballot(any)) continue;
+                const bool writer = (lane & 7u) == 0 && j0 + ms < cnt;
+                if constexpr (MODE == kTotal) {
+                    uint32_t r = wave_sum8_transposed(q);
+                    if (writer && r) atomicAdd(&s_q0[j0 + ms], r);
+                } else {
+                    uint32_t qf[8], qb[8];
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) {
+                        qf[s] = inside ? q[s] : 0u;
+                        qb[s] = inside ? 0u : q[s];
+                    }
+                    uint32_t rf = wave_sum8_transposed(qf);
+                    uint32_t rb = wave_sum8_transposed(qb);
+                    if (writer) {
+                        if (rf) atomicAdd(&s_q0[j0 + ms], rf);
+                        if (rb) atomicAdd(&s_q1[j0 + ms], rb);
                     }
                 }
             }
