@@ -14,14 +14,11 @@
 #include "sfk_device.h"
 #include "sfk_internal.h"
 #include "sfk_math.h"
+#include "sfk_raster_core.h"
 
 namespace sfk {
 
 namespace {
-
-constexpr float kInv255 = 1.0f / 255.0f;
-constexpr float kPowerSkip = 5.6f;
-
 
 __global__ __launch_bounds__(256) void k_loss_l2(const float4* __restrict__ img,
                                                  const float4* __restrict__ target, uint32_t np,
@@ -67,8 +64,11 @@ struct BwdArgs {
 template <int TW>
 __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
     constexpr int NT = TW * TW;
+    constexpr int NW = NT / 64;
     __shared__ float4 s_geo[NT];
     __shared__ float4 s_aux[NT];
+    __shared__ float4 s_box[NT];
+    __shared__ uint16_t s_list[NW][NT];
     __shared__ float s_blue[NT];
     __shared__ uint32_t s_gid[NT];
     __shared__ float s_acc[9][NT];
@@ -78,10 +78,14 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
     const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
     const uint32_t tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const uint32_t t = threadIdx.x;
-    const uint32_t px = tx * TW + (t % TW), py = ty * TW + (t / TW);
+    uint32_t lx, ly;
+    tile_pixel<TW>(t, lx, ly);
+    const uint32_t px = tx * TW + lx, py = ty * TW + ly;
     const bool in_bounds = px < a.w && py < a.h;
     const uint32_t start = a.tile_starts[tile];
     const float pxf = (float)px + 0.5f, pyf = (float)py + 0.5f;
+    const float bx0 = (float)(tx * TW + (lx & ~7u)), by0 = (float)(ty * TW + (ly & ~7u));
+    uint16_t* list = s_list[t / 64];
 
     uint32_t last = start;
     float T = 1.0f, Tfin = 1.0f;
@@ -98,6 +102,10 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
     atomicMax(&s_last, last);
     __syncthreads();
     const uint32_t block_last = s_last;
+    // Past its own last contributor a wave has nothing to do.
+    uint32_t wave_last = last;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wave_last = max(wave_last, (uint32_t)__shfl_xor(wave_last, o, 64));
     float acc_r = 0.f, acc_g = 0.f, acc_b = 0.f;
     const float4* proj = reinterpret_cast<const float4*>(a.proj);
 
@@ -108,27 +116,32 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
         if (t < cnt) {
             uint32_t g = a.gids[cbeg + t];
             const float4* row = proj + (size_t)g * 3;
-            s_geo[t] = row[0];
-            s_aux[t] = row[1];
+            float4 geo = row[0], aux = row[1];
+            s_geo[t] = geo;
+            s_aux[t] = aux;
+            s_box[t] = reach_box(geo, aux);
             s_blue[t] = row[2].x;
             s_gid[t] = g;
 #pragma unroll
             for (int k = 0; k < 9; ++k) s_acc[k][t] = 0.0f;
         }
         __syncthreads();
-        // Splats in groups of 8, back to front: member s of the group whose
-        // top is `jtop` is staged row jtop - s.  The 9 x 8 per-pixel terms
-        // are reduced over the wave in one transposed pass; lane l ends with
-        // the sums of member l >> 3.
-        for (uint32_t jtop = cnt; jtop > 0; jtop = jtop > 8 ? jtop - 8 : 0) {
+        const uint32_t lim = wave_last > cbeg ? min(cnt, wave_last - cbeg) : 0u;
+        const uint32_t nl = wave_reach_list(s_box, lim, bx0, by0, list);
+        // Listed splats in groups of 8, back to front: member s of the group
+        // whose top is `itop` is list entry itop - 1 - s.  The 9 x 8 per-pixel
+        // terms are reduced over the wave in one transposed pass; lane l ends
+        // with the sums of member l >> 3.
+        for (uint32_t itop = nl; itop > 0; itop = itop > 8 ? itop - 8 : 0) {
             float g[9][8];
             bool has = false;
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) g[k][s] = 0.0f;
-                const uint32_t j = jtop - 1 - s;  // wraps when s >= jtop
-                if (s < (int)jtop && cbeg + j < last) {
+                if (s >= (int)itop) continue;
+                const uint32_t j = list[itop - 1 - s];
+                if (cbeg + j < last) {
                     float4 geo = s_geo[j];
                     float4 aux = s_aux[j];
                     float dx = geo.x - pxf, dy = geo.y - pyf;
@@ -140,7 +153,8 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
                         if (alpha >= kInv255) {
                             has = true;
                             float om = 1.0f - alpha;
-                            T = T / om;
+                            float rom = 1.0f / om;
+                            T = T * rom;
                             float wgt = alpha * T;
                             float cb = s_blue[j];
                             g[6][s] = wgt * dC.x;
@@ -148,7 +162,7 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
                             g[8][s] = wgt * dC.z;
                             float dal = T * ((aux.z - acc_r) * dC.x + (aux.w - acc_g) * dC.y +
                                              (cb - acc_b) * dC.z) +
-                                        Tfin / om * dC.w;
+                                        Tfin * rom * dC.w;
                             acc_r = alpha * aux.z + om * acc_r;
                             acc_g = alpha * aux.w + om * acc_g;
                             acc_b = alpha * cb + om * acc_b;
@@ -168,8 +182,8 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
             if (__ballot(has)) {
                 const uint32_t lane = lane_id();
                 const uint32_t ms = lane >> 3;
-                const bool writer = (lane & 7u) == 0 && ms < jtop;
-                const uint32_t j = jtop - 1 - ms;
+                const bool writer = (lane & 7u) == 0 && ms < itop;
+                const uint32_t j = writer ? list[itop - 1 - ms] : 0u;
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
                     float r = wave_sum8_transposed(g[k]);

@@ -5,6 +5,7 @@
 #include "sfk_device.h"
 #include "sfk_internal.h"
 #include "sfk_math.h"
+#include "sfk_raster_core.h"
 
 namespace sfk {
 
@@ -70,11 +71,6 @@ __global__ __launch_bounds__(256) void k_tile_starts(const uint32_t* __restrict_
     for (uint32_t t = lo; t <= hi; ++t) starts[t] = k;
 }
 
-constexpr float kInv255 = 1.0f / 255.0f;
-// exp(-5.6) * 1.0 < 1/255: beyond this Gaussian power no splat (opacity <= 1)
-// reaches the contribution threshold, whatever the last ulp of exp.
-constexpr float kPowerSkip = 5.6f;
-
 // One tile of one view.
 struct TileJob {
     uint32_t w, h, tx, ty, start, end, row_stride;
@@ -87,26 +83,40 @@ struct TileJob {
 };
 
 // rasterize_kernel (raster.rs:87-230) for one tile: TW x TW threads, one per
-// pixel; splat rows staged through LDS in chunks of TW*TW.  The
-// responsibility modes add each splat's per-pixel u32 weights across the
-// wavefront, then across the workgroup in LDS, and issue one global atomic
-// per (splat, tile): the integer sum is order independent, so the result is
-// bit-identical to one atomic per pixel.
-template <int MODE, int TW>
+// pixel; splat rows staged through LDS in chunks of TW*TW.  Each wavefront
+// walks only the staged splats whose reach box meets its 8 x 8 pixel block
+// (the others cannot reach alpha >= 1/255 on any of its pixels, so skipping
+// them changes nothing).  The responsibility modes add each splat's per-pixel
+// u32 weights across the wavefront, then across the workgroup in LDS, and
+// issue one global atomic per (splat, tile): the integer sum is order
+// independent, so the result is bit-identical to one atomic per pixel.
+// PF loads the staged rows of a group of 8 before compositing them, which
+// overlaps their LDS latency at the cost of registers: it pays when the
+// launch has too few waves to hide latency (one view's responsibility render)
+// and costs occupancy otherwise.
+template <int MODE, int TW, bool PF>
 __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& a) {
     constexpr int NT = TW * TW;
+    constexpr int NW = NT / 64;
     __shared__ float4 s_geo[NT];  // mx, my, conic a, conic b
     __shared__ float4 s_aux[NT];  // conic c, opacity, r, g
+    __shared__ float4 s_box[NT];  // reach box
+    __shared__ uint16_t s_list[NW][NT];
     __shared__ float s_blue[MODE == kRgb ? NT : 1];
     __shared__ uint32_t s_gid[MODE == kRgb ? 1 : NT];
     __shared__ uint32_t s_q0[MODE == kRgb ? 1 : NT];
     __shared__ uint32_t s_q1[MODE == kEvidence ? NT : 1];
 
     const uint32_t t = threadIdx.x;
-    const uint32_t px = J.tx * TW + (t % TW), py = J.ty * TW + (t / TW);
+    uint32_t lx, ly;
+    tile_pixel<TW>(t, lx, ly);
+    const uint32_t px = J.tx * TW + lx, py = J.ty * TW + ly;
     const bool in_bounds = px < J.w && py < J.h;
     const uint32_t start = J.start, end = J.end;
     const float pxf = (float)px + 0.5f, pyf = (float)py + 0.5f;
+    const uint32_t wv = t / 64;
+    const float bx0 = (float)(J.tx * TW + (lx & ~7u)), by0 = (float)(J.ty * TW + (ly & ~7u));
+    uint16_t* list = s_list[wv];
 
     uint32_t inside = 0;
     if (MODE == kEvidence && in_bounds)
@@ -122,8 +132,10 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
         if (idx < end) {
             uint32_t g = a.gids[idx];
             const float4* row = J.proj + (size_t)g * J.rows4;
-            s_geo[t] = row[0];
-            s_aux[t] = row[1];
+            float4 geo = row[0], aux = row[1];
+            s_geo[t] = geo;
+            s_aux[t] = aux;
+            s_box[t] = reach_box(geo, aux);
             if constexpr (MODE == kRgb) {
                 s_blue[t] = row[2].x;
             } else {
@@ -135,13 +147,12 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
         // Staging fence and whole-tile early exit (raster.rs:166-171).
         if (__syncthreads_count(!done) == 0) break;
         const uint32_t cnt = (end - chunk) < (uint32_t)NT ? (end - chunk) : (uint32_t)NT;
-        // One splat against this pixel: returns the contribution weight
-        // vis = alpha * T (0 when the splat does not contribute) and advances
-        // T, `last` and `done` exactly as raster.rs:183-217.
-        auto step = [&](uint32_t j, float& vis) -> bool {
-            float4 geo = s_geo[j];
+        const uint32_t nl = __ballot(!done) ? wave_reach_list(s_box, cnt, bx0, by0, list) : 0u;
+        // One splat against this pixel: returns whether it contributes, with
+        // its weight vis = alpha * T, advancing T, `last` and `done` exactly
+        // as raster.rs:183-217.
+        auto step = [&](uint32_t j, float4 geo, float4 aux, float blue, float& vis) -> bool {
             float dx = geo.x - pxf, dy = geo.y - pyf;
-            float4 aux = s_aux[j];
             float power = 0.5f * (geo.z * dx * dx + aux.x * dy * dy) + geo.w * dx * dy;
             if (power > kPowerSkip) return false;
             float alpha = fminf(aux.y * sfk_expf(-power), 0.999f);
@@ -150,41 +161,79 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
             if constexpr (MODE == kRgb) {
                 cr += aux.z * vis;
                 cg += aux.w * vis;
-                cb += s_blue[j] * vis;
+                cb += blue * vis;
             }
             T *= 1.0f - alpha;
             last = chunk + j + 1;
             if (T < kInv255) done = true;
             return true;
         };
-        if constexpr (MODE == kRgb) {
-            for (uint32_t j = 0; j < cnt; ++j) {
-                float vis;
-                if (!done) step(j, vis);
+        // Listed splats in groups of 8, rows loaded up front so that the LDS
+        // reads of a group overlap (entries past the list end repeat the last
+        // entry and are masked off).
+        const uint32_t lane = lane_id();
+        const uint32_t ms = lane >> 3;
+        for (uint32_t i0 = 0; i0 < nl; i0 += 8) {
+            if (!__ballot(!done)) break;
+            uint32_t jj[8];
+            float4 geo[8], aux[8];
+            float blue[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) jj[s] = list[min(i0 + s, nl - 1)];
+            if constexpr (PF) {
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    geo[s] = s_geo[jj[s]];
+                    aux[s] = s_aux[jj[s]];
+                    if constexpr (MODE == kRgb) blue[s] = s_blue[jj[s]];
+                    else blue[s] = 0.0f;
+                }
             }
-        } else {
-            // Groups of 8 splats: member s is staged row j0 + s; the u32
-            // weights of the 8 members are summed over the wave in one
-            // transposed pass, lane l ending with member l >> 3.
-            const uint32_t lane = lane_id();
-            const uint32_t ms = lane >> 3;
-            for (uint32_t j0 = 0; j0 < cnt; j0 += 8) {
+            // Row of member s: preloaded, or read at its turn.
+            auto row = [&](int s, float4& g, float4& x, float& b) {
+                if constexpr (PF) {
+                    g = geo[s]; x = aux[s]; b = blue[s];
+                } else {
+                    g = s_geo[jj[s]]; x = s_aux[jj[s]];
+                    b = MODE == kRgb ? s_blue[jj[s]] : 0.0f;
+                }
+            };
+            if constexpr (MODE == kRgb) {
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    float vis;
+                    if (!done && i0 + s < nl) {
+                        float4 g, x;
+                        float b;
+                        row(s, g, x, b);
+                        step(jj[s], g, x, b, vis);
+                    }
+                }
+            } else {
+                // The u32 weights of the 8 members are summed over the wave in
+                // one transposed pass, lane l ending with member l >> 3.
                 uint32_t q[8];
                 bool any = false;
 #pragma unroll
                 for (int s = 0; s < 8; ++s) {
                     float vis;
                     q[s] = 0;
-                    if (!done && j0 + s < cnt && step(j0 + s, vis)) {
-                        q[s] = sfk_f2u(vis * J.scale);
-                        any |= q[s] != 0;
+                    if (!done && i0 + s < nl) {
+                        float4 g, x;
+                        float b;
+                        row(s, g, x, b);
+                        if (step(jj[s], g, x, b, vis)) {
+                            q[s] = sfk_f2u(vis * J.scale);
+                            any |= q[s] != 0;
+                        }
                     }
                 }
                 if (!__ballot(any)) continue;
-                const bool writer = (lane & 7u) == 0 && j0 + ms < cnt;
+                const bool writer = (lane & 7u) == 0 && i0 + ms < nl;
+                const uint32_t j = writer ? list[i0 + ms] : 0u;
                 if constexpr (MODE == kTotal) {
                     uint32_t r = wave_sum8_transposed(q);
-                    if (writer && r) atomicAdd(&s_q0[j0 + ms], r);
+                    if (writer && r) atomicAdd(&s_q0[j], r);
                 } else {
                     uint32_t qf[8], qb[8];
 #pragma unroll
@@ -195,8 +244,8 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
                     uint32_t rf = wave_sum8_transposed(qf);
                     uint32_t rb = wave_sum8_transposed(qb);
                     if (writer) {
-                        if (rf) atomicAdd(&s_q0[j0 + ms], rf);
-                        if (rb) atomicAdd(&s_q1[j0 + ms], rb);
+                        if (rf) atomicAdd(&s_q0[j], rf);
+                        if (rb) atomicAdd(&s_q1[j], rb);
                     }
                 }
             }
@@ -247,7 +296,7 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
     J.proj = reinterpret_cast<const float4*>(a.proj);
     J.rows4 = 3;
     J.resp = a.resp; J.fg = a.fg; J.bg = a.bg;
-    raster_tile<MODE, TW>(J, a);
+    raster_tile<MODE, TW, MODE != kRgb>(J, a);
 }
 
 // Batched responsibility: one launch over the tiles of every view.
@@ -279,7 +328,7 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize_batch(RasterArgs a) {
     RasterArgs b = a;
     b.final_t = nullptr;
     b.n_contrib = nullptr;
-    raster_tile<kTotal, TW>(J, b);
+    raster_tile<kTotal, TW, false>(J, b);
 }
 
 template <int MODE>
