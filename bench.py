@@ -135,6 +135,8 @@ def kernel_bytes(n, k, vis, ni, px, tile_bits):
         "radix_pass_u32": None,  # depth and tile passes differ; see below
         "scan_gather": 16 * vis,
         "map_isects": 16 * vis + 8 * ni,
+        # order + counts + bbox read, offsets written, isects written
+        "scan_emit": 20 * vis + 8 * ni,
         "tile_starts": 4 * ni,
         "raster_rgb": 40 * ni + 16 * px,
         "loss_l2": 48 * px,

@@ -157,7 +157,6 @@ struct sfk_eig_round {
     uint32_t* keys = nullptr;
     uint32_t* counts = nullptr;
     uint2* bbox = nullptr;
-    uint32_t* offsets = nullptr;
     uint32_t* counters = nullptr;  // [0] total isects (scan); 2 + v*2*kCountSlots: slots
     uint32_t* tile_ids = nullptr;
     uint32_t* gids = nullptr;
@@ -171,7 +170,7 @@ struct sfk_eig_round {
     sfk_radix_scratch* sort_depth = nullptr;
     sfk_radix_scratch* sort_tile = nullptr;
     sfk_scan_scratch* scan = nullptr;
-    uint32_t* scan_sums = nullptr;
+    uint32_t* emit_status = nullptr;  // look-back words of the fused scan/emission
     const uint32_t* sorted_tiles = nullptr;
     const uint32_t* sorted_gids = nullptr;
     hipGraph_t graph = nullptr;
@@ -222,16 +221,11 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
                                        &order);
     if (rc) return rc;
     {
-        SFK_PROF(c, "scan_gather");
-        SFK_HIP(c, launch_scan(c->stream, order, r->counts, r->offsets, (uint32_t)vn,
-                               r->scan_sums, n, r->counters));
-    }
-    {
-        SFK_PROF(c, "map_isects");
-        SFK_HIP(c, launch_map_isects(c->stream, order, r->counts,
-                                     reinterpret_cast<const uint32_t*>(r->bbox), r->offsets,
-                                     (uint32_t)vn, 0, r->capacity, r->tile_ids, r->gids, n,
-                                     r->d_bviews));
+        SFK_PROF(c, "scan_emit");
+        SFK_HIP(c, launch_scan_emit(c->stream, order, r->counts,
+                                    reinterpret_cast<const uint32_t*>(r->bbox), (uint32_t)vn, 0,
+                                    r->capacity, r->tile_ids, r->gids, nullptr, r->counters,
+                                    r->emit_status, n, r->d_bviews));
     }
     uint32_t* st = nullptr;
     uint32_t* sg = nullptr;
@@ -337,7 +331,6 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     RALLOC(r->keys, vn * 4);
     RALLOC(r->counts, vn * 4);
     RALLOC(r->bbox, vn * 8);
-    RALLOC(r->offsets, vn * 4);
     RALLOC(r->counters, (2 + (size_t)nviews * 2 * kCountSlots) * 4);
     RALLOC(r->starts, ((size_t)tiles + 1) * 4);
     RALLOC(r->resp, vn * 4);
@@ -345,7 +338,7 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     RALLOC(r->partials, (size_t)nviews * div_up(n, kEigBlock) * 8);
     RALLOC(r->eig, nviews * 8);
     RALLOC(r->best, 4);
-    RALLOC(r->scan_sums, (size_t)div_up(vn, kScanBlock) * 4);
+    RALLOC(r->emit_status, (size_t)scan_emit_status_words((uint32_t)vn) * 4);
 #undef RALLOC
     if (hipHostMalloc(&r->host_out, (nviews + 1) * 8, hipHostMallocDefault) != hipSuccess)
         return fail_out(fail(c, SFK_E_HIP, "eig round: pinned output"));
@@ -370,9 +363,9 @@ extern "C" void sfk_eig_round_destroy(sfk_eig_round* r) {
     if (r->ctx) hipStreamSynchronize(r->ctx->stream);
     if (r->exec) hipGraphExecDestroy(r->exec);
     if (r->graph) hipGraphDestroy(r->graph);
-    void* bufs[] = {r->d_views, r->d_bviews, r->proj, r->keys, r->counts, r->bbox, r->offsets,
+    void* bufs[] = {r->d_views, r->d_bviews, r->proj, r->keys, r->counts, r->bbox,
                     r->counters, r->starts, r->resp, r->h0, r->partials, r->eig, r->best,
-                    r->scan_sums};
+                    r->emit_status};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (r->host_out) hipHostFree(r->host_out);

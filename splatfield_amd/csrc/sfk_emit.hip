@@ -1,0 +1,196 @@
+// Fused emission-offset scan and isect emission for gfx950: one pass over
+// the depth ranks that does what scan.rs:215-255 (exclusive_scan_gather of
+// tile counts in depth order) followed by raster.rs:12-48 (map_isects) do,
+// with one gather of the per-splat counts and bboxes instead of three.
+//
+// Workgroups of 256 threads own 1024 consecutive ranks (4 per thread).  The
+// workgroup prefix comes from a decoupled look-back over the workgroups
+// before it: each publishes its aggregate, then its inclusive prefix, in one
+// 64-bit status word (flag << 32 | u32 value).  Logical workgroup ids are
+// taken from a ticket counter, so every workgroup a look-back waits on has
+// already started and publishes its aggregate before it waits on anything:
+// every wait ends.  Each wavefront then writes the isects of its 256 ranks,
+// which form one contiguous output range, with consecutive lanes on
+// consecutive slots (coalesced), finding each slot's rank by binary search
+// over the wave's prefix sums in LDS.
+#include "sfk_device.h"
+#include "sfk_internal.h"
+
+namespace sfk {
+
+namespace {
+
+constexpr uint32_t kEmitThreads = 256;
+constexpr uint32_t kEmitRpt = 4;  // ranks per thread
+constexpr uint32_t kEmitRanks = kEmitThreads * kEmitRpt;
+constexpr uint32_t kFlagAggregate = 1, kFlagPrefix = 2;
+
+__device__ __forceinline__ uint64_t status_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint64_t* p, uint32_t flag, uint32_t v) {
+    __hip_atomic_store(p, ((uint64_t)flag << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct EmitArgs {
+    const uint32_t* order;        // depth rank -> splat (in-view index when batched)
+    const uint32_t* tile_counts;  // per splat (view-major when batched)
+    const uint2* bbox;            // packed tile bbox per splat
+    uint32_t n;                   // ranks
+    uint32_t tiles_per_row;
+    uint32_t max_isects;
+    uint32_t* tile_ids;
+    uint32_t* gaussian_ids;
+    uint32_t* offsets;            // optional: per-rank exclusive offsets
+    uint32_t* total_out;          // grand total (raw isect count)
+    uint32_t seg_len;             // batched: ranks per view
+    const BatchView* views;
+    uint32_t* ticket;             // zeroed before launch
+    uint64_t* status;             // one word per workgroup, zeroed before launch
+};
+
+__global__ __launch_bounds__(kEmitThreads) void k_scan_emit(EmitArgs a) {
+    __shared__ uint32_t s_incl[kEmitRanks];  // inclusive prefix within the wave's 256 ranks
+    __shared__ uint2 s_bb[kEmitRanks];
+    __shared__ uint32_t s_g[kEmitRanks];
+    __shared__ uint32_t s_tmp[kEmitThreads / 64 + 1];
+    __shared__ uint32_t s_bid, s_excl;
+
+    const uint32_t t = threadIdx.x;
+    if (t == 0) s_bid = atomicAdd(a.ticket, 1u);
+    __syncthreads();
+    const uint32_t bid = s_bid;
+    const uint32_t r0 = bid * kEmitRanks + t * kEmitRpt;
+
+    // Gather counts and boxes of this thread's 4 consecutive ranks.
+    uint32_t cnt[kEmitRpt];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kEmitRpt; ++k) {
+        const uint32_t r = r0 + k;
+        uint32_t c = 0, g = 0;
+        uint2 bb = make_uint2(0, 0);
+        if (r < a.n) {
+            g = a.order[r];
+            size_t gi = g;
+            if (a.seg_len) gi = (size_t)(r / a.seg_len) * a.seg_len + g;
+            c = a.tile_counts[gi];
+            if (c) bb = a.bbox[gi];
+        }
+        cnt[k] = c;
+        sum += c;
+        s_bb[t * kEmitRpt + k] = bb;
+        s_g[t * kEmitRpt + k] = g;
+    }
+
+    // Workgroup exclusive prefix of the per-thread sums (u32, wrapping as
+    // the reference's u32 scan does).
+    uint32_t total;
+    const uint32_t texcl = block_excl_scan_u32<kEmitThreads>(sum, s_tmp, &total);
+
+    // Decoupled look-back for the prefix of everything before this workgroup.
+    if (t == 0) {
+        uint32_t excl = 0;
+        if (bid == 0) {
+            status_store(&a.status[0], kFlagPrefix, total);
+        } else {
+            status_store(&a.status[bid], kFlagAggregate, total);
+            for (uint32_t p = bid - 1;;) {
+                const uint64_t w = status_load(&a.status[p]);
+                const uint32_t flag = (uint32_t)(w >> 32);
+                if (flag == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += (uint32_t)w;
+                if (flag == kFlagPrefix) break;
+                --p;  // an aggregate: keep walking (block 0 always holds a prefix)
+            }
+            status_store(&a.status[bid], kFlagPrefix, excl + total);
+        }
+        s_excl = excl;
+        if (bid * kEmitRanks + kEmitRanks >= a.n && a.total_out) *a.total_out = excl + total;
+    }
+    __syncthreads();
+    const uint32_t base = s_excl;
+
+    // Per-rank offsets, and the inclusive prefix of each rank within its
+    // wave's 256 ranks for the emission.
+    const uint32_t lane = lane_id();
+    const uint32_t wave_first = (t / 64) * 64;  // first thread of this wave
+    const uint32_t wave_base_excl = __shfl(texcl, 0, 64);
+    uint32_t run = texcl;
+#pragma unroll
+    for (uint32_t k = 0; k < kEmitRpt; ++k) {
+        const uint32_t r = r0 + k;
+        if (a.offsets && r < a.n) a.offsets[r] = base + run;
+        run += cnt[k];
+        s_incl[t * kEmitRpt + k] = run - wave_base_excl;
+    }
+    // Wave range: [base + wave_base_excl, + wave_total).
+    const uint32_t wave_total = __shfl(run, 63, 64) - wave_base_excl;
+    const uint32_t out0 = base + wave_base_excl;
+    const uint32_t* incl = s_incl + wave_first * kEmitRpt;
+    const uint2* bbs = s_bb + wave_first * kEmitRpt;
+    const uint32_t* gs = s_g + wave_first * kEmitRpt;
+    const uint32_t rbase = bid * kEmitRanks + wave_first * kEmitRpt;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t e = lane; e < wave_total; e += 64) {
+        const uint32_t slot = out0 + e;
+        if (slot >= a.max_isects) break;  // slots only grow with e
+        // Owner: first k with incl[k] > e (256 entries, 8 halvings).
+        uint32_t lo = 0;
+#pragma unroll
+        for (uint32_t step = 128; step > 0; step >>= 1)
+            if (incl[lo + step - 1] <= e) lo += step;
+        const uint32_t k = lo;
+        const uint32_t before = k ? incl[k - 1] : 0u;
+        const uint32_t li = e - before;
+        const uint2 bb = bbs[k];
+        const uint32_t min_x = bb.x & 0xffffu, min_y = bb.x >> 16, max_x = bb.y & 0xffffu;
+        const uint32_t wx = max_x - min_x;
+        const uint32_t dy = li / wx, dx = li - dy * wx;
+        uint32_t tpr = a.tiles_per_row, tbase = 0;
+        if (a.seg_len) {
+            const BatchView& v = a.views[(rbase + k) / a.seg_len];
+            tpr = v.tiles_x;
+            tbase = v.tile_base;
+        }
+        a.tile_ids[slot] = tbase + (min_x + dx) + (min_y + dy) * tpr;
+        a.gaussian_ids[slot] = gs[k];
+    }
+}
+
+}  // namespace
+
+uint32_t scan_emit_status_words(uint32_t n) { return 2u + 2u * div_up(n ? n : 1, kEmitRanks); }
+
+hipError_t launch_scan_emit(hipStream_t s, const uint32_t* order, const uint32_t* tile_counts,
+                            const uint32_t* bbox, uint32_t n, uint32_t tiles_per_row,
+                            uint32_t max_isects, uint32_t* tile_ids, uint32_t* gaussian_ids,
+                            uint32_t* offsets, uint32_t* total_out, uint32_t* status_words,
+                            uint32_t seg_len, const BatchView* views) {
+    if (n == 0) return hipSuccess;
+    const uint32_t nb = div_up(n, kEmitRanks);
+    hipError_t e = launch_fill_u32(s, status_words, 0u, scan_emit_status_words(n));
+    if (e != hipSuccess) return e;
+    EmitArgs a;
+    a.order = order;
+    a.tile_counts = tile_counts;
+    a.bbox = reinterpret_cast<const uint2*>(bbox);
+    a.n = n;
+    a.tiles_per_row = tiles_per_row;
+    a.max_isects = max_isects;
+    a.tile_ids = tile_ids;
+    a.gaussian_ids = gaussian_ids;
+    a.offsets = offsets;
+    a.total_out = total_out;
+    a.seg_len = seg_len;
+    a.views = views;
+    a.ticket = status_words;
+    a.status = reinterpret_cast<uint64_t*>(status_words + 2);
+    k_scan_emit<<<nb, kEmitThreads, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace sfk

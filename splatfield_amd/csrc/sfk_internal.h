@@ -122,6 +122,17 @@ hipError_t launch_scan(hipStream_t s, const uint32_t* gids, const uint32_t* coun
                        uint32_t* offsets, uint32_t n, uint32_t* block_sums, uint32_t seg = 0,
                        uint32_t* total_out = nullptr);
 
+// Fused gather scan + isect emission (scan.rs:215-255 then raster.rs:12-48).
+// status_words: scan_emit_status_words(n) u32 of device scratch (cleared by
+// the launch itself with a fill kernel).
+struct BatchView;
+uint32_t scan_emit_status_words(uint32_t n);
+hipError_t launch_scan_emit(hipStream_t s, const uint32_t* order, const uint32_t* tile_counts,
+                            const uint32_t* bbox, uint32_t n, uint32_t tiles_per_row,
+                            uint32_t max_isects, uint32_t* tile_ids, uint32_t* gaussian_ids,
+                            uint32_t* offsets, uint32_t* total_out, uint32_t* status_words,
+                            uint32_t seg_len = 0, const BatchView* views = nullptr);
+
 // Per-view constants of a batched round (device array, one per view).
 struct BatchView {
     uint32_t w, h, tiles_x, tiles_y;

@@ -107,7 +107,7 @@ struct sfk_render_scratch {
     uint32_t* n_contrib = nullptr;
     sfk_radix_scratch* sort_depth = nullptr;
     sfk_radix_scratch* sort_tile = nullptr;
-    sfk_scan_scratch* scan = nullptr;
+    uint32_t* emit_status = nullptr;  // look-back words of the fused scan/emission
     // Previous frame's raw emission, read back asynchronously for growth.
     uint32_t* host_counters = nullptr;
     hipEvent_t counters_ready = nullptr;
@@ -127,7 +127,7 @@ struct sfk_render_scratch {
 static void scratch_free_frame(sfk_render_scratch* s) {
     void* bufs[] = {s->depth_keys, s->projected, s->tile_counts, s->bbox, s->offsets,
                     s->counters, s->tile_starts, s->bitmap, s->rgba, s->final_t, s->n_contrib,
-                    s->tile_ids, s->gaussian_ids, s->grad2d};
+                    s->tile_ids, s->gaussian_ids, s->grad2d, s->emit_status};
     for (void* p : bufs)
         if (p) hipFree(p);
     s->depth_keys = nullptr; s->projected = nullptr; s->tile_counts = nullptr; s->bbox = nullptr;
@@ -135,12 +135,11 @@ static void scratch_free_frame(sfk_render_scratch* s) {
     s->rgba = nullptr; s->final_t = nullptr; s->n_contrib = nullptr; s->tile_ids = nullptr;
     s->gaussian_ids = nullptr;
     s->grad2d = nullptr;
+    s->emit_status = nullptr;
     s->has_view = false;
     sfk_radix_scratch_destroy(s->sort_depth);
     sfk_radix_scratch_destroy(s->sort_tile);
-    sfk_scan_scratch_destroy(s->scan);
     s->sort_depth = s->sort_tile = nullptr;
-    s->scan = nullptr;
     s->depth_order = s->sorted_tiles = s->sorted_gids = nullptr;
 }
 
@@ -185,8 +184,7 @@ static int scratch_build(sfk_render_scratch* s, uint32_t n, uint32_t w, uint32_t
     int rc = sfk_radix_scratch_create(c, n, 4, &s->sort_depth);
     if (rc) return rc;
     sfk_radix_set_digit_bits(s->sort_depth, s->digit_bits);
-    rc = sfk_scan_scratch_create(c, n, &s->scan);
-    if (rc) return rc;
+    SFK_HIP(c, hipMalloc(&s->emit_status, (size_t)scan_emit_status_words(n) * 4));
     uint64_t cap = std::min<uint64_t>(tiles * n, kInitialIsectsCap);
     return alloc_isects(s, next_pow2(cap ? cap : 1));
 }
@@ -344,20 +342,16 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
     if (rc) return rc;
     s->depth_order = order;
     {
-        // Emission offsets in depth order; the scan's grand total is the raw
-        // isect count (counters[0]).
-        SFK_PROF(c, "scan_gather");
-        SFK_HIP(c, launch_scan(c->stream, order, s->tile_counts, s->offsets, n,
-                               scan_scratch_sums(s->scan), 0, s->counters));
+        // Emission offsets in depth order and the isects themselves, in one
+        // pass; the grand total is the raw isect count (counters[0]).
+        SFK_PROF(c, "scan_emit");
+        SFK_HIP(c, launch_scan_emit(c->stream, order, s->tile_counts, s->bbox, n, s->tiles_x,
+                                    s->capacity, s->tile_ids, s->gaussian_ids, s->offsets,
+                                    s->counters, s->emit_status));
     }
     SFK_HIP(c, hipMemcpyAsync(s->host_counters, s->counters, 4, hipMemcpyDeviceToHost, c->stream));
     SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));
     s->counters_pending = true;
-    {
-        SFK_PROF(c, "map_isects");
-        SFK_HIP(c, launch_map_isects(c->stream, order, s->tile_counts, s->bbox, s->offsets, n,
-                                     s->tiles_x, s->capacity, s->tile_ids, s->gaussian_ids));
-    }
     uint32_t* st = nullptr;
     uint32_t* sg = nullptr;
     rc = sfk_argsort_u32(c, s->sort_tile, s->tile_ids, s->gaussian_ids, s->capacity, s->counters,
