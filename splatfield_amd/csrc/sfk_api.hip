@@ -354,8 +354,12 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const u
     const K* cur_k = keys;
     const uint32_t* cur_v = vals;
     int dst = 0;
+    // As few passes as digits of at most digit_bits allow, with the bits
+    // spread evenly over them (13 bits: 7 + 6, not 8 + 5): fewer bins per
+    // pass mean longer coalesced runs in the scatter.
     const uint32_t db = s->digit_bits;
     const uint32_t passes = bits == 0 ? 1u : (bits + db - 1) / db;
+    const uint32_t wlo = bits / passes, wextra = bits % passes;
     SortArgs a;
     a.n_max = nseg > 1 ? 0u : n;
     a.d_n = d_n;
@@ -364,9 +368,9 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const u
     a.hist = s->hist;
     a.stride = nseg > 1 ? radix_max_blocks(n) : s->max_blocks;
     a.totals = s->totals;
+    uint32_t shift = 0;
     for (uint32_t p = 0; p < passes; ++p) {
-        uint32_t shift = p * db;
-        uint32_t width = bits == 0 ? 0u : std::min(db, bits - shift);
+        const uint32_t width = wlo + (p < wextra ? 1u : 0u);
         bool last = p + 1 == passes;
         a.keys_in = cur_k;
         a.vals_in = cur_v;
@@ -375,8 +379,9 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const u
         a.shift = shift;
         a.width = width;
         SFK_PROF(c, sizeof(K) == 4 ? "radix_pass_u32" : "radix_pass_u64");
-        hipError_t e = launch_radix_pass(c->stream, a, sizeof(K), db);
+        hipError_t e = launch_radix_pass(c->stream, a, sizeof(K), wlo + (wextra ? 1u : 0u));
         if (e != hipSuccess) return hip_fail(c, e, "radix pass");
+        shift += width;
         cur_k = static_cast<K*>(s->keys[dst]);
         cur_v = a.vals_out;
         dst ^= 1;

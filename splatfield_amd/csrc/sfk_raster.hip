@@ -56,7 +56,8 @@ __global__ __launch_bounds__(256) void k_map_isects(const uint32_t* __restrict__
 
 // starts[t] = lower_bound(tile_ids, t) for t in [0, num_tiles] (raster.rs:
 // 52-65): isect k writes every t in (tile_ids[k-1], tile_ids[k]], and k = n
-// closes (tile_ids[n-1], num_tiles].  Each t is written exactly once.
+// closes (tile_ids[n-1], num_tiles].  Each t is written exactly once.  A
+// thread covers 4 consecutive isects with one 16-byte load.
 __global__ __launch_bounds__(256) void k_tile_starts(const uint32_t* __restrict__ tile_ids,
                                                      uint32_t n_max,
                                                      const uint32_t* __restrict__ d_n,
@@ -64,11 +65,25 @@ __global__ __launch_bounds__(256) void k_tile_starts(const uint32_t* __restrict_
                                                      uint32_t* __restrict__ starts) {
     uint32_t n = n_max;
     if (d_n) n = *d_n < n_max ? *d_n : n_max;
-    uint32_t k = blockIdx.x * 256u + threadIdx.x;
-    if (k > n) return;
-    uint32_t lo = k == 0 ? 0u : tile_ids[k - 1] + 1u;
-    uint32_t hi = k == n ? num_tiles : tile_ids[k];
-    for (uint32_t t = lo; t <= hi; ++t) starts[t] = k;
+    const uint32_t k0 = (blockIdx.x * 256u + threadIdx.x) * 4u;
+    if (k0 > n) return;
+    uint32_t ids[4];
+    if (k0 + 4 <= n) {
+        const uint4 v = reinterpret_cast<const uint4*>(tile_ids)[k0 / 4];
+        ids[0] = v.x; ids[1] = v.y; ids[2] = v.z; ids[3] = v.w;
+    } else {
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) ids[i] = k0 + i < n ? tile_ids[k0 + i] : num_tiles;
+    }
+    uint32_t lo = k0 == 0 ? 0u : tile_ids[k0 - 1] + 1u;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        const uint32_t k = k0 + i;
+        if (k > n) break;
+        const uint32_t hi = ids[i];  // num_tiles at k == n
+        for (uint32_t t = lo; t <= hi; ++t) starts[t] = k;
+        lo = hi + 1u;
+    }
 }
 
 // One tile of one view.
@@ -360,7 +375,7 @@ hipError_t launch_map_isects(hipStream_t s, const uint32_t* order, const uint32_
 
 hipError_t launch_tile_starts(hipStream_t s, const uint32_t* tile_ids, uint32_t n,
                               const uint32_t* d_n, uint32_t num_tiles, uint32_t* starts) {
-    k_tile_starts<<<div_up((uint64_t)n + 1, 256), 256, 0, s>>>(tile_ids, n, d_n, num_tiles, starts);
+    k_tile_starts<<<div_up((uint64_t)n + 1, 1024), 256, 0, s>>>(tile_ids, n, d_n, num_tiles, starts);
     return hipGetLastError();
 }
 

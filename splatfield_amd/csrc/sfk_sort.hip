@@ -266,11 +266,15 @@ hipError_t launch_radix_pass(hipStream_t st, const SortArgs& a, uint32_t key_byt
                              uint32_t digit_bits) {
     if (key_bytes == 4) {
         if (digit_bits <= 4) return pass_impl<uint32_t, 16>(st, a);
+        if (digit_bits <= 5) return pass_impl<uint32_t, 32>(st, a);
         if (digit_bits <= 6) return pass_impl<uint32_t, 64>(st, a);
+        if (digit_bits <= 7) return pass_impl<uint32_t, 128>(st, a);
         return pass_impl<uint32_t, 256>(st, a);
     }
     if (digit_bits <= 4) return pass_impl<uint64_t, 16>(st, a);
+    if (digit_bits <= 5) return pass_impl<uint64_t, 32>(st, a);
     if (digit_bits <= 6) return pass_impl<uint64_t, 64>(st, a);
+    if (digit_bits <= 7) return pass_impl<uint64_t, 128>(st, a);
     return pass_impl<uint64_t, 256>(st, a);
 }
 
