@@ -149,6 +149,7 @@ struct sfk_eig_round {
     const sfk_splats* splats = nullptr;
     uint32_t n = 0, nviews = 0, tile_w = 16, digit_bits = 8;
     uint32_t total_tiles = 0, capacity = 0;
+    uint32_t max_tiles = 0, tile_bits = 0;  // largest view's tiles, bits_for of it
     std::vector<sfk_view> views;
     std::vector<BatchView> bviews;
     sfk_view* d_views = nullptr;
@@ -225,19 +226,21 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
         SFK_HIP(c, launch_scan_emit(c->stream, order, r->counts,
                                     reinterpret_cast<const uint32_t*>(r->bbox), (uint32_t)vn, 0,
                                     r->capacity, r->tile_ids, r->gids, nullptr, r->counters,
-                                    r->emit_status, n, r->d_bviews));
+                                    r->emit_status, n, r->d_bviews, r->tile_bits));
     }
     uint32_t* st = nullptr;
     uint32_t* sg = nullptr;
+    // Only the local tile bits are sorted: the emission is view-major, so a
+    // stable sort on them orders by (local tile, view, depth rank).
     rc = sfk_argsort_u32(c, r->sort_tile, r->tile_ids, r->gids, r->capacity, r->counters,
-                         sfk_bits_for(r->total_tiles), 1, &st, &sg);
+                         r->tile_bits, 1, &st, &sg);
     if (rc) return rc;
     r->sorted_tiles = st;
     r->sorted_gids = sg;
     {
         SFK_PROF(c, "tile_starts");
         SFK_HIP(c, launch_tile_starts(c->stream, st, r->capacity, r->counters,
-                                      r->total_tiles, r->starts));
+                                      r->max_tiles * V, r->starts, V, r->tile_bits));
     }
     SFK_HIP(c, launch_fill_u32(c->stream, r->resp, 0u, vn));
     {
@@ -311,10 +314,16 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
         bv.tile_base = (uint32_t)tiles;
         bv.scale = (float)(1ull << sfk_fractional_bits(widths[v], heights[v]));
         tiles += (uint64_t)bv.tiles_x * bv.tiles_y;
+        r->max_tiles = std::max(r->max_tiles, bv.tiles_x * bv.tiles_y);
         cap0 += std::min<uint64_t>((uint64_t)bv.tiles_x * bv.tiles_y * n, 1u << 22);
         r->bviews.push_back(bv);
     }
     r->total_tiles = (uint32_t)tiles;
+    r->tile_bits = sfk_bits_for(r->max_tiles);
+    if (r->tile_bits + 6 > 32 || (uint64_t)r->max_tiles * nviews >= 0xffffffffull) {
+        delete r;
+        return fail(c, SFK_E_ARG, "eig round: %u tiles per view too many", r->max_tiles);
+    }
     const size_t vn = (size_t)nviews * n;
     auto fail_out = [&](int rc) {
         sfk_eig_round_destroy(r);
@@ -332,7 +341,7 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     RALLOC(r->counts, vn * 4);
     RALLOC(r->bbox, vn * 8);
     RALLOC(r->counters, (2 + (size_t)nviews * 2 * kCountSlots) * 4);
-    RALLOC(r->starts, ((size_t)tiles + 1) * 4);
+    RALLOC(r->starts, ((size_t)r->max_tiles * nviews + 1) * 4);
     RALLOC(r->resp, vn * 4);
     RALLOC(r->h0, (size_t)n * 8);
     RALLOC(r->partials, (size_t)nviews * div_up(n, kEigBlock) * 8);

@@ -44,6 +44,7 @@ struct EmitArgs {
     uint32_t* offsets;            // optional: per-rank exclusive offsets
     uint32_t* total_out;          // grand total (raw isect count)
     uint32_t seg_len;             // batched: ranks per view
+    uint32_t view_shift;          // batched: key = view << view_shift | local tile
     const BatchView* views;
     uint32_t* ticket;             // zeroed before launch
     uint64_t* status;             // one word per workgroup, zeroed before launch
@@ -150,14 +151,18 @@ __global__ __launch_bounds__(kEmitThreads) void k_scan_emit(EmitArgs a) {
         const uint32_t min_x = bb.x & 0xffffu, min_y = bb.x >> 16, max_x = bb.y & 0xffffu;
         const uint32_t wx = max_x - min_x;
         const uint32_t dy = li / wx, dx = li - dy * wx;
-        uint32_t tpr = a.tiles_per_row, tbase = 0;
         if (a.seg_len) {
-            const BatchView& v = a.views[(rbase + k) / a.seg_len];
-            tpr = v.tiles_x;
-            tbase = v.tile_base;
+            // Batched: the key keeps the view above the local tile (only the
+            // local tile bits are sorted; the input is already view-major),
+            // the value is the view-major splat slot.
+            const uint32_t v = (rbase + k) / a.seg_len;
+            const uint32_t tpr = a.views[v].tiles_x;
+            a.tile_ids[slot] = (v << a.view_shift) | ((min_x + dx) + (min_y + dy) * tpr);
+            a.gaussian_ids[slot] = v * a.seg_len + gs[k];
+        } else {
+            a.tile_ids[slot] = (min_x + dx) + (min_y + dy) * a.tiles_per_row;
+            a.gaussian_ids[slot] = gs[k];
         }
-        a.tile_ids[slot] = tbase + (min_x + dx) + (min_y + dy) * tpr;
-        a.gaussian_ids[slot] = gs[k];
     }
 }
 
@@ -169,7 +174,7 @@ hipError_t launch_scan_emit(hipStream_t s, const uint32_t* order, const uint32_t
                             const uint32_t* bbox, uint32_t n, uint32_t tiles_per_row,
                             uint32_t max_isects, uint32_t* tile_ids, uint32_t* gaussian_ids,
                             uint32_t* offsets, uint32_t* total_out, uint32_t* status_words,
-                            uint32_t seg_len, const BatchView* views) {
+                            uint32_t seg_len, const BatchView* views, uint32_t view_shift) {
     if (n == 0) return hipSuccess;
     const uint32_t nb = div_up(n, kEmitRanks);
     hipError_t e = launch_fill_u32(s, status_words, 0u, scan_emit_status_words(n));
@@ -186,6 +191,7 @@ hipError_t launch_scan_emit(hipStream_t s, const uint32_t* order, const uint32_t
     a.offsets = offsets;
     a.total_out = total_out;
     a.seg_len = seg_len;
+    a.view_shift = view_shift;
     a.views = views;
     a.ticket = status_words;
     a.status = reinterpret_cast<uint64_t*>(status_words + 2);

@@ -131,7 +131,8 @@ hipError_t launch_scan_emit(hipStream_t s, const uint32_t* order, const uint32_t
                             const uint32_t* bbox, uint32_t n, uint32_t tiles_per_row,
                             uint32_t max_isects, uint32_t* tile_ids, uint32_t* gaussian_ids,
                             uint32_t* offsets, uint32_t* total_out, uint32_t* status_words,
-                            uint32_t seg_len = 0, const BatchView* views = nullptr);
+                            uint32_t seg_len = 0, const BatchView* views = nullptr,
+                            uint32_t view_shift = 0);
 
 // Per-view constants of a batched round (device array, one per view).
 struct BatchView {
@@ -147,8 +148,12 @@ hipError_t launch_map_isects(hipStream_t s, const uint32_t* order, const uint32_
                              uint32_t tiles_per_row, uint32_t max_isects, uint32_t* tile_ids,
                              uint32_t* gaussian_ids, uint32_t seg_len = 0,
                              const BatchView* views = nullptr);
+// nviews > 0: keys are view << view_shift | local tile, sorted by local tile
+// (then view, from the view-major input); starts are indexed by
+// local * nviews + view over num_tiles = max local tiles * nviews.
 hipError_t launch_tile_starts(hipStream_t s, const uint32_t* tile_ids, uint32_t n,
-                              const uint32_t* d_n, uint32_t num_tiles, uint32_t* starts);
+                              const uint32_t* d_n, uint32_t num_tiles, uint32_t* starts,
+                              uint32_t nviews = 0, uint32_t view_shift = 0);
 
 enum RasterMode { kRgb = 0, kTotal = 1, kEvidence = 2 };
 struct RasterArgs {

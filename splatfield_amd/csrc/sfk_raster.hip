@@ -62,7 +62,12 @@ __global__ __launch_bounds__(256) void k_tile_starts(const uint32_t* __restrict_
                                                      uint32_t n_max,
                                                      const uint32_t* __restrict__ d_n,
                                                      uint32_t num_tiles,
-                                                     uint32_t* __restrict__ starts) {
+                                                     uint32_t* __restrict__ starts,
+                                                     uint32_t nviews, uint32_t view_shift) {
+    // Batched keys (view << view_shift | local) map to local * nviews + view.
+    auto id = [&](uint32_t key) {
+        return nviews ? (key & ((1u << view_shift) - 1u)) * nviews + (key >> view_shift) : key;
+    };
     uint32_t n = n_max;
     if (d_n) n = *d_n < n_max ? *d_n : n_max;
     const uint32_t k0 = (blockIdx.x * 256u + threadIdx.x) * 4u;
@@ -70,12 +75,12 @@ __global__ __launch_bounds__(256) void k_tile_starts(const uint32_t* __restrict_
     uint32_t ids[4];
     if (k0 + 4 <= n) {
         const uint4 v = reinterpret_cast<const uint4*>(tile_ids)[k0 / 4];
-        ids[0] = v.x; ids[1] = v.y; ids[2] = v.z; ids[3] = v.w;
+        ids[0] = id(v.x); ids[1] = id(v.y); ids[2] = id(v.z); ids[3] = id(v.w);
     } else {
 #pragma unroll
-        for (uint32_t i = 0; i < 4; ++i) ids[i] = k0 + i < n ? tile_ids[k0 + i] : num_tiles;
+        for (uint32_t i = 0; i < 4; ++i) ids[i] = k0 + i < n ? id(tile_ids[k0 + i]) : num_tiles;
     }
-    uint32_t lo = k0 == 0 ? 0u : tile_ids[k0 - 1] + 1u;
+    uint32_t lo = k0 == 0 ? 0u : id(tile_ids[k0 - 1]) + 1u;
 #pragma unroll
     for (uint32_t i = 0; i < 4; ++i) {
         const uint32_t k = k0 + i;
@@ -332,13 +337,16 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize_batch(RasterArgs a) {
     J.w = bv.w; J.h = bv.h;
     J.tx = local % bv.tiles_x;
     J.ty = local / bv.tiles_x;
-    J.start = a.tile_starts[tile];
-    J.end = a.tile_starts[tile + 1];
+    // Sorted by (local tile, view): list (view lo, tile local) is entry
+    // local * nviews + lo of the starts table; ids are view-major slots.
+    const uint32_t slot = local * a.nviews + lo;
+    J.start = a.tile_starts[slot];
+    J.end = a.tile_starts[slot + 1];
     J.row_stride = 0;
     J.scale = bv.scale;
-    J.proj = reinterpret_cast<const float4*>(a.proj) + (size_t)lo * a.seg_len * 2;
+    J.proj = reinterpret_cast<const float4*>(a.proj);
     J.rows4 = 2;
-    J.resp = a.resp + (size_t)lo * a.seg_len;
+    J.resp = a.resp;
     J.fg = nullptr; J.bg = nullptr;
     RasterArgs b = a;
     b.final_t = nullptr;
@@ -374,8 +382,10 @@ hipError_t launch_map_isects(hipStream_t s, const uint32_t* order, const uint32_
 }
 
 hipError_t launch_tile_starts(hipStream_t s, const uint32_t* tile_ids, uint32_t n,
-                              const uint32_t* d_n, uint32_t num_tiles, uint32_t* starts) {
-    k_tile_starts<<<div_up((uint64_t)n + 1, 1024), 256, 0, s>>>(tile_ids, n, d_n, num_tiles, starts);
+                              const uint32_t* d_n, uint32_t num_tiles, uint32_t* starts,
+                              uint32_t nviews, uint32_t view_shift) {
+    k_tile_starts<<<div_up((uint64_t)n + 1, 1024), 256, 0, s>>>(tile_ids, n, d_n, num_tiles, starts,
+                                                                nviews, view_shift);
     return hipGetLastError();
 }
 
