@@ -260,7 +260,7 @@ extern "C" uint32_t sfk_bits_for(uint32_t m) {
 
 struct sfk_radix_scratch {
     sfk_ctx* ctx = nullptr;
-    uint32_t max_elems = 0, key_bytes = 4, digit_bits = 8, max_blocks = 0;
+    uint32_t max_elems = 0, key_bytes = 4, val_bytes = 4, digit_bits = 8, max_blocks = 0;
     void* keys[2] = {nullptr, nullptr};
     uint32_t* vals[2] = {nullptr, nullptr};
     uint32_t* hist = nullptr;
@@ -286,7 +286,7 @@ static int radix_alloc(sfk_ctx* c, sfk_radix_scratch* s, uint32_t max_elems) {
     s->max_blocks = radix_max_blocks(max_elems);
     for (int i = 0; i < 2; ++i) {
         SFK_HIP(c, hipMalloc(&s->keys[i], m * s->key_bytes));
-        SFK_HIP(c, hipMalloc(&s->vals[i], m * 4));
+        SFK_HIP(c, hipMalloc(&s->vals[i], m * s->val_bytes));
     }
     // Room for up to 256 segments (each may round its last block up).
     SFK_HIP(c, hipMalloc(&s->hist, ((size_t)s->max_blocks + 256) * 256 * 4));
@@ -294,14 +294,17 @@ static int radix_alloc(sfk_ctx* c, sfk_radix_scratch* s, uint32_t max_elems) {
     return SFK_OK;
 }
 
-extern "C" int sfk_radix_scratch_create(sfk_ctx* c, uint32_t max_elems, uint32_t key_bytes,
-                                        sfk_radix_scratch** out) {
-    if (!c || !out || (key_bytes != 4 && key_bytes != 8)) return fail(c, SFK_E_ARG, "radix scratch: bad args");
+int sfk::radix_scratch_create_ex(sfk_ctx* c, uint32_t max_elems, uint32_t key_bytes,
+                                 uint32_t val_bytes, sfk_radix_scratch** out) {
+    if (!c || !out || (key_bytes != 4 && key_bytes != 8) || (val_bytes != 4 && val_bytes != 8) ||
+        (val_bytes == 8 && key_bytes != 4))
+        return fail(c, SFK_E_ARG, "radix scratch: bad args");
     *out = nullptr;
     auto* s = new (std::nothrow) sfk_radix_scratch();
     if (!s) return SFK_E_NOMEM;
     s->ctx = c;
     s->key_bytes = key_bytes;
+    s->val_bytes = val_bytes;
     int rc = radix_alloc(c, s, max_elems);
     if (rc) {
         radix_release(s);
@@ -310,6 +313,11 @@ extern "C" int sfk_radix_scratch_create(sfk_ctx* c, uint32_t max_elems, uint32_t
     }
     *out = s;
     return SFK_OK;
+}
+
+extern "C" int sfk_radix_scratch_create(sfk_ctx* c, uint32_t max_elems, uint32_t key_bytes,
+                                        sfk_radix_scratch** out) {
+    return sfk::radix_scratch_create_ex(c, max_elems, key_bytes, 4, out);
 }
 
 extern "C" void sfk_radix_scratch_destroy(sfk_radix_scratch* s) {
@@ -324,12 +332,14 @@ extern "C" int sfk_radix_set_digit_bits(sfk_radix_scratch* s, uint32_t bits) {
     return SFK_OK;
 }
 
-template <typename K>
-static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const uint32_t* vals,
+template <typename K, typename V = uint32_t>
+static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const V* vals,
                         uint32_t n, const uint32_t* d_n, uint32_t bits, int write_keys,
-                        K** out_keys, uint32_t** out_vals, uint32_t nseg = 1) {
+                        K** out_keys, V** out_vals, uint32_t nseg = 1,
+                        const uint32_t* payload = nullptr) {
     if (!c || !s || !out_vals || (n && !keys)) return fail(c, SFK_E_ARG, "argsort: bad args");
     if (sizeof(K) != s->key_bytes) return fail(c, SFK_E_ARG, "argsort: scratch key width %u", s->key_bytes);
+    if (sizeof(V) > s->val_bytes) return fail(c, SFK_E_ARG, "argsort: scratch value width %u", s->val_bytes);
     if (bits > 8 * sizeof(K)) return fail(c, SFK_E_ARG, "argsort: %u bits on %zu-byte keys", bits, sizeof(K));
     if (nseg == 0 || nseg > 256 || (nseg > 1 && d_n))
         return fail(c, SFK_E_ARG, "argsort: %u segments", nseg);
@@ -340,19 +350,19 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const u
     // n <= 1 or bits == 0: nothing moves (sort.rs:319-321).
     if ((n <= 1 || bits == 0) && vals) {
         if (out_keys) *out_keys = const_cast<K*>(keys);
-        *out_vals = const_cast<uint32_t*>(vals);
+        *out_vals = const_cast<V*>(vals);
         return SFK_OK;
     }
     if (n == 0) {
         if (out_keys) *out_keys = const_cast<K*>(keys);
-        *out_vals = s->vals[0];
+        *out_vals = static_cast<V*>(static_cast<void*>(s->vals[0]));
         return SFK_OK;
     }
     // Without values the identity permutation is wanted: a zero-width pass
     // is a stable copy that materializes it.
     if (n <= 1 || bits == 0) bits = 0;
     const K* cur_k = keys;
-    const uint32_t* cur_v = vals;
+    const V* cur_v = vals;
     int dst = 0;
     // As few passes as digits of at most digit_bits allow, with the bits
     // spread evenly over them (13 bits: 7 + 6, not 8 + 5): fewer bins per
@@ -376,19 +386,29 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const u
         a.vals_in = cur_v;
         a.keys_out = (!last || write_keys) ? s->keys[dst] : nullptr;
         a.vals_out = s->vals[dst];
+        a.payload = payload;
         a.shift = shift;
         a.width = width;
         SFK_PROF(c, sizeof(K) == 4 ? "radix_pass_u32" : "radix_pass_u64");
-        hipError_t e = launch_radix_pass(c->stream, a, sizeof(K), wlo + (wextra ? 1u : 0u));
+        hipError_t e = launch_radix_pass(c->stream, a, sizeof(K), sizeof(V),
+                                         wlo + (wextra ? 1u : 0u));
         if (e != hipSuccess) return hip_fail(c, e, "radix pass");
         shift += width;
         cur_k = static_cast<K*>(s->keys[dst]);
-        cur_v = a.vals_out;
+        cur_v = static_cast<const V*>(a.vals_out);
         dst ^= 1;
     }
     if (out_keys) *out_keys = const_cast<K*>(cur_k);
-    *out_vals = const_cast<uint32_t*>(cur_v);
+    *out_vals = const_cast<V*>(cur_v);
     return SFK_OK;
+}
+
+int sfk::argsort_payload_u32(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys,
+                             const uint32_t* payload, uint32_t n, uint32_t nseg, uint32_t bits,
+                             uint2** out) {
+    if (!payload) return fail(c, SFK_E_ARG, "argsort: no payload");
+    return argsort_impl<uint32_t, uint2>(c, s, keys, nullptr, n, nullptr, bits, 0, nullptr, out,
+                                         nseg, payload);
 }
 
 extern "C" int sfk_argsort_u32(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys,

@@ -42,6 +42,7 @@ __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restric
                                                        float4* __restrict__ proj,
                                                        uint32_t* __restrict__ tile_counts,
                                                        uint2* __restrict__ bbox,
+                                                       uint32_t* __restrict__ cbbox,
                                                        uint32_t* __restrict__ counters) {
     __shared__ sfk_view s_views[kMaxViews];
     for (uint32_t k = threadIdx.x; k < nviews * (sizeof(sfk_view) / 4); k += 256)
@@ -64,10 +65,12 @@ __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restric
                 cnt = o.count;
                 proj[gi * 2] = make_float4(o.px, o.py, o.qa, o.qb);
                 proj[gi * 2 + 1] = make_float4(o.qc, s.opacity, 0.0f, 0.0f);
-                bbox[gi] = make_uint2(o.bb_lo, o.bb_hi);
+                if (!cbbox) bbox[gi] = make_uint2(o.bb_lo, o.bb_hi);
             }
             depth_keys[gi] = key;
-            tile_counts[gi] = cnt;
+            // Compact grids: the bbox word is all the emission needs.
+            if (cbbox) cbbox[gi] = vis ? compact_bbox(o) : 0u;
+            else tile_counts[gi] = cnt;
         }
         block_count_pair(cnt, vis, counters + 2 + (size_t)v * 2 * kCountSlots);
     }
@@ -150,6 +153,8 @@ struct sfk_eig_round {
     uint32_t n = 0, nviews = 0, tile_w = 16, digit_bits = 8;
     uint32_t total_tiles = 0, capacity = 0;
     uint32_t max_tiles = 0, tile_bits = 0;  // largest view's tiles, bits_for of it
+    bool compact = true;                     // every view's grid fits the compact bbox
+    uint32_t* cbbox = nullptr;
     std::vector<sfk_view> views;
     std::vector<BatchView> bviews;
     sfk_view* d_views = nullptr;
@@ -214,19 +219,30 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
         SFK_PROF(c, "batch_project");
         k_project_batch<<<div_up(n, 256), 256, 0, c->stream>>>(
             r->d_views, V, attrs, n, 1.0f / (float)r->tile_w, r->keys, r->proj, r->counts,
-            r->bbox, r->counters);
+            r->bbox, r->compact ? r->cbbox : nullptr, r->counters);
         SFK_LAUNCHED(c, "batch project");
     }
-    uint32_t* order = nullptr;
-    int rc = sfk_argsort_u32_segmented(c, r->sort_depth, r->keys, nullptr, V, n, 32, 0, nullptr,
+    EmitSource src;
+    int rc;
+    if (r->compact) {
+        uint2* ordered = nullptr;
+        rc = argsort_payload_u32(c, r->sort_depth, r->keys, r->cbbox, n, V, 32, &ordered);
+        if (rc) return rc;
+        src.ordered = ordered;
+    } else {
+        uint32_t* order = nullptr;
+        rc = sfk_argsort_u32_segmented(c, r->sort_depth, r->keys, nullptr, V, n, 32, 0, nullptr,
                                        &order);
-    if (rc) return rc;
+        if (rc) return rc;
+        src.order = order;
+        src.tile_counts = r->counts;
+        src.bbox = reinterpret_cast<const uint32_t*>(r->bbox);
+    }
     {
         SFK_PROF(c, "scan_emit");
-        SFK_HIP(c, launch_scan_emit(c->stream, order, r->counts,
-                                    reinterpret_cast<const uint32_t*>(r->bbox), (uint32_t)vn, 0,
-                                    r->capacity, r->tile_ids, r->gids, nullptr, r->counters,
-                                    r->emit_status, n, r->d_bviews, r->tile_bits));
+        SFK_HIP(c, launch_scan_emit(c->stream, src, (uint32_t)vn, 0, r->capacity, r->tile_ids,
+                                    r->gids, nullptr, r->counters, r->emit_status, n,
+                                    r->d_bviews, r->tile_bits));
     }
     uint32_t* st = nullptr;
     uint32_t* sg = nullptr;
@@ -315,6 +331,7 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
         bv.scale = (float)(1ull << sfk_fractional_bits(widths[v], heights[v]));
         tiles += (uint64_t)bv.tiles_x * bv.tiles_y;
         r->max_tiles = std::max(r->max_tiles, bv.tiles_x * bv.tiles_y);
+        if (bv.tiles_x > kCompactGrid || bv.tiles_y > kCompactGrid) r->compact = false;
         cap0 += std::min<uint64_t>((uint64_t)bv.tiles_x * bv.tiles_y * n, 1u << 22);
         r->bviews.push_back(bv);
     }
@@ -338,8 +355,12 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     RALLOC(r->d_bviews, nviews * sizeof(BatchView));
     RALLOC(r->proj, vn * 32);
     RALLOC(r->keys, vn * 4);
-    RALLOC(r->counts, vn * 4);
-    RALLOC(r->bbox, vn * 8);
+    if (r->compact) {
+        RALLOC(r->cbbox, vn * 4);
+    } else {
+        RALLOC(r->counts, vn * 4);
+        RALLOC(r->bbox, vn * 8);
+    }
     RALLOC(r->counters, (2 + (size_t)nviews * 2 * kCountSlots) * 4);
     RALLOC(r->starts, ((size_t)r->max_tiles * nviews + 1) * 4);
     RALLOC(r->resp, vn * 4);
@@ -358,7 +379,7 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
                            hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail_out(hip_fail(c, e, "eig round upload"));
-    int rc = sfk_radix_scratch_create(c, (uint32_t)vn, 4, &r->sort_depth);
+    int rc = radix_scratch_create_ex(c, (uint32_t)vn, 4, r->compact ? 8 : 4, &r->sort_depth);
     if (rc) return fail_out(rc);
     sfk_radix_set_digit_bits(r->sort_depth, digit_bits);
     rc = round_alloc_isects(r, next_pow2(cap0 ? cap0 : 1));
@@ -372,7 +393,7 @@ extern "C" void sfk_eig_round_destroy(sfk_eig_round* r) {
     if (r->ctx) hipStreamSynchronize(r->ctx->stream);
     if (r->exec) hipGraphExecDestroy(r->exec);
     if (r->graph) hipGraphDestroy(r->graph);
-    void* bufs[] = {r->d_views, r->d_bviews, r->proj, r->keys, r->counts, r->bbox,
+    void* bufs[] = {r->d_views, r->d_bviews, r->proj, r->keys, r->counts, r->bbox, r->cbbox,
                     r->counters, r->starts, r->resp, r->h0, r->partials, r->eig, r->best,
                     r->emit_status};
     for (void* p : bufs)

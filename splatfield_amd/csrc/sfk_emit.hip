@@ -36,6 +36,8 @@ struct EmitArgs {
     const uint32_t* order;        // depth rank -> splat (in-view index when batched)
     const uint32_t* tile_counts;  // per splat (view-major when batched)
     const uint2* bbox;            // packed tile bbox per splat
+    const uint2* ordered;         // or: (splat, compact bbox) per depth rank
+    uint32_t* order_out;          // with `ordered`: depth order (optional)
     uint32_t n;                   // ranks
     uint32_t tiles_per_row;
     uint32_t max_isects;
@@ -72,11 +74,21 @@ __global__ __launch_bounds__(kEmitThreads) void k_scan_emit(EmitArgs a) {
         uint32_t c = 0, g = 0;
         uint2 bb = make_uint2(0, 0);
         if (r < a.n) {
-            g = a.order[r];
-            size_t gi = g;
-            if (a.seg_len) gi = (size_t)(r / a.seg_len) * a.seg_len + g;
-            c = a.tile_counts[gi];
-            if (c) bb = a.bbox[gi];
+            if (a.ordered) {
+                const uint2 oc = a.ordered[r];
+                g = oc.x;
+                const uint32_t min_x = oc.y & 0xffu, min_y = (oc.y >> 8) & 0xffu,
+                               max_x = (oc.y >> 16) & 0xffu, max_y = oc.y >> 24;
+                c = (max_x - min_x) * (max_y - min_y);
+                bb = make_uint2(min_x | (min_y << 16), max_x | (max_y << 16));
+                if (a.order_out) a.order_out[r] = g;
+            } else {
+                g = a.order[r];
+                size_t gi = g;
+                if (a.seg_len) gi = (size_t)(r / a.seg_len) * a.seg_len + g;
+                c = a.tile_counts[gi];
+                if (c) bb = a.bbox[gi];
+            }
         }
         cnt[k] = c;
         sum += c;
@@ -170,19 +182,21 @@ __global__ __launch_bounds__(kEmitThreads) void k_scan_emit(EmitArgs a) {
 
 uint32_t scan_emit_status_words(uint32_t n) { return 2u + 2u * div_up(n ? n : 1, kEmitRanks); }
 
-hipError_t launch_scan_emit(hipStream_t s, const uint32_t* order, const uint32_t* tile_counts,
-                            const uint32_t* bbox, uint32_t n, uint32_t tiles_per_row,
-                            uint32_t max_isects, uint32_t* tile_ids, uint32_t* gaussian_ids,
-                            uint32_t* offsets, uint32_t* total_out, uint32_t* status_words,
-                            uint32_t seg_len, const BatchView* views, uint32_t view_shift) {
+hipError_t launch_scan_emit(hipStream_t s, const EmitSource& src, uint32_t n,
+                            uint32_t tiles_per_row, uint32_t max_isects, uint32_t* tile_ids,
+                            uint32_t* gaussian_ids, uint32_t* offsets, uint32_t* total_out,
+                            uint32_t* status_words, uint32_t seg_len, const BatchView* views,
+                            uint32_t view_shift) {
     if (n == 0) return hipSuccess;
     const uint32_t nb = div_up(n, kEmitRanks);
     hipError_t e = launch_fill_u32(s, status_words, 0u, scan_emit_status_words(n));
     if (e != hipSuccess) return e;
     EmitArgs a;
-    a.order = order;
-    a.tile_counts = tile_counts;
-    a.bbox = reinterpret_cast<const uint2*>(bbox);
+    a.order = src.order;
+    a.tile_counts = src.tile_counts;
+    a.bbox = reinterpret_cast<const uint2*>(src.bbox);
+    a.ordered = src.ordered;
+    a.order_out = src.order_out;
     a.n = n;
     a.tiles_per_row = tiles_per_row;
     a.max_isects = max_isects;

@@ -93,7 +93,8 @@ hipError_t launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, si
 
 hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, const float* sh,
                           uint32_t n, uint32_t sh_per_ch, uint32_t tile_w, uint32_t* depth_keys,
-                          float* proj, uint32_t* tile_counts, uint32_t* bbox, uint32_t* counters);
+                          float* proj, uint32_t* tile_counts, uint32_t* bbox, uint32_t* counters,
+                          uint32_t* cbbox = nullptr);
 
 // Radix sort: one pass = upsweep + digit scan + scatter.
 constexpr uint32_t kSortThreads = 256;
@@ -102,9 +103,10 @@ constexpr uint32_t kSortBlock = kSortThreads * kSortEpt;  // keys per block
 uint32_t radix_max_blocks(uint32_t max_elems);
 struct SortArgs {
     const void* keys_in;
-    const uint32_t* vals_in;  // nullptr: identity (index within the segment)
+    const void* vals_in;      // nullptr: identity (index within the segment)
+    const uint32_t* payload;  // 8-byte identity values: (index, payload[index])
     void* keys_out;           // nullptr: keys not written
-    uint32_t* vals_out;
+    void* vals_out;
     uint32_t n_max;           // one segment: element count (or its cap with d_n)
     const uint32_t* d_n;      // one segment: device-side count, may be nullptr
     uint32_t nseg, seg_len;   // nseg > 1: segments of seg_len keys each
@@ -114,7 +116,15 @@ struct SortArgs {
     uint32_t* totals;         // nseg * RADIX
 };
 hipError_t launch_radix_pass(hipStream_t s, const SortArgs& a, uint32_t key_bytes,
-                             uint32_t digit_bits);
+                             uint32_t val_bytes, uint32_t digit_bits);
+// Radix scratch with 8-byte values (u32 keys only), and the argsort of u32
+// keys whose identity values carry a u32 payload: out[r] = (index, payload[index])
+// of the element of rank r within its segment.
+int radix_scratch_create_ex(sfk_ctx* c, uint32_t max_elems, uint32_t key_bytes,
+                            uint32_t val_bytes, sfk_radix_scratch** out);
+int argsort_payload_u32(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys,
+                        const uint32_t* payload, uint32_t n, uint32_t nseg, uint32_t bits,
+                        uint2** out);
 
 // Scan.
 constexpr uint32_t kScanBlock = 4096;
@@ -127,12 +137,23 @@ hipError_t launch_scan(hipStream_t s, const uint32_t* gids, const uint32_t* coun
 // the launch itself with a fill kernel).
 struct BatchView;
 uint32_t scan_emit_status_words(uint32_t n);
-hipError_t launch_scan_emit(hipStream_t s, const uint32_t* order, const uint32_t* tile_counts,
-                            const uint32_t* bbox, uint32_t n, uint32_t tiles_per_row,
-                            uint32_t max_isects, uint32_t* tile_ids, uint32_t* gaussian_ids,
-                            uint32_t* offsets, uint32_t* total_out, uint32_t* status_words,
-                            uint32_t seg_len = 0, const BatchView* views = nullptr,
-                            uint32_t view_shift = 0);
+// Largest tile grid side whose bboxes fit the 8-bit compact form.
+constexpr uint32_t kCompactGrid = 255;
+// Either order + tile_counts + bbox (gathered per rank), or `ordered`:
+// (splat, compact bbox) per rank straight from the payload depth sort, in
+// which case order_out (optional) receives the depth order.
+struct EmitSource {
+    const uint32_t* order = nullptr;
+    const uint32_t* tile_counts = nullptr;
+    const uint32_t* bbox = nullptr;
+    const uint2* ordered = nullptr;
+    uint32_t* order_out = nullptr;
+};
+hipError_t launch_scan_emit(hipStream_t s, const EmitSource& src, uint32_t n,
+                            uint32_t tiles_per_row, uint32_t max_isects, uint32_t* tile_ids,
+                            uint32_t* gaussian_ids, uint32_t* offsets, uint32_t* total_out,
+                            uint32_t* status_words, uint32_t seg_len = 0,
+                            const BatchView* views = nullptr, uint32_t view_shift = 0);
 
 // Per-view constants of a batched round (device array, one per view).
 struct BatchView {

@@ -78,10 +78,11 @@ __global__ __launch_bounds__(256) void k_project(sfk_view v, const float* __rest
                                                  float4* __restrict__ proj,
                                                  uint32_t* __restrict__ tile_counts,
                                                  uint2* __restrict__ bbox,
+                                                 uint32_t* __restrict__ cbbox,
                                                  uint32_t* __restrict__ counters) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     bool vis = false;
-    uint32_t cnt = 0;
+    uint32_t cnt = 0, cb = 0;
     if (i < n) {
         SplatIn s = load_splat(A, i, n);
         ProjGeom o;
@@ -102,9 +103,11 @@ __global__ __launch_bounds__(256) void k_project(sfk_view v, const float* __rest
             row[1] = make_float4(o.qc, s.opacity, cr + 0.5f, cg + 0.5f);
             row[2] = make_float4(cbl + 0.5f, 0.0f, 0.0f, 0.0f);
             bbox[i] = make_uint2(o.bb_lo, o.bb_hi);
+            cb = compact_bbox(o);
         }
         depth_keys[i] = key;
         tile_counts[i] = cnt;
+        if (cbbox) cbbox[i] = cb;
     }
     block_count_pair(cnt, vis, counters);
 }
@@ -113,7 +116,8 @@ __global__ __launch_bounds__(256) void k_project(sfk_view v, const float* __rest
 
 hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, const float* sh,
                           uint32_t n, uint32_t sh_per_ch, uint32_t tile_w, uint32_t* depth_keys,
-                          float* proj, uint32_t* tile_counts, uint32_t* bbox, uint32_t* counters) {
+                          float* proj, uint32_t* tile_counts, uint32_t* bbox, uint32_t* counters,
+                          uint32_t* cbbox) {
     if (n == 0) return hipSuccess;
     dim3 grid(div_up(n, 256)), block(256);
     float inv_tile = 1.0f / (float)tile_w;
@@ -122,23 +126,23 @@ hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, 
     switch (sh_per_ch) {
         case 0:
             k_project<0><<<grid, block, 0, s>>>(v, attrs, sh, n, inv_tile, depth_keys, p4,
-                                                 tile_counts, bb, counters);
+                                                 tile_counts, bb, cbbox, counters);
             break;
         case 1:
             k_project<1><<<grid, block, 0, s>>>(v, attrs, sh, n, inv_tile, depth_keys, p4,
-                                                 tile_counts, bb, counters);
+                                                 tile_counts, bb, cbbox, counters);
             break;
         case 4:
             k_project<4><<<grid, block, 0, s>>>(v, attrs, sh, n, inv_tile, depth_keys, p4,
-                                                 tile_counts, bb, counters);
+                                                 tile_counts, bb, cbbox, counters);
             break;
         case 9:
             k_project<9><<<grid, block, 0, s>>>(v, attrs, sh, n, inv_tile, depth_keys, p4,
-                                                 tile_counts, bb, counters);
+                                                 tile_counts, bb, cbbox, counters);
             break;
         case 16:
             k_project<16><<<grid, block, 0, s>>>(v, attrs, sh, n, inv_tile, depth_keys, p4,
-                                                  tile_counts, bb, counters);
+                                                  tile_counts, bb, cbbox, counters);
             break;
         default:
             return hipErrorInvalidValue;

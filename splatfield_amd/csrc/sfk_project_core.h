@@ -109,6 +109,15 @@ __device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s
     return true;
 }
 
+// Tile bbox in one word, 8 bits per bound (min_x, min_y, max_x, max_y from
+// the low byte up), for grids of at most 255 x 255 tiles; 0 for a culled
+// splat (zero tiles).  It rides through the depth sort as the payload of
+// the identity values, so the emission reads it in depth order.
+__device__ __forceinline__ uint32_t compact_bbox(const ProjGeom& o) {
+    return (o.bb_lo & 0xffu) | ((o.bb_lo >> 16) << 8) | ((o.bb_hi & 0xffu) << 16) |
+           ((o.bb_hi >> 16) << 24);
+}
+
 // Informational per-frame counts (isects, visible): reduced across the
 // 256-thread workgroup, then added to one of kCountSlots slot pairs chosen
 // by workgroup id.  One hot address for every workgroup serializes in L2

@@ -108,6 +108,8 @@ struct sfk_render_scratch {
     sfk_radix_scratch* sort_depth = nullptr;
     sfk_radix_scratch* sort_tile = nullptr;
     uint32_t* emit_status = nullptr;  // look-back words of the fused scan/emission
+    uint32_t* cbbox = nullptr;        // compact tile bbox per splat (depth-sort payload)
+    uint32_t* order_buf = nullptr;    // depth order written by the emission
     // Previous frame's raw emission, read back asynchronously for growth.
     uint32_t* host_counters = nullptr;
     hipEvent_t counters_ready = nullptr;
@@ -127,7 +129,8 @@ struct sfk_render_scratch {
 static void scratch_free_frame(sfk_render_scratch* s) {
     void* bufs[] = {s->depth_keys, s->projected, s->tile_counts, s->bbox, s->offsets,
                     s->counters, s->tile_starts, s->bitmap, s->rgba, s->final_t, s->n_contrib,
-                    s->tile_ids, s->gaussian_ids, s->grad2d, s->emit_status};
+                    s->tile_ids, s->gaussian_ids, s->grad2d, s->emit_status, s->cbbox,
+                    s->order_buf};
     for (void* p : bufs)
         if (p) hipFree(p);
     s->depth_keys = nullptr; s->projected = nullptr; s->tile_counts = nullptr; s->bbox = nullptr;
@@ -136,6 +139,7 @@ static void scratch_free_frame(sfk_render_scratch* s) {
     s->gaussian_ids = nullptr;
     s->grad2d = nullptr;
     s->emit_status = nullptr;
+    s->cbbox = s->order_buf = nullptr;
     s->has_view = false;
     sfk_radix_scratch_destroy(s->sort_depth);
     sfk_radix_scratch_destroy(s->sort_tile);
@@ -181,7 +185,9 @@ static int scratch_build(sfk_render_scratch* s, uint32_t n, uint32_t w, uint32_t
     SFK_HIP(c, hipMalloc(&s->rgba, px * 16));
     SFK_HIP(c, hipMalloc(&s->final_t, px * 4));
     SFK_HIP(c, hipMalloc(&s->n_contrib, px * 4));
-    int rc = sfk_radix_scratch_create(c, n, 4, &s->sort_depth);
+    SFK_HIP(c, hipMalloc(&s->cbbox, nn * 4));
+    SFK_HIP(c, hipMalloc(&s->order_buf, nn * 4));
+    int rc = radix_scratch_create_ex(c, n, 4, 8, &s->sort_depth);
     if (rc) return rc;
     sfk_radix_set_digit_bits(s->sort_depth, s->digit_bits);
     SFK_HIP(c, hipMalloc(&s->emit_status, (size_t)scan_emit_status_words(n) * 4));
@@ -326,28 +332,45 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
         return fail(c, SFK_E_ARG, "view tile bounds do not match the scratch tile width %u", s->tile_w);
     const uint32_t n = sp->n;
     const uint32_t tiles = s->tiles_x * s->tiles_y;
+    const bool compact = s->tiles_x <= kCompactGrid && s->tiles_y <= kCompactGrid;
 
     SFK_HIP(c, launch_fill_u32(c->stream, s->counters, 0u, kCounterWords));
     {
         SFK_PROF(c, "project");
         SFK_HIP(c, launch_project(c->stream, *view, sp->attrs, sp->sh, n,
                                   colors ? sp->sh_per_ch : 0u, s->tile_w, s->depth_keys,
-                                  s->projected, s->tile_counts, s->bbox, s->counters + 2));
+                                  s->projected, s->tile_counts, s->bbox, s->counters + 2,
+                                  compact ? s->cbbox : nullptr));
         s->has_colors = colors;
     }
 
-    uint32_t* order = nullptr;
-    rc = sfk_argsort_u32(c, s->sort_depth, s->depth_keys, nullptr, n, nullptr, 32, 0, nullptr,
-                         &order);
-    if (rc) return rc;
-    s->depth_order = order;
+    // Depth order.  On grids of at most 255 x 255 tiles the compact bbox
+    // rides through the sort, and the emission reads (splat, bbox) in depth
+    // order instead of gathering counts and boxes per rank.
+    EmitSource src;
+    if (compact) {
+        uint2* ordered = nullptr;
+        rc = argsort_payload_u32(c, s->sort_depth, s->depth_keys, s->cbbox, n, 1, 32, &ordered);
+        if (rc) return rc;
+        src.ordered = ordered;
+        src.order_out = s->order_buf;
+        s->depth_order = s->order_buf;
+    } else {
+        uint32_t* order = nullptr;
+        rc = sfk_argsort_u32(c, s->sort_depth, s->depth_keys, nullptr, n, nullptr, 32, 0,
+                             nullptr, &order);
+        if (rc) return rc;
+        src.order = order;
+        src.tile_counts = s->tile_counts;
+        src.bbox = s->bbox;
+        s->depth_order = order;
+    }
     {
         // Emission offsets in depth order and the isects themselves, in one
         // pass; the grand total is the raw isect count (counters[0]).
         SFK_PROF(c, "scan_emit");
-        SFK_HIP(c, launch_scan_emit(c->stream, order, s->tile_counts, s->bbox, n, s->tiles_x,
-                                    s->capacity, s->tile_ids, s->gaussian_ids, s->offsets,
-                                    s->counters, s->emit_status));
+        SFK_HIP(c, launch_scan_emit(c->stream, src, n, s->tiles_x, s->capacity, s->tile_ids,
+                                    s->gaussian_ids, s->offsets, s->counters, s->emit_status));
     }
     SFK_HIP(c, hipMemcpyAsync(s->host_counters, s->counters, 4, hipMemcpyDeviceToHost, c->stream));
     SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));

@@ -19,7 +19,9 @@
 // (the batched multi-view depth sort).  With one segment the element count
 // may come from device memory (min(*d_n, n)): blocks past the live count exit
 // at once, so a capacity-sized grid needs no host sync.  Identity values
-// (vals_in == nullptr) are indices within the segment.
+// (vals_in == nullptr) are indices within the segment; with 8-byte values
+// an identity value is (index, payload[index]), so per-element data rides
+// along and comes out in sorted order without a gather.
 #include "sfk_device.h"
 #include "sfk_internal.h"
 
@@ -136,10 +138,22 @@ __global__ __launch_bounds__(NT) void k_digscan(SortArgs a) {
     if (threadIdx.x == 0) a.totals[(size_t)s * RADIX + d] = carry;
 }
 
-template <typename K, int RADIX>
+template <typename V>
+__device__ __forceinline__ V identity_val(uint32_t li, const uint32_t* payload, size_t gi);
+template <>
+__device__ __forceinline__ uint32_t identity_val<uint32_t>(uint32_t li, const uint32_t*, size_t) {
+    return li;
+}
+template <>
+__device__ __forceinline__ uint2 identity_val<uint2>(uint32_t li, const uint32_t* payload,
+                                                     size_t gi) {
+    return make_uint2(li, payload[gi]);
+}
+
+template <typename K, typename V, int RADIX>
 __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
     __shared__ K s_keys[KPB];
-    __shared__ uint32_t s_vals[KPB];
+    __shared__ V s_vals[KPB];
     __shared__ uint32_t s_wh[NW][RADIX];  // per-wave digit counters, then bases
     __shared__ uint32_t s_gofs[RADIX];    // global position minus local position
     __shared__ uint32_t s_tmp[NW + 1];
@@ -170,7 +184,9 @@ __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
     // (the scan's barriers also order the s_wh clear)
 
     K key[EPT];
-    uint32_t val[EPT];
+    V val[EPT];
+    const V* vals_in = static_cast<const V*>(a.vals_in);
+    V* vals_out = static_cast<V*>(a.vals_out);
     uint32_t loc[EPT];
     const uint32_t wbase = blk0 + w * 64 * EPT + lane;
 #pragma unroll
@@ -178,7 +194,13 @@ __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
         uint32_t li = wbase + e * 64;
         bool valid = li < p.seg_n;
         key[e] = valid ? keys_in[p.seg_base + li] : K(0);
-        val[e] = valid ? (a.vals_in ? a.vals_in[p.seg_base + li] : li) : 0u;
+        if constexpr (sizeof(V) == 4) {
+            val[e] = valid ? (vals_in ? vals_in[p.seg_base + li] : li) : 0u;
+        } else {
+            val[e] = valid ? (vals_in ? vals_in[p.seg_base + li]
+                                      : identity_val<V>(li, a.payload, p.seg_base + li))
+                           : V{};
+        }
     }
 
 #pragma unroll
@@ -243,18 +265,18 @@ __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
             uint32_t d = digit_of(k, shift, mask);
             size_t dst = (size_t)(s_gofs[d] + s);
             if (keys_out) keys_out[dst] = k;
-            a.vals_out[dst] = s_vals[s];
+            vals_out[dst] = s_vals[s];
         }
     }
 }
 
-template <typename K, int RADIX>
+template <typename K, typename V, int RADIX>
 hipError_t pass_impl(hipStream_t st, const SortArgs& a) {
     uint32_t grid = a.nseg <= 1 ? div_up(a.n_max, KPB) : a.nseg * div_up(a.seg_len, KPB);
     if (grid == 0) return hipSuccess;
     k_upsweep<K, RADIX><<<grid, NT, 0, st>>>(a);
     k_digscan<RADIX><<<(a.nseg <= 1 ? 1u : a.nseg) << a.width, NT, 0, st>>>(a);
-    k_scatter<K, RADIX><<<grid, NT, 0, st>>>(a);
+    k_scatter<K, V, RADIX><<<grid, NT, 0, st>>>(a);
     return hipGetLastError();
 }
 
@@ -262,20 +284,21 @@ hipError_t pass_impl(hipStream_t st, const SortArgs& a) {
 
 uint32_t radix_max_blocks(uint32_t max_elems) { return div_up(max_elems > 0 ? max_elems : 1, KPB); }
 
+template <typename K, typename V>
+hipError_t pass_radix(hipStream_t st, const SortArgs& a, uint32_t digit_bits) {
+    if (digit_bits <= 4) return pass_impl<K, V, 16>(st, a);
+    if (digit_bits <= 5) return pass_impl<K, V, 32>(st, a);
+    if (digit_bits <= 6) return pass_impl<K, V, 64>(st, a);
+    if (digit_bits <= 7) return pass_impl<K, V, 128>(st, a);
+    return pass_impl<K, V, 256>(st, a);
+}
+
 hipError_t launch_radix_pass(hipStream_t st, const SortArgs& a, uint32_t key_bytes,
-                             uint32_t digit_bits) {
-    if (key_bytes == 4) {
-        if (digit_bits <= 4) return pass_impl<uint32_t, 16>(st, a);
-        if (digit_bits <= 5) return pass_impl<uint32_t, 32>(st, a);
-        if (digit_bits <= 6) return pass_impl<uint32_t, 64>(st, a);
-        if (digit_bits <= 7) return pass_impl<uint32_t, 128>(st, a);
-        return pass_impl<uint32_t, 256>(st, a);
-    }
-    if (digit_bits <= 4) return pass_impl<uint64_t, 16>(st, a);
-    if (digit_bits <= 5) return pass_impl<uint64_t, 32>(st, a);
-    if (digit_bits <= 6) return pass_impl<uint64_t, 64>(st, a);
-    if (digit_bits <= 7) return pass_impl<uint64_t, 128>(st, a);
-    return pass_impl<uint64_t, 256>(st, a);
+                             uint32_t val_bytes, uint32_t digit_bits) {
+    if (key_bytes == 4)
+        return val_bytes == 8 ? pass_radix<uint32_t, uint2>(st, a, digit_bits)
+                              : pass_radix<uint32_t, uint32_t>(st, a, digit_bits);
+    return pass_radix<uint64_t, uint32_t>(st, a, digit_bits);
 }
 
 }  // namespace sfk
