@@ -7,39 +7,48 @@
 
 namespace sfk {
 
-// ln Gamma, Lanczos g = 7, 9 terms (beta.rs:55-70); x >= 1 here.
-__device__ __forceinline__ double d_lgamma(double x) {
-    const double z = x - 1.0;
-    double series = 0.9999999999998099;
-    series += 676.5203681218851 / (z + 1.0);
-    series += -1259.1392167224028 / (z + 2.0);
-    series += 771.3234287776531 / (z + 3.0);
-    series += -176.6150291621406 / (z + 4.0);
-    series += 12.507343278686905 / (z + 5.0);
-    series += -0.13857109526572012 / (z + 6.0);
-    series += 9.984369578019572e-6 / (z + 7.0);
-    series += 1.5056327351493116e-7 / (z + 8.0);
-    const double t = z + 7.5;
-    return 0.91893853320467274178 + (z + 0.5) * log(t) - t + log(series);
-}
-
-// digamma: recurrence to x >= 6, then the asymptotic series (beta.rs:75-86).
-__device__ __forceinline__ double d_digamma(double x) {
-    double r = 0.0;
-    while (x < 6.0) {
-        r -= 1.0 / x;
-        x += 1.0;
+// ln Gamma(x) and digamma(x) together, for x >= 1, in f64.  The reference
+// evaluates them separately (Lanczos g = 7 and a recurrence to x >= 6 with a
+// short asymptotic series, beta.rs:55-86) to fill its f32 table; here both
+// share one shift to z = x + 8 >= 9:
+//   ln Gamma(x) = ln Gamma(z) - ln P,   psi(x) = psi(z) - P'/P,
+//   P = prod_{i<8} (x + i)  (P and P' by one product recurrence),
+// then Stirling's series for ln Gamma(z) and the asymptotic series for
+// psi(z), each to the 1/z^9 (resp. 1/z^10) Bernoulli term: truncation below
+// 1e-13 at z = 9.  Two logs and two divisions per argument, no data-dependent
+// loop (no divergence across the wave).
+struct LgPsi {
+    double lg, psi;
+};
+__device__ __forceinline__ LgPsi d_lgamma_digamma(double x) {
+    double p = 1.0, dp = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const double t = x + (double)i;
+        dp = dp * t + p;
+        p *= t;
     }
-    const double inv = 1.0 / x, inv2 = inv * inv;
-    return r + log(x) - 0.5 * inv - inv2 / 12.0 + inv2 * inv2 / 120.0 -
-           inv2 * inv2 * inv2 / 252.0;
+    const double z = x + 8.0;
+    const double iz = 1.0 / z, iz2 = iz * iz;
+    const double lz = log(z);
+    const double st =
+        iz * (1.0 / 12.0 +
+              iz2 * (-1.0 / 360.0 + iz2 * (1.0 / 1260.0 + iz2 * (-1.0 / 1680.0 + iz2 * (1.0 / 1188.0)))));
+    const double ps =
+        iz2 * (1.0 / 12.0 +
+               iz2 * (-1.0 / 120.0 + iz2 * (1.0 / 252.0 + iz2 * (-1.0 / 240.0 + iz2 * (1.0 / 132.0)))));
+    LgPsi r;
+    r.lg = (z - 0.5) * lz - z + 0.91893853320467274178 + st - log(p);
+    r.psi = lz - 0.5 * iz - ps - dp / p;
+    return r;
 }
 
-// H(a, b) of Beta(a, b) (beta.rs:88-96).
+// H(a, b) of Beta(a, b) (beta.rs:88-96):
+// ln B(a, b) - (a-1) psi(a) - (b-1) psi(b) + (a+b-2) psi(a+b).
 __device__ __forceinline__ double d_beta_entropy(double a, double b) {
     const double s = a + b;
-    return d_lgamma(a) + d_lgamma(b) - d_lgamma(s) - (a - 1.0) * d_digamma(a) -
-           (b - 1.0) * d_digamma(b) + (s - 2.0) * d_digamma(s);
+    const LgPsi la = d_lgamma_digamma(a), lb = d_lgamma_digamma(b), ls = d_lgamma_digamma(s);
+    return la.lg + lb.lg - ls.lg - (a - 1.0) * la.psi - (b - 1.0) * lb.psi + (s - 2.0) * ls.psi;
 }
 
 // max(0, H0 - H(a k, b k)), k = 1 + eps/(a+b), eps = f32(q)/scale; 0 when
