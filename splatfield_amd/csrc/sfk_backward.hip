@@ -137,47 +137,51 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
             bool has = false;
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
+                // Member s, or (masked off) the group's top entry again.
+                const bool member = s < (int)itop;
+                const uint32_t j = list[member ? itop - 1 - s : itop - 1];
+                const float4 geo = s_geo[j];
+                const float4 aux = s_aux[j];
+                const float dx = geo.x - pxf, dy = geo.y - pyf;
+                const float power =
+                    0.5f * (geo.z * dx * dx + aux.x * dy * dy) + geo.w * dx * dy;
+                const bool in = member && cbeg + j < last && !(power > kPowerSkip);
+                if (!__ballot(in)) {
 #pragma unroll
-                for (int k = 0; k < 9; ++k) g[k][s] = 0.0f;
-                if (s >= (int)itop) continue;
-                const uint32_t j = list[itop - 1 - s];
-                if (cbeg + j < last) {
-                    float4 geo = s_geo[j];
-                    float4 aux = s_aux[j];
-                    float dx = geo.x - pxf, dy = geo.y - pyf;
-                    float power = 0.5f * (geo.z * dx * dx + aux.x * dy * dy) + geo.w * dx * dy;
-                    if (!(power > kPowerSkip)) {
-                        float G = sfk_expf(-power);
-                        float araw = aux.y * G;
-                        float alpha = fminf(araw, 0.999f);
-                        if (alpha >= kInv255) {
-                            has = true;
-                            float om = 1.0f - alpha;
-                            float rom = 1.0f / om;
-                            T = T * rom;
-                            float wgt = alpha * T;
-                            float cb = s_blue[j];
-                            g[6][s] = wgt * dC.x;
-                            g[7][s] = wgt * dC.y;
-                            g[8][s] = wgt * dC.z;
-                            float dal = T * ((aux.z - acc_r) * dC.x + (aux.w - acc_g) * dC.y +
-                                             (cb - acc_b) * dC.z) +
-                                        Tfin * rom * dC.w;
-                            acc_r = alpha * aux.z + om * acc_r;
-                            acc_g = alpha * aux.w + om * acc_g;
-                            acc_b = alpha * cb + om * acc_b;
-                            if (araw < 0.999f) {
-                                float dpow = -dal * alpha;
-                                g[5][s] = dal * G;
-                                g[0][s] = dpow * (geo.z * dx + geo.w * dy);
-                                g[1][s] = dpow * (aux.x * dy + geo.w * dx);
-                                g[2][s] = dpow * 0.5f * dx * dx;
-                                g[3][s] = dpow * dx * dy;
-                                g[4][s] = dpow * 0.5f * dy * dy;
-                            }
-                        }
-                    }
+                    for (int k = 0; k < 9; ++k) g[k][s] = 0.0f;
+                    continue;
                 }
+                // Straight-line evaluation for the whole wave; lanes that do
+                // not contribute keep their state through selects.
+                const float G = sfk_expf_neg(power);
+                const float araw = aux.y * G;
+                const float alpha = fminf(araw, 0.999f);
+                const bool con = in && alpha >= kInv255;
+                has |= con;
+                const float om = 1.0f - alpha;  // in [0.001, 1]
+                const float rom = __builtin_amdgcn_rcpf(om);
+                const float Tn = T * rom;
+                T = con ? Tn : T;
+                const float wgt = alpha * Tn;
+                const float cb = s_blue[j];
+                g[6][s] = con ? wgt * dC.x : 0.0f;
+                g[7][s] = con ? wgt * dC.y : 0.0f;
+                g[8][s] = con ? wgt * dC.z : 0.0f;
+                const float dal = Tn * ((aux.z - acc_r) * dC.x + (aux.w - acc_g) * dC.y +
+                                        (cb - acc_b) * dC.z) +
+                                  Tfin * rom * dC.w;
+                acc_r = con ? alpha * aux.z + om * acc_r : acc_r;
+                acc_g = con ? alpha * aux.w + om * acc_g : acc_g;
+                acc_b = con ? alpha * cb + om * acc_b : acc_b;
+                // The clamp at 0.999 has zero derivative.
+                const bool slope = con && araw < 0.999f;
+                const float dpow = -dal * alpha;
+                g[5][s] = slope ? dal * G : 0.0f;
+                g[0][s] = slope ? dpow * (geo.z * dx + geo.w * dy) : 0.0f;
+                g[1][s] = slope ? dpow * (aux.x * dy + geo.w * dx) : 0.0f;
+                g[2][s] = slope ? dpow * 0.5f * dx * dx : 0.0f;
+                g[3][s] = slope ? dpow * dx * dy : 0.0f;
+                g[4][s] = slope ? dpow * 0.5f * dy * dy : 0.0f;
             }
             if (__ballot(has)) {
                 const uint32_t lane = lane_id();

@@ -40,9 +40,32 @@ SFK_HD float sfk_expf(float x) {
     return p * sfk_pow2i(k);
 }
 
-// exp(-p) for the rasterizer's Gaussian falloff when p >= 0 is known not to
-// underflow past the normal range; identical bits to sfk_expf(-p).
-SFK_HD float sfk_expf_neg(float p) { return sfk_expf(-p); }
+// exp(-p) for the rasterizers' Gaussian falloff, without sfk_expf's range
+// branches.  For -p in [-87, 88] it is the same evaluation step for step
+// (the scale 2^k stays a normal number, so it is built directly): identical
+// bits.  NaN comes out as NaN.  -p above 88 is evaluated at 88: exp(88) =
+// 1.65e38, so min(o * G, 0.999) = 0.999 exactly as with the true value for
+// any opacity above 1e-35, and o * G >= 0.999 keeps every caller on the same
+// side of its saturation test.  The callers skip powers above 5.6, so -p
+// below -87 never occurs.
+SFK_HD float sfk_expf_neg(float p) {
+    const float a = -p;
+    const float x = a > 88.0f ? 88.0f : (a == a ? a : 0.0f);
+    const float n = floorf(x * 1.44269504088896341f + 0.5f);
+    float r = fmaf(-n, 0.693359375f, x);
+    r = fmaf(-n, -2.12194440e-4f, r);
+    const float z = r * r;
+    float q = 1.9875691500e-4f;
+    q = fmaf(q, r, 1.3981999507e-3f);
+    q = fmaf(q, r, 8.3334519073e-3f);
+    q = fmaf(q, r, 4.1665795894e-2f);
+    q = fmaf(q, r, 1.6666665459e-1f);
+    q = fmaf(q, r, 5.0000001201e-1f);
+    q = fmaf(q, z, r);
+    q = q + 1.0f;
+    const float y = q * sfk_pow2i((int)n);
+    return a == a ? y : a;
+}
 
 SFK_HD float sfk_logf(float x) {
     if (x != x) return x;

@@ -168,29 +168,12 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
         if (__syncthreads_count(!done) == 0) break;
         const uint32_t cnt = (end - chunk) < (uint32_t)NT ? (end - chunk) : (uint32_t)NT;
         const uint32_t nl = __ballot(!done) ? wave_reach_list(s_box, cnt, bx0, by0, list) : 0u;
-        // One splat against this pixel: returns whether it contributes, with
-        // its weight vis = alpha * T, advancing T, `last` and `done` exactly
-        // as raster.rs:183-217.
-        auto step = [&](uint32_t j, float4 geo, float4 aux, float blue, float& vis) -> bool {
-            float dx = geo.x - pxf, dy = geo.y - pyf;
-            float power = 0.5f * (geo.z * dx * dx + aux.x * dy * dy) + geo.w * dx * dy;
-            if (power > kPowerSkip) return false;
-            float alpha = fminf(aux.y * sfk_expf(-power), 0.999f);
-            if (!(alpha >= kInv255)) return false;
-            vis = alpha * T;
-            if constexpr (MODE == kRgb) {
-                cr += aux.z * vis;
-                cg += aux.w * vis;
-                cb += blue * vis;
-            }
-            T *= 1.0f - alpha;
-            last = chunk + j + 1;
-            if (T < kInv255) done = true;
-            return true;
-        };
-        // Listed splats in groups of 8, rows loaded up front so that the LDS
-        // reads of a group overlap (entries past the list end repeat the last
-        // entry and are masked off).
+        // Listed splats in groups of 8 (entries past the list end repeat the
+        // last entry and are masked off).  Each member is evaluated straight
+        // through for the whole wave once any lane may be reached; lanes that
+        // do not contribute keep their state through selects.  Per lane this
+        // is raster.rs:183-217 exactly: alpha = min(o * exp(-power), 0.999),
+        // contribute iff alpha >= 1/255, T *= 1 - alpha, done iff T < 1/255.
         const uint32_t lane = lane_id();
         const uint32_t ms = lane >> 3;
         for (uint32_t i0 = 0; i0 < nl; i0 += 8) {
@@ -209,45 +192,41 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
                     else blue[s] = 0.0f;
                 }
             }
-            // Row of member s: preloaded, or read at its turn.
-            auto row = [&](int s, float4& g, float4& x, float& b) {
+            uint32_t q[8];
+            bool any = false;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                q[s] = 0;
+                float4 g4, x4;
+                float bl = 0.0f;
                 if constexpr (PF) {
-                    g = geo[s]; x = aux[s]; b = blue[s];
+                    g4 = geo[s]; x4 = aux[s]; bl = blue[s];
                 } else {
-                    g = s_geo[jj[s]]; x = s_aux[jj[s]];
-                    b = MODE == kRgb ? s_blue[jj[s]] : 0.0f;
+                    g4 = s_geo[jj[s]]; x4 = s_aux[jj[s]];
                 }
-            };
-            if constexpr (MODE == kRgb) {
-#pragma unroll
-                for (int s = 0; s < 8; ++s) {
-                    float vis;
-                    if (!done && i0 + s < nl) {
-                        float4 g, x;
-                        float b;
-                        row(s, g, x, b);
-                        step(jj[s], g, x, b, vis);
-                    }
+                const float dx = g4.x - pxf, dy = g4.y - pyf;
+                const float power = 0.5f * (g4.z * dx * dx + x4.x * dy * dy) + g4.w * dx * dy;
+                const bool in = !done && i0 + s < nl && !(power > kPowerSkip);
+                if (!__ballot(in)) continue;
+                const float alpha = fminf(x4.y * sfk_expf_neg(power), 0.999f);
+                const bool con = in && alpha >= kInv255;
+                const float vis = alpha * T;
+                if constexpr (MODE == kRgb) {
+                    if constexpr (!PF) bl = s_blue[jj[s]];
+                    cr = con ? cr + x4.z * vis : cr;
+                    cg = con ? cg + x4.w * vis : cg;
+                    cb = con ? cb + bl * vis : cb;
+                } else {
+                    q[s] = con ? sfk_f2u(vis * J.scale) : 0u;
+                    any |= q[s] != 0;
                 }
-            } else {
-                // The u32 weights of the 8 members are summed over the wave in
-                // one transposed pass, lane l ending with member l >> 3.
-                uint32_t q[8];
-                bool any = false;
-#pragma unroll
-                for (int s = 0; s < 8; ++s) {
-                    float vis;
-                    q[s] = 0;
-                    if (!done && i0 + s < nl) {
-                        float4 g, x;
-                        float b;
-                        row(s, g, x, b);
-                        if (step(jj[s], g, x, b, vis)) {
-                            q[s] = sfk_f2u(vis * J.scale);
-                            any |= q[s] != 0;
-                        }
-                    }
-                }
+                T = con ? T * (1.0f - alpha) : T;
+                last = con ? chunk + jj[s] + 1 : last;
+                done = done || (con && T < kInv255);
+            }
+            if constexpr (MODE != kRgb) {
+                // The u32 weights of the 8 members are summed over the wave
+                // in one transposed pass, lane l ending with member l >> 3.
                 if (!__ballot(any)) continue;
                 const bool writer = (lane & 7u) == 0 && i0 + ms < nl;
                 const uint32_t j = writer ? list[i0 + ms] : 0u;

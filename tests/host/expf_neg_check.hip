@@ -1,0 +1,33 @@
+// Host-side check that the rasterizers' sfk_expf_neg(p) returns the same bits
+// as sfk_expf(-p) (the oracle's exp) over every float p in [-88, 6] visited
+// with the given stride, that NaN propagates, and that arguments past the
+// clamp still saturate alpha = min(o * G, 0.999) for o >= 10/255.
+#include "../../splatfield_amd/csrc/sfk_math.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+int main(int argc, char** argv) {
+    const uint32_t stride = argc > 1 ? (uint32_t)strtoul(argv[1], nullptr, 10) : 1u;
+    unsigned long long bad = 0, checked = 0;
+    auto one = [&](uint32_t u) {
+        float p;
+        memcpy(&p, &u, 4);
+        const float a = sfk_expf(-p), b = sfk_expf_neg(p);
+        if (memcmp(&a, &b, 4) != 0) {
+            if (bad < 5) printf("p=%a exp=%a neg=%a\n", p, a, b);
+            ++bad;
+        }
+        ++checked;
+    };
+    for (uint64_t u = 0; u <= 0x40C00000u; u += stride) one((uint32_t)u);           // [0, 6]
+    for (uint64_t u = 0x80000000u; u <= 0xC2B00000u; u += stride) one((uint32_t)u);  // [-88, 0)
+    const float nan = __builtin_nanf("");
+    const float r = sfk_expf_neg(nan);
+    if (r == r) ++bad;
+    for (float p = -88.0f; p > -1e30f; p *= 1.7f)
+        if (!(fminf((10.0f / 255.0f) * sfk_expf_neg(p), 0.999f) == 0.999f)) ++bad;
+    printf("checked %llu mismatches %llu\n", checked, bad);
+    return bad != 0;
+}
