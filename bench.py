@@ -239,9 +239,13 @@ def main():
         if kb.get(name):
             entry["gbs"] = round(kb[name] / (per_launch * 1e-3) / 1e9, 1)
         per_kernel[name] = entry
-    dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
+    # Dominant kernel: the single kernel with the largest time per step
+    # (radix_pass_* scopes span three kernels and two workloads).
+    single = {k_: v_ for k_, v_ in prof.items() if not k_.startswith("radix_")}
+    dom = max(single.items(), key=lambda kv: kv[1][1])[0]
     dom_ms = prof[dom][1] / prof[dom][0]
     dom_gbs = kb[dom] / (dom_ms * 1e-3) / 1e9 if kb.get(dom) else None
+    traffic = measured_traffic(dom)
 
     b_fwd = (n * (44 + 4 * 3 * k) + 56 * vis + 16 * vis + 8 * ni + 4 * 16 * vis +
              math.ceil(tile_bits / 8) * 16 * ni + 40 * ni + 16 * px)
@@ -278,7 +282,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 1) if dom_gbs else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 4) if dom_gbs else None,
-                     "traffic": None, "kernel": dom,
+                     "traffic": traffic, "kernel": dom,
                      "frame_achieved": round(step_gbs, 1),
                      "frame_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "kernels": per_kernel,
@@ -288,6 +292,20 @@ def main():
     }
     print(json.dumps(line))
     ctx.close()
+
+
+def measured_traffic(scope):
+    """HBM bytes per launch of a kernel, from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE summary committed under profiles/ (each counter scaled by
+    what it reads for a device copy of known size), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "traffic_fwd_bwd_1080p.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["kernels"][scope]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(attrs, sh, n, k, view, w, h, threads, seconds, sc_target=None):
