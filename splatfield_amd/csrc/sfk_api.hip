@@ -376,7 +376,8 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const V
     a.nseg = nseg;
     a.seg_len = nseg > 1 ? n : 0u;
     a.hist = s->hist;
-    a.stride = nseg > 1 ? radix_max_blocks(n) : s->max_blocks;
+    a.kpb = radix_keys_per_block(total);
+    a.stride = nseg > 1 ? div_up(n, a.kpb) : s->max_blocks;
     a.totals = s->totals;
     uint32_t shift = 0;
     for (uint32_t p = 0; p < passes; ++p) {

@@ -98,8 +98,8 @@ hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, 
 
 // Radix sort: one pass = upsweep + digit scan + scatter.
 constexpr uint32_t kSortThreads = 256;
-constexpr uint32_t kSortEpt = 16;
-constexpr uint32_t kSortBlock = kSortThreads * kSortEpt;  // keys per block
+constexpr uint64_t kSortSmallMax = 1u << 22;  // up to this many keys: 1024-key blocks
+uint32_t radix_keys_per_block(uint64_t total_keys);   // 1024 or 4096
 uint32_t radix_max_blocks(uint32_t max_elems);
 struct SortArgs {
     const void* keys_in;
@@ -111,6 +111,7 @@ struct SortArgs {
     const uint32_t* d_n;      // one segment: device-side count, may be nullptr
     uint32_t nseg, seg_len;   // nseg > 1: segments of seg_len keys each
     uint32_t shift, width;
+    uint32_t kpb;             // keys per block (radix_keys_per_block)
     uint32_t* hist;           // nseg * RADIX rows of `stride` blocks
     uint32_t stride;
     uint32_t* totals;         // nseg * RADIX

@@ -25,13 +25,13 @@
 #include "sfk_device.h"
 #include "sfk_internal.h"
 
+#include <algorithm>
+
 namespace sfk {
 
 namespace {
 
 constexpr uint32_t NT = kSortThreads;
-constexpr uint32_t EPT = kSortEpt;
-constexpr uint32_t KPB = kSortBlock;
 constexpr uint32_t NW = NT / 64;
 
 template <typename K>
@@ -55,11 +55,11 @@ __device__ __forceinline__ bool seg_pos(const SortArgs& a, uint32_t b, SegPos& p
         }
         p.s = 0;
         p.bl = b;
-        p.nb = (n + KPB - 1) / KPB;
+        p.nb = (n + a.kpb - 1) / a.kpb;
         p.seg_n = n;
         p.seg_base = 0;
     } else {
-        uint32_t bps = (a.seg_len + KPB - 1) / KPB;
+        uint32_t bps = (a.seg_len + a.kpb - 1) / a.kpb;
         p.s = b / bps;
         p.bl = b % bps;
         p.nb = bps;
@@ -70,8 +70,9 @@ __device__ __forceinline__ bool seg_pos(const SortArgs& a, uint32_t b, SegPos& p
     return p.bl < p.nb;
 }
 
-template <typename K, int RADIX>
+template <typename K, int RADIX, uint32_t EPT>
 __global__ __launch_bounds__(NT) void k_upsweep(SortArgs a) {
+    constexpr uint32_t KPB = NT * EPT;
     __shared__ uint32_t h[NW][RADIX];
     SegPos p;
     if (!seg_pos(a, blockIdx.x, p)) return;
@@ -108,12 +109,13 @@ __global__ __launch_bounds__(NT) void k_digscan(SortArgs a) {
             uint32_t m = *a.d_n;
             n = m < n ? m : n;
         }
-        nb = (n + KPB - 1) / KPB;
+        nb = (n + a.kpb - 1) / a.kpb;
     } else {
-        nb = (a.seg_len + KPB - 1) / KPB;
+        nb = (a.seg_len + a.kpb - 1) / a.kpb;
     }
     uint32_t* row = a.hist + ((size_t)s * RADIX + d) * a.stride;
     uint32_t carry = 0;
+    constexpr uint32_t EPT = 16;
     for (uint32_t c0 = 0; c0 < nb; c0 += NT * EPT) {
         uint32_t v[EPT];
         uint32_t sum = 0;
@@ -150,8 +152,9 @@ __device__ __forceinline__ uint2 identity_val<uint2>(uint32_t li, const uint32_t
     return make_uint2(li, payload[gi]);
 }
 
-template <typename K, typename V, int RADIX>
+template <typename K, typename V, int RADIX, uint32_t EPT>
 __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
+    constexpr uint32_t KPB = NT * EPT;
     __shared__ K s_keys[KPB];
     __shared__ V s_vals[KPB];
     __shared__ uint32_t s_wh[NW][RADIX];  // per-wave digit counters, then bases
@@ -270,27 +273,47 @@ __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
     }
 }
 
-template <typename K, typename V, int RADIX>
+template <typename K, typename V, int RADIX, uint32_t EPT>
 hipError_t pass_impl(hipStream_t st, const SortArgs& a) {
+    constexpr uint32_t KPB = NT * EPT;
     uint32_t grid = a.nseg <= 1 ? div_up(a.n_max, KPB) : a.nseg * div_up(a.seg_len, KPB);
     if (grid == 0) return hipSuccess;
-    k_upsweep<K, RADIX><<<grid, NT, 0, st>>>(a);
+    k_upsweep<K, RADIX, EPT><<<grid, NT, 0, st>>>(a);
     k_digscan<RADIX><<<(a.nseg <= 1 ? 1u : a.nseg) << a.width, NT, 0, st>>>(a);
-    k_scatter<K, V, RADIX><<<grid, NT, 0, st>>>(a);
+    k_scatter<K, V, RADIX, EPT><<<grid, NT, 0, st>>>(a);
     return hipGetLastError();
 }
 
 }  // namespace
 
-uint32_t radix_max_blocks(uint32_t max_elems) { return div_up(max_elems > 0 ? max_elems : 1, KPB); }
+uint32_t radix_keys_per_block(uint64_t total) {
+    return total <= kSortSmallMax ? NT * 4u : NT * 16u;
+}
 
+// Histogram rows for any sort a scratch of max_elems keys can run.
+uint32_t radix_max_blocks(uint32_t max_elems) {
+    const uint64_t m = max_elems > 0 ? max_elems : 1;
+    const uint32_t big = div_up(m, NT * 16u);
+    const uint32_t small = div_up(std::min<uint64_t>(m, kSortSmallMax), NT * 4u);
+    return big > small ? big : small;
+}
+
+template <typename K, typename V, uint32_t EPT>
+hipError_t pass_ept(hipStream_t st, const SortArgs& a, uint32_t digit_bits) {
+    if (digit_bits <= 4) return pass_impl<K, V, 16, EPT>(st, a);
+    if (digit_bits <= 5) return pass_impl<K, V, 32, EPT>(st, a);
+    if (digit_bits <= 6) return pass_impl<K, V, 64, EPT>(st, a);
+    if (digit_bits <= 7) return pass_impl<K, V, 128, EPT>(st, a);
+    return pass_impl<K, V, 256, EPT>(st, a);
+}
+
+// Small sorts (up to kSortSmallMax keys) use 1024-key blocks: four times the
+// workgroups keep enough waves in flight where 4096-key blocks would leave
+// one per CU.
 template <typename K, typename V>
 hipError_t pass_radix(hipStream_t st, const SortArgs& a, uint32_t digit_bits) {
-    if (digit_bits <= 4) return pass_impl<K, V, 16>(st, a);
-    if (digit_bits <= 5) return pass_impl<K, V, 32>(st, a);
-    if (digit_bits <= 6) return pass_impl<K, V, 64>(st, a);
-    if (digit_bits <= 7) return pass_impl<K, V, 128>(st, a);
-    return pass_impl<K, V, 256>(st, a);
+    if (a.kpb == NT * 4u) return pass_ept<K, V, 4>(st, a, digit_bits);
+    return pass_ept<K, V, 16>(st, a, digit_bits);
 }
 
 hipError_t launch_radix_pass(hipStream_t st, const SortArgs& a, uint32_t key_bytes,
