@@ -103,6 +103,64 @@ def bench_eig_round(ctx, args):
             "last_view_evidence_bit_exact": exact, "isects_total": int(rnd.counters()[-1])}
 
 
+def bench_c4(ctx, args):
+    """Mixed-resolution batched round: 2,097,152 splats (SH3), 64
+    Fibonacci-sphere cameras cycling 256^2 / 512^2 / 1024^2.  The autotuner
+    picks tile width x digit width, the chosen round is timed as graph
+    replays, and the plain loop (one scratch per resolution, isect capacity
+    reserved) is timed beside it."""
+    from splatfield_amd import render as R
+    from splatfield_amd import scene as S
+    from splatfield_amd import seg as G
+    n = 1 << 21
+    attrs, sh, k = S.synthetic_scene(n, 3, seed=0)
+    splats = R.Splats(ctx, attrs, sh)
+    cams = G.sphere_cameras(64, radius=3.0)
+    res = (256, 512, 1024)
+    sizes = [(res[i % 3], res[i % 3]) for i in range(64)]
+    rng = np.random.default_rng(2)
+    state = G.BetaState(ctx, (1 + rng.random(n) * 4).astype(np.float32),
+                        (1 + rng.random(n) * 4).astype(np.float32))
+    best, table = G.autotune_round(splats, cams, sizes, state, reps=2)
+    rnd = G.EigRound(splats, cams, sizes, tile_w=best[0], digit_bits=best[1])
+    rnd.run(state, use_graph=True)
+    tb = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        eig_b, best_b = rnd.run(state, use_graph=True)
+        tb.append(time.perf_counter() - t0)
+    isects = int(rnd.counters()[-1])
+    del rnd
+    scratches = {}
+    acc = G.Accumulators(ctx, n)
+
+    def plain():
+        e = []
+        for cam, (w, h) in zip(cams, sizes):
+            sc = scratches.get(w)
+            if sc is None:
+                sc = scratches[w] = R.RenderScratch(ctx, n, w, h)
+                sc.reserve_isects(1 << 25)
+            G.render_responsibility(splats, sc, acc, cam, w, h)
+            e.append(state.eig(acc))
+        return e
+
+    plain()
+    tp = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        eig_p = plain()
+        tp.append(time.perf_counter() - t0)
+    batch_ms, plain_ms = 1e3 * float(np.median(tb)), 1e3 * float(np.median(tp))
+    return {"gaussians": n, "views": 64, "res": list(res),
+            "autotune_ms": {f"tile{t}_digit{d}": round(v, 3) for (t, d), v in table.items()},
+            "tile_w": best[0], "digit_bits": best[1], "batched_ms": round(batch_ms, 3),
+            "plain_ms": round(plain_ms, 3), "ratio": round(batch_ms / plain_ms, 4),
+            "views_per_s": round(64 / (batch_ms * 1e-3), 1), "isects_total": isects,
+            "same_best_view": bool(int(np.argmax(eig_p)) == best_b),
+            "eig_max_rel_diff": float(np.max(np.abs(eig_b - eig_p) / np.abs(eig_p)))}
+
+
 def bench_sort(ctx, args):
     """Standalone stable argsort of 16,777,216 uniform u64 keys + u32 index
     values (HIP events around the sort launches only)."""
@@ -256,6 +314,7 @@ def main():
     if not args.no_extra:
         extra["eig_round"] = bench_eig_round(ctx, args)
         extra["sort_u64"] = bench_sort(ctx, args)
+        extra["eig_round_c4"] = bench_c4(ctx, args)
 
     cpu = None
     if not args.no_cpu:
