@@ -242,7 +242,6 @@ def main():
 
     splats = R.Splats(ctx, attrs, sh)
     scratch = R.RenderScratch(ctx, n, w, h)
-    d_rgba = GpuTensor.empty(ctx, px * 4, np.float32)
     loss = GpuTensor.empty(ctx, 1, np.float64)
     g_attrs = GpuTensor.empty(ctx, n * 11, np.float32)
     g_sh = GpuTensor.empty(ctx, n * 3 * k, np.float32)
@@ -251,8 +250,7 @@ def main():
     def step():
         isects = R.prepare_isects(splats, scratch, view=view, w=w, h=h)
         R.finalize_rgb(scratch, isects)
-        R.loss_l2(ctx, rgba_ptr, target.ptr, px, d_rgba.ptr, loss)
-        R.render_backward(splats, scratch, isects, d_rgba.ptr, g_attrs, g_sh)
+        R.render_backward_l2(splats, scratch, isects, rgba_ptr, target.ptr, g_attrs, g_sh, loss)
         return isects
 
     rgba_ptr = scratch.device("rgba")

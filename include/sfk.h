@@ -320,6 +320,13 @@ int sfk_loss_l2(sfk_ctx* ctx, const float* rgba, const float* target, uint32_t p
 int sfk_render_backward(sfk_ctx* ctx, const sfk_splats* splats, sfk_render_scratch* scratch,
                         const sfk_tile_isects* isects, const float* d_rgba, float* grad_attrs,
                         float* grad_sh);
+/* sfk_loss_l2 followed by sfk_render_backward in one pass: the adjoint
+ * rasterizer forms d_rgba = 2 (rgba - target) per pixel itself (no d_rgba
+ * buffer) and writes the L2 loss, a fixed-order f64 sum, to d_loss (device,
+ * required).  Gradients equal the two-call sequence bit for bit. */
+int sfk_render_backward_l2(sfk_ctx* ctx, const sfk_splats* splats, sfk_render_scratch* scratch,
+                           const sfk_tile_isects* isects, const float* rgba, const float* target,
+                           float* grad_attrs, float* grad_sh, double* d_loss);
 
 /* ------------------------------------------------------ kernel-level calls */
 

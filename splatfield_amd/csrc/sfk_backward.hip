@@ -59,7 +59,25 @@ struct BwdArgs {
     const uint32_t* n_contrib;
     const float* d_img;
     float* grad2d;  // 9 planes of n: mx, my, conic a, conic b, conic c, opacity, r, g, b
+    // Fused L2 loss (d_img == nullptr): dL/dC = 2 (img - target) per pixel
+    // and one f64 loss partial per tile.
+    const float* img;
+    const float* target;
+    double* loss_partials;  // one per tile
 };
+
+// Fixed-order f64 sum of one value per thread over a workgroup of NT threads.
+template <int NT>
+__device__ __forceinline__ double block_sum_f64(double v, double* s_w) {
+    v = wave_sum_f64(v);
+    if (lane_id() == 0) s_w[threadIdx.x / 64] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) t += s_w[w];
+    __syncthreads();
+    return t;
+}
 
 template <int TW>
 __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
@@ -73,6 +91,7 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
     __shared__ uint32_t s_gid[NT];
     __shared__ float s_acc[9][NT];
     __shared__ uint32_t s_last;
+    __shared__ double s_w[NW];
 
     const uint32_t ntiles = a.tiles_x * a.tiles_y;
     const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
@@ -95,7 +114,23 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
         last = a.n_contrib[p];
         Tfin = a.final_t[p];
         T = Tfin;
-        dC = reinterpret_cast<const float4*>(a.d_img)[p];
+        if (a.d_img) dC = reinterpret_cast<const float4*>(a.d_img)[p];
+    }
+    if (!a.d_img) {
+        // Same arithmetic as the standalone loss: d = img - target, dC = 2 d.
+        double sq = 0.0;
+        if (in_bounds) {
+            const size_t p = (size_t)py * a.w + px;
+            const float4 im = reinterpret_cast<const float4*>(a.img)[p];
+            const float4 tg = reinterpret_cast<const float4*>(a.target)[p];
+            const float4 d = make_float4(im.x - tg.x, im.y - tg.y, im.z - tg.z, im.w - tg.w);
+            dC = make_float4(2.0f * d.x, 2.0f * d.y, 2.0f * d.z, 2.0f * d.w);
+            sq = (double)d.x * d.x + (double)d.y * d.y + (double)d.z * d.z + (double)d.w * d.w;
+        }
+        // One partial per tile; the projection adjoint, which runs next,
+        // adds them up in tile order.
+        const double part = block_sum_f64<NT>(sq, s_w);
+        if (t == 0) a.loss_partials[tile] = part;
     }
     if (t == 0) s_last = start;
     __syncthreads();
@@ -162,36 +197,55 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
                 const float rom = __builtin_amdgcn_rcpf(om);
                 const float Tn = T * rom;
                 T = con ? Tn : T;
-                const float wgt = alpha * Tn;
                 const float cb = s_blue[j];
-                g[6][s] = con ? wgt * dC.x : 0.0f;
-                g[7][s] = con ? wgt * dC.y : 0.0f;
-                g[8][s] = con ? wgt * dC.z : 0.0f;
-                const float dal = Tn * ((aux.z - acc_r) * dC.x + (aux.w - acc_g) * dC.y +
-                                        (cb - acc_b) * dC.z) +
-                                  Tfin * rom * dC.w;
-                acc_r = con ? alpha * aux.z + om * acc_r : acc_r;
-                acc_g = con ? alpha * aux.w + om * acc_g : acc_g;
-                acc_b = con ? alpha * cb + om * acc_b : acc_b;
-                // The clamp at 0.999 has zero derivative.
-                const bool slope = con && araw < 0.999f;
-                const float dpow = -dal * alpha;
-                g[5][s] = slope ? dal * G : 0.0f;
-                g[0][s] = slope ? dpow * (geo.z * dx + geo.w * dy) : 0.0f;
-                g[1][s] = slope ? dpow * (aux.x * dy + geo.w * dx) : 0.0f;
-                g[2][s] = slope ? dpow * 0.5f * dx * dx : 0.0f;
-                g[3][s] = slope ? dpow * dx * dy : 0.0f;
-                g[4][s] = slope ? dpow * 0.5f * dy * dy : 0.0f;
+                const float wr = aux.z - acc_r, wg = aux.w - acc_g, wb = cb - acc_b;
+                // Colour weight: 0 on lanes that do not contribute.
+                const float wgt = con ? alpha * Tn : 0.0f;
+                g[6][s] = wgt * dC.x;
+                g[7][s] = wgt * dC.y;
+                g[8][s] = wgt * dC.z;
+                const float dal = Tn * (wr * dC.x + wg * dC.y + wb * dC.z) + Tfin * rom * dC.w;
+                // Back-to-front colour accumulators: acc' = acc + alpha (c - acc).
+                acc_r = con ? acc_r + alpha * wr : acc_r;
+                acc_g = con ? acc_g + alpha * wg : acc_g;
+                acc_b = con ? acc_b + alpha * wb : acc_b;
+                // d alpha / d(opacity, power) are zero where the 0.999 clamp
+                // binds; the power terms are reduced as moments of dpow in
+                // (dx, dy) and turned into mean / conic gradients per member
+                // after the reduction:
+                //   d mean = (a Sx + b Sy, c Sy + b Sx), d conic = (Sxx/2, Sxy, Syy/2).
+                const float dm = (con && araw < 0.999f) ? dal : 0.0f;
+                const float dpow = -dm * alpha;
+                const float px_ = dpow * dx, py_ = dpow * dy;
+                g[5][s] = dm * G;
+                g[0][s] = px_;
+                g[1][s] = py_;
+                g[2][s] = px_ * dx;
+                g[3][s] = px_ * dy;
+                g[4][s] = py_ * dy;
             }
             if (__ballot(has)) {
                 const uint32_t lane = lane_id();
                 const uint32_t ms = lane >> 3;
                 const bool writer = (lane & 7u) == 0 && ms < itop;
                 const uint32_t j = writer ? list[itop - 1 - ms] : 0u;
+                float r[9];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    float r = wave_sum8_transposed(g[k]);
-                    if (writer && r != 0.0f) atomicAdd(&s_acc[k][j], r);
+                for (int k = 0; k < 9; ++k) r[k] = wave_sum8_transposed(g[k]);
+                if (writer) {
+                    const float4 geo = s_geo[j];
+                    const float qc = s_aux[j].x;
+                    float out[9];
+                    out[0] = geo.z * r[0] + geo.w * r[1];  // d mx
+                    out[1] = qc * r[1] + geo.w * r[0];     // d my
+                    out[2] = 0.5f * r[2];                  // d conic a
+                    out[3] = r[3];                         // d conic b
+                    out[4] = 0.5f * r[4];                  // d conic c
+#pragma unroll
+                    for (int k = 5; k < 9; ++k) out[k] = r[k];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k)
+                        if (out[k] != 0.0f) atomicAdd(&s_acc[k][j], out[k]);
                 }
             }
         }
@@ -220,9 +274,21 @@ constexpr float kK9 = -0.5900436f;
 template <int CHS>
 __global__ __launch_bounds__(256) void k_project_bwd(sfk_view v, const float* __restrict__ A,
                                                      const float* __restrict__ shs, uint32_t n,
-                                                     const float* __restrict__ g2,
+                                                     float* __restrict__ g2,
                                                      float* __restrict__ gA,
-                                                     float* __restrict__ gS) {
+                                                     float* __restrict__ gS,
+                                                     const double* __restrict__ loss_partials,
+                                                     uint32_t loss_np,
+                                                     double* __restrict__ loss_out) {
+    if (loss_out && blockIdx.x == 0) {
+        // Fused L2 loss: fixed-order total of the adjoint raster's tile
+        // partials (complete at this kernel's start).
+        __shared__ double s_w[4];
+        double sum = 0.0;
+        for (uint32_t k = threadIdx.x; k < loss_np; k += 256) sum += loss_partials[k];
+        sum = block_sum_f64<256>(sum, s_w);
+        if (threadIdx.x == 0) *loss_out = sum;
+    }
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     const size_t N = n;
@@ -240,9 +306,13 @@ __global__ __launch_bounds__(256) void k_project_bwd(sfk_view v, const float* __
         for (int k = 0; k < 3 * CHS; ++k) gS[(size_t)k * N + i] = 0.0f;
         return;
     }
+    // Read this splat's screen-space gradients and leave zeros behind: the
+    // next backward accumulates into a cleared buffer without a fill pass.
     float G2[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) G2[k] = g2[(size_t)k * N + i];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g2[(size_t)k * N + i] = 0.0f;
 
     const float iz = 1.0f / cam[2];
     float x2 = qx * qx, y2 = qy * qy, z2 = qz * qz, xy = qx * qy, xz = qx * qz, yz = qy * qz;
@@ -419,11 +489,17 @@ uint32_t loss_num_partials(uint32_t np) { return div_up(np ? np : 1, 256); }
 
 hipError_t launch_raster_bwd(hipStream_t s, const RasterArgs& ra, const float* final_t,
                              const uint32_t* n_contrib, const float* d_img, uint32_t n,
-                             float* grad2d) {
+                             float* grad2d, const BwdLoss* loss) {
     BwdArgs a;
     a.w = ra.w; a.h = ra.h; a.tiles_x = ra.tiles_x; a.tiles_y = ra.tiles_y; a.n = n;
     a.tile_starts = ra.tile_starts; a.gids = ra.gids; a.proj = ra.proj;
     a.final_t = final_t; a.n_contrib = n_contrib; a.d_img = d_img; a.grad2d = grad2d;
+    a.img = a.target = nullptr;
+    a.loss_partials = nullptr;
+    if (!d_img) {
+        if (!loss) return hipErrorInvalidValue;
+        a.img = loss->img; a.target = loss->target; a.loss_partials = loss->partials;
+    }
     uint32_t nt = ra.tiles_x * ra.tiles_y;
     if (nt == 0) return hipSuccess;
     switch (ra.tile_w) {
@@ -437,14 +513,18 @@ hipError_t launch_raster_bwd(hipStream_t s, const RasterArgs& ra, const float* f
 
 hipError_t launch_project_bwd(hipStream_t s, const sfk_view& v, const float* attrs,
                               const float* sh, uint32_t n, uint32_t sh_per_ch,
-                              const float* grad2d, float* g_attrs, float* g_sh) {
+                              float* grad2d, float* g_attrs, float* g_sh,
+                              const BwdLoss* loss) {
     if (n == 0) return hipSuccess;
     dim3 grid(div_up(n, 256));
+    const double* lp = loss ? loss->partials : nullptr;
+    const uint32_t lnp = loss ? loss->num_partials : 0u;
+    double* lo = loss ? loss->out : nullptr;
     switch (sh_per_ch) {
-        case 1: k_project_bwd<1><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh); break;
-        case 4: k_project_bwd<4><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh); break;
-        case 9: k_project_bwd<9><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh); break;
-        case 16: k_project_bwd<16><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh); break;
+        case 1: k_project_bwd<1><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh, lp, lnp, lo); break;
+        case 4: k_project_bwd<4><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh, lp, lnp, lo); break;
+        case 9: k_project_bwd<9><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh, lp, lnp, lo); break;
+        case 16: k_project_bwd<16><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh, lp, lnp, lo); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -204,12 +204,23 @@ hipError_t launch_rasterize_batch(hipStream_t s, const RasterArgs& a);
 uint32_t loss_num_partials(uint32_t np);
 hipError_t launch_loss_l2(hipStream_t s, const float* img, const float* target, uint32_t np,
                           float* d_img, double* partials, double* out);
+// With d_img == nullptr the L2 loss is fused: dL/dimg = 2 (img - target)
+// and one f64 partial per tile in the adjoint raster; the projection adjoint
+// (launched next with the same BwdLoss) writes their fixed-order total.
+struct BwdLoss {
+    const float* img;
+    const float* target;
+    double* partials;  // one per tile
+    uint32_t num_partials;
+    double* out;
+};
 hipError_t launch_raster_bwd(hipStream_t s, const RasterArgs& ra, const float* final_t,
                              const uint32_t* n_contrib, const float* d_img, uint32_t n,
-                             float* grad2d);
+                             float* grad2d, const BwdLoss* loss = nullptr);
 hipError_t launch_project_bwd(hipStream_t s, const sfk_view& v, const float* attrs,
                               const float* sh, uint32_t n, uint32_t sh_per_ch,
-                              const float* grad2d, float* g_attrs, float* g_sh);
+                              float* grad2d, float* g_attrs, float* g_sh,
+                              const BwdLoss* loss = nullptr);
 
 // Seg.
 hipError_t launch_eig(hipStream_t s, const float* a, const float* b, const uint32_t* resp,

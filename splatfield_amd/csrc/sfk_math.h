@@ -46,11 +46,12 @@ SFK_HD float sfk_expf(float x) {
 // bits.  NaN comes out as NaN.  -p above 88 is evaluated at 88: exp(88) =
 // 1.65e38, so min(o * G, 0.999) = 0.999 exactly as with the true value for
 // any opacity above 1e-35, and o * G >= 0.999 keeps every caller on the same
-// side of its saturation test.  The callers skip powers above 5.6, so -p
-// below -87 never occurs.
+// side of its saturation test.  -p below -87 is evaluated at -87 (a tiny
+// finite value): the callers evaluate whole waves but only use lanes whose
+// power is at most 5.6, and a finite value keeps 0 * G at 0 elsewhere.
 SFK_HD float sfk_expf_neg(float p) {
     const float a = -p;
-    const float x = a > 88.0f ? 88.0f : (a == a ? a : 0.0f);
+    const float x = a > 88.0f ? 88.0f : (a < -87.0f ? -87.0f : (a == a ? a : 0.0f));
     const float n = floorf(x * 1.44269504088896341f + 0.5f);
     float r = fmaf(-n, 0.693359375f, x);
     r = fmaf(-n, -2.12194440e-4f, r);

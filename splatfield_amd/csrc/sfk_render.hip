@@ -122,7 +122,8 @@ struct sfk_render_scratch {
     bool has_view = false;
     bool has_colors = false;  // the last prepare projected SH colours
     float* grad2d = nullptr;  // 9 planes x n screen-space gradients (backward)
-    double* loss_partials = nullptr;
+    bool grad2d_dirty = true; // grad2d not known to be all zero
+    double* loss_partials = nullptr;  // one per tile (fused L2 backward)
     uint32_t loss_partials_cap = 0;
 };
 
@@ -138,6 +139,7 @@ static void scratch_free_frame(sfk_render_scratch* s) {
     s->rgba = nullptr; s->final_t = nullptr; s->n_contrib = nullptr; s->tile_ids = nullptr;
     s->gaussian_ids = nullptr;
     s->grad2d = nullptr;
+    s->grad2d_dirty = true;
     s->emit_status = nullptr;
     s->cbbox = s->order_buf = nullptr;
     s->has_view = false;
@@ -638,23 +640,58 @@ extern "C" int sfk_loss_l2(sfk_ctx* c, const float* rgba, const float* target, u
     return SFK_OK;
 }
 
-extern "C" int sfk_render_backward(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
-                                   const sfk_tile_isects* t, const float* d_rgba,
-                                   float* grad_attrs, float* grad_sh) {
+static int render_backward_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
+                                const sfk_tile_isects* t, const float* d_rgba,
+                                const float* rgba, const float* target, float* grad_attrs,
+                                float* grad_sh, double* d_loss) {
     int rc = check_isects(c, s, t);
     if (rc) return rc;
-    if (!sp || !d_rgba || !grad_attrs || !grad_sh) return fail(c, SFK_E_ARG, "backward: null buffer");
+    if (!sp || !grad_attrs || !grad_sh || (!d_rgba && (!rgba || !target || !d_loss)))
+        return fail(c, SFK_E_ARG, "backward: null buffer");
     if (sp->n != s->n || !s->has_view) return fail(c, SFK_E_ARG, "backward: scratch was not prepared for these splats");
     if (!s->grad2d) SFK_HIP(c, hipMalloc(&s->grad2d, (size_t)s->n * 9 * 4));
-    SFK_HIP(c, launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(s->grad2d), 0u, (size_t)s->n * 9));
+    // The projection adjoint clears what it reads, so only a fresh buffer (or
+    // one left by an interrupted call) needs the fill.
+    if (s->grad2d_dirty)
+        SFK_HIP(c, launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(s->grad2d), 0u,
+                                   (size_t)s->n * 9));
+    s->grad2d_dirty = true;
     RasterArgs a = raster_args(s, t);
+    BwdLoss loss;
+    if (!d_rgba) {
+        const uint32_t tiles = s->tiles_x * s->tiles_y;
+        if (s->loss_partials_cap < tiles) {
+            if (s->loss_partials) hipFree(s->loss_partials);
+            s->loss_partials = nullptr;
+            s->loss_partials_cap = 0;
+            SFK_HIP(c, hipMalloc(&s->loss_partials, (size_t)tiles * 8));
+            s->loss_partials_cap = tiles;
+        }
+        loss = {rgba, target, s->loss_partials, tiles, d_loss};
+    }
     {
         SFK_PROF(c, "raster_bwd");
         SFK_HIP(c, launch_raster_bwd(c->stream, a, s->final_t, s->n_contrib, d_rgba, s->n,
-                                     s->grad2d));
+                                     s->grad2d, d_rgba ? nullptr : &loss));
     }
     SFK_PROF(c, "project_bwd");
     SFK_HIP(c, launch_project_bwd(c->stream, s->view, sp->attrs, sp->sh, sp->n, sp->sh_per_ch,
-                                  s->grad2d, grad_attrs, grad_sh));
+                                  s->grad2d, grad_attrs, grad_sh, d_rgba ? nullptr : &loss));
+    s->grad2d_dirty = false;
     return SFK_OK;
+}
+
+extern "C" int sfk_render_backward(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
+                                   const sfk_tile_isects* t, const float* d_rgba,
+                                   float* grad_attrs, float* grad_sh) {
+    if (!d_rgba) return c ? fail(c, SFK_E_ARG, "backward: null d_rgba") : SFK_E_ARG;
+    return render_backward_impl(c, sp, s, t, d_rgba, nullptr, nullptr, grad_attrs, grad_sh,
+                                nullptr);
+}
+
+extern "C" int sfk_render_backward_l2(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
+                                      const sfk_tile_isects* t, const float* rgba,
+                                      const float* target, float* grad_attrs, float* grad_sh,
+                                      double* d_loss) {
+    return render_backward_impl(c, sp, s, t, nullptr, rgba, target, grad_attrs, grad_sh, d_loss);
 }

@@ -231,6 +231,16 @@ def loss_l2(ctx, rgba, target, pixels, d_rgba, d_loss=None):
                                 d_loss.ptr if d_loss is not None else None), "loss_l2")
 
 
+def render_backward_l2(splats, scratch, isects, rgba, target, grad_attrs, grad_sh, d_loss):
+    """loss_l2 + render_backward fused (sfk_render_backward_l2): the adjoint
+    forms d_rgba = 2 (rgba - target) itself; d_loss (device f64) receives
+    the loss."""
+    ctx = splats.ctx
+    ctx.check(lib().sfk_render_backward_l2(ctx.ptr, splats.ptr, scratch.ptr, C.byref(isects.c),
+                                           rgba, target, grad_attrs.ptr, grad_sh.ptr, d_loss.ptr),
+              "render_backward_l2")
+
+
 def render_backward(splats, scratch, isects, d_rgba, grad_attrs, grad_sh):
     """Adjoint of finalize(Rgb) + project_splats (sfk_render_backward)."""
     ctx = splats.ctx

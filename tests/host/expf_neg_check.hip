@@ -28,6 +28,15 @@ int main(int argc, char** argv) {
     if (r == r) ++bad;
     for (float p = -88.0f; p > -1e30f; p *= 1.7f)
         if (!(fminf((10.0f / 255.0f) * sfk_expf_neg(p), 0.999f) == 0.999f)) ++bad;
+    // large powers: finite and below the contribution threshold
+    for (float p = 87.0f; p < 1e30f; p *= 1.7f) {
+        const float g = sfk_expf_neg(p);
+        if (!(g >= 0.0f && g < 1.0f / 255.0f) || 0.0f * g != 0.0f) ++bad;
+    }
+    {
+        const float g = sfk_expf_neg(__builtin_inff());
+        if (!(g >= 0.0f && g < 1.0f / 255.0f)) ++bad;
+    }
     printf("checked %llu mismatches %llu\n", checked, bad);
     return bad != 0;
 }

@@ -104,3 +104,27 @@ def test_backward_matches_central_differences(ctx):
         checked += 1
     assert checked == 96
     assert worst <= REL, worst
+
+
+def test_fused_l2_backward_equals_two_calls(ctx):
+    """sfk_render_backward_l2 against sfk_loss_l2 + sfk_render_backward on the
+    same frame, twice in a row (the loss ticket and the cleared gradient
+    buffer are reused): the per-pixel dL/dC is the same arithmetic, so the
+    gradients differ only by the order of the float atomics that gather them
+    across tiles (not fixed between any two runs); the loss to f64 rounding."""
+    d = setup(ctx)
+    n, k, w, h = d["n"], d["k"], d["w"], d["h"]
+    splats = R.Splats(ctx, d["attrs"], d["sh"])
+    scratch = R.RenderScratch(ctx, n, w, h)
+    isects = R.prepare_isects(splats, scratch, view=d["view"], w=w, h=h)
+    R.finalize_rgb(scratch, isects)
+    tgt = GpuTensor.from_numpy(ctx, d["target"])
+    for _ in range(2):
+        ga = GpuTensor.empty(ctx, n * 11, np.float32)
+        gs = GpuTensor.empty(ctx, n * 3 * k, np.float32)
+        loss = GpuTensor.empty(ctx, 1, np.float64)
+        R.render_backward_l2(splats, scratch, isects, scratch.device("rgba"), tgt.ptr, ga, gs,
+                             loss)
+        np.testing.assert_allclose(ga.read(), d["ga"], rtol=1e-4, atol=1e-7)
+        np.testing.assert_allclose(gs.read(), d["gs"], rtol=1e-4, atol=1e-7)
+        assert abs(float(loss.read()[0]) - d["loss"]) <= 1e-12 * d["loss"]
