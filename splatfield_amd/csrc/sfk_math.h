@@ -51,7 +51,9 @@ SFK_HD float sfk_expf(float x) {
 // power is at most 5.6, and a finite value keeps 0 * G at 0 elsewhere.
 SFK_HD float sfk_expf_neg(float p) {
     const float a = -p;
-    const float x = a > 88.0f ? 88.0f : (a < -87.0f ? -87.0f : (a == a ? a : 0.0f));
+    // Clamp without branches (min/max return the number for a NaN operand;
+    // the NaN is restored by the final select).
+    const float x = fminf(fmaxf(a, -87.0f), 88.0f);
     const float n = floorf(x * 1.44269504088896341f + 0.5f);
     float r = fmaf(-n, 0.693359375f, x);
     r = fmaf(-n, -2.12194440e-4f, r);
