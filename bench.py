@@ -62,8 +62,9 @@ def calibrate_copy(ctx):
 def bench_eig_round(ctx, args):
     """EIG-round latency: 20 candidate views at 512^2 around a 1M-splat SH3
     scene, plain (the reference's select_view loop: per view a
-    responsibility render and a synchronous EIG) vs the batched round
-    (one hipGraph replay), with the per-view u32 evidence compared."""
+    responsibility render and a synchronous EIG, at its fastest tile width)
+    vs the batched round (one hipGraph replay, autotuned tile width x digit
+    width), with the last view's u32 evidence compared."""
     from splatfield_amd import render as R
     from splatfield_amd import scene as S
     from splatfield_amd import seg as G
@@ -76,17 +77,25 @@ def bench_eig_round(ctx, args):
     rng = np.random.default_rng(2)
     state = G.BetaState(ctx, (1 + rng.random(n) * 4).astype(np.float32),
                         (1 + rng.random(n) * 4).astype(np.float32))
-    scratch = R.RenderScratch(ctx, n, args.eig_res, args.eig_res)
-    scratch.reserve_isects(1 << 23)
     resp = GpuTensor.empty(ctx, n)
-    eig_p, best_p = G.eig_round_plain(splats, scratch, cams, sizes, state, resp)  # warm
-    t = []
-    for _ in range(args.eig_reps):
-        t0 = time.perf_counter()
-        eig_p, best_p = G.eig_round_plain(splats, scratch, cams, sizes, state, resp)
-        t.append(time.perf_counter() - t0)
-    plain_ms = 1e3 * float(np.median(t))
-    rnd = G.EigRound(splats, cams, sizes)
+    # Plain: the fastest tile width for the single-view path.
+    plain = {}
+    for tw in (8, 16, 32):
+        scratch = R.RenderScratch(ctx, n, args.eig_res, args.eig_res, tile_w=tw)
+        scratch.reserve_isects(1 << 24)
+        eig_p, best_p = G.eig_round_plain(splats, scratch, cams, sizes, state, resp)  # warm
+        t = []
+        for _ in range(args.eig_reps):
+            t0 = time.perf_counter()
+            eig_p, best_p = G.eig_round_plain(splats, scratch, cams, sizes, state, resp)
+            t.append(time.perf_counter() - t0)
+        plain[tw] = (1e3 * float(np.median(t)), eig_p, best_p)
+        del scratch
+    plain_tw = min(plain, key=lambda k_: plain[k_][0])
+    plain_ms, eig_p, best_p = plain[plain_tw]
+    # Batched: the autotuner's choice of tile width x digit width.
+    (tw_b, db_b), table = G.autotune_round(splats, cams, sizes, state, reps=3)
+    rnd = G.EigRound(splats, cams, sizes, tile_w=tw_b, digit_bits=db_b)
     rnd.run(state, use_graph=True)
     tb = []
     for _ in range(args.eig_reps):
@@ -94,9 +103,18 @@ def bench_eig_round(ctx, args):
         eig_b, best_b = rnd.run(state, use_graph=True)
         tb.append(time.perf_counter() - t0)
     batch_ms = 1e3 * float(np.median(tb))
-    exact = bool(np.array_equal(rnd.resp(args.eig_views - 1)[0], resp.read()))
+    # evidence of the last view against a TW-matched plain render
+    sc = R.RenderScratch(ctx, n, args.eig_res, args.eig_res, tile_w=tw_b)
+    sc.reserve_isects(1 << 24)
+    acc = G.Accumulators(ctx, n)
+    G.render_responsibility(splats, sc, acc, cams[-1], args.eig_res, args.eig_res)
+    exact = bool(np.array_equal(rnd.resp(args.eig_views - 1)[0], acc.bits.read()))
     return {"views": args.eig_views, "res": args.eig_res, "plain_ms": round(plain_ms, 3),
-            "batched_ms": round(batch_ms, 3), "ratio": round(batch_ms / plain_ms, 4),
+            "plain_tile_w": plain_tw,
+            "plain_ms_by_tile_w": {str(k_): round(v_[0], 3) for k_, v_ in plain.items()},
+            "batched_ms": round(batch_ms, 3), "batched_tile_w": tw_b, "batched_digit_bits": db_b,
+            "autotune_ms": {f"tile{t_}_digit{d_}": round(v_, 3) for (t_, d_), v_ in table.items()},
+            "ratio": round(batch_ms / plain_ms, 4),
             "views_per_s": round(args.eig_views / (batch_ms * 1e-3), 1),
             "best_plain": best_p, "best_batched": best_b,
             "eig_max_abs_diff": float(np.max(np.abs(eig_b - eig_p))),
