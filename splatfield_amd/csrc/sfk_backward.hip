@@ -50,6 +50,8 @@ __global__ __launch_bounds__(256) void k_sum_f64(const double* __restrict__ part
     if (threadIdx.x == 0) *out = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
 }
 
+constexpr int kBwdMembers = 4;
+
 struct BwdArgs {
     uint32_t w, h, tiles_x, tiles_y, n;
     const uint32_t* tile_starts;
@@ -79,7 +81,22 @@ __device__ __forceinline__ double block_sum_f64(double v, double* s_w) {
     return t;
 }
 
-template <int TW>
+template <int M>
+struct MemberSum;
+template <>
+struct MemberSum<8> {
+    __device__ static float sum(const float (&v)[8]) { return wave_sum8_transposed(v); }
+    static constexpr uint32_t kShift = 3;  // lane l holds member l >> 3
+};
+template <>
+struct MemberSum<4> {
+    __device__ static float sum(const float (&v)[4]) { return wave_sum4_transposed(v); }
+    static constexpr uint32_t kShift = 4;  // lane l holds member l >> 4
+};
+
+// M = members per group: the gradients of M splats are summed over the wave
+// in one transposed pass; M = 4 halves the registers held across a group.
+template <int TW, int M>
 __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
     constexpr int NT = TW * TW;
     constexpr int NW = NT / 64;
@@ -167,11 +184,11 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
         // whose top is `itop` is list entry itop - 1 - s.  The 9 x 8 per-pixel
         // terms are reduced over the wave in one transposed pass; lane l ends
         // with the sums of member l >> 3.
-        for (uint32_t itop = nl; itop > 0; itop = itop > 8 ? itop - 8 : 0) {
-            float g[9][8];
+        for (uint32_t itop = nl; itop > 0; itop = itop > (uint32_t)M ? itop - M : 0) {
+            float g[9][M];
             bool has = false;
 #pragma unroll
-            for (int s = 0; s < 8; ++s) {
+            for (int s = 0; s < M; ++s) {
                 // Member s, or (masked off) the group's top entry again.
                 const bool member = s < (int)itop;
                 const uint32_t j = list[member ? itop - 1 - s : itop - 1];
@@ -226,12 +243,12 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
             }
             if (__ballot(has)) {
                 const uint32_t lane = lane_id();
-                const uint32_t ms = lane >> 3;
-                const bool writer = (lane & 7u) == 0 && ms < itop;
+                const uint32_t ms = lane >> MemberSum<M>::kShift;
+                const bool writer = (lane & ((1u << MemberSum<M>::kShift) - 1u)) == 0 && ms < itop;
                 const uint32_t j = writer ? list[itop - 1 - ms] : 0u;
                 float r[9];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) r[k] = wave_sum8_transposed(g[k]);
+                for (int k = 0; k < 9; ++k) r[k] = MemberSum<M>::sum(g[k]);
                 if (writer) {
                     const float4 geo = s_geo[j];
                     const float qc = s_aux[j].x;
@@ -502,10 +519,11 @@ hipError_t launch_raster_bwd(hipStream_t s, const RasterArgs& ra, const float* f
     }
     uint32_t nt = ra.tiles_x * ra.tiles_y;
     if (nt == 0) return hipSuccess;
+    constexpr int M = kBwdMembers;
     switch (ra.tile_w) {
-        case 8: k_raster_bwd<8><<<nt, 64, 0, s>>>(a); break;
-        case 16: k_raster_bwd<16><<<nt, 256, 0, s>>>(a); break;
-        case 32: k_raster_bwd<32><<<nt, 1024, 0, s>>>(a); break;
+        case 8: k_raster_bwd<8, M><<<nt, 64, 0, s>>>(a); break;
+        case 16: k_raster_bwd<16, M><<<nt, 256, 0, s>>>(a); break;
+        case 32: k_raster_bwd<32, M><<<nt, 1024, 0, s>>>(a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

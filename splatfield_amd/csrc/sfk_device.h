@@ -98,6 +98,32 @@ __device__ __forceinline__ T wave_sum8_transposed(const T (&v)[8]) {
     return x;
 }
 
+// Four sums over the 64 lanes at once: the same transposed butterfly as
+// wave_sum8_transposed stopped one level earlier.  Lane l returns the wave
+// sum of v[l >> 4] (each sum in the 16 lanes of row l >> 4); 10 VALU
+// operations for four sums.
+template <typename T>
+__device__ __forceinline__ T wave_sum4_transposed(const T (&v)[4]) {
+    static_assert(sizeof(T) == 4, "32-bit lanes");
+    auto bits = [](T x) { return __builtin_bit_cast(uint32_t, x); };
+    auto val = [](uint32_t x) { return __builtin_bit_cast(T, x); };
+    // u[i]: lanes 0-31 v[i], lanes 32-63 v[i + 2], summed over l, l ^ 32.
+    T u[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        auto r = __builtin_amdgcn_permlane32_swap(bits(v[i]), bits(v[i + 2]), false, false);
+        u[i] = val(r[0]) + val(r[1]);
+    }
+    // Row r holds v[r] summed over l, l ^ 16, l ^ 32, l ^ 48.
+    auto r = __builtin_amdgcn_permlane16_swap(bits(u[0]), bits(u[1]), false, false);
+    T x = val(r[0]) + val(r[1]);
+    x += val((uint32_t)SFK_DPP_I((int)bits(x), 0x128, 0xf));  // row_ror:8
+    x += val((uint32_t)SFK_DPP_I((int)bits(x), 0x141, 0xf));  // row_half_mirror
+    x += val((uint32_t)SFK_DPP_I((int)bits(x), 0x4e, 0xf));   // quad_perm [2,3,0,1]
+    x += val((uint32_t)SFK_DPP_I((int)bits(x), 0xb1, 0xf));   // quad_perm [1,0,3,2]
+    return x;
+}
+
 // Inclusive prefix sum across the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
     uint32_t lane = lane_id();
