@@ -263,6 +263,13 @@ int sfk_eig_device(sfk_ctx* ctx, const float* a, const float* b, const uint32_t*
 /* BetaState::update (src/seg/beta.rs:484-500): a += fg/scale, b += bg/scale. */
 int sfk_beta_update(sfk_ctx* ctx, float* a, float* b, const uint32_t* fg, const uint32_t* bg,
                     uint32_t n, float scale);
+/* views::localize (src/seg/views.rs:32-67) on the device: centre = sum(m p) /
+ * sum(m) and radius = sum(m |p - centre|) / sum(m) over splats with a > b,
+ * m = a / (a + b), p = the splat mean, accumulated in f64 (SYNC: 40 bytes
+ * read back instead of the reference's 8 B/splat a,b readback).
+ * *found = 0 when no splat is foreground (centre and radius then 0). */
+int sfk_localize(sfk_ctx* ctx, const sfk_splats* splats, const float* a, const float* b,
+                 float* center3, float* radius, int* found);
 
 /* ------------------------------------- batched EIG round (no reference
  * counterpart: the reference scores candidates one plain render at a time,

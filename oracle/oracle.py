@@ -258,6 +258,25 @@ def beta_update(a, b, fg, bg, scale):
     return a, b
 
 
+def localize(attrs, n, a, b):
+    """views::localize (src/seg/views.rs:32-67) in f64: centre = sum(m p) /
+    sum(m), radius = sum(m |p - centre|) / sum(m) over a > b, m = a/(a+b);
+    None when no splat is foreground.  (The reference accumulates in f32 in
+    index order; f64 is the value that loop approximates.)"""
+    A = np.asarray(attrs, np.float32).reshape(-1)
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    fg = a > b
+    if not fg.any():
+        return None
+    m = (a[fg] / (a[fg] + b[fg])).astype(np.float64)
+    p = np.stack([A[0 * n:1 * n], A[1 * n:2 * n], A[2 * n:3 * n]], 1)[fg].astype(np.float64)
+    sw = m.sum()
+    c = (m[:, None] * p).sum(0) / sw
+    r = (m * np.linalg.norm(p - c, axis=1)).sum() / sw
+    return c, r
+
+
 def tile_keys_u64(tile_ids, gaussian_ids, depth_keys):
     """The sorted u64 ``tile << 32 | depth`` keys of the two-stage sort."""
     return (np.asarray(tile_ids, np.uint64) << np.uint64(32)) | \

@@ -107,6 +107,7 @@ SIGNATURES = {
     "sfk_loss_l2": (i32, [P, P, P, u32, P, P]),
     "sfk_render_backward": (i32, [P, P, P, C.POINTER(TileIsects), P, P, P]),
     "sfk_render_backward_l2": (i32, [P, P, P, C.POINTER(TileIsects), P, P, P, P, P]),
+    "sfk_localize": (i32, [P, P, P, P, P, P, P]),
     "sfk_project": (i32, [P, C.POINTER(View), P, P, u32, u32, u32, P, P, P, P, P]),
     "sfk_map_isects": (i32, [P, P, P, P, P, u32, u32, u32, P, P]),
     "sfk_tile_starts": (i32, [P, P, u32, P, u32, P]),

@@ -228,5 +228,10 @@ hipError_t launch_eig(hipStream_t s, const float* a, const float* b, const uint3
 uint32_t eig_num_partials(uint32_t n);
 hipError_t launch_beta_update(hipStream_t s, float* a, float* b, const uint32_t* fg,
                               const uint32_t* bg, uint32_t n, float scale);
+// views::localize on the device; partials: 4 * localize_blocks(n) doubles;
+// out (device): [cx, cy, cz, sum of weights, radius].
+uint32_t localize_blocks(uint32_t n);
+hipError_t launch_localize(hipStream_t s, const float* attrs, const float* a, const float* b,
+                           uint32_t n, double* partials, double* out);
 
 }  // namespace sfk

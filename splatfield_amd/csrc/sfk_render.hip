@@ -557,8 +557,8 @@ extern "C" int sfk_render_responsibility(sfk_ctx* c, const sfk_splats* sp,
     return sfk_accumulate_prepared(c, s, &t, nullptr, 0, resp, nullptr, nullptr, scale_out);
 }
 
-static int eig_workspace(sfk_ctx* c, uint32_t n) {
-    uint32_t np = eig_num_partials(n);
+// At least `np` doubles of context workspace.
+static int workspace_doubles(sfk_ctx* c, uint32_t np) {
     if (np > c->eig_partials_cap) {
         if (c->eig_partials) hipFree(c->eig_partials);
         c->eig_partials = nullptr;
@@ -568,6 +568,8 @@ static int eig_workspace(sfk_ctx* c, uint32_t n) {
     }
     return SFK_OK;
 }
+
+static int eig_workspace(sfk_ctx* c, uint32_t n) { return workspace_doubles(c, eig_num_partials(n)); }
 
 extern "C" int sfk_eig_device(sfk_ctx* c, const float* a, const float* b, const uint32_t* resp,
                               uint32_t n, float scale, double* d_out) {
@@ -634,7 +636,7 @@ extern "C" int sfk_tile_starts(sfk_ctx* c, const uint32_t* tile_ids, uint32_t n,
 extern "C" int sfk_loss_l2(sfk_ctx* c, const float* rgba, const float* target, uint32_t pixels,
                            float* d_rgba, double* d_loss) {
     if (!c || (pixels && (!rgba || !target || !d_rgba))) return SFK_E_ARG;
-    int rc = eig_workspace(c, loss_num_partials(pixels) * 1024u);
+    int rc = workspace_doubles(c, loss_num_partials(pixels));
     if (rc) return rc;
     SFK_HIP(c, launch_loss_l2(c->stream, rgba, target, pixels, d_rgba, c->eig_partials, d_loss));
     return SFK_OK;
@@ -678,6 +680,25 @@ static int render_backward_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scr
     SFK_HIP(c, launch_project_bwd(c->stream, s->view, sp->attrs, sp->sh, sp->n, sp->sh_per_ch,
                                   s->grad2d, grad_attrs, grad_sh, d_rgba ? nullptr : &loss));
     s->grad2d_dirty = false;
+    return SFK_OK;
+}
+
+extern "C" int sfk_localize(sfk_ctx* c, const sfk_splats* sp, const float* a, const float* b,
+                            float* center3, float* radius, int* found) {
+    if (!c || !sp || !a || !b || !center3 || !radius || !found) return SFK_E_ARG;
+    const uint32_t nb = localize_blocks(sp->n);
+    int rc = workspace_doubles(c, nb * 4u + 8u);
+    if (rc) return rc;
+    double* out = c->eig_partials + (size_t)nb * 4;
+    SFK_HIP(c, launch_localize(c->stream, sp->attrs, a, b, sp->n, c->eig_partials, out));
+    double h[5];
+    SFK_HIP(c, hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    SFK_HIP(c, hipStreamSynchronize(c->stream));
+    *found = h[3] > 0.0 ? 1 : 0;
+    center3[0] = (float)h[0];
+    center3[1] = (float)h[1];
+    center3[2] = (float)h[2];
+    *radius = (float)h[4];
     return SFK_OK;
 }
 

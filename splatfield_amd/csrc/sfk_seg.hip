@@ -85,7 +85,97 @@ __global__ __launch_bounds__(256) void k_beta_update(float* __restrict__ a, floa
     }
 }
 
+// views::localize (views.rs:32-67) on the device: posterior-weighted centre
+// and mean distance of the splats the MAP labels foreground (a > b, weight
+// m = a / (a + b)).  Pass 0 sums (m, m x, m y, m z); pass 1 sums m |p - c|
+// with c from pass 0.  f64 partials per block, summed in block order by the
+// last launch of each pass (deterministic; the reference's sequential f32
+// loop carries its own rounding).
+constexpr uint32_t kLocThreads = 256;
+
+template <int PASS>
+__global__ __launch_bounds__(kLocThreads) void k_localize(const float* __restrict__ A,
+                                                          const float* __restrict__ a,
+                                                          const float* __restrict__ b, uint32_t n,
+                                                          const double* __restrict__ center,
+                                                          double* __restrict__ partials) {
+    constexpr int NV = PASS == 0 ? 4 : 1;
+    __shared__ double s_w[NV][kLocThreads / 64];
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    const size_t N = n;
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    if constexpr (PASS == 1) {
+        cx = center[0]; cy = center[1]; cz = center[2];
+    }
+    for (uint32_t i = blockIdx.x * kLocThreads + threadIdx.x; i < n;
+         i += gridDim.x * kLocThreads) {
+        const float ai = a[i], bi = b[i];
+        if (!(ai > bi)) continue;
+        const double m = (double)(ai / (ai + bi));
+        const double x = A[i], y = A[N + i], z = A[2 * N + i];
+        if constexpr (PASS == 0) {
+            acc[0] += m;
+            acc[1] += m * x;
+            acc[2] += m * y;
+            acc[3] += m * z;
+        } else {
+            const double dx = x - cx, dy = y - cy, dz = z - cz;
+            acc[0] += m * sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double v = wave_sum_f64(acc[k]);
+        if (lane_id() == 0) s_w[k][threadIdx.x / 64] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)NV) {
+        double t = 0.0;
+        for (uint32_t w = 0; w < kLocThreads / 64; ++w) t += s_w[threadIdx.x][w];
+        partials[(size_t)blockIdx.x * NV + threadIdx.x] = t;
+    }
+}
+
+// Fixed-order reduction of the partials; pass 0 turns (sw, sx, sy, sz) into
+// the centre, pass 1 writes the radius.  out = [cx, cy, cz, sw, radius].
+template <int PASS>
+__global__ void k_localize_final(const double* __restrict__ partials, uint32_t nb,
+                                 double* __restrict__ out) {
+    constexpr int NV = PASS == 0 ? 4 : 1;
+    if (threadIdx.x != 0) return;
+    double t[NV];
+    for (int k = 0; k < NV; ++k) t[k] = 0.0;
+    for (uint32_t i = 0; i < nb; ++i)
+        for (int k = 0; k < NV; ++k) t[k] += partials[(size_t)i * NV + k];
+    if constexpr (PASS == 0) {
+        const double sw = t[0];
+        out[3] = sw;
+        out[0] = sw > 0.0 ? t[1] / sw : 0.0;
+        out[1] = sw > 0.0 ? t[2] / sw : 0.0;
+        out[2] = sw > 0.0 ? t[3] / sw : 0.0;
+    } else {
+        out[4] = out[3] > 0.0 ? t[0] / out[3] : 0.0;
+    }
+}
+
 }  // namespace
+
+uint32_t localize_blocks(uint32_t n) {
+    const uint32_t nb = div_up(n ? n : 1, kLocThreads * 16u);
+    return nb < 1024u ? nb : 1024u;
+}
+
+hipError_t launch_localize(hipStream_t s, const float* attrs, const float* a, const float* b,
+                           uint32_t n, double* partials, double* out) {
+    const uint32_t nb = localize_blocks(n);
+    k_localize<0><<<nb, kLocThreads, 0, s>>>(attrs, a, b, n, nullptr, partials);
+    k_localize_final<0><<<1, 64, 0, s>>>(partials, nb, out);
+    k_localize<1><<<nb, kLocThreads, 0, s>>>(attrs, a, b, n, out, partials);
+    k_localize_final<1><<<1, 64, 0, s>>>(partials, nb, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* buf, uint32_t v, size_t count) {
     if (count == 0) return hipSuccess;

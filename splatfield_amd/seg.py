@@ -114,6 +114,21 @@ def map_labels(a, b):
     return a > b
 
 
+def localize(splats, state):
+    """views::localize (views.rs:32-67) on the device: (centre, radius) of
+    the posterior-weighted foreground (a > b, weight a / (a + b)), or None
+    when no splat is foreground."""
+    ctx = splats.ctx
+    center = np.zeros(3, np.float32)
+    radius = C.c_float()
+    found = C.c_int()
+    ctx.check(lib().sfk_localize(ctx.ptr, splats.ptr, state.a.ptr, state.b.ptr,
+                                 center.ctypes.data, C.byref(radius), C.byref(found)), "localize")
+    if not found.value:
+        return None
+    return center, radius.value
+
+
 CANDIDATE_FOV = BASE_FOV  # views.rs:69-71
 
 
@@ -227,3 +242,74 @@ def sphere_cameras(count, radius=3.0, center=(0.0, 0.0, 0.0), fov=CANDIDATE_FOV)
     c = np.asarray(center, np.float32)
     dirs = fibonacci_directions(count)
     return [Camera.look_at(c + np.float32(radius) * d, c, fov=(fov, fov)) for d in dirs]
+
+
+def bitmap_to_rgb(bitmap_words, width, height, row_stride):
+    """active.rs:240-256: the RGBA8 bitmap (row stride in words) unpacked to
+    tightly packed RGB bytes (alpha dropped), row-major."""
+    w = np.asarray(bitmap_words, np.uint32).reshape(height, row_stride)[:, :width]
+    return np.ascontiguousarray(w.view(np.uint8).reshape(height, width, 4)[:, :, :3]).reshape(-1)
+
+
+class Segmenter:
+    """Segmenter (active.rs:60-183) over the device pipeline: each step
+    localizes the posterior foreground on the device, scores candidate views
+    (batched round or the plain select_view loop), renders the best view,
+    hands its RGB bytes to the caller's 2D oracle, lifts the returned mask to
+    per-splat evidence over the same intersections and updates the Beta
+    posterior."""
+
+    def __init__(self, splats, attributes, camera, resolution=512, num_candidates=20,
+                 batched=True, tile_w=16):
+        from .render import RenderScratch
+        self.splats = splats
+        self.ctx = splats.ctx
+        self.camera = camera
+        self.res = int(resolution)
+        self.num_candidates = int(num_candidates)
+        self.batched = bool(batched)
+        self.tile_w = tile_w
+        n = splats.n
+        A = np.asarray(attributes, np.float32).reshape(-1)
+        p = np.stack([A[0:n], A[n:2 * n], A[2 * n:3 * n]], 1)
+        self.bounds = (p.min(0), p.max(0))  # render.rs:242-269
+        self.scratch = RenderScratch(self.ctx, n, self.res, self.res, tile_w=tile_w)
+        self.acc = Accumulators(self.ctx, n)
+        self.state = BetaState.new_uniform(self.ctx, n)
+        self.bootstrapped = False
+
+    def select_view(self, center, radius):
+        """active.rs:113-129: the EIG-best of the candidate cameras."""
+        cams = candidates(center, radius, self.num_candidates)
+        sizes = [(self.res, self.res)] * len(cams)
+        if self.batched:
+            rnd = EigRound(self.splats, cams, sizes, tile_w=self.tile_w)
+            eig, best = rnd.run(self.state, use_graph=False)
+            del rnd
+        else:
+            resp = GpuTensor.empty(self.ctx, self.splats.n)
+            eig, best = eig_round_plain(self.splats, self.scratch, cams, sizes, self.state, resp)
+        return cams[best], float(eig[best])
+
+    def step(self, oracle):
+        """active.rs:133-183.  oracle(rgb_bytes, (w, h), camera) -> one mask
+        byte per pixel.  Returns (camera, eig, foreground pixels)."""
+        from .render import finalize_rgb, prepare_isects
+        loc = localize(self.splats, self.state)
+        if loc is not None:
+            camera, eig = self.select_view(*loc)
+        elif not self.bootstrapped:
+            camera, eig = self.camera, 0.0
+        else:
+            lo, hi = self.bounds
+            camera, eig = self.select_view((lo + hi) * 0.5, float((hi - lo).max()) * 0.5)
+        self.bootstrapped = True
+        w = h = self.res
+        isects = prepare_isects(self.splats, self.scratch, camera, w, h)
+        finalize_rgb(self.scratch, isects)
+        b = self.scratch.buffers()
+        rgb = bitmap_to_rgb(self.scratch.read("bitmap"), w, h, b.row_stride)
+        mask = Mask(self.ctx, w, h, oracle(rgb, (w, h), camera))
+        accumulate_prepared(self.scratch, isects, self.acc, mask)
+        self.state.update(self.acc)
+        return camera, eig, mask.fg_pixels

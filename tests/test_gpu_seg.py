@@ -177,3 +177,59 @@ def test_eig_reference_cases(ctx):
     assert abs(fresh - (O.beta_entropy(1, 1) - O.beta_entropy(1.25, 1.25))) < 5e-9
     sat = eig1([5e5, 5e5, 1, 1], [5e5, 5e5, 1, 1], [0.5, 0, 0, 0])
     assert sat < 1e-2 < fresh
+
+
+def test_localize_matches_oracle(ctx):
+    """Device views::localize against the oracle's f64 restatement on a 1M
+    splat scene with a random posterior (f32 outputs, 1e-6 relative)."""
+    n = 1 << 20
+    attrs, sh, k = S.synthetic_scene(n, 0, seed=5)
+    sp = R.Splats(ctx, attrs, sh)
+    rng = np.random.default_rng(6)
+    a = (1 + rng.random(n) * 4).astype(np.float32)
+    b = (1 + rng.random(n) * 4).astype(np.float32)
+    st = G.BetaState(ctx, a, b)
+    got = G.localize(sp, st)
+    want = O.localize(attrs, n, a, b)
+    assert got is not None
+    np.testing.assert_allclose(got[0], want[0], rtol=1e-6, atol=1e-6)
+    assert abs(got[1] - want[1]) <= 1e-6 * want[1]
+    none = G.localize(sp, G.BetaState(ctx, b * 0 + 1, b * 0 + 1))
+    assert none is None
+
+
+def test_segmenter_steps(ctx):
+    """Segmenter::step (active.rs:133-183) over the device pipeline with a
+    synthetic oracle (a centred disk): round 0 observes from the bootstrap
+    camera with EIG 0; the disk's evidence moves the posterior; the next
+    round localizes on the device and the batched and plain view scorings
+    pick the same candidate with the same EIG."""
+    n = 65536
+    attrs, sh, k = S.synthetic_scene(n, 0, seed=0)
+    sp = R.Splats(ctx, attrs, sh)
+    cam0 = R.Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0))
+    res = 128
+    yy, xx = np.mgrid[0:res, 0:res]
+    disk = (((xx + 0.5 - res / 2) ** 2 + (yy + 0.5 - res / 2) ** 2) < (res / 4) ** 2).astype(np.uint8)
+    seen = []
+
+    def oracle(rgb, size, camera):
+        assert rgb.size == size[0] * size[1] * 3
+        seen.append(camera)
+        return disk.reshape(-1)
+
+    seg = G.Segmenter(sp, attrs, cam0, resolution=res, num_candidates=8, batched=True)
+    cam, eig, fg = seg.step(oracle)
+    assert cam is cam0 and eig == 0.0 and fg == int(disk.sum())
+    a, b = seg.state.posteriors()
+    assert (a >= 1).all() and (b >= 1).all() and (a > 1).any() and (b > 1).any()
+    loc = G.localize(sp, seg.state)
+    assert loc is not None
+    cb, eb = seg.select_view(*loc)
+    seg.batched = False
+    cp, ep = seg.select_view(*loc)
+    assert cb.position == cp.position and abs(eb - ep) <= 1e-6 * max(1.0, abs(ep))
+    seg.batched = True
+    cam2, eig2, _ = seg.step(oracle)
+    assert cam2.position == cb.position and eig2 > 0.0
+    assert len(seen) == 2

@@ -406,3 +406,36 @@ def test_candidate_sphere():
 
 def test_jitter_fixture():
     assert np.all(np.abs(np.array([jitter(i) for i in range(100)])) <= 0.5)
+
+
+def test_localize_closed_forms():
+    """views.rs:32-67: equal-weight foreground on a sphere of radius 2 about
+    (1, -1, 0.5) gives that centre and radius 2; background and ties (a == b)
+    are ignored; no foreground gives None; weights a/(a+b) pull the centre
+    toward the heavier splat."""
+    rng = np.random.default_rng(3)
+    n = 4000
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    c0 = np.array([1.0, -1.0, 0.5])
+    # antipodal pairs: the centroid is exactly c0
+    pts = np.concatenate([c0 + 2.0 * d[: n // 2], c0 - 2.0 * d[: n // 2]])
+    attrs = np.zeros((11, n), np.float32)
+    attrs[:3] = pts.T
+    a = np.full(n, 3.0, np.float32)
+    b = np.ones(n, np.float32)
+    c, r = O.localize(attrs, n, a, b)
+    np.testing.assert_allclose(c, c0, atol=2e-6)
+    assert abs(r - 2.0) < 2e-6
+    b2 = b.copy()
+    b2[n // 4:] = 3.0  # ties (a == b) are background
+    c2, _ = O.localize(attrs, n, a, b2)
+    sel = pts[: n // 4]
+    np.testing.assert_allclose(c2, sel.mean(0), atol=1e-6)
+    assert O.localize(attrs, n, b, a) is None
+    attrs3 = np.zeros((11, 2), np.float32)
+    attrs3[0] = [0.0, 1.0]
+    c3, r3 = O.localize(attrs3, 2, np.array([3.0, 1.5], np.float32), np.array([1.0, 1.0], np.float32))
+    w0, w1 = 0.75, 0.6
+    assert abs(c3[0] - w1 / (w0 + w1)) < 1e-7
+    assert abs(r3 - 2 * w0 * w1 / (w0 + w1) ** 2) < 1e-7
