@@ -98,8 +98,8 @@ hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, 
 
 // Radix sort: one pass = upsweep + digit scan + scatter.
 constexpr uint32_t kSortThreads = 256;
-constexpr uint64_t kSortSmallMax = 1u << 22;  // up to this many keys: 1024-key blocks
-uint32_t radix_keys_per_block(uint64_t total_keys);   // 1024 or 4096
+constexpr uint64_t kSortSmallMax = 1u << 22;  // up to this many keys: 2048-key blocks
+uint32_t radix_keys_per_block(uint64_t total_keys);   // 2048 or 4096
 uint32_t radix_max_blocks(uint32_t max_elems);
 struct SortArgs {
     const void* keys_in;

@@ -287,14 +287,14 @@ hipError_t pass_impl(hipStream_t st, const SortArgs& a) {
 }  // namespace
 
 uint32_t radix_keys_per_block(uint64_t total) {
-    return total <= kSortSmallMax ? NT * 4u : NT * 16u;
+    return total <= kSortSmallMax ? NT * 8u : NT * 16u;
 }
 
 // Histogram rows for any sort a scratch of max_elems keys can run.
 uint32_t radix_max_blocks(uint32_t max_elems) {
     const uint64_t m = max_elems > 0 ? max_elems : 1;
     const uint32_t big = div_up(m, NT * 16u);
-    const uint32_t small = div_up(std::min<uint64_t>(m, kSortSmallMax), NT * 4u);
+    const uint32_t small = div_up(std::min<uint64_t>(m, kSortSmallMax), NT * 8u);
     return big > small ? big : small;
 }
 
@@ -307,12 +307,12 @@ hipError_t pass_ept(hipStream_t st, const SortArgs& a, uint32_t digit_bits) {
     return pass_impl<K, V, 256, EPT>(st, a);
 }
 
-// Small sorts (up to kSortSmallMax keys) use 1024-key blocks: four times the
-// workgroups keep enough waves in flight where 4096-key blocks would leave
-// one per CU.
+// Small sorts (up to kSortSmallMax keys) use 2048-key blocks: twice the
+// workgroups keep more waves in flight where 4096-key blocks would leave one
+// per CU (measured: 1024-key blocks are slower again).
 template <typename K, typename V>
 hipError_t pass_radix(hipStream_t st, const SortArgs& a, uint32_t digit_bits) {
-    if (a.kpb == NT * 4u) return pass_ept<K, V, 4>(st, a, digit_bits);
+    if (a.kpb == NT * 8u) return pass_ept<K, V, 8>(st, a, digit_bits);
     return pass_ept<K, V, 16>(st, a, digit_bits);
 }
 
