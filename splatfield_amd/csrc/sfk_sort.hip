@@ -82,11 +82,23 @@ __global__ __launch_bounds__(NT) void k_upsweep(SortArgs a) {
     for (uint32_t i = threadIdx.x; i < NW * RADIX; i += NT) (&h[0][0])[i] = 0;
     __syncthreads();
     const uint32_t loc0 = p.bl * KPB + threadIdx.x;
+    // All loads first, then the LDS counts, so the loads overlap; a block
+    // fully inside its segment loads without per-element guards.
+    uint32_t dg[EPT];
+    const K* kp = keys + p.seg_base + loc0;
+    if (p.bl * KPB + KPB <= p.seg_n) {
 #pragma unroll
-    for (uint32_t e = 0; e < EPT; ++e) {
-        uint32_t loc = loc0 + e * NT;
-        if (loc < p.seg_n) atomicAdd(&h[w][digit_of(keys[p.seg_base + loc], a.shift, mask)], 1u);
+        for (uint32_t e = 0; e < EPT; ++e) dg[e] = digit_of(kp[e * NT], a.shift, mask);
+    } else {
+#pragma unroll
+        for (uint32_t e = 0; e < EPT; ++e) {
+            const uint32_t loc = loc0 + e * NT;
+            dg[e] = loc < p.seg_n ? digit_of(kp[e * NT], a.shift, mask) : ~0u;
+        }
     }
+#pragma unroll
+    for (uint32_t e = 0; e < EPT; ++e)
+        if (dg[e] != ~0u) atomicAdd(&h[w][dg[e]], 1u);
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < RADIX; d += NT) {
         uint32_t t = 0;
