@@ -169,6 +169,12 @@ int sfk_render_scratch_set_tile_width(sfk_render_scratch* s, uint32_t tile_w);
 int sfk_render_scratch_reserve_isects(sfk_render_scratch* s, uint32_t isects);
 /* Radix digit width of the scratch's sorts (4..8). */
 int sfk_render_scratch_set_digit_bits(sfk_render_scratch* s, uint32_t digit_bits);
+/* Isect stage on grids of at most 255 x 255 tiles: on (the default), the
+ * binned emission writes every tile's isects in depth order without sorting
+ * them; off, emission + stable tile argsort as render.rs:271-364 does.  Both
+ * give the same arrays word for word (a tuning and test knob, no counterpart
+ * in the reference). */
+int sfk_render_scratch_set_binned(sfk_render_scratch* s, int on);
 
 /* TileIsects (src/render.rs:463-471) plus the device-side counters. */
 typedef struct {

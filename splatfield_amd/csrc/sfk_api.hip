@@ -326,6 +326,15 @@ extern "C" void sfk_radix_scratch_destroy(sfk_radix_scratch* s) {
     delete s;
 }
 
+// An argsort reads its input in the first pass only, and that pass writes
+// keys[0] / vals[0]: the second buffer pair may hold the input itself.
+int sfk::radix_scratch_spare_u32(sfk_radix_scratch* s, uint32_t** keys, uint32_t** vals) {
+    if (!s || s->key_bytes != 4 || s->val_bytes != 4) return SFK_E_ARG;
+    *keys = static_cast<uint32_t*>(s->keys[1]);
+    *vals = s->vals[1];
+    return SFK_OK;
+}
+
 extern "C" int sfk_radix_set_digit_bits(sfk_radix_scratch* s, uint32_t bits) {
     if (!s || bits < 4 || bits > 8) return SFK_E_ARG;
     s->digit_bits = bits;

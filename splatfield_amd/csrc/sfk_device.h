@@ -124,6 +124,89 @@ __device__ __forceinline__ T wave_sum4_transposed(const T (&v)[4]) {
     return x;
 }
 
+// 64 x 64 bit-matrix transpose across a wavefront: lane l holds row l (bit j
+// = element (l, j)) and receives column l (bit i = element (i, l)).  Stage k
+// (k = 32, 16, 8, 4, 2, 1) swaps bit k of the lane index with bit k of the
+// bit index: element (l, j) with (l & k) != (j & k) trades places with
+// (l ^ k, j ^ k).  Lanes move by v_permlane32_swap (k = 32),
+// v_permlane16_swap (16), DPP row_ror:8 (8), a bpermute (4) and DPP
+// quad_perm (2, 1); bits by v_perm byte selects (k >= 8) or a rotate and a
+// bit-field insert (k < 8).  About 30 VALU operations for the 4096 bits.
+struct TransposeLanes {
+    uint32_t sel16, sel8;     // v_perm selectors of the byte stages
+    uint32_t rot[3], msk[3];  // rotate amount and insert mask of stages 4, 2, 1
+};
+
+__device__ __forceinline__ TransposeLanes transpose_lanes() {
+    const uint32_t l = lane_id();
+    TransposeLanes c;
+    c.sel16 = (l & 16u) ? 0x07060302u : 0x05040100u;
+    c.sel8 = (l & 8u) ? 0x03070105u : 0x06020400u;
+    // Lanes with the stage bit clear take the partner's low field into their
+    // high field (rotate left by k); the others the partner's high field into
+    // their low field (rotate right by k).
+    c.rot[0] = (l & 4u) ? 4u : 28u;
+    c.msk[0] = (l & 4u) ? 0x0f0f0f0fu : 0xf0f0f0f0u;
+    c.rot[1] = (l & 2u) ? 2u : 30u;
+    c.msk[1] = (l & 2u) ? 0x33333333u : 0xccccccccu;
+    c.rot[2] = (l & 1u) ? 1u : 31u;
+    c.msk[2] = (l & 1u) ? 0x55555555u : 0xaaaaaaaau;
+    return c;
+}
+
+__device__ __forceinline__ uint32_t transpose_insert(uint32_t w, uint32_t partner, uint32_t rot,
+                                                     uint32_t msk) {
+    const uint32_t t = __builtin_amdgcn_alignbit(partner, partner, rot);  // rotate right
+    return (t & msk) | (w & ~msk);
+}
+
+__device__ __forceinline__ uint64_t wave_transpose64(uint64_t row, const TransposeLanes& c) {
+    uint32_t lo = (uint32_t)row, hi = (uint32_t)(row >> 32);
+    // k = 32: lanes 0-31 take their partner's low word as the high word,
+    // lanes 32-63 their partner's high word as the low word.
+    {
+        auto r = __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+        lo = r[0];
+        hi = r[1];
+    }
+    // k = 16: permlane16_swap(w, w) leaves (w, partner) in (r[0], r[1]) on
+    // even rows and (partner, w) on odd rows.
+    {
+        auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        lo = __builtin_amdgcn_perm(a[1], a[0], c.sel16);
+        auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        hi = __builtin_amdgcn_perm(b[1], b[0], c.sel16);
+    }
+    // k = 8: row_ror:8 is lane ^ 8.
+    {
+        const uint32_t plo = (uint32_t)SFK_DPP_I((int)lo, 0x128, 0xf);
+        const uint32_t phi = (uint32_t)SFK_DPP_I((int)hi, 0x128, 0xf);
+        lo = __builtin_amdgcn_perm(plo, lo, c.sel8);
+        hi = __builtin_amdgcn_perm(phi, hi, c.sel8);
+    }
+    // k = 4
+    {
+        const uint32_t plo = (uint32_t)__shfl_xor((int)lo, 4, 64);
+        const uint32_t phi = (uint32_t)__shfl_xor((int)hi, 4, 64);
+        lo = transpose_insert(lo, plo, c.rot[0], c.msk[0]);
+        hi = transpose_insert(hi, phi, c.rot[0], c.msk[0]);
+    }
+    // k = 2: quad_perm [2,3,0,1]; k = 1: quad_perm [1,0,3,2].
+    {
+        const uint32_t plo = (uint32_t)SFK_DPP_I((int)lo, 0x4e, 0xf);
+        const uint32_t phi = (uint32_t)SFK_DPP_I((int)hi, 0x4e, 0xf);
+        lo = transpose_insert(lo, plo, c.rot[1], c.msk[1]);
+        hi = transpose_insert(hi, phi, c.rot[1], c.msk[1]);
+    }
+    {
+        const uint32_t plo = (uint32_t)SFK_DPP_I((int)lo, 0xb1, 0xf);
+        const uint32_t phi = (uint32_t)SFK_DPP_I((int)hi, 0xb1, 0xf);
+        lo = transpose_insert(lo, plo, c.rot[2], c.msk[2]);
+        hi = transpose_insert(hi, phi, c.rot[2], c.msk[2]);
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Inclusive prefix sum across the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
     uint32_t lane = lane_id();

@@ -75,8 +75,9 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 
 // Slot pairs of the informational (isects, visible) counts of a projection.
 constexpr uint32_t kCountSlots = 64;
-// Render counters: [0] total isects (written by the gather scan), [1] unused,
-// then kCountSlots (isects, visible) pairs added by the projection.
+// Render counters: [0] total isects (written by the gather scan), [1] the
+// binned emission's pair count, then kCountSlots (isects, visible) pairs
+// added by the projection.
 constexpr uint32_t kCounterWords = 2 + 2 * kCountSlots;
 
 uint32_t* scan_scratch_sums(sfk_scan_scratch* s);
@@ -126,6 +127,9 @@ int radix_scratch_create_ex(sfk_ctx* c, uint32_t max_elems, uint32_t key_bytes,
 int argsort_payload_u32(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys,
                         const uint32_t* payload, uint32_t n, uint32_t nseg, uint32_t bits,
                         uint2** out);
+// The buffer pair of a 4-byte-key, 4-byte-value scratch that an argsort on it
+// may take as input (the first pass writes the other pair).
+int radix_scratch_spare_u32(sfk_radix_scratch* s, uint32_t** keys, uint32_t** vals);
 
 // Scan.
 constexpr uint32_t kScanBlock = 4096;
@@ -163,6 +167,54 @@ struct BatchView {
     float scale;         // fixed-point scale 2^fractional_bits(w, h)
     uint32_t pad[2];
 };
+
+// Binned isect emission (compact grids): the isects of every tile in depth
+// order, as emission + stable tile argsort + tile ranges produce them,
+// without sorting the isects (sfk_bin.hip).
+constexpr uint32_t kBinSuper = 8;     // tiles per supertile side
+constexpr uint32_t kBinChunk = 512;   // sorted pairs per wavefront chunk
+struct BinView {
+    uint32_t tiles_x, tiles_y;  // this view's tile grid
+    uint32_t supers_x;          // supertiles per row, ceil(tiles_x / 8)
+    uint32_t super_base;        // first global supertile id of the view
+    uint32_t out_mul, out_add;  // tile_starts index = local tile * out_mul + out_add
+    uint32_t gid_add;           // written splat id = in-view splat + gid_add
+    uint32_t pad;
+};
+inline uint32_t bin_supers(uint32_t tiles_x, uint32_t tiles_y) {
+    return div_up(tiles_x, kBinSuper) * div_up(tiles_y, kBinSuper);
+}
+// Chunk slots for `cap` pairs over `supers` supertiles: chunk k of
+// supertile s is slot start[s] / kBinChunk + s + k (disjoint ranges).
+inline uint32_t bin_chunk_slots(uint32_t cap, uint32_t supers) {
+    return cap / kBinChunk + supers + 1;
+}
+struct BinJob {
+    const uint2* ordered;   // (in-view splat, compact bbox) per depth rank
+    uint32_t n;             // ranks (all views)
+    uint32_t seg_len;       // ranks per view when batched, else 0
+    const BinView* views;   // device, one per view (batched), or nullptr:
+    BinView view0;          //   the single view, by value
+    uint32_t nviews;
+    uint32_t supers;        // supertiles over all views
+    uint32_t out_tiles;     // tile_starts entries before the total
+    bool zero_starts;       // some tile_starts entries belong to no supertile
+    uint32_t cap;           // isect capacity (the reference's max_isects), >= 2
+    uint32_t* order_out;    // optional: depth order
+    uint32_t* offsets;      // per rank: exclusive isect offset (n words)
+    sfk_radix_scratch* sort;  // >= cap elements, 4-byte keys and values; the
+                              // (supertile, rank) pairs live in its spare pair
+    uint32_t* sstart;       // supers + 1
+    uint32_t* chunk_map;    // 4 words per chunk slot: bin_chunk_slots(cap, supers)
+    uint32_t* chunk_counts; // 64 per chunk slot
+    uint32_t* scan_sums;    // div_up(out_tiles, kScanBlock) words
+    uint32_t* tile_starts;  // out_tiles + 1
+    uint32_t* gids;         // cap: splat per isect
+    uint32_t* tile_ids;     // cap or nullptr: tile per isect
+    uint32_t* counters;     // [0] raw isect count, [1] pair count
+    uint32_t* status;       // scan_emit_status_words(n)
+};
+int bin_emit(sfk_ctx* c, const BinJob& j);
 
 // Raster stage.
 hipError_t launch_map_isects(hipStream_t s, const uint32_t* order, const uint32_t* tile_counts,

@@ -125,13 +125,23 @@ struct sfk_render_scratch {
     bool grad2d_dirty = true; // grad2d not known to be all zero
     double* loss_partials = nullptr;  // one per tile (fused L2 backward)
     uint32_t loss_partials_cap = 0;
+    // Binned emission (compact grids): supertile ranges, chunk map, per-chunk
+    // tile counts (the last two sized by the isect capacity).
+    bool binned = true;
+    uint32_t* bin_sstart = nullptr;
+    uint32_t* bin_map = nullptr;
+    uint32_t* bin_counts = nullptr;
 };
+
+static bool compact_grid(const sfk_render_scratch* s) {
+    return s->tiles_x <= kCompactGrid && s->tiles_y <= kCompactGrid;
+}
 
 static void scratch_free_frame(sfk_render_scratch* s) {
     void* bufs[] = {s->depth_keys, s->projected, s->tile_counts, s->bbox, s->offsets,
                     s->counters, s->tile_starts, s->bitmap, s->rgba, s->final_t, s->n_contrib,
                     s->tile_ids, s->gaussian_ids, s->grad2d, s->emit_status, s->cbbox,
-                    s->order_buf};
+                    s->order_buf, s->bin_sstart, s->bin_map, s->bin_counts};
     for (void* p : bufs)
         if (p) hipFree(p);
     s->depth_keys = nullptr; s->projected = nullptr; s->tile_counts = nullptr; s->bbox = nullptr;
@@ -142,6 +152,7 @@ static void scratch_free_frame(sfk_render_scratch* s) {
     s->grad2d_dirty = true;
     s->emit_status = nullptr;
     s->cbbox = s->order_buf = nullptr;
+    s->bin_sstart = s->bin_map = s->bin_counts = nullptr;
     s->has_view = false;
     sfk_radix_scratch_destroy(s->sort_depth);
     sfk_radix_scratch_destroy(s->sort_tile);
@@ -151,14 +162,20 @@ static void scratch_free_frame(sfk_render_scratch* s) {
 
 static int alloc_isects(sfk_render_scratch* s, uint32_t cap) {
     sfk_ctx* c = s->ctx;
-    if (s->tile_ids) hipFree(s->tile_ids);
-    if (s->gaussian_ids) hipFree(s->gaussian_ids);
-    s->tile_ids = s->gaussian_ids = nullptr;
+    void* bufs[] = {s->tile_ids, s->gaussian_ids, s->bin_map, s->bin_counts};
+    for (void* p : bufs)
+        if (p) hipFree(p);
+    s->tile_ids = s->gaussian_ids = s->bin_map = s->bin_counts = nullptr;
     sfk_radix_scratch_destroy(s->sort_tile);
     s->sort_tile = nullptr;
     s->sorted_tiles = s->sorted_gids = nullptr;
     SFK_HIP(c, hipMalloc(&s->tile_ids, (size_t)cap * 4));
     SFK_HIP(c, hipMalloc(&s->gaussian_ids, (size_t)cap * 4));
+    if (compact_grid(s)) {
+        const size_t slots = bin_chunk_slots(cap, bin_supers(s->tiles_x, s->tiles_y));
+        SFK_HIP(c, hipMalloc(&s->bin_map, slots * 16));
+        SFK_HIP(c, hipMalloc(&s->bin_counts, slots * 64 * 4));
+    }
     int rc = sfk_radix_scratch_create(c, cap, 4, &s->sort_tile);
     if (rc) return rc;
     sfk_radix_set_digit_bits(s->sort_tile, s->digit_bits);
@@ -193,6 +210,9 @@ static int scratch_build(sfk_render_scratch* s, uint32_t n, uint32_t w, uint32_t
     if (rc) return rc;
     sfk_radix_set_digit_bits(s->sort_depth, s->digit_bits);
     SFK_HIP(c, hipMalloc(&s->emit_status, (size_t)scan_emit_status_words(n) * 4));
+    if (compact_grid(s))
+        SFK_HIP(c, hipMalloc(&s->bin_sstart,
+                             ((size_t)bin_supers(s->tiles_x, s->tiles_y) + 1) * 4));
     uint64_t cap = std::min<uint64_t>(tiles * n, kInitialIsectsCap);
     return alloc_isects(s, next_pow2(cap ? cap : 1));
 }
@@ -236,6 +256,12 @@ extern "C" int sfk_render_scratch_set_tile_width(sfk_render_scratch* s, uint32_t
     int rc = scratch_build(s, s->n, s->w, s->h);
     s->last_isects_raw = raw;
     return rc;
+}
+
+extern "C" int sfk_render_scratch_set_binned(sfk_render_scratch* s, int on) {
+    if (!s) return SFK_E_ARG;
+    s->binned = on != 0;
+    return SFK_OK;
 }
 
 extern "C" int sfk_render_scratch_set_digit_bits(sfk_render_scratch* s, uint32_t bits) {
@@ -334,7 +360,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
         return fail(c, SFK_E_ARG, "view tile bounds do not match the scratch tile width %u", s->tile_w);
     const uint32_t n = sp->n;
     const uint32_t tiles = s->tiles_x * s->tiles_y;
-    const bool compact = s->tiles_x <= kCompactGrid && s->tiles_y <= kCompactGrid;
+    const bool compact = compact_grid(s);
 
     SFK_HIP(c, launch_fill_u32(c->stream, s->counters, 0u, kCounterWords));
     {
@@ -354,6 +380,45 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
         uint2* ordered = nullptr;
         rc = argsort_payload_u32(c, s->sort_depth, s->depth_keys, s->cbbox, n, 1, 32, &ordered);
         if (rc) return rc;
+        if (s->binned && s->capacity >= 2) {
+            // Each tile's isects straight from the depth order, no isect sort.
+            BinJob j;
+            memset(&j, 0, sizeof j);
+            j.ordered = ordered;
+            j.n = n;
+            j.view0.tiles_x = s->tiles_x;
+            j.view0.tiles_y = s->tiles_y;
+            j.view0.supers_x = div_up(s->tiles_x, kBinSuper);
+            j.view0.out_mul = 1;
+            j.nviews = 1;
+            j.supers = bin_supers(s->tiles_x, s->tiles_y);
+            j.out_tiles = tiles;
+            j.cap = s->capacity;
+            j.order_out = s->order_buf;
+            j.offsets = s->offsets;
+            j.sort = s->sort_tile;
+            j.sstart = s->bin_sstart;
+            j.chunk_map = s->bin_map;
+            j.chunk_counts = s->bin_counts;
+            j.tile_starts = s->tile_starts;
+            j.gids = s->gaussian_ids;
+            j.tile_ids = s->tile_ids;
+            j.counters = s->counters;
+            j.status = s->emit_status;
+            rc = bin_emit(c, j);
+            if (rc) return rc;
+            SFK_HIP(c, hipMemcpyAsync(s->host_counters, s->counters, 4, hipMemcpyDeviceToHost,
+                                      c->stream));
+            SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));
+            s->counters_pending = true;
+            s->depth_order = s->order_buf;
+            s->sorted_tiles = s->tile_ids;
+            s->sorted_gids = s->gaussian_ids;
+            s->view = *view;
+            s->has_view = true;
+            fill_isects(s, out);
+            return SFK_OK;
+        }
         src.ordered = ordered;
         src.order_out = s->order_buf;
         s->depth_order = s->order_buf;

@@ -147,6 +147,11 @@ class RenderScratch:
     def set_digit_bits(self, bits):
         self.ctx.check(lib().sfk_render_scratch_set_digit_bits(self.ptr, bits), "digit bits")
 
+    def set_binned(self, on):
+        """Binned isect emission (default) or emission + stable tile argsort;
+        both give the same isect arrays on grids of at most 255 x 255 tiles."""
+        self.ctx.check(lib().sfk_render_scratch_set_binned(self.ptr, 1 if on else 0), "binned")
+
     def reserve_isects(self, n):
         self.ctx.check(lib().sfk_render_scratch_reserve_isects(self.ptr, n), "reserve")
 

@@ -207,3 +207,68 @@ def test_ties_deterministic(ctx):
     # ties rank by splat index
     order = s1.read("depth_order")
     np.testing.assert_array_equal(order, np.arange(n))
+
+
+def _isect_arrays(scratch, isects):
+    raw, nvis, ni = isects.counts()
+    return dict(counts=(raw, nvis, ni), tile_ids=isects.tile_ids(ni),
+                gids=isects.gaussian_ids(ni), starts=isects.tile_starts(),
+                order=scratch.read("depth_order")[:nvis], offsets=scratch.read("offsets")[:nvis])
+
+
+@pytest.mark.parametrize("w,h,tile_w,n,boost", [
+    (256, 256, 16, 65536, 0.0),    # 16 x 16 tiles: 4 supertiles
+    (320, 180, 16, 16384, 1.0),    # 20 x 12 tiles: partial supertiles
+    (64, 64, 16, 2048, 1.5),       # one supertile (a 1-bit pair sort)
+    (1920, 1080, 8, 65536, 0.5),   # 240 x 135 tiles: 510 supertiles, two pair passes
+    (1024, 768, 32, 32768, 1.0),   # 32 x 24 tiles
+])
+def test_binned_matches_sorted(ctx, w, h, tile_w, n, boost):
+    """The binned emission (supertile pairs, per-tile counts from a wave bit
+    transpose, deterministic placement) against emission + stable tile
+    argsort: every isect array word for word."""
+    attrs, sh, k = S.synthetic_scene(n, 0, seed=11)
+    attrs = attrs.reshape(S.ATTR_PLANES, n)
+    attrs[7:10] += np.float32(boost)  # larger splats: more tiles and supertiles each
+    attrs = attrs.reshape(-1)
+    fy = np.float32(np.pi / 3)
+    fx = np.float32(S.fov_for_aspect(w, h, fy))
+    cam = R.Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), fov=(fx, fy))
+    splats = R.Splats(ctx, attrs, sh)
+    out = []
+    for binned in (True, False):
+        scratch = R.RenderScratch(ctx, n, w, h, tile_w=tile_w)
+        scratch.set_binned(binned)
+        scratch.reserve_isects(1 << 24)
+        view = cam.view(w, h, tile_w)
+        out.append(_isect_arrays(scratch, R.prepare_isects(splats, scratch, view=view, w=w, h=h)))
+    a, b = out
+    assert a["counts"] == b["counts"] and a["counts"][2] > 0
+    for key in ("tile_ids", "gids", "starts", "order", "offsets"):
+        np.testing.assert_array_equal(a[key], b[key], key)
+
+
+def test_truncated_first_frame(ctx):
+    """A fresh scratch holds min(tiles * n, 2^22) isects (render.rs:143-167);
+    a first frame emitting more keeps exactly the isects whose emission slot
+    is below that (raster.rs:12-48), in both emission paths, as the oracle
+    does with the same max_isects."""
+    n, w, h = 65536, 1920, 1080
+    attrs, sh, k = S.synthetic_scene(n, 0, seed=5)
+    attrs = attrs.reshape(S.ATTR_PLANES, n)
+    attrs[7:10] += np.float32(1.6)
+    attrs = attrs.reshape(-1)
+    fy = np.float32(np.pi / 3)
+    fx = np.float32(S.fov_for_aspect(w, h, fy))
+    cam = R.Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), fov=(fx, fy))
+    view = cam.view(w, h)
+    cap = 1 << 22
+    ref = O.prepare(oview(view), attrs, sh, n, k, w, h, 16, max_isects=cap, nthreads=8)
+    assert ref["num_isects_raw"] > cap, "scene must overflow the first-frame capacity"
+    splats = R.Splats(ctx, attrs, sh)
+    for binned in (True, False):
+        scratch = R.RenderScratch(ctx, n, w, h)
+        scratch.set_binned(binned)
+        isects = R.prepare_isects(splats, scratch, view=view, w=w, h=h)
+        assert isects.counts()[2] == cap
+        check_prepare(scratch, isects, ref, n)
