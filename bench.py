@@ -59,6 +59,11 @@ def calibrate_copy(ctx):
                       "ms": round(ms, 4), "gbs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}))
 
 
+def stage(msg):
+    """Progress line on stderr (the JSON result is the last stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def bench_eig_round(ctx, args):
     """EIG-round latency: 20 candidate views at 512^2 around a 1M-splat SH3
     scene, plain (the reference's select_view loop: per view a
@@ -81,6 +86,7 @@ def bench_eig_round(ctx, args):
     # Plain: the fastest tile width for the single-view path.
     plain = {}
     for tw in (8, 16, 32):
+        stage(f"eig round: plain loop, tile width {tw}")
         scratch = R.RenderScratch(ctx, n, args.eig_res, args.eig_res, tile_w=tw)
         scratch.reserve_isects(1 << 24)
         eig_p, best_p = G.eig_round_plain(splats, scratch, cams, sizes, state, resp)  # warm
@@ -94,6 +100,7 @@ def bench_eig_round(ctx, args):
     plain_tw = min(plain, key=lambda k_: plain[k_][0])
     plain_ms, eig_p, best_p = plain[plain_tw]
     # Batched: the autotuner's choice of tile width x digit width.
+    stage("eig round: autotune")
     (tw_b, db_b), table = G.autotune_round(splats, cams, sizes, state, reps=3)
     rnd = G.EigRound(splats, cams, sizes, tile_w=tw_b, digit_bits=db_b)
     rnd.run(state, use_graph=True)
@@ -139,6 +146,7 @@ def bench_c4(ctx, args):
     rng = np.random.default_rng(2)
     state = G.BetaState(ctx, (1 + rng.random(n) * 4).astype(np.float32),
                         (1 + rng.random(n) * 4).astype(np.float32))
+    stage("c4 round: autotune")
     best, table = G.autotune_round(splats, cams, sizes, state, reps=2)
     rnd = G.EigRound(splats, cams, sizes, tile_w=best[0], digit_bits=best[1])
     rnd.run(state, use_graph=True)
@@ -163,6 +171,7 @@ def bench_c4(ctx, args):
             e.append(state.eig(acc))
         return e
 
+    stage("c4 round: plain loop")
     plain()
     tp = []
     for _ in range(2):
@@ -239,6 +248,7 @@ def main():
         return
     n, w, h = args.n, args.width, args.height
     px = w * h
+    stage(f"frame: {n} splats, {w}x{h}, forward+backward")
     attrs, sh, k = S.synthetic_scene(n, 3, seed=0)
     rng = np.random.default_rng(1)
     pa = (attrs + rng.normal(0, 0.01, attrs.size)).astype(np.float32)
@@ -329,6 +339,7 @@ def main():
     extra = {}
     if not args.no_extra:
         extra["eig_round"] = bench_eig_round(ctx, args)
+        stage("sort: 16M u64 keys")
         extra["sort_u64"] = bench_sort(ctx, args)
         extra["eig_round_c4"] = bench_c4(ctx, args)
 

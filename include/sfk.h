@@ -297,6 +297,10 @@ void sfk_eig_round_destroy(sfk_eig_round* r);
  * one synchronization. */
 int sfk_eig_round_run(sfk_ctx* ctx, sfk_eig_round* r, const float* a, const float* b,
                       int use_graph, double* eig_out, uint32_t* best_out);
+/* Binned isect emission (default, grids of at most 255 x 255 tiles) or
+ * emission + stable tile argsort; the same isect lists either way.  Drops a
+ * captured graph. */
+int sfk_eig_round_set_binned(sfk_eig_round* r, int on);
 /* Device pointer of one view's u32 responsibility words and its scale. */
 int sfk_eig_round_info(const sfk_eig_round* r, uint32_t view, const uint32_t** resp,
                        float* scale, uint32_t* capacity, uint32_t* total_tiles);

@@ -179,6 +179,16 @@ struct sfk_eig_round {
     uint32_t* emit_status = nullptr;  // look-back words of the fused scan/emission
     const uint32_t* sorted_tiles = nullptr;
     const uint32_t* sorted_gids = nullptr;
+    // Binned emission (compact grids): per-view supertile grids, supertile
+    // ranges, chunk map and counts, per-rank isect offsets.
+    bool binned = true;
+    uint32_t supers = 0;
+    BinView* d_binviews = nullptr;
+    uint32_t* offsets = nullptr;
+    uint32_t* bin_sstart = nullptr;
+    uint32_t* bin_map = nullptr;
+    uint32_t* bin_counts = nullptr;
+    uint32_t* scan_sums = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const float* graph_a = nullptr;
@@ -188,9 +198,10 @@ struct sfk_eig_round {
 extern "C" void sfk_eig_round_destroy(sfk_eig_round* r);
 
 static void round_free_isects(sfk_eig_round* r) {
-    if (r->tile_ids) hipFree(r->tile_ids);
-    if (r->gids) hipFree(r->gids);
-    r->tile_ids = r->gids = nullptr;
+    void* bufs[] = {r->tile_ids, r->gids, r->bin_map, r->bin_counts};
+    for (void* p : bufs)
+        if (p) hipFree(p);
+    r->tile_ids = r->gids = r->bin_map = r->bin_counts = nullptr;
     sfk_radix_scratch_destroy(r->sort_tile);
     r->sort_tile = nullptr;
 }
@@ -200,6 +211,11 @@ static int round_alloc_isects(sfk_eig_round* r, uint32_t cap) {
     round_free_isects(r);
     SFK_HIP(c, hipMalloc(&r->tile_ids, (size_t)cap * 4));
     SFK_HIP(c, hipMalloc(&r->gids, (size_t)cap * 4));
+    if (r->compact && cap <= kBinMaxCap) {
+        const size_t slots = bin_chunk_slots(cap, r->supers);
+        SFK_HIP(c, hipMalloc(&r->bin_map, slots * 16));
+        SFK_HIP(c, hipMalloc(&r->bin_counts, slots * 64 * 4));
+    }
     int rc = sfk_radix_scratch_create(c, cap, 4, &r->sort_tile);
     if (rc) return rc;
     sfk_radix_set_digit_bits(r->sort_tile, r->digit_bits);
@@ -224,11 +240,42 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
     }
     EmitSource src;
     int rc;
+    bool emitted = false;
     if (r->compact) {
         uint2* ordered = nullptr;
         rc = argsort_payload_u32(c, r->sort_depth, r->keys, r->cbbox, n, V, 32, &ordered);
         if (rc) return rc;
         src.ordered = ordered;
+        if (r->binned && r->capacity >= 2 && r->capacity <= kBinMaxCap) {
+            // Every (view, tile) list in depth order, no isect sort; tile
+            // ranges indexed local * V + view as the batched raster reads them.
+            BinJob j;
+            memset(&j, 0, sizeof j);
+            j.ordered = ordered;
+            j.n = (uint32_t)vn;
+            j.seg_len = n;
+            j.views = r->d_binviews;
+            j.nviews = V;
+            j.supers = r->supers;
+            j.out_tiles = r->max_tiles * V;
+            j.zero_starts = true;
+            j.cap = r->capacity;
+            j.offsets = r->offsets;
+            j.sort = r->sort_tile;
+            j.sstart = r->bin_sstart;
+            j.chunk_map = r->bin_map;
+            j.chunk_counts = r->bin_counts;
+            j.scan_sums = r->scan_sums;
+            j.tile_starts = r->starts;
+            j.gids = r->gids;
+            j.counters = r->counters;
+            j.status = r->emit_status;
+            rc = bin_emit(c, j);
+            if (rc) return rc;
+            r->sorted_tiles = nullptr;
+            r->sorted_gids = r->gids;
+            emitted = true;
+        }
     } else {
         uint32_t* order = nullptr;
         rc = sfk_argsort_u32_segmented(c, r->sort_depth, r->keys, nullptr, V, n, 32, 0, nullptr,
@@ -238,25 +285,27 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
         src.tile_counts = r->counts;
         src.bbox = reinterpret_cast<const uint32_t*>(r->bbox);
     }
-    {
-        SFK_PROF(c, "scan_emit");
-        SFK_HIP(c, launch_scan_emit(c->stream, src, (uint32_t)vn, 0, r->capacity, r->tile_ids,
-                                    r->gids, nullptr, r->counters, r->emit_status, n,
-                                    r->d_bviews, r->tile_bits));
-    }
-    uint32_t* st = nullptr;
-    uint32_t* sg = nullptr;
-    // Only the local tile bits are sorted: the emission is view-major, so a
-    // stable sort on them orders by (local tile, view, depth rank).
-    rc = sfk_argsort_u32(c, r->sort_tile, r->tile_ids, r->gids, r->capacity, r->counters,
-                         r->tile_bits, 1, &st, &sg);
-    if (rc) return rc;
-    r->sorted_tiles = st;
-    r->sorted_gids = sg;
-    {
-        SFK_PROF(c, "tile_starts");
-        SFK_HIP(c, launch_tile_starts(c->stream, st, r->capacity, r->counters,
-                                      r->max_tiles * V, r->starts, V, r->tile_bits));
+    if (!emitted) {
+        {
+            SFK_PROF(c, "scan_emit");
+            SFK_HIP(c, launch_scan_emit(c->stream, src, (uint32_t)vn, 0, r->capacity,
+                                        r->tile_ids, r->gids, nullptr, r->counters,
+                                        r->emit_status, n, r->d_bviews, r->tile_bits));
+        }
+        uint32_t* st = nullptr;
+        uint32_t* sg = nullptr;
+        // Only the local tile bits are sorted: the emission is view-major, so
+        // a stable sort on them orders by (local tile, view, depth rank).
+        rc = sfk_argsort_u32(c, r->sort_tile, r->tile_ids, r->gids, r->capacity, r->counters,
+                             r->tile_bits, 1, &st, &sg);
+        if (rc) return rc;
+        r->sorted_tiles = st;
+        r->sorted_gids = sg;
+        {
+            SFK_PROF(c, "tile_starts");
+            SFK_HIP(c, launch_tile_starts(c->stream, st, r->capacity, r->counters,
+                                          r->max_tiles * V, r->starts, V, r->tile_bits));
+        }
     }
     SFK_HIP(c, launch_fill_u32(c->stream, r->resp, 0u, vn));
     {
@@ -265,7 +314,7 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
         memset(&ra, 0, sizeof ra);
         ra.tile_w = r->tile_w;
         ra.tile_starts = r->starts;
-        ra.gids = sg;
+        ra.gids = r->sorted_gids;
         ra.proj = reinterpret_cast<const float*>(r->proj);
         ra.resp = r->resp;
         ra.views = r->d_bviews;
@@ -337,6 +386,20 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     }
     r->total_tiles = (uint32_t)tiles;
     r->tile_bits = sfk_bits_for(r->max_tiles);
+    // Supertile grids of the binned emission, view after view.
+    std::vector<BinView> binviews(nviews);
+    for (uint32_t v = 0; v < nviews; ++v) {
+        BinView& b = binviews[v];
+        memset(&b, 0, sizeof b);
+        b.tiles_x = r->bviews[v].tiles_x;
+        b.tiles_y = r->bviews[v].tiles_y;
+        b.supers_x = div_up(b.tiles_x, kBinSuper);
+        b.super_base = r->supers;
+        b.out_mul = nviews;
+        b.out_add = v;
+        b.gid_add = v * n;
+        r->supers += bin_supers(b.tiles_x, b.tiles_y);
+    }
     if (r->tile_bits + 6 > 32 || (uint64_t)r->max_tiles * nviews >= 0xffffffffull) {
         delete r;
         return fail(c, SFK_E_ARG, "eig round: %u tiles per view too many", r->max_tiles);
@@ -357,6 +420,10 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     RALLOC(r->keys, vn * 4);
     if (r->compact) {
         RALLOC(r->cbbox, vn * 4);
+        RALLOC(r->d_binviews, nviews * sizeof(BinView));
+        RALLOC(r->offsets, vn * 4);
+        RALLOC(r->bin_sstart, ((size_t)r->supers + 1) * 4);
+        RALLOC(r->scan_sums, (size_t)div_up((uint64_t)r->max_tiles * nviews, kScanBlock) * 4);
     } else {
         RALLOC(r->counts, vn * 4);
         RALLOC(r->bbox, vn * 8);
@@ -377,6 +444,9 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     if (e == hipSuccess)
         e = hipMemcpyAsync(r->d_bviews, r->bviews.data(), nviews * sizeof(BatchView),
                            hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && r->d_binviews)
+        e = hipMemcpyAsync(r->d_binviews, binviews.data(), nviews * sizeof(BinView),
+                           hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail_out(hip_fail(c, e, "eig round upload"));
     int rc = radix_scratch_create_ex(c, (uint32_t)vn, 4, r->compact ? 8 : 4, &r->sort_depth);
@@ -395,7 +465,7 @@ extern "C" void sfk_eig_round_destroy(sfk_eig_round* r) {
     if (r->graph) hipGraphDestroy(r->graph);
     void* bufs[] = {r->d_views, r->d_bviews, r->proj, r->keys, r->counts, r->bbox, r->cbbox,
                     r->counters, r->starts, r->resp, r->h0, r->partials, r->eig, r->best,
-                    r->emit_status};
+                    r->emit_status, r->d_binviews, r->offsets, r->bin_sstart, r->scan_sums};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (r->host_out) hipHostFree(r->host_out);
@@ -429,6 +499,19 @@ static int round_fit_capacity(sfk_eig_round* r, const float* a, const float* b) 
         if (rc) return rc;
         SFK_HIP(c, hipStreamSynchronize(c->stream));
     }
+    return SFK_OK;
+}
+
+extern "C" int sfk_eig_round_set_binned(sfk_eig_round* r, int on) {
+    if (!r) return SFK_E_ARG;
+    if (r->binned == (on != 0)) return SFK_OK;
+    r->binned = on != 0;
+    // A captured round holds the other emission: capture again on next use.
+    if (r->ctx) hipStreamSynchronize(r->ctx->stream);
+    if (r->exec) hipGraphExecDestroy(r->exec);
+    if (r->graph) hipGraphDestroy(r->graph);
+    r->exec = nullptr;
+    r->graph = nullptr;
     return SFK_OK;
 }
 

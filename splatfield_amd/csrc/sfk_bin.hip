@@ -342,19 +342,19 @@ __device__ __forceinline__ uint64_t cover_mask(uint32_t cb, uint32_t ox, uint32_
 
 // One wavefront per chunk of kBinChunk sorted pairs, looping over the chunk
 // slots.  Lane t owns tile (ox + t % 8, oy + t / 8) of the chunk's supertile.
-// The place pass stages the chunk's isects in LDS tile by tile, as 16-bit
-// (round, lane) indices into the chunk's splat ids, in groups of consecutive
-// rounds holding at most kStage isects (a round holds at most 64 x 64), and
-// then copies every tile's run of the group out with consecutive lanes on
-// consecutive slots: one coalesced store per tile instead of stores that
-// touch 64 tile ranges each.
+// The place pass stages the chunk's isects in LDS tile by tile, each as a
+// 16-bit word (tile << 9 | round << 6 | lane of the splat), in groups of
+// consecutive rounds holding at most kStage isects (a round holds at most
+// 64 x 64), then copies every group out with consecutive lanes on
+// consecutive staged words: a tile's isects of a group form one contiguous
+// run of its range, so each store covers a few runs instead of 64 tiles.
 constexpr uint32_t kStage = 4096;
 static_assert(kRounds == 8, "round totals use the 8-way transposed wave sum");
 
 template <bool PLACE>
 __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
     constexpr uint32_t SW = PLACE ? 4 : 1;
-    __shared__ uint16_t s_idx[SW][PLACE ? kStage : 1];
+    __shared__ uint16_t s_st[SW][PLACE ? kStage : 1];
     __shared__ uint32_t s_gid[SW][PLACE ? kBinChunk : 1];
     const uint32_t lane = lane_id(), w = threadIdx.x / 64u;
     const uint32_t wave = blockIdx.x * 4u + w, nwaves = gridDim.x * 4u;
@@ -437,31 +437,38 @@ __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
                 for (uint32_t k = 0; k < kRounds; ++k)
                     if (k < nr && grp[k] == q) mine += cnt[k];
                 const uint32_t lo = wave_incl_scan_u32(mine) - mine;
+                uint32_t gsum = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < kRounds; ++k)
+                    if (k < nr && grp[k] == q) gsum += __builtin_amdgcn_readlane(tk, 8 * k);
                 // Stage: tile t's isects of the group at lo.., in rank order.
                 uint32_t pos = lo;
 #pragma unroll
                 for (uint32_t k = 0; k < kRounds; ++k) {
                     if (!(k < nr && grp[k] == q)) continue;
-                    uint64_t cm = col[k];
-                    while (cm) {
-                        s_idx[w][pos++] = (uint16_t)(k * 64 + __builtin_ctzll(cm));
-                        cm &= cm - 1;
+                    // The two 32-bit halves in turn: 32-bit bit scans.
+#pragma unroll
+                    for (uint32_t h = 0; h < 2; ++h) {
+                        uint32_t cm = (uint32_t)(col[k] >> (32 * h));
+                        const uint32_t tag = (lane << 9) | (k * 64 + 32 * h);
+                        while (cm) {
+                            s_st[w][pos++] = (uint16_t)(tag + __builtin_ctz(cm));
+                            cm &= cm - 1;
+                        }
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
-                // Copy out, one tile's run at a time.
-                for (uint32_t j = 0; j < 64; ++j) {
-                    const uint32_t n_j = __builtin_amdgcn_readlane(mine, j);
-                    if (n_j == 0) continue;
-                    const uint32_t lo_j = __builtin_amdgcn_readlane(lo, j);
-                    const uint32_t out_j = __builtin_amdgcn_readlane(base, j);
-                    const uint32_t tile_j = __builtin_amdgcn_readlane(tile, j);
-                    for (uint32_t e = lane; e < n_j; e += 64) {
-                        const uint32_t slot = out_j + e;
-                        if (slot < a.cap) {  // holds by construction; a guard, not a clamp
-                            a.gids[slot] = s_gid[w][s_idx[w][lo_j + e]];
-                            if (a.tile_ids) a.tile_ids[slot] = tile_j;
-                        }
+                // Copy out: staged word e of tile j goes to slot e + base_j - lo_j.
+                const uint32_t delta = base - lo;
+                for (uint32_t e0 = 0; e0 < gsum; e0 += 64) {
+                    const uint32_t e = e0 + lane;
+                    const uint32_t v = e < gsum ? (uint32_t)s_st[w][e] : 0u;
+                    const int j = (int)(v >> 9);
+                    const uint32_t slot = e + (uint32_t)__shfl((int)delta, j, 64);
+                    const uint32_t tid = a.tile_ids ? (uint32_t)__shfl((int)tile, j, 64) : 0u;
+                    if (e < gsum && slot < a.cap) {  // holds by construction; a guard
+                        a.gids[slot] = s_gid[w][v & 511u];
+                        if (a.tile_ids) a.tile_ids[slot] = tid;
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
@@ -550,7 +557,9 @@ __global__ __launch_bounds__(kSmallScanThreads) void k_scan_small(uint32_t* __re
 
 int bin_emit(sfk_ctx* c, const BinJob& j) {
     hipStream_t st = c->stream;
-    if (j.n == 0 || j.cap < 2 || j.supers == 0 || j.supers >= (1u << 24) || j.nviews > 255 ||
+    static_assert(kBinMaxCap < kSat, "a saturated isect total must exceed every capacity");
+    if (j.n == 0 || j.cap < 2 || j.cap > kBinMaxCap || j.supers == 0 || j.supers >= (1u << 24) ||
+        j.nviews > 255 ||
         !j.offsets || !j.gids || !j.tile_starts || !j.counters || !j.status)
         return fail(c, SFK_E_ARG, "binned emission: bad job");
     uint32_t* pair_keys = nullptr;

@@ -181,6 +181,10 @@ struct BinView {
     uint32_t gid_add;           // written splat id = in-view splat + gid_add
     uint32_t pad;
 };
+// Largest isect capacity of the binned emission.  Its prefixes saturate at
+// 2^31 - 1, so a saturated total must still read as above the capacity
+// (truncation on); larger capacities take emission + stable tile argsort.
+constexpr uint32_t kBinMaxCap = 1u << 30;
 inline uint32_t bin_supers(uint32_t tiles_x, uint32_t tiles_y) {
     return div_up(tiles_x, kBinSuper) * div_up(tiles_y, kBinSuper);
 }
@@ -199,7 +203,7 @@ struct BinJob {
     uint32_t supers;        // supertiles over all views
     uint32_t out_tiles;     // tile_starts entries before the total
     bool zero_starts;       // some tile_starts entries belong to no supertile
-    uint32_t cap;           // isect capacity (the reference's max_isects), >= 2
+    uint32_t cap;           // isect capacity (the reference's max_isects), 2..kBinMaxCap
     uint32_t* order_out;    // optional: depth order
     uint32_t* offsets;      // per rank: exclusive isect offset (n words)
     sfk_radix_scratch* sort;  // >= cap elements, 4-byte keys and values; the

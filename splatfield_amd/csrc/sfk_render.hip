@@ -171,7 +171,7 @@ static int alloc_isects(sfk_render_scratch* s, uint32_t cap) {
     s->sorted_tiles = s->sorted_gids = nullptr;
     SFK_HIP(c, hipMalloc(&s->tile_ids, (size_t)cap * 4));
     SFK_HIP(c, hipMalloc(&s->gaussian_ids, (size_t)cap * 4));
-    if (compact_grid(s)) {
+    if (compact_grid(s) && cap <= kBinMaxCap) {
         const size_t slots = bin_chunk_slots(cap, bin_supers(s->tiles_x, s->tiles_y));
         SFK_HIP(c, hipMalloc(&s->bin_map, slots * 16));
         SFK_HIP(c, hipMalloc(&s->bin_counts, slots * 64 * 4));
@@ -380,7 +380,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
         uint2* ordered = nullptr;
         rc = argsort_payload_u32(c, s->sort_depth, s->depth_keys, s->cbbox, n, 1, 32, &ordered);
         if (rc) return rc;
-        if (s->binned && s->capacity >= 2) {
+        if (s->binned && s->capacity >= 2 && s->capacity <= kBinMaxCap) {
             // Each tile's isects straight from the depth order, no isect sort.
             BinJob j;
             memset(&j, 0, sizeof j);

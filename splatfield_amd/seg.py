@@ -169,6 +169,10 @@ class EigRound:
                        "eig round create")
         self.ptr = out.value
 
+    def set_binned(self, on):
+        """Binned isect emission (default) or emission + stable tile argsort."""
+        self.ctx.check(lib().sfk_eig_round_set_binned(self.ptr, 1 if on else 0), "binned")
+
     def run(self, state, use_graph=True, read=True):
         """Returns (eig per view, best view) when read is set."""
         eig = np.zeros(self.nviews, np.float64)
