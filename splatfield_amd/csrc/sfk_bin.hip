@@ -182,32 +182,50 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
     }
     const uint32_t texcl_i = wbase_i + inc_i - ci, texcl_p = wbase_p + inc_p - cp;
 
-    // Decoupled look-back over the workgroups before this one.
-    if (t == 0) {
+    // Decoupled look-back over the workgroups before this one, by the first
+    // wavefront: each step reads 64 predecessors' status words at once
+    // (lane l: workgroup top - l), adds the values up to and including the
+    // nearest prefix, and steps 64 further back only if none of them is one.
+    if (w == 0) {
         uint32_t ei = 0, ep = 0;
         if (bid == 0) {
-            status_store(&a.status[0], kFlagPrefix, tot_i, tot_p);
+            if (lane == 0) status_store(&a.status[0], kFlagPrefix, tot_i, tot_p);
         } else {
-            status_store(&a.status[bid], kFlagAggregate, tot_i, tot_p);
-            for (uint32_t p = bid - 1;;) {
-                const uint64_t sw = status_load(&a.status[p]);
+            if (lane == 0) status_store(&a.status[bid], kFlagAggregate, tot_i, tot_p);
+            for (int32_t top = (int32_t)bid - 1;;) {
+                const int32_t p = top - (int32_t)lane;
+                // Below workgroup 0: a zero prefix (never reached past
+                // workgroup 0's own prefix, which is always published).
+                const uint64_t sw = p >= 0 ? status_load(&a.status[p]) : (kFlagPrefix << 62);
                 const uint32_t flag = (uint32_t)(sw >> 62);
-                if (flag == 0) {
+                const uint64_t pre = __ballot(flag == kFlagPrefix);
+                const uint64_t need = pre ? (2ull << __builtin_ctzll(pre)) - 1ull : ~0ull;
+                if (__ballot(flag == 0) & need) {
                     __builtin_amdgcn_s_sleep(1);
-                    continue;
+                    continue;  // a needed predecessor has not published yet
                 }
-                ei = sat_add(ei, (uint32_t)sw & kSat);
-                ep = sat_add(ep, (uint32_t)(sw >> 31) & kSat);
-                if (flag == kFlagPrefix) break;
-                --p;  // an aggregate: keep walking (workgroup 0 always holds a prefix)
+                const bool take = (need >> lane) & 1ull;
+                uint64_t vi = take ? (sw & kSat) : 0ull, vp = take ? ((sw >> 31) & kSat) : 0ull;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    vi += __shfl_xor(vi, o, 64);
+                    vp += __shfl_xor(vp, o, 64);
+                }
+                ei = (uint32_t)min<uint64_t>((uint64_t)ei + vi, kSat);
+                ep = (uint32_t)min<uint64_t>((uint64_t)ep + vp, kSat);
+                if (pre) break;
+                top -= 64;
             }
-            status_store(&a.status[bid], kFlagPrefix, sat_add(ei, tot_i), sat_add(ep, tot_p));
+            if (lane == 0)
+                status_store(&a.status[bid], kFlagPrefix, sat_add(ei, tot_i), sat_add(ep, tot_p));
         }
-        s_ei = ei;
-        s_ep = ep;
-        if (bid * kRanks + kRanks >= a.n) {
-            a.counters[0] = sat_add(ei, tot_i);
-            a.counters[1] = min(sat_add(ep, tot_p), a.cap);
+        if (lane == 0) {
+            s_ei = ei;
+            s_ep = ep;
+            if (bid * kRanks + kRanks >= a.n) {
+                a.counters[0] = sat_add(ei, tot_i);
+                a.counters[1] = min(sat_add(ep, tot_p), a.cap);
+            }
         }
     }
     __syncthreads();
@@ -534,19 +552,38 @@ __global__ __launch_bounds__(kSmallScanThreads) void k_scan_small(uint32_t* __re
     uint32_t carry = 0;
     for (uint32_t c0 = 0; c0 < n; c0 += kSmallScanThreads * kSmallScanEpt) {
         const uint32_t first = c0 + threadIdx.x * kSmallScanEpt;
+        const bool full = first + kSmallScanEpt <= n;
         uint32_t v[kSmallScanEpt];
+        if (full) {
+#pragma unroll
+            for (uint32_t q = 0; q < kSmallScanEpt / 4; ++q) {
+                const uint4 x = reinterpret_cast<const uint4*>(buf + first)[q];
+                v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t e = 0; e < kSmallScanEpt; ++e) v[e] = first + e < n ? buf[first + e] : 0u;
+        }
         uint32_t s = 0;
 #pragma unroll
-        for (uint32_t e = 0; e < kSmallScanEpt; ++e) {
-            v[e] = first + e < n ? buf[first + e] : 0u;
-            s += v[e];
-        }
+        for (uint32_t e = 0; e < kSmallScanEpt; ++e) s += v[e];
         uint32_t total;
         uint32_t run = carry + block_excl_scan_u32<kSmallScanThreads>(s, tmp, &total);
+        uint32_t o[kSmallScanEpt];
 #pragma unroll
         for (uint32_t e = 0; e < kSmallScanEpt; ++e) {
-            if (first + e < n) buf[first + e] = run;
+            o[e] = run;
             run += v[e];
+        }
+        if (full) {
+#pragma unroll
+            for (uint32_t q = 0; q < kSmallScanEpt / 4; ++q)
+                reinterpret_cast<uint4*>(buf + first)[q] =
+                    make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+        } else {
+#pragma unroll
+            for (uint32_t e = 0; e < kSmallScanEpt; ++e)
+                if (first + e < n) buf[first + e] = o[e];
         }
         carry += total;
     }
