@@ -42,7 +42,11 @@ namespace sfk {
 
 namespace {
 
-constexpr uint32_t kThreads = 256;
+// Workgroups of 1024 threads, 4 ranks each: the ticket counter (one device
+// atomic per workgroup on one word) and the look-back chain see 4096 ranks
+// per workgroup; at 1024-rank workgroups the ticket alone cost ~10 us per
+// million ranks on MI355X.
+constexpr uint32_t kThreads = 1024;
 constexpr uint32_t kRpt = 4;  // ranks per thread
 constexpr uint32_t kRanks = kThreads * kRpt;
 constexpr uint32_t kSat = 0x7fffffffu;
@@ -93,7 +97,7 @@ struct BinScanArgs {
     uint64_t* status;
 };
 
-// Workgroups of 256 threads own 1024 consecutive ranks (4 per thread);
+// Workgroups of 1024 threads own 4096 consecutive ranks (4 per thread);
 // logical ids come from a ticket, so every workgroup a look-back waits on has
 // started and publishes its aggregate before it waits itself.  Each
 // wavefront then writes the pairs of its 256 ranks (a contiguous range),
