@@ -328,12 +328,13 @@ extern "C" void sfk_radix_scratch_destroy(sfk_radix_scratch* s) {
 
 // An argsort reads its input in the first pass only, and that pass writes
 // keys[0] / vals[0]: the second buffer pair may hold the input itself.
-int sfk::radix_scratch_spare_u32(sfk_radix_scratch* s, uint32_t** keys, uint32_t** vals) {
-    if (!s || s->key_bytes != 4 || s->val_bytes != 4) return SFK_E_ARG;
+int sfk::radix_scratch_spare_u32x2(sfk_radix_scratch* s, uint32_t** keys, uint2** vals) {
+    if (!s || s->key_bytes != 4 || s->val_bytes != 8) return SFK_E_ARG;
     *keys = static_cast<uint32_t*>(s->keys[1]);
-    *vals = s->vals[1];
+    *vals = reinterpret_cast<uint2*>(s->vals[1]);
     return SFK_OK;
 }
+
 
 extern "C" int sfk_radix_set_digit_bits(sfk_radix_scratch* s, uint32_t bits) {
     if (!s || bits < 4 || bits > 8) return SFK_E_ARG;
@@ -419,6 +420,13 @@ int sfk::argsort_payload_u32(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* k
     if (!payload) return fail(c, SFK_E_ARG, "argsort: no payload");
     return argsort_impl<uint32_t, uint2>(c, s, keys, nullptr, n, nullptr, bits, 0, nullptr, out,
                                          nseg, payload);
+}
+
+int sfk::argsort_u32_vals8(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys,
+                           const uint2* vals, uint32_t n, const uint32_t* d_n, uint32_t bits,
+                           uint32_t** out_keys, uint2** out_vals) {
+    if (!vals) return fail(c, SFK_E_ARG, "argsort: no values");
+    return argsort_impl<uint32_t, uint2>(c, s, keys, vals, n, d_n, bits, 1, out_keys, out_vals);
 }
 
 extern "C" int sfk_argsort_u32(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys,

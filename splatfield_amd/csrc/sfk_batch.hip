@@ -184,7 +184,7 @@ struct sfk_eig_round {
     bool binned = true;
     uint32_t supers = 0;
     BinView* d_binviews = nullptr;
-    uint32_t* offsets = nullptr;
+    uint32_t* offsets = nullptr;  // isect offsets of truncation candidates, by view-major splat
     uint32_t* bin_sstart = nullptr;
     uint32_t* bin_map = nullptr;
     uint32_t* bin_counts = nullptr;
@@ -216,7 +216,9 @@ static int round_alloc_isects(sfk_eig_round* r, uint32_t cap) {
         SFK_HIP(c, hipMalloc(&r->bin_map, slots * 16));
         SFK_HIP(c, hipMalloc(&r->bin_counts, slots * 64 * 4));
     }
-    int rc = sfk_radix_scratch_create(c, cap, 4, &r->sort_tile);
+    // 8-byte values where the binned emission may sort its pairs.
+    int rc = radix_scratch_create_ex(c, cap, 4, r->compact && cap <= kBinMaxCap ? 8 : 4,
+                                     &r->sort_tile);
     if (rc) return rc;
     sfk_radix_set_digit_bits(r->sort_tile, r->digit_bits);
     r->capacity = cap;
@@ -260,7 +262,7 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
             j.out_tiles = r->max_tiles * V;
             j.zero_starts = true;
             j.cap = r->capacity;
-            j.offsets = r->offsets;
+            j.offs_by_splat = r->offsets;
             j.sort = r->sort_tile;
             j.sstart = r->bin_sstart;
             j.chunk_map = r->bin_map;

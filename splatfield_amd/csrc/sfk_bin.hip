@@ -9,12 +9,13 @@
 //
 //  1. k_bin_scan, one pass over the depth ranks: a decoupled look-back gives
 //     two exclusive prefixes per rank, the isect offset of the reference's
-//     emission (kept per rank: it decides the truncation) and the offset of
-//     the rank's (supertile, rank) pairs, which it writes.  A supertile is a
+//     emission (it decides the truncation) and the offset of the rank's
+//     (supertile, splat | bbox) pairs, which it writes.  A supertile is a
 //     block of 8 x 8 tiles; a bbox meets few of them (1.6 per splat for a
 //     1M-splat 1080p frame, against 9.4 tiles).
 //  2. a stable radix argsort of the pairs by supertile id (one pass up to 256
-//     supertiles): the ranks of every supertile in depth order.
+//     supertiles): every supertile's splats and bboxes in depth order, read
+//     coalesced by the tile passes (no gather by rank).
 //  3. each supertile's range of sorted pairs (k_tile_starts), cut into
 //     chunks of kBinChunk pairs, one wavefront each (k_bin_chunk_map).
 //  4. k_bin_chunks<false>: per chunk and per tile of its supertile, the isect
@@ -90,8 +91,9 @@ struct BinScanArgs {
     uint32_t cap;
     uint32_t* order_out;
     uint32_t* offsets;
+    uint32_t* offs_by_splat;
     uint32_t* pair_keys;
-    uint32_t* pair_vals;
+    uint2* pair_vals;
     uint32_t* counters;
     uint32_t* ticket;
     uint64_t* status;
@@ -107,6 +109,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
     __shared__ uint32_t s_incl[kRanks];  // inclusive pair prefix within the wave's ranks
     __shared__ uint32_t s_key0[kRanks];  // key of the rank's first supertile
     __shared__ uint32_t s_span[kRanks];  // supertile columns | view supertiles per row << 8
+    __shared__ uint2 s_val[kRanks];      // pair value: (splat | may-truncate << 31, bbox)
     __shared__ uint32_t s_wi[kThreads / 64], s_wp[kThreads / 64];
     __shared__ uint32_t s_bid, s_ei, s_ep;
 
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
     }
     // View of the first rank (batched); later ranks step over boundaries.
     uint32_t v = a.seg_len ? (r0 < a.n ? r0 : a.n - 1) / a.seg_len : 0u;
-    uint32_t cnt[kRpt], sc[kRpt];
+    uint32_t cnt[kRpt], sc[kRpt], gadd[kRpt];
     uint32_t ci = 0, cp = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kRpt; ++k) {
@@ -152,6 +155,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
         cnt[k] = (x1 - x0) * (y1 - y0);
         sc[k] = 0;
         uint32_t key0 = 0, span = 0;
+        gadd[k] = 0;
         if (cnt[k]) {
             const uint32_t sx0 = x0 / kBinSuper, sy0 = y0 / kBinSuper;
             const uint32_t swx = (x1 - 1) / kBinSuper - sx0 + 1;
@@ -160,6 +164,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
             const BinView vw = view_of(a.views, a.view0, v);
             key0 = vw.super_base + sy0 * vw.supers_x + sx0;
             span = swx | (vw.supers_x << 8);
+            gadd[k] = vw.gid_add;
         }
         s_key0[t * kRpt + k] = key0;
         s_span[t * kRpt + k] = span;
@@ -245,13 +250,22 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
         run_i += cnt[k];
         run_p += sc[k];
         s_incl[t * kRpt + k] = run_p - wave_excl_p;
+        // A rank whose isects may reach the capacity carries a flag in its
+        // pairs and leaves its offset at its splat: the only ranks whose
+        // offset the tile passes need (raster.rs:12-48 drops slots >= cap).
+        const bool cut = cnt[k] && offs[k] + cnt[k] > a.cap;
+        s_val[t * kRpt + k] = make_uint2(oc[k].x | (cut ? 0x80000000u : 0u), oc[k].y);
+        if (cut) a.offs_by_splat[gadd[k] + oc[k].x] = offs[k];
     }
-    if (full) {
-        reinterpret_cast<uint4*>(a.offsets)[r0 / 4] = make_uint4(offs[0], offs[1], offs[2], offs[3]);
-    } else {
+    if (a.offsets) {
+        if (full) {
+            reinterpret_cast<uint4*>(a.offsets)[r0 / 4] =
+                make_uint4(offs[0], offs[1], offs[2], offs[3]);
+        } else {
 #pragma unroll
-        for (uint32_t k = 0; k < kRpt; ++k)
-            if (r0 + k < a.n) a.offsets[r0 + k] = offs[k];
+            for (uint32_t k = 0; k < kRpt; ++k)
+                if (r0 + k < a.n) a.offsets[r0 + k] = offs[k];
+        }
     }
 
     // The wave's pairs: slots [pbase, pbase + wave_total) below the capacity.
@@ -262,7 +276,6 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
     const uint32_t pbase = sat_add(ep, wave_excl_p);
     const uint32_t first = w * 64 * kRpt;
     const uint32_t* incl = s_incl + first;
-    const uint32_t rbase = bid * kRanks + first;
     __builtin_amdgcn_wave_barrier();
     if (pbase >= a.cap) return;
     constexpr uint32_t Q = 4;
@@ -288,7 +301,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
             const uint32_t swx = span & 0xffu, row = span >> 8;
             const uint32_t dy = j < swx ? 0u : j / swx;
             a.pair_keys[slot] = s_key0[first + k] + dy * row + (j - dy * swx);
-            a.pair_vals[slot] = rbase + k;
+            a.pair_vals[slot] = s_val[first + k];
         }
     }
 }
@@ -314,12 +327,10 @@ __global__ __launch_bounds__(256) void k_bin_chunk_map(const uint32_t* __restric
 }
 
 struct ChunkArgs {
-    const uint32_t* pvals;  // sorted pair values: global depth ranks
+    const uint2* pvals;     // sorted pair values: (splat | may-truncate << 31, bbox)
     const uint32_t* sstart;
     const uint4* map;
-    const uint2* ordered;
-    const uint32_t* offsets;
-    const uint32_t* counters;
+    const uint32_t* offs_by_splat;
     const BinView* views;
     BinView view0;
     uint32_t supers, cap;
@@ -381,7 +392,6 @@ __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
     const uint32_t lane = lane_id(), w = threadIdx.x / 64u;
     const uint32_t wave = blockIdx.x * 4u + w, nwaves = gridDim.x * 4u;
     const uint32_t slots = a.sstart[a.supers] / kBinChunk + a.supers;
-    const bool trunc = a.counters[0] > a.cap;
     const TransposeLanes tl = transpose_lanes();
     for (uint32_t c = wave; c < slots; c += nwaves) {
         const uint4 d = a.map[c];
@@ -400,20 +410,17 @@ __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
             if (tx < vw.tiles_x && ty < vw.tiles_y)
                 base += a.tile_starts[(ty * vw.tiles_x + tx) * vw.out_mul + vw.out_add];
         }
-        // All rounds' loads first: the rank, then (splat, bbox) and offset.
-        uint32_t rr[kRounds];
+        // All rounds' pairs first (coalesced), then the offsets of flagged ones.
         uint2 oc[kRounds];
         uint32_t off[kRounds];
 #pragma unroll
         for (uint32_t k = 0; k < kRounds; ++k) {
             const uint32_t i = p0 + k * 64 + lane;
-            rr[k] = i < pe ? a.pvals[i] : ~0u;
+            oc[k] = i < pe ? a.pvals[i] : make_uint2(0u, 0u);  // bbox 0: no tiles
         }
 #pragma unroll
-        for (uint32_t k = 0; k < kRounds; ++k) {
-            oc[k] = rr[k] != ~0u ? a.ordered[rr[k]] : make_uint2(0u, 0u);
-            off[k] = trunc && rr[k] != ~0u ? a.offsets[rr[k]] : 0u;
-        }
+        for (uint32_t k = 0; k < kRounds; ++k)
+            off[k] = (oc[k].x >> 31) ? a.offs_by_splat[vw.gid_add + (oc[k].x & 0x7fffffffu)] : 0u;
         // Per round: lane t's mask of the round's splats covering tile t.
         uint64_t col[kRounds];
         uint32_t cnt[kRounds];
@@ -422,7 +429,7 @@ __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
             col[k] = 0;
             if (k < nr) {
                 const uint64_t mask =
-                    rr[k] != ~0u ? cover_mask(oc[k].y, ox, oy, trunc, off[k], a.cap) : 0ull;
+                    cover_mask(oc[k].y, ox, oy, (oc[k].x >> 31) != 0, off[k], a.cap);
                 col[k] = wave_transpose64(mask, tl);
             }
             cnt[k] = (uint32_t)__popcll(col[k]);
@@ -434,7 +441,8 @@ __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
             a.counts[(size_t)c * 64 + lane] = total;
         } else {
 #pragma unroll
-            for (uint32_t k = 0; k < kRounds; ++k) s_gid[w][k * 64 + lane] = oc[k].x + vw.gid_add;
+            for (uint32_t k = 0; k < kRounds; ++k)
+                s_gid[w][k * 64 + lane] = (oc[k].x & 0x7fffffffu) + vw.gid_add;
             const uint32_t tile = ty * vw.tiles_x + tx;
             // Round totals, then groups of consecutive rounds of at most
             // kStage isects (all wave-uniform).
@@ -601,12 +609,14 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     static_assert(kBinMaxCap < kSat, "a saturated isect total must exceed every capacity");
     if (j.n == 0 || j.cap < 2 || j.cap > kBinMaxCap || j.supers == 0 || j.supers >= (1u << 24) ||
         j.nviews > 255 ||
-        !j.offsets || !j.gids || !j.tile_starts || !j.counters || !j.status)
+        !j.gids || !j.tile_starts || !j.counters || !j.status)
         return fail(c, SFK_E_ARG, "binned emission: bad job");
     uint32_t* pair_keys = nullptr;
-    uint32_t* pair_vals = nullptr;
-    if (radix_scratch_spare_u32(j.sort, &pair_keys, &pair_vals))
+    uint2* pair_vals = nullptr;
+    if (radix_scratch_spare_u32x2(j.sort, &pair_keys, &pair_vals))
         return fail(c, SFK_E_ARG, "binned emission: pair sort scratch");
+    if ((j.seg_len ? j.seg_len : j.n) > 0x7fffffffu || !j.offs_by_splat)
+        return fail(c, SFK_E_ARG, "binned emission: splat ids need 31 bits");
     SFK_HIP(c, launch_fill_u32(st, j.status, 0u, scan_emit_status_words(j.n)));
     {
         SFK_PROF(c, "bin_scan");
@@ -619,6 +629,7 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
         a.cap = j.cap;
         a.order_out = j.order_out;
         a.offsets = j.offsets;
+        a.offs_by_splat = j.offs_by_splat;
         a.pair_keys = pair_keys;
         a.pair_vals = pair_vals;
         a.counters = j.counters;
@@ -630,9 +641,9 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     // Pairs by supertile, stably (at least one bit, so the sorted pairs never
     // alias the input).
     uint32_t* sk = nullptr;
-    uint32_t* sv = nullptr;
-    int rc = sfk_argsort_u32(c, j.sort, pair_keys, pair_vals, j.cap, j.counters + 1,
-                             std::max(1u, sfk_bits_for(j.supers)), 1, &sk, &sv);
+    uint2* sv = nullptr;
+    int rc = argsort_u32_vals8(c, j.sort, pair_keys, pair_vals, j.cap, j.counters + 1,
+                               std::max(1u, sfk_bits_for(j.supers)), &sk, &sv);
     if (rc) return rc;
     {
         SFK_PROF(c, "bin_ranges");
@@ -645,9 +656,7 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     ca.pvals = sv;
     ca.sstart = j.sstart;
     ca.map = reinterpret_cast<const uint4*>(j.chunk_map);
-    ca.ordered = j.ordered;
-    ca.offsets = j.offsets;
-    ca.counters = j.counters;
+    ca.offs_by_splat = j.offs_by_splat;
     ca.views = j.views;
     ca.view0 = j.view0;
     ca.supers = j.supers;

@@ -127,9 +127,14 @@ int radix_scratch_create_ex(sfk_ctx* c, uint32_t max_elems, uint32_t key_bytes,
 int argsort_payload_u32(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys,
                         const uint32_t* payload, uint32_t n, uint32_t nseg, uint32_t bits,
                         uint2** out);
-// The buffer pair of a 4-byte-key, 4-byte-value scratch that an argsort on it
-// may take as input (the first pass writes the other pair).
-int radix_scratch_spare_u32(sfk_radix_scratch* s, uint32_t** keys, uint32_t** vals);
+// The buffer pair of a 4-byte-key, 8-byte-value scratch that an argsort on it
+// may take as input (the first pass writes the other pair), and the stable
+// argsort of u32 keys carrying 8-byte values (one segment, optional device
+// count; sorted keys written).
+int radix_scratch_spare_u32x2(sfk_radix_scratch* s, uint32_t** keys, uint2** vals);
+int argsort_u32_vals8(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys, const uint2* vals,
+                      uint32_t n, const uint32_t* d_n, uint32_t bits, uint32_t** out_keys,
+                      uint2** out_vals);
 
 // Scan.
 constexpr uint32_t kScanBlock = 4096;
@@ -205,9 +210,11 @@ struct BinJob {
     bool zero_starts;       // some tile_starts entries belong to no supertile
     uint32_t cap;           // isect capacity (the reference's max_isects), 2..kBinMaxCap
     uint32_t* order_out;    // optional: depth order
-    uint32_t* offsets;      // per rank: exclusive isect offset (n words)
-    sfk_radix_scratch* sort;  // >= cap elements, 4-byte keys and values; the
-                              // (supertile, rank) pairs live in its spare pair
+    uint32_t* offsets;      // optional, per rank: exclusive isect offset (n words)
+    uint32_t* offs_by_splat;  // n words: the isect offset of every rank whose
+                              // isects may be truncated, at its (view-major) splat
+    sfk_radix_scratch* sort;  // >= cap elements, 4-byte keys, 8-byte values; the
+                              // (supertile, splat | bbox) pairs live in its spare pair
     uint32_t* sstart;       // supers + 1
     uint32_t* chunk_map;    // 4 words per chunk slot: bin_chunk_slots(cap, supers)
     uint32_t* chunk_counts; // 64 per chunk slot

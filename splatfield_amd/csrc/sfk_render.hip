@@ -131,6 +131,7 @@ struct sfk_render_scratch {
     uint32_t* bin_sstart = nullptr;
     uint32_t* bin_map = nullptr;
     uint32_t* bin_counts = nullptr;
+    uint32_t* bin_offs = nullptr;  // isect offsets of truncation candidates, by splat
 };
 
 static bool compact_grid(const sfk_render_scratch* s) {
@@ -141,7 +142,7 @@ static void scratch_free_frame(sfk_render_scratch* s) {
     void* bufs[] = {s->depth_keys, s->projected, s->tile_counts, s->bbox, s->offsets,
                     s->counters, s->tile_starts, s->bitmap, s->rgba, s->final_t, s->n_contrib,
                     s->tile_ids, s->gaussian_ids, s->grad2d, s->emit_status, s->cbbox,
-                    s->order_buf, s->bin_sstart, s->bin_map, s->bin_counts};
+                    s->order_buf, s->bin_sstart, s->bin_map, s->bin_counts, s->bin_offs};
     for (void* p : bufs)
         if (p) hipFree(p);
     s->depth_keys = nullptr; s->projected = nullptr; s->tile_counts = nullptr; s->bbox = nullptr;
@@ -152,7 +153,7 @@ static void scratch_free_frame(sfk_render_scratch* s) {
     s->grad2d_dirty = true;
     s->emit_status = nullptr;
     s->cbbox = s->order_buf = nullptr;
-    s->bin_sstart = s->bin_map = s->bin_counts = nullptr;
+    s->bin_sstart = s->bin_map = s->bin_counts = s->bin_offs = nullptr;
     s->has_view = false;
     sfk_radix_scratch_destroy(s->sort_depth);
     sfk_radix_scratch_destroy(s->sort_tile);
@@ -176,7 +177,9 @@ static int alloc_isects(sfk_render_scratch* s, uint32_t cap) {
         SFK_HIP(c, hipMalloc(&s->bin_map, slots * 16));
         SFK_HIP(c, hipMalloc(&s->bin_counts, slots * 64 * 4));
     }
-    int rc = sfk_radix_scratch_create(c, cap, 4, &s->sort_tile);
+    // 8-byte values where the binned emission may sort its pairs.
+    const bool bin = compact_grid(s) && cap <= kBinMaxCap;
+    int rc = radix_scratch_create_ex(c, cap, 4, bin ? 8 : 4, &s->sort_tile);
     if (rc) return rc;
     sfk_radix_set_digit_bits(s->sort_tile, s->digit_bits);
     s->capacity = cap;
@@ -210,9 +213,11 @@ static int scratch_build(sfk_render_scratch* s, uint32_t n, uint32_t w, uint32_t
     if (rc) return rc;
     sfk_radix_set_digit_bits(s->sort_depth, s->digit_bits);
     SFK_HIP(c, hipMalloc(&s->emit_status, (size_t)scan_emit_status_words(n) * 4));
-    if (compact_grid(s))
+    if (compact_grid(s)) {
         SFK_HIP(c, hipMalloc(&s->bin_sstart,
                              ((size_t)bin_supers(s->tiles_x, s->tiles_y) + 1) * 4));
+        SFK_HIP(c, hipMalloc(&s->bin_offs, nn * 4));
+    }
     uint64_t cap = std::min<uint64_t>(tiles * n, kInitialIsectsCap);
     return alloc_isects(s, next_pow2(cap ? cap : 1));
 }
@@ -396,6 +401,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
             j.cap = s->capacity;
             j.order_out = s->order_buf;
             j.offsets = s->offsets;
+            j.offs_by_splat = s->bin_offs;
             j.sort = s->sort_tile;
             j.sstart = s->bin_sstart;
             j.chunk_map = s->bin_map;
