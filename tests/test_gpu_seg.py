@@ -179,6 +179,23 @@ def test_eig_reference_cases(ctx):
     assert sat < 1e-2 < fresh
 
 
+def test_entropy_drop_grid_against_scipy(ctx):
+    """The device Beta entropy (the three arguments' shift products
+    normalized and fused: four logs, three divisions) against scipy's
+    lgamma / digamma: single-splat entropy drops over a grid of posteriors
+    and evidence, within 1e-10 nats."""
+    scale = 4096.0
+    for a0, b0 in [(1.0, 1.0), (1.0, 40.0), (2.5, 7.25), (33.0, 33.0), (900.0, 3.0), (300.0, 700.0)]:
+        for eps in (0.25, 3.0, 60.0):
+            acc = G.Accumulators(ctx, 1)
+            acc.bits.write(np.array([np.float32(eps) * np.float32(scale)], np.float32).astype(np.uint32))
+            acc.scale = scale
+            got = G.BetaState(ctx, np.array([a0], np.float32), np.array([b0], np.float32)).eig(acc)
+            k = 1.0 + float(np.float32(eps)) / (a0 + b0)
+            want = max(0.0, _entropy_exact(a0, b0) - _entropy_exact(a0 * k, b0 * k))
+            assert abs(got - want) <= 1e-10, (a0, b0, eps, got, want)
+
+
 def test_localize_matches_oracle(ctx):
     """Device views::localize against the oracle's f64 restatement on a 1M
     splat scene with a random posterior (f32 outputs, 1e-6 relative)."""
