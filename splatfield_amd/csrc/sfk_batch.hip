@@ -182,7 +182,7 @@ struct sfk_eig_round {
     // Binned emission (compact grids): per-view supertile grids, supertile
     // ranges, chunk map and counts, per-rank isect offsets.
     bool binned = true;
-    uint32_t supers = 0;
+    uint32_t supers = 0, bin_shift = 1;  // supertile ids, bits of the local ones
     BinView* d_binviews = nullptr;
     uint32_t* offsets = nullptr;  // isect offsets of truncation candidates, by view-major splat
     uint32_t* bin_sstart = nullptr;
@@ -258,6 +258,7 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
             j.seg_len = n;
             j.views = r->d_binviews;
             j.nviews = V;
+            j.view_shift = r->bin_shift;
             j.supers = r->supers;
             j.out_tiles = r->max_tiles * V;
             j.zero_starts = true;
@@ -388,19 +389,25 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     }
     r->total_tiles = (uint32_t)tiles;
     r->tile_bits = sfk_bits_for(r->max_tiles);
-    // Supertile grids of the binned emission, view after view.
+    // Supertile grids of the binned emission.  Ids are local * views + view:
+    // the view-major pairs sorted stably on their local bits alone come out
+    // in that order (one radix pass up to 256 supertiles per view).
     std::vector<BinView> binviews(nviews);
+    uint32_t max_supers = 1;
+    for (uint32_t v = 0; v < nviews; ++v)
+        max_supers = std::max(max_supers, bin_supers(r->bviews[v].tiles_x, r->bviews[v].tiles_y));
+    r->bin_shift = std::max(1u, sfk_bits_for(max_supers));
+    r->supers = max_supers * nviews;
     for (uint32_t v = 0; v < nviews; ++v) {
         BinView& b = binviews[v];
         memset(&b, 0, sizeof b);
         b.tiles_x = r->bviews[v].tiles_x;
         b.tiles_y = r->bviews[v].tiles_y;
         b.supers_x = div_up(b.tiles_x, kBinSuper);
-        b.super_base = r->supers;
+        b.key_base = v << r->bin_shift;
         b.out_mul = nviews;
         b.out_add = v;
         b.gid_add = v * n;
-        r->supers += bin_supers(b.tiles_x, b.tiles_y);
     }
     if (r->tile_bits + 6 > 32 || (uint64_t)r->max_tiles * nviews >= 0xffffffffull) {
         delete r;

@@ -74,14 +74,6 @@ __device__ __forceinline__ BinView view_of(const BinView* views, const BinView& 
     return views ? views[v] : view0;
 }
 
-// View holding global supertile s (views ordered by super_base).
-__device__ __forceinline__ uint32_t view_of_super(const BinView* views, uint32_t nviews,
-                                                  uint32_t s) {
-    uint32_t v = 0;
-    if (views)
-        while (v + 1 < nviews && views[v + 1].super_base <= s) ++v;
-    return v;
-}
 
 struct BinScanArgs {
     const uint2* ordered;
@@ -162,7 +154,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
             const uint32_t swy = (y1 - 1) / kBinSuper - sy0 + 1;
             sc[k] = swx * swy;
             const BinView vw = view_of(a.views, a.view0, v);
-            key0 = vw.super_base + sy0 * vw.supers_x + sx0;
+            key0 = vw.key_base + sy0 * vw.supers_x + sx0;
             span = swx | (vw.supers_x << 8);
             gadd[k] = vw.gid_add;
         }
@@ -317,7 +309,7 @@ __global__ __launch_bounds__(256) void k_bin_chunk_map(const uint32_t* __restric
                                                        uint4* __restrict__ map) {
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
     if (s >= supers) return;
-    const uint32_t v = view_of_super(views, nviews, s);
+    const uint32_t v = s % nviews;
     const uint32_t st = sstart[s], en = sstart[s + 1];
     const uint32_t c0 = st / kBinChunk + s, c1 = en / kBinChunk + s + 1;
     for (uint32_t c = c0; c < c1; ++c) {
@@ -333,7 +325,7 @@ struct ChunkArgs {
     const uint32_t* offs_by_splat;
     const BinView* views;
     BinView view0;
-    uint32_t supers, cap;
+    uint32_t supers, cap, nviews;
     uint32_t* counts;              // count pass: per chunk per tile; place: chunk prefixes
     const uint32_t* tile_starts;   // place pass
     uint32_t* gids;
@@ -400,7 +392,7 @@ __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
         const uint32_t s = d.x & 0xffffffu;
         const uint32_t nr = (pe - p0 + 63) / 64;  // rounds in this chunk
         const BinView vw = view_of(a.views, a.view0, d.x >> 24);
-        const uint32_t ls = s - vw.super_base;
+        const uint32_t ls = s / a.nviews;
         const uint32_t sy = ls / vw.supers_x, sx = ls - sy * vw.supers_x;
         const uint32_t ox = sx * kBinSuper, oy = sy * kBinSuper;
         const uint32_t tx = ox + (lane & 7u), ty = oy + (lane >> 3);
@@ -545,8 +537,8 @@ __global__ __launch_bounds__(256) void k_bin_chunk_scan(const uint32_t* __restri
         run += cnt;
     }
     if (g == 0) {
-        const BinView vw = view_of(views, view0, view_of_super(views, nviews, s));
-        const uint32_t ls = s - vw.super_base;
+        const BinView vw = view_of(views, view0, s % nviews);
+        const uint32_t ls = s / nviews;
         const uint32_t sy = ls / vw.supers_x, sx = ls - sy * vw.supers_x;
         const uint32_t tx = sx * kBinSuper + (lane & 7u), ty = sy * kBinSuper + (lane >> 3);
         if (tx < vw.tiles_x && ty < vw.tiles_y)
@@ -608,7 +600,8 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     hipStream_t st = c->stream;
     static_assert(kBinMaxCap < kSat, "a saturated isect total must exceed every capacity");
     if (j.n == 0 || j.cap < 2 || j.cap > kBinMaxCap || j.supers == 0 || j.supers >= (1u << 24) ||
-        j.nviews > 255 ||
+        j.nviews == 0 || j.nviews > 255 || j.view_shift == 0 || j.view_shift + 8 > 32 ||
+        j.supers % j.nviews ||
         !j.gids || !j.tile_starts || !j.counters || !j.status)
         return fail(c, SFK_E_ARG, "binned emission: bad job");
     uint32_t* pair_keys = nullptr;
@@ -643,11 +636,12 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     uint32_t* sk = nullptr;
     uint2* sv = nullptr;
     int rc = argsort_u32_vals8(c, j.sort, pair_keys, pair_vals, j.cap, j.counters + 1,
-                               std::max(1u, sfk_bits_for(j.supers)), &sk, &sv);
+                               j.view_shift, &sk, &sv);
     if (rc) return rc;
     {
         SFK_PROF(c, "bin_ranges");
-        SFK_HIP(c, launch_tile_starts(st, sk, j.cap, j.counters + 1, j.supers, j.sstart));
+        SFK_HIP(c, launch_tile_starts(st, sk, j.cap, j.counters + 1, j.supers, j.sstart,
+                                      j.nviews > 1 ? j.nviews : 0u, j.view_shift));
         k_bin_chunk_map<<<div_up(j.supers, 256), 256, 0, st>>>(
             j.sstart, j.supers, j.views, j.nviews, reinterpret_cast<uint4*>(j.chunk_map));
         SFK_LAUNCHED(c, "bin chunk map");
@@ -660,6 +654,7 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     ca.views = j.views;
     ca.view0 = j.view0;
     ca.supers = j.supers;
+    ca.nviews = j.nviews;
     ca.cap = j.cap;
     ca.counts = j.chunk_counts;
     ca.tile_starts = j.tile_starts;

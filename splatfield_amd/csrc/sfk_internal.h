@@ -181,7 +181,7 @@ constexpr uint32_t kBinChunk = 512;   // sorted pairs per wavefront chunk
 struct BinView {
     uint32_t tiles_x, tiles_y;  // this view's tile grid
     uint32_t supers_x;          // supertiles per row, ceil(tiles_x / 8)
-    uint32_t super_base;        // first global supertile id of the view
+    uint32_t key_base;          // view << view_shift: pair keys are key_base + local supertile
     uint32_t out_mul, out_add;  // tile_starts index = local tile * out_mul + out_add
     uint32_t gid_add;           // written splat id = in-view splat + gid_add
     uint32_t pad;
@@ -193,7 +193,7 @@ constexpr uint32_t kBinMaxCap = 1u << 30;
 inline uint32_t bin_supers(uint32_t tiles_x, uint32_t tiles_y) {
     return div_up(tiles_x, kBinSuper) * div_up(tiles_y, kBinSuper);
 }
-// Chunk slots for `cap` pairs over `supers` supertiles: chunk k of
+// Chunk slots for `cap` pairs over `supers` supertile ids: chunk k of
 // supertile s is slot start[s] / kBinChunk + s + k (disjoint ranges).
 inline uint32_t bin_chunk_slots(uint32_t cap, uint32_t supers) {
     return cap / kBinChunk + supers + 1;
@@ -205,7 +205,8 @@ struct BinJob {
     const BinView* views;   // device, one per view (batched), or nullptr:
     BinView view0;          //   the single view, by value
     uint32_t nviews;
-    uint32_t supers;        // supertiles over all views
+    uint32_t view_shift;    // bits of the local supertile ids (the sorted bits), >= 1
+    uint32_t supers;        // supertile ids: local * nviews + view, max local x nviews
     uint32_t out_tiles;     // tile_starts entries before the total
     bool zero_starts;       // some tile_starts entries belong to no supertile
     uint32_t cap;           // isect capacity (the reference's max_isects), 2..kBinMaxCap

@@ -397,6 +397,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
             j.view0.out_mul = 1;
             j.nviews = 1;
             j.supers = bin_supers(s->tiles_x, s->tiles_y);
+            j.view_shift = std::max(1u, sfk_bits_for(j.supers));
             j.out_tiles = tiles;
             j.cap = s->capacity;
             j.order_out = s->order_buf;
