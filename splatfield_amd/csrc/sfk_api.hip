@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <new>
 
 #include "sfk_internal.h"
@@ -346,7 +347,7 @@ template <typename K, typename V = uint32_t>
 static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const V* vals,
                         uint32_t n, const uint32_t* d_n, uint32_t bits, int write_keys,
                         K** out_keys, V** out_vals, uint32_t nseg = 1,
-                        const uint32_t* payload = nullptr) {
+                        const uint32_t* payload = nullptr, const KeyRebase* rebase = nullptr) {
     if (!c || !s || !out_vals || (n && !keys)) return fail(c, SFK_E_ARG, "argsort: bad args");
     if (sizeof(K) != s->key_bytes) return fail(c, SFK_E_ARG, "argsort: scratch key width %u", s->key_bytes);
     if (sizeof(V) > s->val_bytes) return fail(c, SFK_E_ARG, "argsort: scratch value width %u", s->val_bytes);
@@ -389,6 +390,9 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const V
     a.kpb = radix_keys_per_block(total);
     a.stride = nseg > 1 ? div_up(n, a.kpb) : s->max_blocks;
     a.totals = s->totals;
+    a.rebase_mode = 0;
+    a.rebase_base = 0;
+    a.rebase_bases = nullptr;
     uint32_t shift = 0;
     for (uint32_t p = 0; p < passes; ++p) {
         const uint32_t width = wlo + (p < wextra ? 1u : 0u);
@@ -400,6 +404,11 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const V
         a.payload = payload;
         a.shift = shift;
         a.width = width;
+        a.rebase_mode = (p == 0 && rebase && sizeof(K) == 4) ? (rebase->bases ? 2u : 1u) : 0u;
+        if (a.rebase_mode) {
+            a.rebase_base = rebase->base;
+            a.rebase_bases = rebase->bases;
+        }
         SFK_PROF(c, sizeof(K) == 4 ? "radix_pass_u32" : "radix_pass_u64");
         hipError_t e = launch_radix_pass(c->stream, a, sizeof(K), sizeof(V),
                                          wlo + (wextra ? 1u : 0u));
@@ -416,10 +425,41 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const V
 
 int sfk::argsort_payload_u32(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys,
                              const uint32_t* payload, uint32_t n, uint32_t nseg, uint32_t bits,
-                             uint2** out) {
+                             uint2** out, const KeyRebase* rebase) {
     if (!payload) return fail(c, SFK_E_ARG, "argsort: no payload");
     return argsort_impl<uint32_t, uint2>(c, s, keys, nullptr, n, nullptr, bits, 0, nullptr, out,
-                                         nseg, payload);
+                                         nseg, payload, rebase);
+}
+
+static uint32_t f32_bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+
+// project_geom computes cz = dot(rot2, mean) + trans.z in f32 and culls
+// cz <= 0.1.  Over the AABB the exact value lies in [lo, hi]; the f32 one
+// within a few ulps of the magnitudes involved, covered by a relative
+// margin of 1e-5.
+bool sfk::depth_rebase(const sfk_view& v, const float bmin[3], const float bmax[3],
+                       uint32_t* base) {
+    double lo = v.trans[2], hi = v.trans[2], mag = fabs((double)v.trans[2]);
+    for (int i = 0; i < 3; ++i) {
+        const double r = v.rot2[i], p = r * bmin[i], q = r * bmax[i];
+        lo += std::min(p, q);
+        hi += std::max(p, q);
+        mag += fabs(r) * std::max(fabs((double)bmin[i]), fabs((double)bmax[i]));
+    }
+    const double err = 1e-5 * mag + 1e-30;
+    const float near = 0.1f;
+    float flo = (float)(lo - err);
+    if ((double)flo > lo - err) flo = nextafterf(flo, -INFINITY);
+    float fhi = (float)(hi + err);
+    if ((double)fhi < hi + err) fhi = nextafterf(fhi, INFINITY);
+    const uint32_t kmin = flo > near ? f32_bits(flo) : f32_bits(near);
+    const uint32_t kmax = fhi > near ? f32_bits(fhi) : kmin;
+    *base = kmin;
+    return kmax - kmin < 0xfffffeu && std::isfinite(fhi);
 }
 
 int sfk::argsort_u32_vals8(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys,

@@ -189,6 +189,10 @@ struct sfk_eig_round {
     uint32_t* bin_map = nullptr;
     uint32_t* bin_counts = nullptr;
     uint32_t* scan_sums = nullptr;
+    // Depth keys rebased per view to 24 bits when every view's AABB depth
+    // span allows it (depth_rebase).
+    uint32_t depth_bits = 32;
+    uint32_t* d_rebase = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const float* graph_a = nullptr;
@@ -245,7 +249,10 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
     bool emitted = false;
     if (r->compact) {
         uint2* ordered = nullptr;
-        rc = argsort_payload_u32(c, r->sort_depth, r->keys, r->cbbox, n, V, 32, &ordered);
+        KeyRebase rb;
+        rb.bases = r->d_rebase;
+        rc = argsort_payload_u32(c, r->sort_depth, r->keys, r->cbbox, n, V, r->depth_bits,
+                                 &ordered, r->d_rebase ? &rb : nullptr);
         if (rc) return rc;
         src.ordered = ordered;
         if (r->binned && r->capacity >= 2 && r->capacity <= kBinMaxCap) {
@@ -427,8 +434,20 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     RALLOC(r->d_bviews, nviews * sizeof(BatchView));
     RALLOC(r->proj, vn * 32);
     RALLOC(r->keys, vn * 4);
+    std::vector<uint32_t> rebase(nviews);
+    bool depth_fit = true;
+    {
+        float bmin[3], bmax[3];
+        sfk_splats_info(sp, nullptr, nullptr, bmin, bmax, nullptr, nullptr);
+        for (uint32_t v = 0; v < nviews; ++v)
+            depth_fit = depth_rebase(r->views[v], bmin, bmax, &rebase[v]) && depth_fit;
+    }
     if (r->compact) {
         RALLOC(r->cbbox, vn * 4);
+        if (depth_fit) {
+            RALLOC(r->d_rebase, nviews * 4);
+            r->depth_bits = 24;
+        }
         RALLOC(r->d_binviews, nviews * sizeof(BinView));
         RALLOC(r->offsets, vn * 4);
         RALLOC(r->bin_sstart, ((size_t)r->supers + 1) * 4);
@@ -453,6 +472,9 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     if (e == hipSuccess)
         e = hipMemcpyAsync(r->d_bviews, r->bviews.data(), nviews * sizeof(BatchView),
                            hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && r->d_rebase)
+        e = hipMemcpyAsync(r->d_rebase, rebase.data(), nviews * 4, hipMemcpyHostToDevice,
+                           c->stream);
     if (e == hipSuccess && r->d_binviews)
         e = hipMemcpyAsync(r->d_binviews, binviews.data(), nviews * sizeof(BinView),
                            hipMemcpyHostToDevice, c->stream);
@@ -474,7 +496,8 @@ extern "C" void sfk_eig_round_destroy(sfk_eig_round* r) {
     if (r->graph) hipGraphDestroy(r->graph);
     void* bufs[] = {r->d_views, r->d_bviews, r->proj, r->keys, r->counts, r->bbox, r->cbbox,
                     r->counters, r->starts, r->resp, r->h0, r->partials, r->eig, r->best,
-                    r->emit_status, r->d_binviews, r->offsets, r->bin_sstart, r->scan_sums};
+                    r->emit_status, r->d_binviews, r->offsets, r->bin_sstart, r->scan_sums,
+                    r->d_rebase};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (r->host_out) hipHostFree(r->host_out);

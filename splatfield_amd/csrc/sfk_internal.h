@@ -116,7 +116,25 @@ struct SortArgs {
     uint32_t* hist;           // nseg * RADIX rows of `stride` blocks
     uint32_t stride;
     uint32_t* totals;         // nseg * RADIX
+    // Pass 0 of u32 keys may rebase them first: key - base (base per segment
+    // from rebase_bases when mode 2, rebase_base when mode 1), 0xffffffff
+    // kept, the rest clamped to [0, 0xfffffe].  Order-preserving for keys in
+    // [base, base + 0xfffffe]; later passes see the rebased keys.
+    uint32_t rebase_mode;
+    uint32_t rebase_base;
+    const uint32_t* rebase_bases;
 };
+// Depth keys of a view (f32 bits of camera z of visible splats, 0xffffffff
+// for culled ones) rebased so that 24 bits order them: see depth_rebase.
+struct KeyRebase {
+    uint32_t base = 0;                  // one segment
+    const uint32_t* bases = nullptr;    // or one per segment (device)
+};
+// A lower bound `base` on the depth keys of every splat visible in view v
+// whose mean lies in [bmin, bmax] (the splats' AABB); true when all of them
+// lie in [base, base + 0xfffffe], i.e. a 24-bit sort of the rebased keys
+// orders them as the 32-bit sort of the keys does (culled keys last).
+bool depth_rebase(const sfk_view& v, const float bmin[3], const float bmax[3], uint32_t* base);
 hipError_t launch_radix_pass(hipStream_t s, const SortArgs& a, uint32_t key_bytes,
                              uint32_t val_bytes, uint32_t digit_bits);
 // Radix scratch with 8-byte values (u32 keys only), and the argsort of u32
@@ -126,7 +144,7 @@ int radix_scratch_create_ex(sfk_ctx* c, uint32_t max_elems, uint32_t key_bytes,
                             uint32_t val_bytes, sfk_radix_scratch** out);
 int argsort_payload_u32(sfk_ctx* c, sfk_radix_scratch* s, const uint32_t* keys,
                         const uint32_t* payload, uint32_t n, uint32_t nseg, uint32_t bits,
-                        uint2** out);
+                        uint2** out, const KeyRebase* rebase = nullptr);
 // The buffer pair of a 4-byte-key, 8-byte-value scratch that an argsort on it
 // may take as input (the first pass writes the other pair), and the stable
 // argsort of u32 keys carrying 8-byte values (one segment, optional device

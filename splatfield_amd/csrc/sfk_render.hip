@@ -382,8 +382,14 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
     // order instead of gathering counts and boxes per rank.
     EmitSource src;
     if (compact) {
+        // Where the splats' AABB bounds this view's visible depths to a 2^24
+        // span of f32 bit patterns, 24 bits of the rebased keys give the
+        // same order as the 32-bit keys: three radix passes, not four.
+        KeyRebase rb;
+        const bool fit = depth_rebase(*view, sp->bmin, sp->bmax, &rb.base);
         uint2* ordered = nullptr;
-        rc = argsort_payload_u32(c, s->sort_depth, s->depth_keys, s->cbbox, n, 1, 32, &ordered);
+        rc = argsort_payload_u32(c, s->sort_depth, s->depth_keys, s->cbbox, n, 1, fit ? 24 : 32,
+                                 &ordered, fit ? &rb : nullptr);
         if (rc) return rc;
         if (s->binned && s->capacity >= 2 && s->capacity <= kBinMaxCap) {
             // Each tile's isects straight from the depth order, no isect sort.

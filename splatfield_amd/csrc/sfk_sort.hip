@@ -39,6 +39,20 @@ __device__ __forceinline__ uint32_t digit_of(K key, uint32_t shift, uint32_t mas
     return (uint32_t)(key >> shift) & mask;
 }
 
+// The key as pass 0 sees it when rebasing (SortArgs::rebase_mode).
+template <typename K>
+__device__ __forceinline__ K rebased(K key, const SortArgs& a, uint32_t seg) {
+    if constexpr (sizeof(K) == 4) {
+        if (a.rebase_mode) {
+            const uint32_t b = a.rebase_mode == 2 ? a.rebase_bases[seg] : a.rebase_base;
+            if (key == 0xffffffffu) return key;
+            const uint32_t d = key >= b ? key - b : 0u;
+            return d < 0xfffffeu ? d : 0xfffffeu;
+        }
+    }
+    return key;
+}
+
 // Which segment and block-within-segment this workgroup handles, and the
 // segment's key range; false if the block has no live keys.
 struct SegPos {
@@ -88,12 +102,12 @@ __global__ __launch_bounds__(NT) void k_upsweep(SortArgs a) {
     const K* kp = keys + p.seg_base + loc0;
     if (p.bl * KPB + KPB <= p.seg_n) {
 #pragma unroll
-        for (uint32_t e = 0; e < EPT; ++e) dg[e] = digit_of(kp[e * NT], a.shift, mask);
+        for (uint32_t e = 0; e < EPT; ++e) dg[e] = digit_of(rebased(kp[e * NT], a, p.s), a.shift, mask);
     } else {
 #pragma unroll
         for (uint32_t e = 0; e < EPT; ++e) {
             const uint32_t loc = loc0 + e * NT;
-            dg[e] = loc < p.seg_n ? digit_of(kp[e * NT], a.shift, mask) : ~0u;
+            dg[e] = loc < p.seg_n ? digit_of(rebased(kp[e * NT], a, p.s), a.shift, mask) : ~0u;
         }
     }
 #pragma unroll
@@ -208,7 +222,7 @@ __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
     for (uint32_t e = 0; e < EPT; ++e) {
         uint32_t li = wbase + e * 64;
         bool valid = li < p.seg_n;
-        key[e] = valid ? keys_in[p.seg_base + li] : K(0);
+        key[e] = valid ? rebased(keys_in[p.seg_base + li], a, p.s) : K(0);
         if constexpr (sizeof(V) == 4) {
             val[e] = valid ? (vals_in ? vals_in[p.seg_base + li] : li) : 0u;
         } else {

@@ -272,3 +272,24 @@ def test_truncated_first_frame(ctx):
         isects = R.prepare_isects(splats, scratch, view=view, w=w, h=h)
         assert isects.counts()[2] == cap
         check_prepare(scratch, isects, ref, n)
+
+
+@pytest.mark.parametrize("zspan", [(-1.0, 1.0), (-40.0, 2.0)])
+def test_depth_sort_key_window(ctx, zspan):
+    """The depth sort runs on 24 bits of keys rebased to the view's depth
+    window when the splats' AABB confines the visible depths to 2^24 f32 bit
+    patterns (camera 3 units from a [-1,1]^3 scene), and on all 32 bits
+    otherwise (depths 1..43): both orders equal the oracle's, ties and
+    culled splats included."""
+    n = 16384
+    attrs, sh, k = S.synthetic_scene(n, 0, seed=21)
+    attrs = attrs.reshape(S.ATTR_PLANES, n)
+    rng = np.random.default_rng(21)
+    attrs[2] = rng.uniform(zspan[0], zspan[1], n).astype(np.float32)
+    attrs[2, :64] = attrs[2, 64:128]   # exact depth ties
+    attrs[0, -32:] = 500.0             # off-screen: culled
+    attrs = attrs.reshape(-1)
+    cam = R.Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0))
+    splats, scratch, view, isects, ref = run_both(ctx, attrs, sh, n, k, cam, 320, 240)
+    check_prepare(scratch, isects, ref, n)
+    check_rgb(scratch, isects, ref, n, 320, 240, 16)
