@@ -99,8 +99,9 @@ struct BinScanArgs {
 // search over the wave's inclusive pair prefix in LDS.
 __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
     __shared__ uint32_t s_incl[kRanks];  // inclusive pair prefix within the wave's ranks
-    __shared__ uint32_t s_key0[kRanks];  // key of the rank's first supertile
-    __shared__ uint32_t s_span[kRanks];  // supertile columns | view supertiles per row << 8
+    // Per rank: key of its first supertile (18 bits) | supertile columns
+    // spanned << 18 | view supertiles per row << 24.
+    __shared__ uint32_t s_span[kRanks];
     __shared__ uint2 s_val[kRanks];      // pair value: (splat | may-truncate << 31, bbox)
     __shared__ uint32_t s_wi[kThreads / 64], s_wp[kThreads / 64];
     __shared__ uint32_t s_bid, s_ei, s_ep;
@@ -155,11 +156,10 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
             sc[k] = swx * swy;
             const BinView vw = view_of(a.views, a.view0, v);
             key0 = vw.key_base + sy0 * vw.supers_x + sx0;
-            span = swx | (vw.supers_x << 8);
+            span = swx | (vw.supers_x << 6);
             gadd[k] = vw.gid_add;
         }
-        s_key0[t * kRpt + k] = key0;
-        s_span[t * kRpt + k] = span;
+        s_span[t * kRpt + k] = key0 | (span << 18);
         ci += cnt[k];
         cp += sc[k];
     }
@@ -290,9 +290,10 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
             const uint32_t k = lo[q];
             const uint32_t j = e - (k ? incl[k - 1] : 0u);
             const uint32_t span = s_span[first + k];
-            const uint32_t swx = span & 0xffu, row = span >> 8;
+            const uint32_t key0 = span & 0x3ffffu;
+            const uint32_t swx = (span >> 18) & 63u, row = span >> 24;
             const uint32_t dy = j < swx ? 0u : j / swx;
-            a.pair_keys[slot] = s_key0[first + k] + dy * row + (j - dy * swx);
+            a.pair_keys[slot] = key0 + dy * row + (j - dy * swx);
             a.pair_vals[slot] = s_val[first + k];
         }
     }
@@ -600,7 +601,7 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     hipStream_t st = c->stream;
     static_assert(kBinMaxCap < kSat, "a saturated isect total must exceed every capacity");
     if (j.n == 0 || j.cap < 2 || j.cap > kBinMaxCap || j.supers == 0 || j.supers >= (1u << 24) ||
-        j.nviews == 0 || j.nviews > 255 || j.view_shift == 0 || j.view_shift + 8 > 32 ||
+        j.nviews == 0 || j.nviews > 255 || j.view_shift == 0 || j.view_shift > 10 ||
         j.supers % j.nviews ||
         !j.gids || !j.tile_starts || !j.counters || !j.status)
         return fail(c, SFK_E_ARG, "binned emission: bad job");
