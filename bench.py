@@ -211,18 +211,20 @@ def bench_sort(ctx, args):
             "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
-def kernel_bytes(n, k, vis, ni, px, tile_bits):
-    """Algorithmic HBM bytes per launch of each step of the frame, counting
-    the reference's two-stage pipeline (8-bit digits): reads + writes a
-    minimal implementation must make."""
+def kernel_bytes(n, k, vis, ni, px, pairs):
+    """Algorithmic HBM bytes per launch of each step of the frame: the reads
+    and writes the implemented pipeline must make (not counting cache
+    reuse).  Radix passes move key + value and read the key once more for
+    the histogram: 28 B per element with 8-byte values."""
     return {
         "project": n * (44 + 4 * 3 * k) + 56 * vis,
-        "radix_pass_u32": None,  # depth and tile passes differ; see below
-        "scan_gather": 16 * vis,
-        "map_isects": 16 * vis + 8 * ni,
-        # order + counts + bbox read, offsets written, isects written
-        "scan_emit": 20 * vis + 8 * ni,
-        "tile_starts": 4 * ni,
+        "radix_pass_u32": None,  # depth and pair passes differ; see main()
+        # (splat, bbox) per rank read; depth order and offsets written;
+        # (supertile, splat, bbox) pairs written
+        "bin_scan": 16 * n + 12 * pairs,
+        "bin_count": 8 * pairs + 256 * (pairs // 512),
+        # pairs read again; splat id and tile id written per isect
+        "bin_place": 8 * pairs + 8 * ni,
         "raster_rgb": 40 * ni + 16 * px,
         "loss_l2": 48 * px,
         "raster_bwd": 40 * ni + 32 * px + 36 * vis,
@@ -309,13 +311,14 @@ def main():
     prof = ctx.prof_report()
     ctx.prof_enable(False)
 
-    tiles = ((w + 15) // 16) * ((h + 15) // 16)
-    tile_bits = int(math.ceil(math.log2(tiles)))
-    kb = kernel_bytes(n, k, vis, ni, px, tile_bits)
-    depth_passes, tile_passes = 4, math.ceil(tile_bits / 8)
-    # radix passes: depth passes move 16 B per visible key, tile passes 16 B per isect
-    kb["radix_pass_u32"] = (depth_passes * 16 * vis + tile_passes * 16 * ni) / (
-        depth_passes + tile_passes)
+    pairs = int(scratch.read("counters")[1])
+    kb = kernel_bytes(n, k, vis, ni, px, pairs)
+    # Radix passes per frame: the depth sort (three passes on rebased 24-bit
+    # keys where the view's depth window allows, else four) over all n keys,
+    # then one pass over the supertile pairs.
+    radix_launches = prof.get("radix_pass_u32", (0, 0.0))[0] / args.steps
+    depth_passes = max(int(round(radix_launches)) - 1, 0)
+    kb["radix_pass_u32"] = (depth_passes * 28 * n + 28 * pairs) / max(depth_passes + 1, 1)
     per_kernel = {}
     for name, (cnt, ms) in prof.items():
         per_launch = ms / max(cnt, 1)
@@ -331,8 +334,8 @@ def main():
     dom_gbs = kb[dom] / (dom_ms * 1e-3) / 1e9 if kb.get(dom) else None
     traffic = measured_traffic(dom)
 
-    b_fwd = (n * (44 + 4 * 3 * k) + 56 * vis + 16 * vis + 8 * ni + 4 * 16 * vis +
-             math.ceil(tile_bits / 8) * 16 * ni + 40 * ni + 16 * px)
+    b_fwd = (kb["project"] + depth_passes * 28 * n + 28 * pairs + kb["bin_scan"] +
+             kb["bin_count"] + kb["bin_place"] + kb["raster_rgb"])
     b_bwd = 40 * ni + 32 * px + 36 * vis + n * (44 + 4 * 3 * k) + 36 * vis + n * (44 + 4 * 3 * k)
     step_gbs = (b_fwd + b_bwd) / (ms_per_step * 1e-3) / 1e9
 
@@ -364,7 +367,7 @@ def main():
                 "opacity logits N(0,1), SH3; target = same scene perturbed by N(0,0.01))",
         "config": {"workload": "1080p forward+backward frame, 1,048,576 Gaussians SH3, L2 loss",
                    "gaussians": n, "width": w, "height": h, "isects": ni, "visible": vis,
-                   "loss": loss_v},
+                   "supertile_pairs": pairs, "depth_sort_passes": depth_passes, "loss": loss_v},
         "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 1) if dom_gbs else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 4) if dom_gbs else None,

@@ -329,6 +329,11 @@ extern "C" void sfk_radix_scratch_destroy(sfk_radix_scratch* s) {
 
 // An argsort reads its input in the first pass only, and that pass writes
 // keys[0] / vals[0]: the second buffer pair may hold the input itself.
+const uint32_t* sfk::radix_scratch_totals(sfk_radix_scratch* s, uint32_t* digit_bits) {
+    if (digit_bits) *digit_bits = s ? s->digit_bits : 0u;
+    return s ? s->totals : nullptr;
+}
+
 int sfk::radix_scratch_spare_u32x2(sfk_radix_scratch* s, uint32_t** keys, uint2** vals) {
     if (!s || s->key_bytes != 4 || s->val_bytes != 8) return SFK_E_ARG;
     *keys = static_cast<uint32_t*>(s->keys[1]);

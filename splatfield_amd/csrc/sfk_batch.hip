@@ -236,7 +236,9 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
     const size_t vn = (size_t)V * n;
     const float* attrs = nullptr;
     sfk_splats_info(r->splats, nullptr, nullptr, nullptr, nullptr, &attrs, nullptr);
-    SFK_HIP(c, launch_fill_u32(c->stream, r->counters, 0u, 2 + (size_t)V * 2 * kCountSlots));
+    const bool binned = r->compact && r->binned && r->capacity >= 2 && r->capacity <= kBinMaxCap;
+    SFK_HIP(c, launch_zero2_u32(c->stream, r->counters, 2 + (size_t)V * 2 * kCountSlots,
+                                r->emit_status, binned ? scan_emit_status_words((uint32_t)vn) : 0u));
     {
         SFK_PROF(c, "batch_project");
         k_project_batch<<<div_up(n, 256), 256, 0, c->stream>>>(
@@ -255,7 +257,7 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
                                  &ordered, r->d_rebase ? &rb : nullptr);
         if (rc) return rc;
         src.ordered = ordered;
-        if (r->binned && r->capacity >= 2 && r->capacity <= kBinMaxCap) {
+        if (binned) {
             // Every (view, tile) list in depth order, no isect sort; tile
             // ranges indexed local * V + view as the batched raster reads them.
             BinJob j;
@@ -280,6 +282,7 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
             j.gids = r->gids;
             j.counters = r->counters;
             j.status = r->emit_status;
+            j.status_zeroed = true;
             rc = bin_emit(c, j);
             if (rc) return rc;
             r->sorted_tiles = nullptr;

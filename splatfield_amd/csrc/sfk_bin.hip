@@ -87,6 +87,7 @@ struct BinScanArgs {
     uint32_t* pair_keys;
     uint2* pair_vals;
     uint32_t* counters;
+    uint32_t* host_total;
     uint32_t* ticket;
     uint64_t* status;
 };
@@ -226,6 +227,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
             if (bid * kRanks + kRanks >= a.n) {
                 a.counters[0] = sat_add(ei, tot_i);
                 a.counters[1] = min(sat_add(ep, tot_p), a.cap);
+                if (a.host_total) *a.host_total = sat_add(ei, tot_i);
             }
         }
     }
@@ -316,6 +318,39 @@ __global__ __launch_bounds__(256) void k_bin_chunk_map(const uint32_t* __restric
     for (uint32_t c = c0; c < c1; ++c) {
         const uint32_t p0 = st + (c - c0) * kBinChunk;
         map[c] = make_uint4((v << 24) | s, p0, min(p0 + kBinChunk, en), 0u);
+    }
+}
+
+// Single view, pairs sorted in one radix pass: the pass's digit totals are
+// the supertiles' pair counts, so one workgroup scans them into the
+// supertile ranges and writes the chunk map (k_tile_starts over the sorted
+// keys and k_bin_chunk_map in one launch).
+__global__ __launch_bounds__(256) void k_bin_ranges1(const uint32_t* __restrict__ totals,
+                                                     uint32_t supers,
+                                                     uint32_t* __restrict__ sstart,
+                                                     uint4* __restrict__ map) {
+    __shared__ uint32_t tmp[256 / 64 + 1];
+    __shared__ uint32_t s_start[257];
+    const uint32_t t = threadIdx.x;
+    const uint32_t c = t < supers ? totals[t] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan_u32<256>(c, tmp, &total);
+    if (t < supers) {
+        s_start[t] = ex;
+        sstart[t] = ex;
+    }
+    if (t == 0) {
+        s_start[supers] = total;
+        sstart[supers] = total;
+    }
+    __syncthreads();
+    if (t < supers) {
+        const uint32_t st = ex, en = s_start[t + 1];
+        const uint32_t c0 = st / kBinChunk + t, c1 = en / kBinChunk + t + 1;
+        for (uint32_t k = c0; k < c1; ++k) {
+            const uint32_t p0 = st + (k - c0) * kBinChunk;
+            map[k] = make_uint4(t, p0, min(p0 + kBinChunk, en), 0u);
+        }
     }
 }
 
@@ -611,7 +646,8 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
         return fail(c, SFK_E_ARG, "binned emission: pair sort scratch");
     if ((j.seg_len ? j.seg_len : j.n) > 0x7fffffffu || !j.offs_by_splat)
         return fail(c, SFK_E_ARG, "binned emission: splat ids need 31 bits");
-    SFK_HIP(c, launch_fill_u32(st, j.status, 0u, scan_emit_status_words(j.n)));
+    if (!j.status_zeroed)
+        SFK_HIP(c, launch_fill_u32(st, j.status, 0u, scan_emit_status_words(j.n)));
     {
         SFK_PROF(c, "bin_scan");
         BinScanArgs a;
@@ -627,6 +663,7 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
         a.pair_keys = pair_keys;
         a.pair_vals = pair_vals;
         a.counters = j.counters;
+        a.host_total = j.host_total;
         a.ticket = j.status;
         a.status = reinterpret_cast<uint64_t*>(j.status + 2);
         k_bin_scan<<<div_up(j.n, kRanks), kThreads, 0, st>>>(a);
@@ -641,11 +678,19 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     if (rc) return rc;
     {
         SFK_PROF(c, "bin_ranges");
-        SFK_HIP(c, launch_tile_starts(st, sk, j.cap, j.counters + 1, j.supers, j.sstart,
-                                      j.nviews > 1 ? j.nviews : 0u, j.view_shift));
-        k_bin_chunk_map<<<div_up(j.supers, 256), 256, 0, st>>>(
-            j.sstart, j.supers, j.views, j.nviews, reinterpret_cast<uint4*>(j.chunk_map));
-        SFK_LAUNCHED(c, "bin chunk map");
+        uint32_t db = 0;
+        const uint32_t* totals = radix_scratch_totals(j.sort, &db);
+        if (j.nviews == 1 && j.view_shift <= db && j.supers <= 256) {
+            k_bin_ranges1<<<1, 256, 0, st>>>(totals, j.supers, j.sstart,
+                                             reinterpret_cast<uint4*>(j.chunk_map));
+            SFK_LAUNCHED(c, "bin ranges");
+        } else {
+            SFK_HIP(c, launch_tile_starts(st, sk, j.cap, j.counters + 1, j.supers, j.sstart,
+                                          j.nviews > 1 ? j.nviews : 0u, j.view_shift));
+            k_bin_chunk_map<<<div_up(j.supers, 256), 256, 0, st>>>(
+                j.sstart, j.supers, j.views, j.nviews, reinterpret_cast<uint4*>(j.chunk_map));
+            SFK_LAUNCHED(c, "bin chunk map");
+        }
     }
     ChunkArgs ca;
     ca.pvals = sv;

@@ -81,6 +81,9 @@ constexpr uint32_t kCountSlots = 64;
 constexpr uint32_t kCounterWords = 2 + 2 * kCountSlots;
 
 uint32_t* scan_scratch_sums(sfk_scan_scratch* s);
+// Per-digit totals of the last radix pass of a single-segment argsort on `s`
+// (its digit width is s's digit bits).
+const uint32_t* radix_scratch_totals(sfk_radix_scratch* s, uint32_t* digit_bits);
 inline uint32_t next_pow2(uint64_t v) {
     uint64_t p = 1;
     while (p < v) p <<= 1;
@@ -90,6 +93,7 @@ inline uint32_t next_pow2(uint64_t v) {
 // ---- kernel launchers (each defined in the .hip file holding the kernel)
 
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* buf, uint32_t v, size_t count);
+hipError_t launch_zero2_u32(hipStream_t s, uint32_t* a, size_t na, uint32_t* b, size_t nb);
 hipError_t launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, size_t count);
 
 hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, const float* sh,
@@ -243,6 +247,9 @@ struct BinJob {
     uint32_t* tile_ids;     // cap or nullptr: tile per isect
     uint32_t* counters;     // [0] raw isect count, [1] pair count
     uint32_t* status;       // scan_emit_status_words(n)
+    bool status_zeroed;     // the caller cleared `status` (else bin_emit does)
+    uint32_t* host_total;   // optional device view of mapped pinned host memory:
+                            // receives the raw isect count as well
 };
 int bin_emit(sfk_ctx* c, const BinJob& j);
 

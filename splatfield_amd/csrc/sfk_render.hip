@@ -112,6 +112,7 @@ struct sfk_render_scratch {
     uint32_t* order_buf = nullptr;    // depth order written by the emission
     // Previous frame's raw emission, read back asynchronously for growth.
     uint32_t* host_counters = nullptr;
+    uint32_t* host_counters_dev = nullptr;  // device view of the mapped pinned word
     hipEvent_t counters_ready = nullptr;
     bool counters_pending = false;
     uint32_t last_isects_raw = 0;
@@ -231,7 +232,9 @@ extern "C" int sfk_render_scratch_create(sfk_ctx* c, uint32_t n, uint32_t w, uin
     if (!s) return SFK_E_NOMEM;
     s->ctx = c;
     int rc = SFK_OK;
-    if (hipHostMalloc(&s->host_counters, 16, hipHostMallocDefault) != hipSuccess ||
+    if (hipHostMalloc(&s->host_counters, 16, hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&s->host_counters_dev), s->host_counters,
+                                0) != hipSuccess ||
         hipEventCreateWithFlags(&s->counters_ready, hipEventDisableTiming) != hipSuccess)
         rc = fail(c, SFK_E_HIP, "render scratch: pinned counters");
     if (!rc) rc = scratch_build(s, n, w, h);
@@ -367,7 +370,10 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
     const uint32_t tiles = s->tiles_x * s->tiles_y;
     const bool compact = compact_grid(s);
 
-    SFK_HIP(c, launch_fill_u32(c->stream, s->counters, 0u, kCounterWords));
+    // Counters (and, for the binned emission, its look-back words) start at 0.
+    const bool binned = compact && s->binned && s->capacity >= 2 && s->capacity <= kBinMaxCap;
+    SFK_HIP(c, launch_zero2_u32(c->stream, s->counters, kCounterWords, s->emit_status,
+                                binned ? scan_emit_status_words(n) : 0u));
     {
         SFK_PROF(c, "project");
         SFK_HIP(c, launch_project(c->stream, *view, sp->attrs, sp->sh, n,
@@ -391,7 +397,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
         rc = argsort_payload_u32(c, s->sort_depth, s->depth_keys, s->cbbox, n, 1, fit ? 24 : 32,
                                  &ordered, fit ? &rb : nullptr);
         if (rc) return rc;
-        if (s->binned && s->capacity >= 2 && s->capacity <= kBinMaxCap) {
+        if (binned) {
             // Each tile's isects straight from the depth order, no isect sort.
             BinJob j;
             memset(&j, 0, sizeof j);
@@ -418,10 +424,10 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
             j.tile_ids = s->tile_ids;
             j.counters = s->counters;
             j.status = s->emit_status;
+            j.status_zeroed = true;
+            j.host_total = s->host_counters_dev;  // the raw count, for next frame's capacity
             rc = bin_emit(c, j);
             if (rc) return rc;
-            SFK_HIP(c, hipMemcpyAsync(s->host_counters, s->counters, 4, hipMemcpyDeviceToHost,
-                                      c->stream));
             SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));
             s->counters_pending = true;
             s->depth_order = s->order_buf;

@@ -16,6 +16,17 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ buf, ui
     for (; i < count; i += stride) buf[i] = v;
 }
 
+// Two buffers zeroed in one launch (a frame's counters and look-back words).
+__global__ __launch_bounds__(256) void k_zero2_u32(uint32_t* __restrict__ a, size_t na,
+                                                   uint32_t* __restrict__ b, size_t nb) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (; i < na + nb; i += stride) {
+        if (i < na) a[i] = 0u;
+        else b[i - na] = 0u;
+    }
+}
+
 // Plain streaming copy (a known-traffic kernel for counter calibration).
 __global__ __launch_bounds__(256) void k_copy_u32(uint4* __restrict__ dst,
                                                   const uint4* __restrict__ src, size_t count4) {
@@ -182,6 +193,14 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t* buf, uint32_t v, size_t coun
     size_t blocks = (count + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     k_fill_u32<<<(uint32_t)blocks, 256, 0, s>>>(buf, v, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero2_u32(hipStream_t s, uint32_t* a, size_t na, uint32_t* b, size_t nb) {
+    if (na + nb == 0) return hipSuccess;
+    size_t blocks = (na + nb + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    k_zero2_u32<<<(uint32_t)blocks, 256, 0, s>>>(a, na, b, nb);
     return hipGetLastError();
 }
 
