@@ -14,6 +14,7 @@
 #include "sfk_device.h"
 #include "sfk_internal.h"
 #include "sfk_math.h"
+#include "sfk_project_core.h"
 #include "sfk_raster_core.h"
 
 namespace sfk {
@@ -290,7 +291,8 @@ constexpr float kK9 = -0.5900436f;
 // Projection adjoint, one thread per splat.  CHS = SH coefficients per channel.
 template <int CHS>
 __global__ __launch_bounds__(256) void k_project_bwd(sfk_view v, const float* __restrict__ A,
-                                                     const float* __restrict__ shs, uint32_t n,
+                                                     const float* __restrict__ shs,
+                                                     const float* __restrict__ dcol, uint32_t n,
                                                      float* __restrict__ g2,
                                                      float* __restrict__ gA,
                                                      float* __restrict__ gS,
@@ -376,22 +378,15 @@ __global__ __launch_bounds__(256) void k_project_bwd(sfk_view v, const float* __
     float ivl = 1.0f / vlen;
     float x = d[0] * ivl, y = d[1] * ivl, z = d[2] * ivl;
     float B[16];
-    float D[16][3];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) { B[k] = 0.f; D[k][0] = D[k][1] = D[k][2] = 0.f; }
+    for (int k = 0; k < 16; ++k) B[k] = 0.f;
     B[0] = kShC0;
     if constexpr (CHS >= 4) {
         B[1] = -kShC1 * y; B[2] = kShC1 * z; B[3] = -kShC1 * x;
-        D[1][1] = -kShC1; D[2][2] = kShC1; D[3][0] = -kShC1;
     }
     if constexpr (CHS >= 9) {
         B[4] = kShC23 * 2.0f * x * y; B[5] = kShC22 * z * y; B[6] = kShC20 * z * z - kShC21;
         B[7] = kShC22 * z * x; B[8] = kShC23 * (x * x - y * y);
-        D[4][0] = 2.f * kShC23 * y; D[4][1] = 2.f * kShC23 * x;
-        D[5][1] = kShC22 * z; D[5][2] = kShC22 * y;
-        D[6][2] = 2.f * kShC20 * z;
-        D[7][0] = kShC22 * z; D[7][2] = kShC22 * x;
-        D[8][0] = 2.f * kShC23 * x; D[8][1] = -2.f * kShC23 * y;
     }
     if constexpr (CHS >= 16) {
         float c1x = x * x - y * y, c1y = 2.f * x * y;
@@ -401,28 +396,36 @@ __global__ __launch_bounds__(256) void k_project_bwd(sfk_view v, const float* __
         B[11] = tt0 * y; B[12] = tt0 * x;
         B[13] = z * (1.8658817f * z * z - 1.119529f);
         B[14] = tt1 * c1y; B[15] = tt1 * c1x;
-        D[9][0] = kK9 * c1y + kK9 * x * 2.f * y + y * 2.f * x;
-        D[9][1] = kK9 * x * 2.f * x + c1x - 2.f * y * y;
-        D[10][0] = kK9 * c1x + kK9 * x * 2.f * x - 2.f * y * y;
-        D[10][1] = -kK9 * x * 2.f * y - c1y - 2.f * x * y;
-        D[11][1] = tt0; D[11][2] = -4.570458f * z * y;
-        D[12][0] = tt0; D[12][2] = -4.570458f * z * x;
-        D[13][2] = 5.5976451f * z * z - 1.119529f;
-        D[14][0] = tt1 * 2.f * y; D[14][1] = tt1 * 2.f * x; D[14][2] = 1.4453057f * c1y;
-        D[15][0] = tt1 * 2.f * x; D[15][1] = -tt1 * 2.f * y; D[15][2] = 1.4453057f * c1x;
     }
-    float gd0 = 0.f, gd1 = 0.f, gd2 = 0.f;
 #pragma unroll
     for (int k = 0; k < CHS; ++k)
 #pragma unroll
+        for (int c = 0; c < 3; ++c) gS[(size_t)(3 * k + c) * N + i] = B[k] * G2[6 + c];
+    // Gradient w.r.t. the unit view direction through the colour: from the
+    // forward's d colour / d direction when it kept it (9 floats read
+    // instead of the 3 x CHS SH coefficients), else from the coefficients.
+    float gd0 = 0.f, gd1 = 0.f, gd2 = 0.f;
+    if (dcol) {
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
-            float gc = G2[6 + c];
-            gS[(size_t)(3 * k + c) * N + i] = B[k] * gc;
-            float sv = shs[(size_t)(3 * k + c) * N + i] * gc;
-            gd0 += sv * D[k][0];
-            gd1 += sv * D[k][1];
-            gd2 += sv * D[k][2];
+            const float gc = G2[6 + c];
+            gd0 += gc * dcol[(size_t)(3 * c + 0) * N + i];
+            gd1 += gc * dcol[(size_t)(3 * c + 1) * N + i];
+            gd2 += gc * dcol[(size_t)(3 * c + 2) * N + i];
         }
+    } else {
+        float D[16][3];
+        sh_basis_dir_grad<CHS>(x, y, z, D);
+#pragma unroll
+        for (int k = 0; k < CHS; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float sv = shs[(size_t)(3 * k + c) * N + i] * G2[6 + c];
+                gd0 += sv * D[k][0];
+                gd1 += sv * D[k][1];
+                gd2 += sv * D[k][2];
+            }
+    }
     float dd = gd0 * x + gd1 * y + gd2 * z;
     float gm[3] = {(gd0 - x * dd) * ivl, (gd1 - y * dd) * ivl, (gd2 - z * dd) * ivl};
 
@@ -532,17 +535,17 @@ hipError_t launch_raster_bwd(hipStream_t s, const RasterArgs& ra, const float* f
 hipError_t launch_project_bwd(hipStream_t s, const sfk_view& v, const float* attrs,
                               const float* sh, uint32_t n, uint32_t sh_per_ch,
                               float* grad2d, float* g_attrs, float* g_sh,
-                              const BwdLoss* loss) {
+                              const BwdLoss* loss, const float* dcol) {
     if (n == 0) return hipSuccess;
     dim3 grid(div_up(n, 256));
     const double* lp = loss ? loss->partials : nullptr;
     const uint32_t lnp = loss ? loss->num_partials : 0u;
     double* lo = loss ? loss->out : nullptr;
     switch (sh_per_ch) {
-        case 1: k_project_bwd<1><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh, lp, lnp, lo); break;
-        case 4: k_project_bwd<4><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh, lp, lnp, lo); break;
-        case 9: k_project_bwd<9><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh, lp, lnp, lo); break;
-        case 16: k_project_bwd<16><<<grid, 256, 0, s>>>(v, attrs, sh, n, grad2d, g_attrs, g_sh, lp, lnp, lo); break;
+        case 1: k_project_bwd<1><<<grid, 256, 0, s>>>(v, attrs, sh, dcol, n, grad2d, g_attrs, g_sh, lp, lnp, lo); break;
+        case 4: k_project_bwd<4><<<grid, 256, 0, s>>>(v, attrs, sh, dcol, n, grad2d, g_attrs, g_sh, lp, lnp, lo); break;
+        case 9: k_project_bwd<9><<<grid, 256, 0, s>>>(v, attrs, sh, dcol, n, grad2d, g_attrs, g_sh, lp, lnp, lo); break;
+        case 16: k_project_bwd<16><<<grid, 256, 0, s>>>(v, attrs, sh, dcol, n, grad2d, g_attrs, g_sh, lp, lnp, lo); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

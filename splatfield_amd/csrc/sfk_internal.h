@@ -99,7 +99,7 @@ hipError_t launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, si
 hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, const float* sh,
                           uint32_t n, uint32_t sh_per_ch, uint32_t tile_w, uint32_t* depth_keys,
                           float* proj, uint32_t* tile_counts, uint32_t* bbox, uint32_t* counters,
-                          uint32_t* cbbox = nullptr);
+                          uint32_t* cbbox = nullptr, float* dcol = nullptr);
 
 // Radix sort: one pass = upsweep + digit scan + scatter.
 constexpr uint32_t kSortThreads = 256;
@@ -309,7 +309,7 @@ hipError_t launch_raster_bwd(hipStream_t s, const RasterArgs& ra, const float* f
 hipError_t launch_project_bwd(hipStream_t s, const sfk_view& v, const float* attrs,
                               const float* sh, uint32_t n, uint32_t sh_per_ch,
                               float* grad2d, float* g_attrs, float* g_sh,
-                              const BwdLoss* loss = nullptr);
+                              const BwdLoss* loss = nullptr, const float* dcol = nullptr);
 
 // Seg.
 hipError_t launch_eig(hipStream_t s, const float* a, const float* b, const uint32_t* resp,

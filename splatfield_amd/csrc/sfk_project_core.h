@@ -113,6 +113,40 @@ __device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s
 // the low byte up), for grids of at most 255 x 255 tiles; 0 for a culled
 // splat (zero tiles).  It rides through the depth sort as the payload of
 // the identity values, so the emission reads it in depth order.
+// Derivatives D[k][j] of the SH basis functions of sh_to_rgb
+// (project.rs:403-449, band 3 as written there) with respect to the unit
+// view direction component j (x, y, z), coefficients k < CHS; the rest 0.
+template <int CHS>
+__device__ __forceinline__ void sh_basis_dir_grad(float x, float y, float z, float (&D)[16][3]) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) D[k][0] = D[k][1] = D[k][2] = 0.f;
+    constexpr float c1 = 0.48860252f, c20 = 0.9461747f, c22 = -1.0925485f, c23 = 0.54627424f;
+    constexpr float k9 = -0.5900436f;
+    if constexpr (CHS >= 4) {
+        D[1][1] = -c1; D[2][2] = c1; D[3][0] = -c1;
+    }
+    if constexpr (CHS >= 9) {
+        D[4][0] = 2.f * c23 * y; D[4][1] = 2.f * c23 * x;
+        D[5][1] = c22 * z; D[5][2] = c22 * y;
+        D[6][2] = 2.f * c20 * z;
+        D[7][0] = c22 * z; D[7][2] = c22 * x;
+        D[8][0] = 2.f * c23 * x; D[8][1] = -2.f * c23 * y;
+    }
+    if constexpr (CHS >= 16) {
+        const float c1x = x * x - y * y, c1y = 2.f * x * y;
+        const float tt0 = -2.285229f * z * z + 0.4570458f, tt1 = 1.4453057f * z;
+        D[9][0] = k9 * c1y + k9 * x * 2.f * y + y * 2.f * x;
+        D[9][1] = k9 * x * 2.f * x + c1x - 2.f * y * y;
+        D[10][0] = k9 * c1x + k9 * x * 2.f * x - 2.f * y * y;
+        D[10][1] = -k9 * x * 2.f * y - c1y - 2.f * x * y;
+        D[11][1] = tt0; D[11][2] = -4.570458f * z * y;
+        D[12][0] = tt0; D[12][2] = -4.570458f * z * x;
+        D[13][2] = 5.5976451f * z * z - 1.119529f;
+        D[14][0] = tt1 * 2.f * y; D[14][1] = tt1 * 2.f * x; D[14][2] = 1.4453057f * c1y;
+        D[15][0] = tt1 * 2.f * x; D[15][1] = -tt1 * 2.f * y; D[15][2] = 1.4453057f * c1x;
+    }
+}
+
 __device__ __forceinline__ uint32_t compact_bbox(const ProjGeom& o) {
     return (o.bb_lo & 0xffu) | ((o.bb_lo >> 16) << 8) | ((o.bb_hi & 0xffu) << 16) |
            ((o.bb_hi >> 16) << 24);

@@ -133,6 +133,12 @@ struct sfk_render_scratch {
     uint32_t* bin_map = nullptr;
     uint32_t* bin_counts = nullptr;
     uint32_t* bin_offs = nullptr;  // isect offsets of truncation candidates, by splat
+    // d colour / d view direction per splat (9 planes), kept by the RGB
+    // projection once a backward has run, so the next adjoint reads 9 floats
+    // per splat instead of the SH coefficients.
+    float* dcol = nullptr;
+    bool want_dcol = false;
+    bool has_dcol = false;  // the last prepare wrote dcol
 };
 
 static bool compact_grid(const sfk_render_scratch* s) {
@@ -143,7 +149,8 @@ static void scratch_free_frame(sfk_render_scratch* s) {
     void* bufs[] = {s->depth_keys, s->projected, s->tile_counts, s->bbox, s->offsets,
                     s->counters, s->tile_starts, s->bitmap, s->rgba, s->final_t, s->n_contrib,
                     s->tile_ids, s->gaussian_ids, s->grad2d, s->emit_status, s->cbbox,
-                    s->order_buf, s->bin_sstart, s->bin_map, s->bin_counts, s->bin_offs};
+                    s->order_buf, s->bin_sstart, s->bin_map, s->bin_counts, s->bin_offs,
+                    s->dcol};
     for (void* p : bufs)
         if (p) hipFree(p);
     s->depth_keys = nullptr; s->projected = nullptr; s->tile_counts = nullptr; s->bbox = nullptr;
@@ -155,6 +162,8 @@ static void scratch_free_frame(sfk_render_scratch* s) {
     s->emit_status = nullptr;
     s->cbbox = s->order_buf = nullptr;
     s->bin_sstart = s->bin_map = s->bin_counts = s->bin_offs = nullptr;
+    s->dcol = nullptr;
+    s->has_dcol = false;
     s->has_view = false;
     sfk_radix_scratch_destroy(s->sort_depth);
     sfk_radix_scratch_destroy(s->sort_tile);
@@ -376,11 +385,13 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
                                 binned ? scan_emit_status_words(n) : 0u));
     {
         SFK_PROF(c, "project");
+        float* dcol = colors && s->want_dcol && sp->sh_per_ch > 1 ? s->dcol : nullptr;
         SFK_HIP(c, launch_project(c->stream, *view, sp->attrs, sp->sh, n,
                                   colors ? sp->sh_per_ch : 0u, s->tile_w, s->depth_keys,
                                   s->projected, s->tile_counts, s->bbox, s->counters + 2,
-                                  compact ? s->cbbox : nullptr));
+                                  compact ? s->cbbox : nullptr, dcol));
         s->has_colors = colors;
+        s->has_dcol = dcol != nullptr;
     }
 
     // Depth order.  On grids of at most 255 x 255 tiles the compact bbox
@@ -762,8 +773,12 @@ static int render_backward_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scr
     }
     SFK_PROF(c, "project_bwd");
     SFK_HIP(c, launch_project_bwd(c->stream, s->view, sp->attrs, sp->sh, sp->n, sp->sh_per_ch,
-                                  s->grad2d, grad_attrs, grad_sh, d_rgba ? nullptr : &loss));
+                                  s->grad2d, grad_attrs, grad_sh, d_rgba ? nullptr : &loss,
+                                  s->has_dcol ? s->dcol : nullptr));
     s->grad2d_dirty = false;
+    // From the next frame on, the forward keeps d colour / d direction.
+    if (!s->dcol && sp->sh_per_ch > 1) SFK_HIP(c, hipMalloc(&s->dcol, (size_t)s->n * 9 * 4));
+    s->want_dcol = s->dcol != nullptr;
     return SFK_OK;
 }
 

@@ -128,3 +128,33 @@ def test_fused_l2_backward_equals_two_calls(ctx):
         np.testing.assert_allclose(ga.read(), d["ga"], rtol=1e-4, atol=1e-7)
         np.testing.assert_allclose(gs.read(), d["gs"], rtol=1e-4, atol=1e-7)
         assert abs(float(loss.read()[0]) - d["loss"]) <= 1e-12 * d["loss"]
+
+
+def test_backward_colour_jacobian_frames(ctx):
+    """After a first backward the RGB projection keeps d colour / d view
+    direction (9 floats per splat) and the next adjoint reads it instead of
+    the SH coefficients: the second frame's gradients equal the first's to
+    float summation order, and both meet the f64 adjoint at 1e-3."""
+    d = setup(ctx)
+    n, k, w, h = d["n"], d["k"], d["w"], d["h"]
+    splats = R.Splats(ctx, d["attrs"], d["sh"])
+    scratch = R.RenderScratch(ctx, n, w, h)
+    tgt = GpuTensor.from_numpy(ctx, d["target"])
+    L = d["L"]
+    ra, rs = L.grad(d["attrs"].astype(np.float64), d["sh"].astype(np.float64))
+    ra = ra.reshape(11, n)
+    for frame in range(3):  # SH coefficients, then the kept Jacobian twice
+        isects = R.prepare_isects(splats, scratch, view=d["view"], w=w, h=h)
+        R.finalize_rgb(scratch, isects)
+        ga = GpuTensor.empty(ctx, n * 11, np.float32)
+        gs = GpuTensor.empty(ctx, n * 3 * k, np.float32)
+        loss = GpuTensor.empty(ctx, 1, np.float64)
+        R.render_backward_l2(splats, scratch, isects, scratch.device("rgba"), tgt.ptr, ga, gs,
+                             loss)
+        g = ga.read()
+        np.testing.assert_allclose(g, d["ga"], rtol=2e-4, atol=1e-6, err_msg=f"frame {frame}")
+        np.testing.assert_allclose(gs.read(), d["gs"], rtol=1e-4, atol=1e-7)
+        g = g.reshape(11, n)
+        for p in range(11):
+            assert group_rel(g[p], ra[p]) <= REL, (frame, p)
+        assert group_rel(gs.read(), rs) <= REL
