@@ -3,7 +3,8 @@
 //
 // One pass over a `width`-bit digit (<= 8) is three launches:
 //   upsweep   per-block digit histogram (LDS, per-wave sub-histograms),
-//             written digit-major per segment: hist[(s*RADIX + d)*stride + b];
+//             written digit-major per segment: hist[(s*RADIX + d)*stride + col(b)]
+//             (col groups the blocks of one XCD, see hist_col);
 //   digscan   one workgroup per (segment, digit): exclusive scan of its row
 //             over the segment's blocks, row total into totals[s*RADIX + d];
 //   scatter   per block: stable rank of every key inside the block with
@@ -84,6 +85,15 @@ __device__ __forceinline__ bool seg_pos(const SortArgs& a, uint32_t b, SegPos& p
     return p.bl < p.nb;
 }
 
+// Column of block `bl` (of nb) in a histogram row.  Workgroups are handed
+// to the 8 XCDs round-robin, so blocks bl, bl + 8, bl + 16 ... share an L2:
+// giving them adjacent columns lets their 4-byte row entries merge into whole
+// cache lines there instead of each XCD writing back (and later fetching) a
+// partial line per entry.
+__device__ __forceinline__ uint32_t hist_col(uint32_t bl, uint32_t nb) {
+    return (bl & 7u) * ((nb + 7u) >> 3) + (bl >> 3);
+}
+
 template <typename K, int RADIX, uint32_t EPT>
 __global__ __launch_bounds__(NT) void k_upsweep(SortArgs a) {
     constexpr uint32_t KPB = NT * EPT;
@@ -118,7 +128,7 @@ __global__ __launch_bounds__(NT) void k_upsweep(SortArgs a) {
         uint32_t t = 0;
 #pragma unroll
         for (uint32_t k = 0; k < NW; ++k) t += h[k][d];
-        a.hist[((size_t)p.s * RADIX + d) * a.stride + p.bl] = t;
+        a.hist[((size_t)p.s * RADIX + d) * a.stride + hist_col(p.bl, p.nb)] = t;
     }
 }
 
@@ -149,7 +159,7 @@ __global__ __launch_bounds__(NT) void k_digscan(SortArgs a) {
 #pragma unroll
         for (uint32_t e = 0; e < EPT; ++e) {
             uint32_t idx = first + e;
-            v[e] = idx < nb ? row[idx] : 0u;
+            v[e] = idx < nb ? row[hist_col(idx, nb)] : 0u;
             sum += v[e];
         }
         uint32_t total;
@@ -158,7 +168,7 @@ __global__ __launch_bounds__(NT) void k_digscan(SortArgs a) {
 #pragma unroll
         for (uint32_t e = 0; e < EPT; ++e) {
             uint32_t idx = first + e;
-            if (idx < nb) row[idx] = run;
+            if (idx < nb) row[hist_col(idx, nb)] = run;
             run += v[e];
         }
         carry += total;
@@ -207,7 +217,8 @@ __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
         uint32_t ex = block_excl_scan_u32<NT>(tot, s_tmp, &all);
         if (t < (uint32_t)RADIX)
             s_gofs[t] = t < nbins ? (uint32_t)p.seg_base + ex +
-                                        a.hist[((size_t)p.s * RADIX + t) * a.stride + p.bl]
+                                        a.hist[((size_t)p.s * RADIX + t) * a.stride +
+                                               hist_col(p.bl, p.nb)]
                                   : 0u;
     }
     // (the scan's barriers also order the s_wh clear)
@@ -321,7 +332,7 @@ uint32_t radix_max_blocks(uint32_t max_elems) {
     const uint64_t m = max_elems > 0 ? max_elems : 1;
     const uint32_t big = div_up(m, NT * 16u);
     const uint32_t small = div_up(std::min<uint64_t>(m, kSortSmallMax), NT * 8u);
-    return big > small ? big : small;
+    return ((big > small ? big : small) + 7u) & ~7u;
 }
 
 template <typename K, typename V, uint32_t EPT>
