@@ -289,9 +289,8 @@ static int radix_alloc(sfk_ctx* c, sfk_radix_scratch* s, uint32_t max_elems) {
         SFK_HIP(c, hipMalloc(&s->keys[i], m * s->key_bytes));
         SFK_HIP(c, hipMalloc(&s->vals[i], m * s->val_bytes));
     }
-    // Room for up to 256 segments (each may round its last block up, and its
-    // row length up to a multiple of 8).
-    SFK_HIP(c, hipMalloc(&s->hist, ((size_t)s->max_blocks + 8 * 256) * 256 * 4));
+    // Room for up to 256 segments (each may round its last block up).
+    SFK_HIP(c, hipMalloc(&s->hist, ((size_t)s->max_blocks + 256) * 256 * 4));
     SFK_HIP(c, hipMalloc(&s->totals, 256 * 256 * 4));
     return SFK_OK;
 }
@@ -394,9 +393,7 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const V
     a.seg_len = nseg > 1 ? n : 0u;
     a.hist = s->hist;
     a.kpb = radix_keys_per_block(total);
-    // Rows hold whole groups of 8 columns (the XCD-grouped column order).
     a.stride = nseg > 1 ? div_up(n, a.kpb) : s->max_blocks;
-    a.stride = (a.stride + 7u) & ~7u;
     a.totals = s->totals;
     a.rebase_mode = 0;
     a.rebase_base = 0;

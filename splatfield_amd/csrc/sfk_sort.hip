@@ -3,8 +3,7 @@
 //
 // One pass over a `width`-bit digit (<= 8) is three launches:
 //   upsweep   per-block digit histogram (LDS, per-wave sub-histograms),
-//             written digit-major per segment: hist[(s*RADIX + d)*stride + col(b)]
-//             (col groups the blocks of one XCD, see hist_col);
+//             written digit-major per segment: hist[(s*RADIX + d)*stride + b];
 //   digscan   one workgroup per (segment, digit): exclusive scan of its row
 //             over the segment's blocks, row total into totals[s*RADIX + d];
 //   scatter   per block: stable rank of every key inside the block with
@@ -55,7 +54,12 @@ __device__ __forceinline__ K rebased(K key, const SortArgs& a, uint32_t seg) {
 }
 
 // Which segment and block-within-segment this workgroup handles, and the
-// segment's key range; false if the block has no live keys.
+// segment's key range; false if the block has no live keys.  Blocks (chunks
+// of kpb keys, numbered segment-major) go to workgroups through xcd_remap, so
+// each XCD takes one contiguous range of them: the runs that consecutive
+// blocks scatter into a digit's output range, and their entries of a
+// histogram row, are then adjacent within one L2 and merge there into whole
+// cache lines, where round-robin blocks would each write back partial lines.
 struct SegPos {
     uint32_t s, bl, nb, seg_n;
     size_t seg_base;
@@ -68,30 +72,24 @@ __device__ __forceinline__ bool seg_pos(const SortArgs& a, uint32_t b, SegPos& p
             uint32_t m = *a.d_n;
             n = m < n ? m : n;
         }
-        p.s = 0;
-        p.bl = b;
         p.nb = (n + a.kpb - 1) / a.kpb;
+        if (b >= p.nb) return false;
+        p.s = 0;
+        p.bl = xcd_remap(b, p.nb);
         p.seg_n = n;
         p.seg_base = 0;
     } else {
-        uint32_t bps = (a.seg_len + a.kpb - 1) / a.kpb;
-        p.s = b / bps;
-        p.bl = b % bps;
+        const uint32_t bps = (a.seg_len + a.kpb - 1) / a.kpb;
+        const uint32_t all = bps * a.nseg;
+        if (b >= all) return false;
+        const uint32_t c = xcd_remap(b, all);
+        p.s = c / bps;
+        p.bl = c % bps;
         p.nb = bps;
         p.seg_n = a.seg_len;
         p.seg_base = (size_t)p.s * a.seg_len;
-        if (p.s >= a.nseg) return false;
     }
-    return p.bl < p.nb;
-}
-
-// Column of block `bl` (of nb) in a histogram row.  Workgroups are handed
-// to the 8 XCDs round-robin, so blocks bl, bl + 8, bl + 16 ... share an L2:
-// giving them adjacent columns lets their 4-byte row entries merge into whole
-// cache lines there instead of each XCD writing back (and later fetching) a
-// partial line per entry.
-__device__ __forceinline__ uint32_t hist_col(uint32_t bl, uint32_t nb) {
-    return (bl & 7u) * ((nb + 7u) >> 3) + (bl >> 3);
+    return true;
 }
 
 template <typename K, int RADIX, uint32_t EPT>
@@ -128,7 +126,7 @@ __global__ __launch_bounds__(NT) void k_upsweep(SortArgs a) {
         uint32_t t = 0;
 #pragma unroll
         for (uint32_t k = 0; k < NW; ++k) t += h[k][d];
-        a.hist[((size_t)p.s * RADIX + d) * a.stride + hist_col(p.bl, p.nb)] = t;
+        a.hist[((size_t)p.s * RADIX + d) * a.stride + p.bl] = t;
     }
 }
 
@@ -159,7 +157,7 @@ __global__ __launch_bounds__(NT) void k_digscan(SortArgs a) {
 #pragma unroll
         for (uint32_t e = 0; e < EPT; ++e) {
             uint32_t idx = first + e;
-            v[e] = idx < nb ? row[hist_col(idx, nb)] : 0u;
+            v[e] = idx < nb ? row[idx] : 0u;
             sum += v[e];
         }
         uint32_t total;
@@ -168,7 +166,7 @@ __global__ __launch_bounds__(NT) void k_digscan(SortArgs a) {
 #pragma unroll
         for (uint32_t e = 0; e < EPT; ++e) {
             uint32_t idx = first + e;
-            if (idx < nb) row[hist_col(idx, nb)] = run;
+            if (idx < nb) row[idx] = run;
             run += v[e];
         }
         carry += total;
@@ -217,8 +215,7 @@ __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
         uint32_t ex = block_excl_scan_u32<NT>(tot, s_tmp, &all);
         if (t < (uint32_t)RADIX)
             s_gofs[t] = t < nbins ? (uint32_t)p.seg_base + ex +
-                                        a.hist[((size_t)p.s * RADIX + t) * a.stride +
-                                               hist_col(p.bl, p.nb)]
+                                        a.hist[((size_t)p.s * RADIX + t) * a.stride + p.bl]
                                   : 0u;
     }
     // (the scan's barriers also order the s_wh clear)
@@ -332,7 +329,7 @@ uint32_t radix_max_blocks(uint32_t max_elems) {
     const uint64_t m = max_elems > 0 ? max_elems : 1;
     const uint32_t big = div_up(m, NT * 16u);
     const uint32_t small = div_up(std::min<uint64_t>(m, kSortSmallMax), NT * 8u);
-    return ((big > small ? big : small) + 7u) & ~7u;
+    return big > small ? big : small;
 }
 
 template <typename K, typename V, uint32_t EPT>
