@@ -160,6 +160,8 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) wave_last = max(wave_last, (uint32_t)__shfl_xor(wave_last, o, 64));
     float acc_r = 0.f, acc_g = 0.f, acc_b = 0.f;
+    // d loss / d alpha through the final transmittance: Tfin dC.w / (1 - alpha).
+    const float tfin_da = Tfin * dC.w;
     const float4* proj = reinterpret_cast<const float4*>(a.proj);
 
     for (uint32_t cend = block_last; cend > start;) {
@@ -222,11 +224,15 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
                 g[6][s] = wgt * dC.x;
                 g[7][s] = wgt * dC.y;
                 g[8][s] = wgt * dC.z;
-                const float dal = Tn * (wr * dC.x + wg * dC.y + wb * dC.z) + Tfin * rom * dC.w;
+                // Gradient-only arithmetic (alpha, T and the contributor
+                // tests above keep the forward's exact operation sequence)
+                // uses fused multiply-adds.
+                const float dal =
+                    fmaf(tfin_da, rom, Tn * fmaf(wb, dC.z, fmaf(wg, dC.y, wr * dC.x)));
                 // Back-to-front colour accumulators: acc' = acc + alpha (c - acc).
-                acc_r = con ? acc_r + alpha * wr : acc_r;
-                acc_g = con ? acc_g + alpha * wg : acc_g;
-                acc_b = con ? acc_b + alpha * wb : acc_b;
+                acc_r = con ? fmaf(alpha, wr, acc_r) : acc_r;
+                acc_g = con ? fmaf(alpha, wg, acc_g) : acc_g;
+                acc_b = con ? fmaf(alpha, wb, acc_b) : acc_b;
                 // d alpha / d(opacity, power) are zero where the 0.999 clamp
                 // binds; the power terms are reduced as moments of dpow in
                 // (dx, dy) and turned into mean / conic gradients per member
