@@ -255,21 +255,28 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
                 const uint32_t j = writer ? list[itop - 1 - ms] : 0u;
                 float r[9];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) r[k] = MemberSum<M>::sum(g[k]);
+                for (int k = 0; k < 9; ++k) {
+                    r[k] = MemberSum<M>::sum(g[k]);
+                    // Complete the sum here: sunk into the writers' branch its
+                    // last DPP step would no longer fold into the add.
+                    asm volatile("" : "+v"(r[k]));
+                }
+                // Every lane forms the member gradients (same cost per wave as
+                // the writers alone); writers add them without testing for
+                // zeros, which would cost a compare and branch per plane.
+                const float4 geo = s_geo[j];
+                const float qc = s_aux[j].x;
+                float out[9];
+                out[0] = fmaf(geo.z, r[0], geo.w * r[1]);  // d mx
+                out[1] = fmaf(qc, r[1], geo.w * r[0]);     // d my
+                out[2] = 0.5f * r[2];                      // d conic a
+                out[3] = r[3];                             // d conic b
+                out[4] = 0.5f * r[4];                      // d conic c
+#pragma unroll
+                for (int k = 5; k < 9; ++k) out[k] = r[k];
                 if (writer) {
-                    const float4 geo = s_geo[j];
-                    const float qc = s_aux[j].x;
-                    float out[9];
-                    out[0] = geo.z * r[0] + geo.w * r[1];  // d mx
-                    out[1] = qc * r[1] + geo.w * r[0];     // d my
-                    out[2] = 0.5f * r[2];                  // d conic a
-                    out[3] = r[3];                         // d conic b
-                    out[4] = 0.5f * r[4];                  // d conic c
 #pragma unroll
-                    for (int k = 5; k < 9; ++k) out[k] = r[k];
-#pragma unroll
-                    for (int k = 0; k < 9; ++k)
-                        if (out[k] != 0.0f) atomicAdd(&s_acc[k][j], out[k]);
+                    for (int k = 0; k < 9; ++k) atomicAdd(&s_acc[k][j], out[k]);
                 }
             }
         }
