@@ -208,7 +208,13 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
                 }
                 // Straight-line evaluation for the whole wave; lanes that do
                 // not contribute keep their state through selects.
-                const float G = sfk_expf_neg(power);
+                // exp(-power) by the hardware exp2: within an ulp or two of the
+                // forward's exp, so alpha and T agree with the forward to
+                // rounding (a contributor test can only differ where alpha is
+                // within an ulp of 1/255).  The argument is capped so that a
+                // NaN or negative power gives a finite G (alpha then clamps
+                // to 0.999 as in the forward and dm = 0 keeps dm * G at 0).
+                const float G = __builtin_amdgcn_exp2f(fminf(-power * 1.44269504f, 126.0f));
                 const float araw = aux.y * G;
                 const float alpha = fminf(araw, 0.999f);
                 const bool con = in && alpha >= kInv255;
