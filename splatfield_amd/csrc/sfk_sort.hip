@@ -39,18 +39,33 @@ __device__ __forceinline__ uint32_t digit_of(K key, uint32_t shift, uint32_t mas
     return (uint32_t)(key >> shift) & mask;
 }
 
-// The key as pass 0 sees it when rebasing (SortArgs::rebase_mode).
-template <typename K>
-__device__ __forceinline__ K rebased(K key, const SortArgs& a, uint32_t seg) {
-    if constexpr (sizeof(K) == 4) {
-        if (a.rebase_mode) {
-            const uint32_t b = a.rebase_mode == 2 ? a.rebase_bases[seg] : a.rebase_base;
-            if (key == 0xffffffffu) return key;
-            const uint32_t d = key >= b ? key - b : 0u;
-            return d < 0xfffffeu ? d : 0xfffffeu;
-        }
+// The key as pass 0 sees it when rebasing (SortArgs::rebase_mode): fixed
+// per segment, applied with selects only, after the block's loads are all
+// issued (a branch per key on a loaded value makes the compiler wait for each
+// load before issuing the next).
+struct KeyBase {
+    uint32_t on, base;
+};
+
+__device__ __forceinline__ KeyBase key_base(const SortArgs& a, uint32_t seg) {
+    KeyBase r{0u, 0u};
+    if (a.rebase_mode) {
+        r.on = 1u;
+        r.base = a.rebase_mode == 2 ? a.rebase_bases[seg] : a.rebase_base;
     }
-    return key;
+    return r;
+}
+
+template <typename K>
+__device__ __forceinline__ K rebased(K key, KeyBase kb) {
+    if constexpr (sizeof(K) == 4) {
+        const uint32_t d = key >= kb.base ? key - kb.base : 0u;
+        const uint32_t c = d < 0xfffffeu ? d : 0xfffffeu;
+        const uint32_t r = key == 0xffffffffu ? key : c;
+        return kb.on ? r : key;
+    } else {
+        return key;
+    }
 }
 
 // Which segment and block-within-segment this workgroup handles, and the
@@ -106,18 +121,21 @@ __global__ __launch_bounds__(NT) void k_upsweep(SortArgs a) {
     const uint32_t loc0 = p.bl * KPB + threadIdx.x;
     // All loads first, then the LDS counts, so the loads overlap; a block
     // fully inside its segment loads without per-element guards.
-    uint32_t dg[EPT];
+    K kr[EPT];
     const K* kp = keys + p.seg_base + loc0;
-    if (p.bl * KPB + KPB <= p.seg_n) {
+    const bool full = p.bl * KPB + KPB <= p.seg_n;
+    if (full) {
 #pragma unroll
-        for (uint32_t e = 0; e < EPT; ++e) dg[e] = digit_of(rebased(kp[e * NT], a, p.s), a.shift, mask);
+        for (uint32_t e = 0; e < EPT; ++e) kr[e] = kp[e * NT];
     } else {
 #pragma unroll
-        for (uint32_t e = 0; e < EPT; ++e) {
-            const uint32_t loc = loc0 + e * NT;
-            dg[e] = loc < p.seg_n ? digit_of(rebased(kp[e * NT], a, p.s), a.shift, mask) : ~0u;
-        }
+        for (uint32_t e = 0; e < EPT; ++e) kr[e] = loc0 + e * NT < p.seg_n ? kp[e * NT] : K(0);
     }
+    const KeyBase kb = key_base(a, p.s);
+    uint32_t dg[EPT];
+#pragma unroll
+    for (uint32_t e = 0; e < EPT; ++e)
+        dg[e] = full || loc0 + e * NT < p.seg_n ? digit_of(rebased(kr[e], kb), a.shift, mask) : ~0u;
 #pragma unroll
     for (uint32_t e = 0; e < EPT; ++e)
         if (dg[e] != ~0u) atomicAdd(&h[w][dg[e]], 1u);
@@ -208,7 +226,37 @@ __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
 
     for (uint32_t i = t; i < NW * RADIX; i += NT) (&s_wh[0][0])[i] = 0;
 
-    // Segment-global digit base: exclusive scan of the segment's digit totals.
+    K key[EPT];
+    V val[EPT];
+    const V* vals_in = static_cast<const V*>(a.vals_in);
+    V* vals_out = static_cast<V*>(a.vals_out);
+    uint32_t loc[EPT];
+    const uint32_t wbase = blk0 + w * 64 * EPT + lane;
+    const size_t gb = p.seg_base;
+    // Every load of the block is issued before any is used (see KeyBase).
+    if (blk0 + KPB <= p.seg_n) {
+#pragma unroll
+        for (uint32_t e = 0; e < EPT; ++e) key[e] = keys_in[gb + wbase + e * 64];
+        if (vals_in) {
+#pragma unroll
+            for (uint32_t e = 0; e < EPT; ++e) val[e] = vals_in[gb + wbase + e * 64];
+        } else {
+#pragma unroll
+            for (uint32_t e = 0; e < EPT; ++e)
+                val[e] = identity_val<V>(wbase + e * 64, a.payload, gb + wbase + e * 64);
+        }
+    } else {
+#pragma unroll
+        for (uint32_t e = 0; e < EPT; ++e) {
+            const uint32_t li = wbase + e * 64;
+            const bool valid = li < p.seg_n;
+            key[e] = valid ? keys_in[gb + li] : K(0);
+            val[e] = valid ? (vals_in ? vals_in[gb + li] : identity_val<V>(li, a.payload, gb + li))
+                           : V{};
+        }
+    }
+    // Segment-global digit base (exclusive scan of the segment's digit
+    // totals) while the block's keys and values are in flight.
     {
         uint32_t tot = t < nbins ? a.totals[(size_t)p.s * RADIX + t] : 0u;
         uint32_t all;
@@ -219,25 +267,10 @@ __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
                                   : 0u;
     }
     // (the scan's barriers also order the s_wh clear)
-
-    K key[EPT];
-    V val[EPT];
-    const V* vals_in = static_cast<const V*>(a.vals_in);
-    V* vals_out = static_cast<V*>(a.vals_out);
-    uint32_t loc[EPT];
-    const uint32_t wbase = blk0 + w * 64 * EPT + lane;
+    {
+        const KeyBase kb = key_base(a, p.s);
 #pragma unroll
-    for (uint32_t e = 0; e < EPT; ++e) {
-        uint32_t li = wbase + e * 64;
-        bool valid = li < p.seg_n;
-        key[e] = valid ? rebased(keys_in[p.seg_base + li], a, p.s) : K(0);
-        if constexpr (sizeof(V) == 4) {
-            val[e] = valid ? (vals_in ? vals_in[p.seg_base + li] : li) : 0u;
-        } else {
-            val[e] = valid ? (vals_in ? vals_in[p.seg_base + li]
-                                      : identity_val<V>(li, a.payload, p.seg_base + li))
-                           : V{};
-        }
+        for (uint32_t e = 0; e < EPT; ++e) key[e] = rebased(key[e], kb);
     }
 
 #pragma unroll
