@@ -330,10 +330,32 @@ __global__ __launch_bounds__(256) void k_project_bwd(sfk_view v, const float* __
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     const size_t N = n;
-    float mx = A[0 * N + i], my = A[1 * N + i], mz = A[2 * N + i];
-    float qw = A[3 * N + i], qx = A[4 * N + i], qy = A[5 * N + i], qz = A[6 * N + i];
-    float s[3] = {sfk_expf(A[7 * N + i]), sfk_expf(A[8 * N + i]), sfk_expf(A[9 * N + i])};
-    float op = 1.0f / (1.0f + sfk_expf(-A[10 * N + i]));
+    // Every per-splat input is loaded before any is used, and the loaded
+    // values are pinned ahead of the visibility branch: left to itself the
+    // compiler sinks the loads that only the visible path needs below that
+    // branch, where each group waits out its own memory round trip.
+    float ar[11], G2[9], J[9];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) ar[k] = A[(size_t)k * N + i];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) G2[k] = g2[(size_t)k * N + i];
+    if (dcol) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) J[k] = dcol[(size_t)k * N + i];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) J[k] = 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < 11; ++k) asm volatile("" : "+v"(ar[k]));
+#pragma unroll
+    for (int k = 0; k < 9; ++k) asm volatile("" : "+v"(G2[k]));
+#pragma unroll
+    for (int k = 0; k < 9; ++k) asm volatile("" : "+v"(J[k]));
+    float mx = ar[0], my = ar[1], mz = ar[2];
+    float qw = ar[3], qx = ar[4], qy = ar[5], qz = ar[6];
+    float s[3] = {sfk_expf(ar[7]), sfk_expf(ar[8]), sfk_expf(ar[9])};
+    float op = 1.0f / (1.0f + sfk_expf(-ar[10]));
     float cam[3];
     cam[0] = v.rot0[0] * mx + v.rot0[1] * my + v.rot0[2] * mz + v.trans[0];
     cam[1] = v.rot1[0] * mx + v.rot1[1] * my + v.rot1[2] * mz + v.trans[1];
@@ -344,11 +366,8 @@ __global__ __launch_bounds__(256) void k_project_bwd(sfk_view v, const float* __
         for (int k = 0; k < 3 * CHS; ++k) gS[(size_t)k * N + i] = 0.0f;
         return;
     }
-    // Read this splat's screen-space gradients and leave zeros behind: the
-    // next backward accumulates into a cleared buffer without a fill pass.
-    float G2[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) G2[k] = g2[(size_t)k * N + i];
+    // Leave zeros behind in the screen-space gradients just read: the next
+    // backward accumulates into a cleared buffer without a fill pass.
 #pragma unroll
     for (int k = 0; k < 9; ++k) g2[(size_t)k * N + i] = 0.0f;
 
@@ -428,18 +447,21 @@ __global__ __launch_bounds__(256) void k_project_bwd(sfk_view v, const float* __
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float gc = G2[6 + c];
-            gd0 += gc * dcol[(size_t)(3 * c + 0) * N + i];
-            gd1 += gc * dcol[(size_t)(3 * c + 1) * N + i];
-            gd2 += gc * dcol[(size_t)(3 * c + 2) * N + i];
+            gd0 += gc * J[3 * c + 0];
+            gd1 += gc * J[3 * c + 1];
+            gd2 += gc * J[3 * c + 2];
         }
     } else {
+        float shc[3 * CHS];
+#pragma unroll
+        for (int k = 0; k < 3 * CHS; ++k) shc[k] = shs[(size_t)k * N + i];
         float D[16][3];
         sh_basis_dir_grad<CHS>(x, y, z, D);
 #pragma unroll
         for (int k = 0; k < CHS; ++k)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                float sv = shs[(size_t)(3 * k + c) * N + i] * G2[6 + c];
+                float sv = shc[3 * k + c] * G2[6 + c];
                 gd0 += sv * D[k][0];
                 gd1 += sv * D[k][1];
                 gd2 += sv * D[k][2];

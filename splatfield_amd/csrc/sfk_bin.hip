@@ -137,12 +137,19 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
     }
     // View of the first rank (batched); later ranks step over boundaries.
     uint32_t v = a.seg_len ? (r0 < a.n ? r0 : a.n - 1) / a.seg_len : 0u;
+    // The view's record is loaded once (pinned, so that it is not re-issued
+    // inside the per-rank branches below) and again only past a boundary.
+    BinView vw = view_of(a.views, a.view0, v);
+    asm volatile("" : "+v"(vw.key_base), "+v"(vw.supers_x), "+v"(vw.gid_add));
     uint32_t cnt[kRpt], sc[kRpt], gadd[kRpt];
     uint32_t ci = 0, cp = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kRpt; ++k) {
         const uint32_t r = r0 + k;
-        if (a.seg_len && r < a.n && r >= (v + 1) * a.seg_len) ++v;
+        if (a.seg_len && r < a.n && r >= (v + 1) * a.seg_len) {
+            ++v;
+            vw = view_of(a.views, a.view0, v);
+        }
         const uint32_t cb = oc[k].y;
         const uint32_t x0 = cb & 0xffu, y0 = (cb >> 8) & 0xffu, x1 = (cb >> 16) & 0xffu,
                        y1 = cb >> 24;
@@ -155,7 +162,6 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
             const uint32_t swx = (x1 - 1) / kBinSuper - sx0 + 1;
             const uint32_t swy = (y1 - 1) / kBinSuper - sy0 + 1;
             sc[k] = swx * swy;
-            const BinView vw = view_of(a.views, a.view0, v);
             key0 = vw.key_base + sy0 * vw.supers_x + sx0;
             span = swx | (vw.supers_x << 6);
             gadd[k] = vw.gid_add;

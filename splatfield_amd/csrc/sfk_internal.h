@@ -102,7 +102,7 @@ hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, 
                           uint32_t* cbbox = nullptr, float* dcol = nullptr);
 
 // Radix sort: one pass = upsweep + digit scan + scatter.
-constexpr uint32_t kSortThreads = 256;
+constexpr uint32_t kSortThreads = 512;
 constexpr uint64_t kSortSmallMax = 1u << 22;  // up to this many keys: 2048-key blocks
 uint32_t radix_keys_per_block(uint64_t total_keys);   // 2048 or 4096
 uint32_t radix_max_blocks(uint32_t max_elems);

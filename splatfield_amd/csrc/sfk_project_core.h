@@ -30,16 +30,31 @@ __device__ __forceinline__ float dot3f(float ax, float ay, float az, float bx, f
 
 // Loads splat i of the field-major planes, applying exp/sigmoid as the
 // reference does before its culls (project.rs:313-333).
+// The 11 loads are issued together and their values pinned here: the
+// scales are only used past the culls, and left to itself the compiler sinks
+// their loads below the cull branches, one memory round trip each.
 __device__ __forceinline__ SplatIn load_splat(const float* __restrict__ A, uint32_t i, uint32_t n) {
     const size_t N = n;
+    float r[11];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) r[k] = A[(size_t)k * N + i];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) asm volatile("" : "+v"(r[k]));
     SplatIn s;
-    s.mx = A[0 * N + i]; s.my = A[1 * N + i]; s.mz = A[2 * N + i];
-    s.qw = A[3 * N + i]; s.qx = A[4 * N + i]; s.qy = A[5 * N + i]; s.qz = A[6 * N + i];
-    s.sx = sfk_expf(A[7 * N + i]);
-    s.sy = sfk_expf(A[8 * N + i]);
-    s.sz = sfk_expf(A[9 * N + i]);
-    s.opacity = 1.0f / (1.0f + sfk_expf(-A[10 * N + i]));
+    s.mx = r[0]; s.my = r[1]; s.mz = r[2];
+    s.qw = r[3]; s.qx = r[4]; s.qy = r[5]; s.qz = r[6];
+    s.sx = sfk_expf(r[7]);
+    s.sy = sfk_expf(r[8]);
+    s.sz = sfk_expf(r[9]);
+    s.opacity = 1.0f / (1.0f + sfk_expf(-r[10]));
     return s;
+}
+
+// The two culls of project_geom alone (opacity, then camera depth).
+__device__ __forceinline__ bool passes_culls(const sfk_view& v, const SplatIn& s) {
+    if (s.opacity < kAlphaCutoff) return false;
+    const float cz = dot3f(v.rot2[0], v.rot2[1], v.rot2[2], s.mx, s.my, s.mz) + v.trans[2];
+    return !(cz <= 0.1f);
 }
 
 // Culls, EWA covariance, conic, mean2d, extent and tile bbox for one view.

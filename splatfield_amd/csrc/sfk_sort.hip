@@ -11,8 +11,12 @@
 //             32-lane assumption as in sort.rs:57-87), LDS staging in sorted
 //             order, then coalesced runs to segment base + digit prefix +
 //             block prefix.
-// Block = 256 threads x 16 keys; wave w owns keys [w*1024, (w+1)*1024) of its
-// block, item i lane l -> w*1024 + i*64 + l, so ranks follow input order.
+// Block = 4096 keys (2048 for small sorts) over kSortThreads threads; wave w
+// owns keys [w*64*EPT, (w+1)*64*EPT) of its block, item i lane l ->
+// w*64*EPT + i*64 + l, so ranks follow input order.  512 threads of 8 keys
+// beat 256 of 16 (measured): each wave's ranking walks a chain of dependent
+// LDS counter updates, one per key, and shorter chains over more waves
+// overlap better.
 //
 // Segments: nseg independent sorts of seg_len keys each, laid out back to
 // back (segment s at [s*seg_len, (s+1)*seg_len)), sorted in the same launches
@@ -33,6 +37,7 @@ namespace {
 
 constexpr uint32_t NT = kSortThreads;
 constexpr uint32_t NW = NT / 64;
+constexpr uint32_t kEptBig = 4096 / NT, kEptSmall = 2048 / NT;
 
 template <typename K>
 __device__ __forceinline__ uint32_t digit_of(K key, uint32_t shift, uint32_t mask) {
@@ -354,14 +359,14 @@ hipError_t pass_impl(hipStream_t st, const SortArgs& a) {
 }  // namespace
 
 uint32_t radix_keys_per_block(uint64_t total) {
-    return total <= kSortSmallMax ? NT * 8u : NT * 16u;
+    return total <= kSortSmallMax ? NT * kEptSmall : NT * kEptBig;
 }
 
 // Histogram rows for any sort a scratch of max_elems keys can run.
 uint32_t radix_max_blocks(uint32_t max_elems) {
     const uint64_t m = max_elems > 0 ? max_elems : 1;
-    const uint32_t big = div_up(m, NT * 16u);
-    const uint32_t small = div_up(std::min<uint64_t>(m, kSortSmallMax), NT * 8u);
+    const uint32_t big = div_up(m, NT * kEptBig);
+    const uint32_t small = div_up(std::min<uint64_t>(m, kSortSmallMax), NT * kEptSmall);
     return big > small ? big : small;
 }
 
@@ -379,8 +384,8 @@ hipError_t pass_ept(hipStream_t st, const SortArgs& a, uint32_t digit_bits) {
 // per CU (measured: 1024-key blocks are slower again).
 template <typename K, typename V>
 hipError_t pass_radix(hipStream_t st, const SortArgs& a, uint32_t digit_bits) {
-    if (a.kpb == NT * 8u) return pass_ept<K, V, 8>(st, a, digit_bits);
-    return pass_ept<K, V, 16>(st, a, digit_bits);
+    if (a.kpb == NT * kEptSmall) return pass_ept<K, V, kEptSmall>(st, a, digit_bits);
+    return pass_ept<K, V, kEptBig>(st, a, digit_bits);
 }
 
 hipError_t launch_radix_pass(hipStream_t st, const SortArgs& a, uint32_t key_bytes,
