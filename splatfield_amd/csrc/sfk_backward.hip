@@ -172,9 +172,11 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
             uint32_t g = a.gids[cbeg + t];
             const float4* row = proj + (size_t)g * 3;
             float4 geo = row[0], aux = row[1];
-            s_geo[t] = geo;
-            s_aux[t] = aux;
             s_box[t] = reach_box(geo, aux);
+            // Conic a and c are staged halved: the power below is then
+            // dx (a/2 dx + b dy) + c/2 dy^2, five operations.
+            s_geo[t] = make_float4(geo.x, geo.y, 0.5f * geo.z, geo.w);
+            s_aux[t] = make_float4(0.5f * aux.x, aux.y, aux.z, aux.w);
             s_blue[t] = row[2].x;
             s_gid[t] = g;
 #pragma unroll
@@ -198,8 +200,10 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
                 const float4 geo = s_geo[j];
                 const float4 aux = s_aux[j];
                 const float dx = geo.x - pxf, dy = geo.y - pyf;
-                const float power =
-                    0.5f * (geo.z * dx * dx + aux.x * dy * dy) + geo.w * dx * dy;
+                // The forward's power up to rounding (fused here); a rounding
+                // difference can only move the kPowerSkip test where alpha
+                // is below 1/255 anyway.
+                const float power = fmaf(dx, fmaf(geo.z, dx, geo.w * dy), aux.x * dy * dy);
                 const bool in = member && cbeg + j < last && !(power > kPowerSkip);
                 if (!__ballot(in)) {
 #pragma unroll
@@ -271,10 +275,10 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
                 // the writers alone); writers add them without testing for
                 // zeros, which would cost a compare and branch per plane.
                 const float4 geo = s_geo[j];
-                const float qc = s_aux[j].x;
+                const float qc = 2.0f * s_aux[j].x;  // staged halved
                 float out[9];
-                out[0] = fmaf(geo.z, r[0], geo.w * r[1]);  // d mx
-                out[1] = fmaf(qc, r[1], geo.w * r[0]);     // d my
+                out[0] = fmaf(2.0f * geo.z, r[0], geo.w * r[1]);  // d mx
+                out[1] = fmaf(qc, r[1], geo.w * r[0]);            // d my
                 out[2] = 0.5f * r[2];                      // d conic a
                 out[3] = r[3];                             // d conic b
                 out[4] = 0.5f * r[4];                      // d conic c
