@@ -104,6 +104,7 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
     __shared__ float4 s_geo[NT];
     __shared__ float4 s_aux[NT];
     __shared__ float4 s_box[NT];
+    __shared__ float s_tau[NT];
     __shared__ uint16_t s_list[NW][NT];
     __shared__ float s_blue[NT];
     __shared__ uint32_t s_gid[NT];
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
             uint32_t g = a.gids[cbeg + t];
             const float4* row = proj + (size_t)g * 3;
             float4 geo = row[0], aux = row[1];
-            s_box[t] = reach_box(geo, aux);
+            s_box[t] = reach_box(geo, aux, &s_tau[t]);
             // Conic a and c are staged halved: the power below is then
             // dx (a/2 dx + b dy) + c/2 dy^2, five operations.
             s_geo[t] = make_float4(geo.x, geo.y, 0.5f * geo.z, geo.w);
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
         }
         __syncthreads();
         const uint32_t lim = wave_last > cbeg ? min(cnt, wave_last - cbeg) : 0u;
-        const uint32_t nl = wave_reach_list(s_box, lim, bx0, by0, list);
+        const uint32_t nl = wave_reach_list<2>(s_box, s_tau, s_geo, s_aux, lim, bx0, by0, list);
         // Listed splats in groups of 8, back to front: member s of the group
         // whose top is `itop` is list entry itop - 1 - s.  The 9 x 8 per-pixel
         // terms are reduced over the wave in one transposed pass; lane l ends

@@ -121,6 +121,7 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
     __shared__ float4 s_geo[NT];  // mx, my, conic a, conic b
     __shared__ float4 s_aux[NT];  // conic c, opacity, r, g
     __shared__ float4 s_box[NT];  // reach box
+    __shared__ float s_tau[NT];   // power bound of the ellipse test
     __shared__ uint16_t s_list[NW][NT];
     __shared__ float s_blue[MODE == kRgb ? NT : 1];
     __shared__ uint32_t s_gid[MODE == kRgb ? 1 : NT];
@@ -155,7 +156,7 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
             float4 geo = row[0], aux = row[1];
             s_geo[t] = geo;
             s_aux[t] = aux;
-            s_box[t] = reach_box(geo, aux);
+            s_box[t] = reach_box(geo, aux, &s_tau[t]);
             if constexpr (MODE == kRgb) {
                 s_blue[t] = row[2].x;
             } else {
@@ -167,7 +168,7 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
         // Staging fence and whole-tile early exit (raster.rs:166-171).
         if (__syncthreads_count(!done) == 0) break;
         const uint32_t cnt = (end - chunk) < (uint32_t)NT ? (end - chunk) : (uint32_t)NT;
-        const uint32_t nl = __ballot(!done) ? wave_reach_list(s_box, cnt, bx0, by0, list) : 0u;
+        const uint32_t nl = __ballot(!done) ? wave_reach_list(s_box, s_tau, s_geo, s_aux, cnt, bx0, by0, list) : 0u;
         // Listed splats in groups of 8 (entries past the list end repeat the
         // last entry and are masked off).  Each member is evaluated straight
         // through for the whole wave once any lane may be reached; lanes that
