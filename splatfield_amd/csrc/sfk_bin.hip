@@ -416,7 +416,14 @@ __device__ __forceinline__ uint64_t cover_mask(uint32_t cb, uint32_t ox, uint32_
 // consecutive staged words: a tile's isects of a group form one contiguous
 // run of its range, so each store covers a few runs instead of 64 tiles.
 constexpr uint32_t kStage = 4096;
-static_assert(kRounds == 8, "round totals use the 8-way transposed wave sum");
+static_assert(kRounds == 8 || kRounds == 4, "round totals use a transposed wave sum");
+// Lane l of round_totals' result holds the wave total of round l / (64 / kRounds).
+template <uint32_t N>
+__device__ __forceinline__ uint32_t round_totals(const uint32_t (&cnt)[N]) {
+    if constexpr (N == 8) return wave_sum8_transposed(cnt);
+    else return wave_sum4_transposed(cnt);
+}
+constexpr uint32_t kRoundLane = 64 / kRounds;
 
 template <bool PLACE>
 __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
@@ -480,12 +487,12 @@ __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
             const uint32_t tile = ty * vw.tiles_x + tx;
             // Round totals, then groups of consecutive rounds of at most
             // kStage isects (all wave-uniform).
-            const uint32_t tk = wave_sum8_transposed(cnt);
+            const uint32_t tk = round_totals(cnt);
             uint32_t grp[kRounds];
             uint32_t ng = 0, acc = 0;
 #pragma unroll
             for (uint32_t k = 0; k < kRounds; ++k) {
-                const uint32_t t = __builtin_amdgcn_readlane(tk, 8 * k);
+                const uint32_t t = __builtin_amdgcn_readlane(tk, kRoundLane * k);
                 if (k < nr) {
                     if (k > 0 && acc + t > kStage) {
                         ++ng;
@@ -504,7 +511,7 @@ __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
                 uint32_t gsum = 0;
 #pragma unroll
                 for (uint32_t k = 0; k < kRounds; ++k)
-                    if (k < nr && grp[k] == q) gsum += __builtin_amdgcn_readlane(tk, 8 * k);
+                    if (k < nr && grp[k] == q) gsum += __builtin_amdgcn_readlane(tk, kRoundLane * k);
                 // Stage: tile t's isects of the group at lo.., in rank order.
                 uint32_t pos = lo;
 #pragma unroll
@@ -531,7 +538,7 @@ __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
                     const uint32_t slot = e + (uint32_t)__shfl((int)delta, j, 64);
                     const uint32_t tid = a.tile_ids ? (uint32_t)__shfl((int)tile, j, 64) : 0u;
                     if (e < gsum && slot < a.cap) {  // holds by construction; a guard
-                        a.gids[slot] = s_gid[w][v & 511u];
+                        a.gids[slot] = s_gid[w][v & 511u];  // chunk index (9 bits)
                         if (a.tile_ids) a.tile_ids[slot] = tid;
                     }
                 }
