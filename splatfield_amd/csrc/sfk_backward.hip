@@ -95,6 +95,16 @@ struct MemberSum<4> {
     static constexpr uint32_t kShift = 4;  // lane l holds member l >> 4
 };
 
+// Workgroup barrier ordering LDS only: unlike __syncthreads it does not wait
+// for this wave's outstanding global loads (a prefetch stays in flight).  For
+// code where no thread reads global memory another thread of the workgroup
+// wrote before the barrier.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // M = members per group: the gradients of M splats are summed over the wave
 // in one transposed pass; M = 4 halves the registers held across a group.
 template <int TW, int M>
@@ -165,20 +175,37 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
     const float tfin_da = Tfin * dC.w;
     const float4* proj = reinterpret_cast<const float4*>(a.proj);
 
+    // The splat ids of the next chunk are loaded while the current one is
+    // walked (one memory round trip less at each staging).
+    auto chunk_begin = [&](uint32_t ce) { return ce - start > (uint32_t)NT ? ce - NT : start; };
+    // Loads at clamped indices, with no branch around them: the wait
+    // counters then see the same loads on every path and need not drain.
+    uint32_t g_next = 0;
+    if (block_last > start) {
+        const uint32_t cb0 = chunk_begin(block_last);
+        g_next = a.gids[cb0 + min(t, block_last - cb0 - 1u)];
+    }
     for (uint32_t cend = block_last; cend > start;) {
-        const uint32_t cbeg = cend - start > (uint32_t)NT ? cend - NT : start;
+        const uint32_t cbeg = chunk_begin(cend);
         const uint32_t cnt = cend - cbeg;
-        __syncthreads();
+        const uint32_t g = g_next;
+        lds_barrier();
+        const float4* row = proj + (size_t)g * 3;  // lanes past cnt: a copy of a live id
+        const float4 geo = row[0], aux = row[1];
+        const float blue = row[2].x;
+        // Issued after the rows (the scheduling barrier keeps it there): loads
+        // complete in issue order, so this one stays in flight past the wait
+        // for the rows.
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t nb = chunk_begin(cbeg);
+        g_next = a.gids[cbeg > start ? nb + min(t, cbeg - nb - 1u) : cbeg];
         if (t < cnt) {
-            uint32_t g = a.gids[cbeg + t];
-            const float4* row = proj + (size_t)g * 3;
-            float4 geo = row[0], aux = row[1];
             s_box[t] = reach_box(geo, aux, &s_tau[t]);
             // Conic a and c are staged halved: the power below is then
             // dx (a/2 dx + b dy) + c/2 dy^2, five operations.
             s_geo[t] = make_float4(geo.x, geo.y, 0.5f * geo.z, geo.w);
             s_aux[t] = make_float4(0.5f * aux.x, aux.y, aux.z, aux.w);
-            s_blue[t] = row[2].x;
+            s_blue[t] = blue;
             s_gid[t] = g;
 #pragma unroll
             for (int k = 0; k < 9; ++k) s_acc[k][t] = 0.0f;
