@@ -50,6 +50,7 @@ namespace {
 constexpr uint32_t kThreads = 1024;
 constexpr uint32_t kRpt = 4;  // ranks per thread
 constexpr uint32_t kRanks = kThreads * kRpt;
+constexpr uint32_t kOwnWin = 512;  // pair slots per owner window of a wave (2 workgroups per CU)
 constexpr uint32_t kSat = 0x7fffffffu;
 constexpr uint64_t kFlagAggregate = 1, kFlagPrefix = 2;
 constexpr uint32_t kRounds = kBinChunk / 64;
@@ -104,6 +105,9 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
     // spanned << 18 | view supertiles per row << 24.
     __shared__ uint32_t s_span[kRanks];
     __shared__ uint2 s_val[kRanks];      // pair value: (splat | may-truncate << 31, bbox)
+    // Per wave: the owning rank (index among the wave's 256) of each pair slot
+    // of the current window of kOwnWin slots.
+    __shared__ uint8_t s_own[kThreads / 64][kOwnWin];
     __shared__ uint32_t s_wi[kThreads / 64], s_wp[kThreads / 64];
     __shared__ uint32_t s_bid, s_ei, s_ep;
 
@@ -243,13 +247,14 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
     // Per-rank isect offsets; the wave-local inclusive pair prefix.
     const uint32_t wave_excl_p = __shfl(texcl_p, 0, 64);
     uint32_t run_i = texcl_i, run_p = texcl_p;
-    uint32_t offs[kRpt];
+    uint32_t offs[kRpt], phi[kRpt];
 #pragma unroll
     for (uint32_t k = 0; k < kRpt; ++k) {
         offs[k] = sat_add(ei, run_i);
         run_i += cnt[k];
         run_p += sc[k];
-        s_incl[t * kRpt + k] = run_p - wave_excl_p;
+        phi[k] = run_p - wave_excl_p;
+        s_incl[t * kRpt + k] = phi[k];
         // A rank whose isects may reach the capacity carries a flag in its
         // pairs and leaves its offset at its splat: the only ranks whose
         // offset the tile passes need (raster.rs:12-48 drops slots >= cap).
@@ -278,24 +283,24 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
     const uint32_t* incl = s_incl + first;
     __builtin_amdgcn_wave_barrier();
     if (pbase >= a.cap) return;
-    constexpr uint32_t Q = 4;
-    for (uint32_t e0 = lane; e0 < wave_total; e0 += 64 * Q) {
-        uint32_t lo[Q];
+    // Owners by scatter, a window of kOwnWin slots at a time: each rank writes
+    // its index into the slots of its pairs, then each slot reads its owner
+    // (the wave's ranks own consecutive slot ranges, so one write per pair
+    // and one read replace a binary search over the wave's pair prefix).
+    for (uint32_t w0 = 0; w0 < wave_total; w0 += kOwnWin) {
+        if (pbase + w0 >= a.cap) break;
+        const uint32_t w1 = min(wave_total, w0 + kOwnWin);
 #pragma unroll
-        for (uint32_t q = 0; q < Q; ++q) lo[q] = 0;
-        // Owner: first k with incl[k] > e (256 entries, 8 halvings).
-#pragma unroll
-        for (uint32_t step = 128; step > 0; step >>= 1) {
-#pragma unroll
-            for (uint32_t q = 0; q < Q; ++q)
-                if (incl[lo[q] + step - 1] <= e0 + 64 * q) lo[q] += step;
+        for (uint32_t k = 0; k < kRpt; ++k) {
+            const uint32_t hi = min(phi[k], w1);
+            for (uint32_t q = max(phi[k] - sc[k], w0); q < hi; ++q)
+                s_own[w][q - w0] = (uint8_t)(lane * kRpt + k);
         }
-#pragma unroll
-        for (uint32_t q = 0; q < Q; ++q) {
-            const uint32_t e = e0 + 64 * q;
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t e = w0 + lane; e < w1; e += 64) {
             const uint32_t slot = pbase + e;
-            if (e >= wave_total || slot >= a.cap) continue;
-            const uint32_t k = lo[q];
+            if (slot >= a.cap) break;  // slots only grow with e
+            const uint32_t k = s_own[w][e - w0];
             const uint32_t j = e - (k ? incl[k - 1] : 0u);
             const uint32_t span = s_span[first + k];
             const uint32_t key0 = span & 0x3ffffu;
@@ -304,6 +309,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_scan(BinScanArgs a) {
             a.pair_keys[slot] = key0 + dy * row + (j - dy * swx);
             a.pair_vals[slot] = s_val[first + k];
         }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
