@@ -322,12 +322,13 @@ __global__ __launch_bounds__(256) void k_bin_chunk_map(const uint32_t* __restric
                                                        const BinView* __restrict__ views,
                                                        uint32_t nviews,
                                                        uint4* __restrict__ map) {
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    // One workgroup per supertile, one thread per chunk: contiguous records.
+    const uint32_t s = blockIdx.x;
     if (s >= supers) return;
     const uint32_t v = s % nviews;
     const uint32_t st = sstart[s], en = sstart[s + 1];
     const uint32_t c0 = st / kBinChunk + s, c1 = en / kBinChunk + s + 1;
-    for (uint32_t c = c0; c < c1; ++c) {
+    for (uint32_t c = c0 + threadIdx.x; c < c1; c += 256u) {
         const uint32_t p0 = st + (c - c0) * kBinChunk;
         map[c] = make_uint4((v << 24) | s, p0, min(p0 + kBinChunk, en), 0u);
     }
@@ -706,7 +707,7 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
         } else {
             SFK_HIP(c, launch_tile_starts(st, sk, j.cap, j.counters + 1, j.supers, j.sstart,
                                           j.nviews > 1 ? j.nviews : 0u, j.view_shift));
-            k_bin_chunk_map<<<div_up(j.supers, 256), 256, 0, st>>>(
+            k_bin_chunk_map<<<j.supers, 256, 0, st>>>(
                 j.sstart, j.supers, j.views, j.nviews, reinterpret_cast<uint4*>(j.chunk_map));
             SFK_LAUNCHED(c, "bin chunk map");
         }

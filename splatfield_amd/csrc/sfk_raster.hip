@@ -70,24 +70,27 @@ __global__ __launch_bounds__(256) void k_tile_starts(const uint32_t* __restrict_
     };
     uint32_t n = n_max;
     if (d_n) n = *d_n < n_max ? *d_n : n_max;
-    const uint32_t k0 = (blockIdx.x * 256u + threadIdx.x) * 4u;
-    if (k0 > n) return;
-    uint32_t ids[4];
-    if (k0 + 4 <= n) {
-        const uint4 v = reinterpret_cast<const uint4*>(tile_ids)[k0 / 4];
-        ids[0] = id(v.x); ids[1] = id(v.y); ids[2] = id(v.z); ids[3] = id(v.w);
-    } else {
+    // Grid-stride: a capacity-sized range runs on a bounded grid.
+    for (uint64_t k00 = (uint64_t)(blockIdx.x * 256u + threadIdx.x) * 4u; k00 <= n;
+         k00 += (uint64_t)gridDim.x * 1024u) {
+        const uint32_t k0 = (uint32_t)k00;
+        uint32_t ids[4];
+        if (k0 + 4 <= n) {
+            const uint4 v = reinterpret_cast<const uint4*>(tile_ids)[k0 / 4];
+            ids[0] = id(v.x); ids[1] = id(v.y); ids[2] = id(v.z); ids[3] = id(v.w);
+        } else {
 #pragma unroll
-        for (uint32_t i = 0; i < 4; ++i) ids[i] = k0 + i < n ? id(tile_ids[k0 + i]) : num_tiles;
-    }
-    uint32_t lo = k0 == 0 ? 0u : id(tile_ids[k0 - 1]) + 1u;
+            for (uint32_t i = 0; i < 4; ++i) ids[i] = k0 + i < n ? id(tile_ids[k0 + i]) : num_tiles;
+        }
+        uint32_t lo = k0 == 0 ? 0u : id(tile_ids[k0 - 1]) + 1u;
 #pragma unroll
-    for (uint32_t i = 0; i < 4; ++i) {
-        const uint32_t k = k0 + i;
-        if (k > n) break;
-        const uint32_t hi = ids[i];  // num_tiles at k == n
-        for (uint32_t t = lo; t <= hi; ++t) starts[t] = k;
-        lo = hi + 1u;
+        for (uint32_t i = 0; i < 4; ++i) {
+            const uint32_t k = k0 + i;
+            if (k > n) break;
+            const uint32_t hi = ids[i];  // num_tiles at k == n
+            for (uint32_t t = lo; t <= hi; ++t) starts[t] = k;
+            lo = hi + 1u;
+        }
     }
 }
 
@@ -364,8 +367,9 @@ hipError_t launch_map_isects(hipStream_t s, const uint32_t* order, const uint32_
 hipError_t launch_tile_starts(hipStream_t s, const uint32_t* tile_ids, uint32_t n,
                               const uint32_t* d_n, uint32_t num_tiles, uint32_t* starts,
                               uint32_t nviews, uint32_t view_shift) {
-    k_tile_starts<<<div_up((uint64_t)n + 1, 1024), 256, 0, s>>>(tile_ids, n, d_n, num_tiles, starts,
-                                                                nviews, view_shift);
+    uint32_t grid = div_up((uint64_t)n + 1, 1024);
+    if (d_n && grid > 4096) grid = 4096;  // the live count may be far below n
+    k_tile_starts<<<grid, 256, 0, s>>>(tile_ids, n, d_n, num_tiles, starts, nviews, view_shift);
     return hipGetLastError();
 }
 

@@ -113,11 +113,9 @@ __device__ __forceinline__ bool seg_pos(const SortArgs& a, uint32_t b, SegPos& p
 }
 
 template <typename K, int RADIX, uint32_t EPT>
-__global__ __launch_bounds__(NT) void k_upsweep(SortArgs a) {
+__device__ __forceinline__ void upsweep_block(const SortArgs& a, const SegPos& p) {
     constexpr uint32_t KPB = NT * EPT;
     __shared__ uint32_t h[NW][RADIX];
-    SegPos p;
-    if (!seg_pos(a, blockIdx.x, p)) return;
     const K* keys = static_cast<const K*>(a.keys_in);
     const uint32_t w = threadIdx.x / 64;
     const uint32_t mask = (1u << a.width) - 1u;
@@ -150,6 +148,24 @@ __global__ __launch_bounds__(NT) void k_upsweep(SortArgs a) {
 #pragma unroll
         for (uint32_t k = 0; k < NW; ++k) t += h[k][d];
         a.hist[((size_t)p.s * RADIX + d) * a.stride + p.bl] = t;
+    }
+}
+
+// LOOP: blocks b = blockIdx.x, + gridDim.x, ... while live.  A grid sized to
+// a capacity (the element count read from device memory) is bounded on the
+// host instead of launching a workgroup per capacity block that would only
+// load the count and exit; gridDim.x is then a multiple of 8, so block b still
+// runs on XCD b % 8 (see seg_pos).  Without LOOP, one block per workgroup.
+template <typename K, int RADIX, uint32_t EPT, bool LOOP>
+__global__ __launch_bounds__(NT) void k_upsweep(SortArgs a) {
+    SegPos p;
+    if constexpr (LOOP) {
+        for (uint32_t b = blockIdx.x; seg_pos(a, b, p); b += gridDim.x) {
+            upsweep_block<K, RADIX, EPT>(a, p);
+            __syncthreads();  // LDS reused by the next block
+        }
+    } else {
+        if (seg_pos(a, blockIdx.x, p)) upsweep_block<K, RADIX, EPT>(a, p);
     }
 }
 
@@ -210,7 +226,7 @@ __device__ __forceinline__ uint2 identity_val<uint2>(uint32_t li, const uint32_t
 }
 
 template <typename K, typename V, int RADIX, uint32_t EPT>
-__global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
+__device__ __forceinline__ void scatter_block(const SortArgs& a, const SegPos& p) {
     constexpr uint32_t KPB = NT * EPT;
     __shared__ K s_keys[KPB];
     __shared__ V s_vals[KPB];
@@ -218,8 +234,6 @@ __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
     __shared__ uint32_t s_gofs[RADIX];    // global position minus local position
     __shared__ uint32_t s_tmp[NW + 1];
 
-    SegPos p;
-    if (!seg_pos(a, blockIdx.x, p)) return;
     const K* keys_in = static_cast<const K*>(a.keys_in);
     K* keys_out = static_cast<K*>(a.keys_out);
     const uint32_t t = threadIdx.x, w = t / 64, lane = lane_id();
@@ -345,14 +359,37 @@ __global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
     }
 }
 
+template <typename K, typename V, int RADIX, uint32_t EPT, bool LOOP>
+__global__ __launch_bounds__(NT) void k_scatter(SortArgs a) {
+    SegPos p;
+    if constexpr (LOOP) {
+        for (uint32_t b = blockIdx.x; seg_pos(a, b, p); b += gridDim.x) {
+            scatter_block<K, V, RADIX, EPT>(a, p);
+            __syncthreads();  // LDS reused by the next block
+        }
+    } else {
+        if (seg_pos(a, blockIdx.x, p)) scatter_block<K, V, RADIX, EPT>(a, p);
+    }
+}
+
+// Workgroups of a pass whose element count is read from device memory.
+constexpr uint32_t kMaxLiveGrid = 2048;
+static_assert(kMaxLiveGrid % 8 == 0, "keeps block b on XCD b % 8");
+
 template <typename K, typename V, int RADIX, uint32_t EPT>
 hipError_t pass_impl(hipStream_t st, const SortArgs& a) {
     constexpr uint32_t KPB = NT * EPT;
     uint32_t grid = a.nseg <= 1 ? div_up(a.n_max, KPB) : a.nseg * div_up(a.seg_len, KPB);
     if (grid == 0) return hipSuccess;
-    k_upsweep<K, RADIX, EPT><<<grid, NT, 0, st>>>(a);
-    k_digscan<RADIX><<<(a.nseg <= 1 ? 1u : a.nseg) << a.width, NT, 0, st>>>(a);
-    k_scatter<K, V, RADIX, EPT><<<grid, NT, 0, st>>>(a);
+    if (a.nseg <= 1 && a.d_n && grid > kMaxLiveGrid) {
+        k_upsweep<K, RADIX, EPT, true><<<kMaxLiveGrid, NT, 0, st>>>(a);
+        k_digscan<RADIX><<<1u << a.width, NT, 0, st>>>(a);
+        k_scatter<K, V, RADIX, EPT, true><<<kMaxLiveGrid, NT, 0, st>>>(a);
+    } else {
+        k_upsweep<K, RADIX, EPT, false><<<grid, NT, 0, st>>>(a);
+        k_digscan<RADIX><<<(a.nseg <= 1 ? 1u : a.nseg) << a.width, NT, 0, st>>>(a);
+        k_scatter<K, V, RADIX, EPT, false><<<grid, NT, 0, st>>>(a);
+    }
     return hipGetLastError();
 }
 
