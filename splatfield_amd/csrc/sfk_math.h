@@ -70,6 +70,30 @@ SFK_HD float sfk_expf_neg(float p) {
     return a == a ? y : a;
 }
 
+// sfk_expf_neg for the lanes a forward rasterizer uses: the same bits for
+// every p <= 87 and for NaN, without the lower clamp of the argument (it only
+// binds past 87; there the value is unspecified, and the rasterizers discard
+// it: a lane uses G only when its power is at most 5.6 or NaN).  Dropping the
+// clamp lets the negation fold into the remaining min.
+SFK_HD float sfk_expf_neg_in(float p) {
+    const float a = -p;
+    const float x = fminf(a, 88.0f);
+    const float n = floorf(x * 1.44269504088896341f + 0.5f);
+    float r = fmaf(-n, 0.693359375f, x);
+    r = fmaf(-n, -2.12194440e-4f, r);
+    const float z = r * r;
+    float q = 1.9875691500e-4f;
+    q = fmaf(q, r, 1.3981999507e-3f);
+    q = fmaf(q, r, 8.3334519073e-3f);
+    q = fmaf(q, r, 4.1665795894e-2f);
+    q = fmaf(q, r, 1.6666665459e-1f);
+    q = fmaf(q, r, 5.0000001201e-1f);
+    q = fmaf(q, z, r);
+    q = q + 1.0f;
+    const float y = q * sfk_pow2i((int)n);
+    return a == a ? y : a;
+}
+
 SFK_HD float sfk_logf(float x) {
     if (x != x) return x;
     if (x < 0.0f) return __builtin_nanf("");

@@ -212,7 +212,7 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
                 const float power = 0.5f * (g4.z * dx * dx + x4.x * dy * dy) + g4.w * dx * dy;
                 const bool in = !done && i0 + s < nl && !(power > kPowerSkip);
                 if (!__ballot(in)) continue;
-                const float alpha = fminf(x4.y * sfk_expf_neg(power), 0.999f);
+                const float alpha = fminf(x4.y * sfk_expf_neg_in(power), 0.999f);
                 const bool con = in && alpha >= kInv255;
                 const float vis = alpha * T;
                 if constexpr (MODE == kRgb) {

@@ -14,9 +14,9 @@ int main(int argc, char** argv) {
     auto one = [&](uint32_t u) {
         float p;
         memcpy(&p, &u, 4);
-        const float a = sfk_expf(-p), b = sfk_expf_neg(p);
-        if (memcmp(&a, &b, 4) != 0) {
-            if (bad < 5) printf("p=%a exp=%a neg=%a\n", p, a, b);
+        const float a = sfk_expf(-p), b = sfk_expf_neg(p), c = sfk_expf_neg_in(p);
+        if (memcmp(&a, &b, 4) != 0 || memcmp(&a, &c, 4) != 0) {
+            if (bad < 5) printf("p=%a exp=%a neg=%a neg_in=%a\n", p, a, b, c);
             ++bad;
         }
         ++checked;
@@ -26,6 +26,14 @@ int main(int argc, char** argv) {
     const float nan = __builtin_nanf("");
     const float r = sfk_expf_neg(nan);
     if (r == r) ++bad;
+    const float r_in = sfk_expf_neg_in(nan);
+    if (r_in == r_in) ++bad;
+    // sfk_expf_neg_in: the same bits as sfk_expf_neg below 87 and at the
+    // upper clamp (p < -88) too.
+    for (float p = -1e30f; p < 87.0f; p = p < -1.0f ? p / 1.7f : (p < 1.0f ? p + 0.37f : p * 1.3f)) {
+        const float g = sfk_expf_neg(p), h = sfk_expf_neg_in(p);
+        if (memcmp(&g, &h, 4) != 0) ++bad;
+    }
     for (float p = -88.0f; p > -1e30f; p *= 1.7f)
         if (!(fminf((10.0f / 255.0f) * sfk_expf_neg(p), 0.999f) == 0.999f)) ++bad;
     // large powers: finite and below the contribution threshold
