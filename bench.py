@@ -211,6 +211,42 @@ def bench_sort(ctx, args):
             "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def bench_paint(ctx, splats, scratch, cam, w, h, n, reps=20):
+    """Posterior-mean render (src/seg/paint.rs:22-47) against the colour
+    render on the frame's scene.  The paint rides in the projection, which
+    then reads 8 bytes of (a, b) per splat instead of 4K of SH."""
+    import ctypes as C
+    from splatfield_amd._lib import lib
+    from splatfield_amd.runtime import GpuTensor
+    rng = np.random.default_rng(2)
+    a = GpuTensor.from_numpy(ctx, (1 + rng.gamma(1.0, 2.0, n)).astype(np.float32))
+    b = GpuTensor.from_numpy(ctx, (1 + rng.gamma(1.0, 2.0, n)).astype(np.float32))
+    c = cam.to_c()
+
+    def colours():  # no bitmap readback inside the timed region
+        ctx.check(lib().sfk_render_with(ctx.ptr, splats.ptr, scratch.ptr, C.byref(c), w, h,
+                                        None, None), "render_with")
+
+    def posterior():
+        ctx.check(lib().sfk_render_posterior(ctx.ptr, splats.ptr, scratch.ptr, a.ptr, b.ptr, n,
+                                             C.byref(c), w, h, None, None), "render_posterior")
+    out = {}
+    for name, fn in (("render_with_ms", colours), ("render_posterior_ms", posterior)):
+        fn()
+        ctx.timer_start()
+        for _ in range(reps):
+            fn()
+        out[name] = round(ctx.timer_stop_ms() / reps, 4)
+    for name, fn in (("project_colour_ms", colours), ("project_posterior_ms", posterior)):
+        ctx.prof_enable(True)
+        for _ in range(reps):
+            fn()
+        cnt, ms = ctx.prof_report()["project"]
+        ctx.prof_enable(False)
+        out[name] = round(ms / cnt, 5)
+    return out
+
+
 def kernel_bytes(n, k, vis, ni, px, pairs):
     """Algorithmic HBM bytes per launch of each step of the frame: the reads
     and writes the implemented pipeline must make (not counting cache
@@ -345,6 +381,8 @@ def main():
         stage("sort: 16M u64 keys")
         extra["sort_u64"] = bench_sort(ctx, args)
         extra["eig_round_c4"] = bench_c4(ctx, args)
+        stage("paint: posterior-mean render at the frame's scene")
+        extra["paint"] = bench_paint(ctx, splats, scratch, cam, w, h, n)
 
     cpu = None
     if not args.no_cpu:

@@ -277,6 +277,25 @@ int sfk_beta_update(sfk_ctx* ctx, float* a, float* b, const uint32_t* fg, const 
 int sfk_localize(sfk_ctx* ctx, const sfk_splats* splats, const float* a, const float* b,
                  float* center3, float* radius, int* found);
 
+/* ------------------------------------------------ paint (src/seg/paint.rs) */
+
+/* Splats::render_posterior (src/seg/paint.rs:22-47): render_with with every
+ * splat coloured by its posterior mean m = a/(a+b) as grayscale, no +0.5
+ * offset (paint_posterior, paint.rs:99-109).  n_ab != the splat count
+ * renders the normal colours instead, as the reference does. */
+int sfk_render_posterior(sfk_ctx* ctx, const sfk_splats* splats, sfk_render_scratch* scratch,
+                         const float* a, const float* b, uint32_t n_ab, const sfk_camera* cam,
+                         uint32_t w, uint32_t h, const uint32_t** bitmap, const float** rgba);
+/* Splats::tint (paint.rs:49-65, tint_colors 113-122): DC planes of the
+ * splats with a > b set to color, in place.  n_ab must equal the splat count
+ * (SFK_E_ARG otherwise). */
+int sfk_tint(sfk_ctx* ctx, sfk_splats* splats, const float* a, const float* b, uint32_t n_ab,
+             const float color[3]);
+/* Splats::save_colors / restore_colors (paint.rs:67-93): device copy of the
+ * three DC planes (3 x n floats, field-major) to / from caller memory. */
+int sfk_save_colors(sfk_ctx* ctx, const sfk_splats* splats, float* saved);
+int sfk_restore_colors(sfk_ctx* ctx, sfk_splats* splats, const float* saved);
+
 /* ------------------------------------- batched EIG round (no reference
  * counterpart: the reference scores candidates one plain render at a time,
  * src/seg/active.rs:113-129) */

@@ -59,6 +59,8 @@ def _lib():
         "sfo_beta_entropy": (f64, [f64, f64]),
         "sfo_eig": (f64, [P, P, P, u32, f32]),
         "sfo_beta_update": (None, [f32, P, P, u32, P, P]),
+        "sfo_paint_posterior": (None, [P, P, P, u32]),
+        "sfo_tint_colors": (None, [P, P, P, u32, f32, f32, f32]),
         "sfo_expf_export": (f32, [f32]),
         "sfo_logf_export": (f32, [f32]),
         "sfo_backward": (None, [P, P, P, u32, u32, u32, u32, u32, P, P, P, P, P, P, P, i32,
@@ -256,6 +258,28 @@ def beta_update(a, b, fg, bg, scale):
     a = _f32(a).copy(); b = _f32(b).copy()
     LIB.sfo_beta_update(scale, _p(_u32(fg)), _p(_u32(bg)), len(a), _p(a), _p(b))
     return a, b
+
+
+def render_posterior(view, attrs, sh, n, sh_per_ch, w, h, a, b, tile_w=16, nthreads=1):
+    """Splats::render_posterior (src/seg/paint.rs:22-47): prepare, paint each
+    projected row with m = a/(a+b), RGB raster."""
+    pr = prepare(view, attrs, sh, n, sh_per_ch, w, h, tile_w, nthreads=nthreads)
+    a = _f32(a); b = _f32(b)
+    proj = pr["projected"].copy()
+    LIB.sfo_paint_posterior(_p(proj), _p(a), _p(b), n)
+    ras = rasterize(MODE_RGB, w, h, pr["ranges"], pr["gaussian_ids"], proj, n, tile_w,
+                    nthreads=nthreads)
+    pr.update(ras)
+    pr["projected"] = proj
+    return pr
+
+
+def tint(sh, a, b, color):
+    """Splats::tint (src/seg/paint.rs:49-65) on a copy of the SH planes."""
+    sh = _f32(sh).copy(); a = _f32(a); b = _f32(b)
+    LIB.sfo_tint_colors(_p(sh), _p(a), _p(b), len(a), float(color[0]), float(color[1]),
+                        float(color[2]))
+    return sh
 
 
 def localize(attrs, n, a, b):

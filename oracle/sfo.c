@@ -599,5 +599,29 @@ void sfo_beta_update(float scale, const uint32_t* fg, const uint32_t* bg, uint32
     }
 }
 
+/* paint_posterior (src/seg/paint.rs:99-109): row colour := a/(a+b) at the
+ * reference row's offsets 5..7, no +0.5. */
+void sfo_paint_posterior(float* projected, const float* a, const float* b, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) {
+        float m = a[i] / (a[i] + b[i]);
+        float* P = projected + (size_t)i * SFO_PROJ_FLOATS;
+        P[5] = m;
+        P[6] = m;
+        P[7] = m;
+    }
+}
+
+/* tint_colors (src/seg/paint.rs:113-122): DC channel c of splat i at
+ * c*n + i, overwritten where a > b. */
+void sfo_tint_colors(float* sh, const float* a, const float* b, uint32_t n, float r, float g,
+                     float bl) {
+    for (uint32_t i = 0; i < n; i++)
+        if (a[i] > b[i]) {
+            sh[i] = r;
+            sh[(size_t)n + i] = g;
+            sh[2 * (size_t)n + i] = bl;
+        }
+}
+
 float sfo_expf_export(float x) { return sfo_expf(x); }
 float sfo_logf_export(float x) { return sfo_logf(x); }

@@ -69,6 +69,9 @@ double sfo_lgamma(double x);
 double sfo_digamma(double x);
 double sfo_beta_entropy(double a, double b);
 double sfo_eig(const float* a, const float* b, const uint32_t* resp, uint32_t n, float scale);
+void sfo_paint_posterior(float* projected, const float* a, const float* b, uint32_t n);
+void sfo_tint_colors(float* sh, const float* a, const float* b, uint32_t n, float r, float g,
+                     float bl);
 void sfo_beta_update(float scale, const uint32_t* fg, const uint32_t* bg, uint32_t n,
                      float* a, float* b);
 

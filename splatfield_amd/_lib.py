@@ -110,6 +110,10 @@ SIGNATURES = {
     "sfk_render_backward": (i32, [P, P, P, C.POINTER(TileIsects), P, P, P]),
     "sfk_render_backward_l2": (i32, [P, P, P, C.POINTER(TileIsects), P, P, P, P, P]),
     "sfk_localize": (i32, [P, P, P, P, P, P, P]),
+    "sfk_render_posterior": (i32, [P, P, P, P, P, u32, C.POINTER(Camera), u32, u32, P, P]),
+    "sfk_tint": (i32, [P, P, P, P, u32, P]),
+    "sfk_save_colors": (i32, [P, P, P]),
+    "sfk_restore_colors": (i32, [P, P, P]),
     "sfk_project": (i32, [P, C.POINTER(View), P, P, u32, u32, u32, P, P, P, P, P]),
     "sfk_map_isects": (i32, [P, P, P, P, P, u32, u32, u32, P, P]),
     "sfk_tile_starts": (i32, [P, P, u32, P, u32, P]),

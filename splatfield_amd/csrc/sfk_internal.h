@@ -99,7 +99,8 @@ hipError_t launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, si
 hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, const float* sh,
                           uint32_t n, uint32_t sh_per_ch, uint32_t tile_w, uint32_t* depth_keys,
                           float* proj, uint32_t* tile_counts, uint32_t* bbox, uint32_t* counters,
-                          uint32_t* cbbox = nullptr, float* dcol = nullptr);
+                          uint32_t* cbbox = nullptr, float* dcol = nullptr,
+                          const float* post_a = nullptr, const float* post_b = nullptr);
 
 // Radix sort: one pass = upsweep + digit scan + scatter.
 constexpr uint32_t kSortThreads = 512;
@@ -317,6 +318,11 @@ hipError_t launch_eig(hipStream_t s, const float* a, const float* b, const uint3
 uint32_t eig_num_partials(uint32_t n);
 hipError_t launch_beta_update(hipStream_t s, float* a, float* b, const uint32_t* fg,
                               const uint32_t* bg, uint32_t n, float scale);
+// paint.rs kernels: DC tint of the a > b splats, f32 copy (the posterior
+// paint is fused into the projection, launch_project's post_a / post_b).
+hipError_t launch_tint_colors(hipStream_t s, float* sh, const float* a, const float* b,
+                              uint32_t n, float r, float g, float bl);
+hipError_t launch_copy_f32(hipStream_t s, float* dst, const float* src, size_t count);
 // views::localize on the device; partials: 4 * localize_blocks(n) doubles;
 // out (device): [cx, cy, cz, sum of weights, radius].
 uint32_t localize_blocks(uint32_t n);

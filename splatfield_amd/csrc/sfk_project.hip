@@ -75,7 +75,9 @@ __device__ __forceinline__ void sh_to_rgb(float x, float y, float z, const float
 }
 
 // CHS = SH coefficients per channel (1, 4, 9, 16); CHS = 0 skips colour
-// (geometry-only projection for the responsibility modes).
+// (geometry-only projection for the responsibility modes); CHS = -1 paints
+// the posterior mean m = a/(a+b) as grayscale with no +0.5 offset
+// (paint_posterior, src/seg/paint.rs:99-109, fused here: no SH is read).
 template <int CHS, bool JAC>
 __global__ __launch_bounds__(256) void k_project(sfk_view v, const float* __restrict__ A,
                                                  const float* __restrict__ shs, uint32_t n,
@@ -86,7 +88,9 @@ __global__ __launch_bounds__(256) void k_project(sfk_view v, const float* __rest
                                                  uint2* __restrict__ bbox,
                                                  uint32_t* __restrict__ cbbox,
                                                  uint32_t* __restrict__ counters,
-                                                 float* __restrict__ dcol) {
+                                                 float* __restrict__ dcol,
+                                                 const float* __restrict__ post_a,
+                                                 const float* __restrict__ post_b) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     bool vis = false;
     uint32_t cnt = 0, cb = 0;
@@ -99,6 +103,11 @@ __global__ __launch_bounds__(256) void k_project(sfk_view v, const float* __rest
             key = __builtin_bit_cast(uint32_t, o.cz);
             cnt = o.count;
             float cr = 0.0f, cg = 0.0f, cbl = 0.0f;
+            if constexpr (CHS < 0) {
+                const float pa = post_a[i];
+                const float m = pa / (pa + post_b[i]);
+                cr = cg = cbl = m;
+            }
             if constexpr (CHS > 0) {
                 float dx = s.mx - v.world_pos[0], dy = s.my - v.world_pos[1],
                       dz = s.mz - v.world_pos[2];
@@ -126,8 +135,9 @@ __global__ __launch_bounds__(256) void k_project(sfk_view v, const float* __rest
             }
             float4* row = proj + (size_t)i * 3;
             row[0] = make_float4(o.px, o.py, o.qa, o.qb);
-            row[1] = make_float4(o.qc, s.opacity, cr + 0.5f, cg + 0.5f);
-            row[2] = make_float4(cbl + 0.5f, 0.0f, 0.0f, 0.0f);
+            constexpr float kOff = CHS < 0 ? 0.0f : 0.5f;  // project.rs:308-310
+            row[1] = make_float4(o.qc, s.opacity, cr + kOff, cg + kOff);
+            row[2] = make_float4(cbl + kOff, 0.0f, 0.0f, 0.0f);
             bbox[i] = make_uint2(o.bb_lo, o.bb_hi);
             cb = compact_bbox(o);
         }
@@ -143,7 +153,8 @@ __global__ __launch_bounds__(256) void k_project(sfk_view v, const float* __rest
 hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, const float* sh,
                           uint32_t n, uint32_t sh_per_ch, uint32_t tile_w, uint32_t* depth_keys,
                           float* proj, uint32_t* tile_counts, uint32_t* bbox, uint32_t* counters,
-                          uint32_t* cbbox, float* dcol) {
+                          uint32_t* cbbox, float* dcol, const float* post_a,
+                          const float* post_b) {
     if (n == 0) return hipSuccess;
     dim3 grid(div_up(n, 256)), block(256);
     float inv_tile = 1.0f / (float)tile_w;
@@ -153,12 +164,20 @@ hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, 
     do {                                                                                   \
         if (dcol)                                                                          \
             k_project<C, true><<<grid, block, 0, s>>>(v, attrs, sh, n, inv_tile, depth_keys, p4, \
-                                                      tile_counts, bb, cbbox, counters, dcol); \
+                                                      tile_counts, bb, cbbox, counters, dcol,  \
+                                                      nullptr, nullptr);                       \
         else                                                                               \
             k_project<C, false><<<grid, block, 0, s>>>(v, attrs, sh, n, inv_tile, depth_keys,   \
                                                        p4, tile_counts, bb, cbbox, counters,   \
-                                                       nullptr);                               \
+                                                       nullptr, nullptr, nullptr);             \
     } while (0)
+    if (post_a || post_b) {
+        if (!post_a || !post_b || dcol) return hipErrorInvalidValue;
+        k_project<-1, false><<<grid, block, 0, s>>>(v, attrs, nullptr, n, inv_tile, depth_keys, p4,
+                                                    tile_counts, bb, cbbox, counters, nullptr,
+                                                    post_a, post_b);
+        return hipGetLastError();
+    }
     switch (sh_per_ch) {
         case 0: SFK_PROJ(0); break;
         case 1: SFK_PROJ(1); break;

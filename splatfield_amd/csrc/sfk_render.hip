@@ -355,7 +355,8 @@ static void fill_isects(const sfk_render_scratch* s, sfk_tile_isects* o) {
 // the tile sort over the device-side isect count.
 static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
                         const sfk_view* view, uint32_t w, uint32_t h, sfk_tile_isects* out,
-                        bool colors);
+                        bool colors, const float* post_a = nullptr,
+                        const float* post_b = nullptr);
 
 extern "C" int sfk_prepare_isects_view(sfk_ctx* c, const sfk_splats* sp,
                                        sfk_render_scratch* s, const sfk_view* view, uint32_t w,
@@ -365,9 +366,10 @@ extern "C" int sfk_prepare_isects_view(sfk_ctx* c, const sfk_splats* sp,
 
 // colors = false projects geometry only (no SH): the responsibility modes
 // never read the colour, and skipping it leaves every epsilon word unchanged.
+// post_a / post_b (colors false): colour rows hold the posterior mean.
 static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
                         const sfk_view* view, uint32_t w, uint32_t h, sfk_tile_isects* out,
-                        bool colors) {
+                        bool colors, const float* post_a, const float* post_b) {
     if (!c || !sp || !s || !view || !out) return SFK_E_ARG;
     if ((uint32_t)view->img[0] != w || (uint32_t)view->img[1] != h)
         return fail(c, SFK_E_ARG, "view is for %gx%g, not %ux%u", view->img[0], view->img[1], w, h);
@@ -389,8 +391,8 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
         SFK_HIP(c, launch_project(c->stream, *view, sp->attrs, sp->sh, n,
                                   colors ? sp->sh_per_ch : 0u, s->tile_w, s->depth_keys,
                                   s->projected, s->tile_counts, s->bbox, s->counters + 2,
-                                  compact ? s->cbbox : nullptr, dcol));
-        s->has_colors = colors;
+                                  compact ? s->cbbox : nullptr, dcol, post_a, post_b));
+        s->has_colors = colors || post_a;
         s->has_dcol = dcol != nullptr;
     }
 
@@ -690,6 +692,52 @@ extern "C" int sfk_beta_update(sfk_ctx* c, float* a, float* b, const uint32_t* f
                                const uint32_t* bg, uint32_t n, float scale) {
     if (!c || (n && (!a || !b || !fg || !bg))) return SFK_E_ARG;
     SFK_HIP(c, launch_beta_update(c->stream, a, b, fg, bg, n, scale));
+    return SFK_OK;
+}
+
+// -------------------------------------------------------------- paint
+
+extern "C" int sfk_render_posterior(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
+                                    const float* a, const float* b, uint32_t n_ab,
+                                    const sfk_camera* cam, uint32_t w, uint32_t h,
+                                    const uint32_t** bitmap, const float** rgba) {
+    if (!c || !sp || !s) return SFK_E_ARG;
+    // A posterior sized for another splat count renders the normal colours
+    // (paint.rs:30-32).
+    if (!a || !b || n_ab != sp->n) return sfk_render_with(c, sp, s, cam, w, h, bitmap, rgba);
+    if (!cam) return SFK_E_ARG;
+    sfk_view v;
+    int rc = sfk_camera_view(cam, w, h, s->tile_w, &v);
+    if (rc) return fail(c, rc, "camera view: bad arguments");
+    // The paint rides in the projection: rows get m = a/(a+b) and no SH is
+    // read (paint.rs projects SH colours, then overwrites them).
+    sfk_tile_isects t;
+    rc = prepare_impl(c, sp, s, &v, w, h, &t, false, a, b);
+    if (rc) return rc;
+    rc = sfk_finalize_rgb(c, s, &t);
+    if (rc) return rc;
+    if (bitmap) *bitmap = s->bitmap;
+    if (rgba) *rgba = s->rgba;
+    return SFK_OK;
+}
+
+extern "C" int sfk_tint(sfk_ctx* c, sfk_splats* sp, const float* a, const float* b,
+                        uint32_t n_ab, const float color[3]) {
+    if (!c || !sp || !a || !b || !color) return SFK_E_ARG;
+    if (n_ab != sp->n) return fail(c, SFK_E_ARG, "tint: %u posteriors for %u splats", n_ab, sp->n);
+    SFK_HIP(c, launch_tint_colors(c->stream, sp->sh, a, b, sp->n, color[0], color[1], color[2]));
+    return SFK_OK;
+}
+
+extern "C" int sfk_save_colors(sfk_ctx* c, const sfk_splats* sp, float* saved) {
+    if (!c || !sp || !saved) return SFK_E_ARG;
+    SFK_HIP(c, launch_copy_f32(c->stream, saved, sp->sh, (size_t)3 * sp->n));
+    return SFK_OK;
+}
+
+extern "C" int sfk_restore_colors(sfk_ctx* c, sfk_splats* sp, const float* saved) {
+    if (!c || !sp || !saved) return SFK_E_ARG;
+    SFK_HIP(c, launch_copy_f32(c->stream, sp->sh, saved, (size_t)3 * sp->n));
     return SFK_OK;
 }
 

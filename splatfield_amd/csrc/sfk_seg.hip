@@ -96,6 +96,28 @@ __global__ __launch_bounds__(256) void k_beta_update(float* __restrict__ a, floa
     }
 }
 
+// tint_colors (src/seg/paint.rs:113-122): DC channels of the splats with
+// a > b (ties stay) set to (r, g, bl); DC channel k of splat i at k*n + i.
+__global__ __launch_bounds__(256) void k_tint_colors(float* __restrict__ sh,
+                                                     const float* __restrict__ a,
+                                                     const float* __restrict__ b, uint32_t n,
+                                                     float r, float g, float bl) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n && a[i] > b[i]) {
+        sh[i] = r;
+        sh[(size_t)n + i] = g;
+        sh[2 * (size_t)n + i] = bl;
+    }
+}
+
+// copy_f32 (src/seg/paint.rs:128-134) over a known count.
+__global__ __launch_bounds__(256) void k_copy_f32(float* __restrict__ dst,
+                                                  const float* __restrict__ src, size_t count) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (; i < count; i += stride) dst[i] = src[i];
+}
+
 // views::localize (views.rs:32-67) on the device: posterior-weighted centre
 // and mean distance of the splats the MAP labels foreground (a > b, weight
 // m = a / (a + b)).  Pass 0 sums (m, m x, m y, m z); pass 1 sums m |p - c|
@@ -229,6 +251,21 @@ hipError_t launch_beta_update(hipStream_t s, float* a, float* b, const uint32_t*
                               const uint32_t* bg, uint32_t n, float scale) {
     if (n == 0) return hipSuccess;
     k_beta_update<<<div_up(n, 256), 256, 0, s>>>(a, b, fg, bg, n, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_tint_colors(hipStream_t s, float* sh, const float* a, const float* b,
+                              uint32_t n, float r, float g, float bl) {
+    if (n == 0) return hipSuccess;
+    k_tint_colors<<<div_up(n, 256), 256, 0, s>>>(sh, a, b, n, r, g, bl);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_f32(hipStream_t s, float* dst, const float* src, size_t count) {
+    if (count == 0) return hipSuccess;
+    size_t blocks = (count + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    k_copy_f32<<<(uint32_t)blocks, 256, 0, s>>>(dst, src, count);
     return hipGetLastError();
 }
 

@@ -81,6 +81,44 @@ class Splats:
         self.bounds = (xyz.min(axis=1), xyz.max(axis=1))
         self.positions = xyz.copy()
 
+    def render_posterior(self, scratch, a, b, camera, w, h):
+        """Splats::render_posterior (src/seg/paint.rs:22-47): the frame with
+        every splat painted by its posterior mean a/(a+b) as grayscale; a
+        posterior of another length renders the normal colours.  a, b are
+        device GpuTensors; returns the RGBA8 bitmap (h x row_stride u32)."""
+        ctx = self.ctx
+        c = camera.to_c()
+        bm = C.c_void_p()
+        n_ab = a.shape[0] if a.shape == b.shape else 0
+        ctx.check(lib().sfk_render_posterior(ctx.ptr, self.ptr, scratch.ptr, a.ptr, b.ptr, n_ab,
+                                             C.byref(c), w, h, C.byref(bm), None),
+                  "render_posterior")
+        bf = scratch.buffers()
+        return GpuTensor(ctx, bm.value, (h, bf.row_stride), np.uint32, False, scratch).read()
+
+    def tint(self, a, b, color):
+        """Splats::tint (src/seg/paint.rs:49-65): DC planes of the splats
+        with a > b set to color, in place on the device."""
+        if a.shape != b.shape:
+            raise ValueError(f"tint: a {a.shape} vs b {b.shape}")
+        col = np.ascontiguousarray(color, np.float32)
+        self.ctx.check(lib().sfk_tint(self.ctx.ptr, self.ptr, a.ptr, b.ptr, a.shape[0],
+                                      col.ctypes.data), "tint")
+
+    def save_colors(self):
+        """Splats::save_colors (paint.rs:67-74): device snapshot [3, n] of
+        the DC planes."""
+        saved = GpuTensor.empty(self.ctx, (3, self.n), np.float32)
+        self.ctx.check(lib().sfk_save_colors(self.ctx.ptr, self.ptr, saved.ptr), "save_colors")
+        return saved
+
+    def restore_colors(self, saved):
+        """Splats::restore_colors (paint.rs:76-79)."""
+        if saved.shape != (3, self.n):
+            raise ValueError(f"restore_colors: snapshot {saved.shape} for {self.n} splats")
+        self.ctx.check(lib().sfk_restore_colors(self.ctx.ptr, self.ptr, saved.ptr),
+                       "restore_colors")
+
     def device_ptrs(self):
         a, s = C.c_void_p(), C.c_void_p()
         lib().sfk_splats_info(self.ptr, None, None, None, None, C.byref(a), C.byref(s))

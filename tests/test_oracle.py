@@ -439,3 +439,55 @@ def test_localize_closed_forms():
     w0, w1 = 0.75, 0.6
     assert abs(c3[0] - w1 / (w0 + w1)) < 1e-7
     assert abs(r3 - 2 * w0 * w1 / (w0 + w1) ** 2) < 1e-7
+
+
+# ------------------------------------------------------------------ paint
+
+def test_paint_posterior_pins():
+    """paint.rs:96-109: the posterior render is sum(m_i alpha_i T_i) with no
+    +0.5.  m = 1 gives 1 - T (the alpha channel), m = 0 gives black, halving
+    every m halves every channel exactly (a power of two), and T is the
+    colour render's: a paint that touched a non-colour field, added the
+    offset or missed a channel fails one of these."""
+    n = 40
+    attrs = overdraw_attributes()
+    sh = np.zeros(3 * n, np.float32)
+    view = default_view(70, 70)
+    plain = O.render(view, attrs, sh, n, 1, 70, 70)
+    ones, zeros = np.ones(n, np.float32), np.zeros(n, np.float32)
+    white = O.render_posterior(view, attrs, sh, n, 1, 70, 70, ones, zeros)
+    black = O.render_posterior(view, attrs, sh, n, 1, 70, 70, zeros, ones)
+    rng = np.random.default_rng(3)
+    a = rng.integers(1, 9, n).astype(np.float32)
+    half = O.render_posterior(view, attrs, sh, n, 1, 70, 70, a, a)
+    quarter = O.render_posterior(view, attrs, sh, n, 1, 70, 70, a, 3 * a)
+    for r in (white, black, half, quarter):
+        np.testing.assert_array_equal(r["final_t"], plain["final_t"])
+        px = r["rgba"].reshape(-1, 4)
+        np.testing.assert_array_equal(px[:, 3], plain["rgba"].reshape(-1, 4)[:, 3])
+        assert np.all(px[:, 0] == px[:, 1]) and np.all(px[:, 1] == px[:, 2])
+    w4 = white["rgba"].reshape(-1, 4)
+    assert w4[:, 3].max() > 0.9
+    np.testing.assert_allclose(w4[:, 0], w4[:, 3], atol=2e-6)
+    assert not black["rgba"].reshape(-1, 4)[:, :3].any()
+    np.testing.assert_array_equal(quarter["rgba"].reshape(-1, 4)[:, :3] * 2,
+                                  half["rgba"].reshape(-1, 4)[:, :3])
+    np.testing.assert_array_equal(half["rgba"].reshape(-1, 4)[:, :3] * 2, w4[:, :3])
+
+
+def test_tint_overwrites_foreground_dc_only():
+    """tint_colors (paint.rs:113-122): DC planes c*n + i take the colour
+    where a > b (ties are background); every other float is unchanged."""
+    n = 1000
+    rng = np.random.default_rng(4)
+    sh = rng.normal(size=3 * 16 * n).astype(np.float32)
+    a = rng.integers(0, 4, n).astype(np.float32)
+    b = rng.integers(0, 4, n).astype(np.float32)
+    out = O.tint(sh, a, b, (0.5, -1.0, 2.0))
+    fg = a > b
+    assert 0 < fg.sum() < n and np.any(a == b)
+    dc = out[:3 * n].reshape(3, n)
+    for c, v in enumerate((0.5, -1.0, 2.0)):
+        assert np.all(dc[c, fg] == np.float32(v))
+        np.testing.assert_array_equal(dc[c, ~fg], sh[c * n:(c + 1) * n][~fg])
+    np.testing.assert_array_equal(out[3 * n:], sh[3 * n:])
