@@ -277,6 +277,40 @@ int sfk_beta_update(sfk_ctx* ctx, float* a, float* b, const uint32_t* fg, const 
 int sfk_localize(sfk_ctx* ctx, const sfk_splats* splats, const float* a, const float* b,
                  float* center3, float* radius, int* found);
 
+/* ------------------------------------------ scene ingestion (src/ply.rs) */
+
+#define SFK_PLY_MAX_REST 45    /* f_rest_* properties: SH degree 3 */
+#define SFK_PLY_MAX_STRIDE 128 /* float properties per vertex */
+
+/* Header of a binary little-endian 3DGS PLY (parse_ply, src/ply.rs:9-75). */
+typedef struct {
+    uint64_t n;           /* vertices */
+    uint64_t data_offset; /* bytes before the first vertex row */
+    uint32_t stride;      /* float properties per vertex row */
+    uint32_t n_rest;      /* f_rest_* properties */
+    uint32_t sh_planes;   /* n_rest + 3 SH planes written (ply.rs:75) */
+    uint32_t sh_per_ch;   /* n_rest / 3 + 1 */
+    /* columns of x y z scale_0..2 opacity rot_0..3 f_dc_0..2 */
+    int32_t base[14];
+    int32_t rest[SFK_PLY_MAX_REST]; /* f_rest_k columns, ascending k */
+} sfk_ply_layout;
+
+/* parse_ply's header pass and checks (ply.rs:13-75): format, vertex count,
+ * float-only properties, the 14 named properties (the first missing one
+ * reported), f_rest_k sorted by k, and that every row is in the buffer.
+ * Host only; errors are SFK_E_ARG with the reference's message in err. */
+int sfk_ply_parse_header(const void* bytes, size_t len, sfk_ply_layout* out, char* err,
+                         int err_len);
+/* parse_ply's row loop (ply.rs:78-106) on the device: rows (n x stride
+ * floats, as in the file) to the 11 attribute planes (rotation normalised)
+ * and sh_planes SH planes (DC, then f_rest_k channel-major in the file to
+ * coefficient-interleaved planes). */
+int sfk_ply_decode(sfk_ctx* ctx, const sfk_ply_layout* layout, const float* rows, float* attrs,
+                   float* sh);
+/* Splats from a PLY file image in host memory (lib.rs:45-50 + Splats::new):
+ * header on the host, rows decoded on the device.  SH degree 0-3. */
+int sfk_splats_from_ply(sfk_ctx* ctx, const void* bytes, size_t len, sfk_splats** out);
+
 /* ------------------------------------------------ paint (src/seg/paint.rs) */
 
 /* Splats::render_posterior (src/seg/paint.rs:22-47): render_with with every

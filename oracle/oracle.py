@@ -260,6 +260,84 @@ def beta_update(a, b, fg, bg, scale):
     return a, b
 
 
+def parse_ply(data):
+    """parse_ply (src/ply.rs:9-112) in numpy: header rules, then the rows
+    transposed to field-major planes; the rotation normalised as glam's
+    x86-64 Quat::normalize evaluates it ((x^2 + z^2) + (y^2 + w^2), sqrt,
+    one f32 division per component).  Returns (attrs, sh) or raises
+    ValueError with the reference's message."""
+    data = bytes(data)
+    pos, props, count, off = 0, [], 0, len(data)
+    while pos < len(data):
+        e = data.find(b"\n", pos)
+        e = len(data) if e < 0 else e
+        tok = data[pos:e].decode("latin-1").split()
+        pos = min(e + 1, len(data))
+        if not tok:
+            continue
+        if tok[0] == "end_header":
+            off = pos
+            break
+        if tok[0] == "format" and len(tok) >= 2 and tok[1] != "binary_little_endian":
+            raise ValueError(f"Unsupported PLY format: {tok[1]}")
+        if len(tok) == 3 and tok[:2] == ["element", "vertex"]:
+            count = int(tok[2])
+        elif len(tok) == 3 and tok[0] == "property":
+            if tok[1] != "float":
+                raise ValueError(f"Unsupported property type {tok[1]}: {tok[2]}")
+            props.append(tok[2])
+    if count == 0:
+        raise ValueError("PLY contains no vertices")
+    names = ["x", "y", "z", "scale_0", "scale_1", "scale_2", "opacity", "rot_0", "rot_1",
+             "rot_2", "rot_3", "f_dc_0", "f_dc_1", "f_dc_2"]
+    col = {}
+    for nm in names:
+        if nm not in props:
+            raise ValueError(f"Missing property: {nm}")
+        col[nm] = props.index(nm)
+    rest = sorted(((int(p[7:]), c) for c, p in enumerate(props)
+                   if p.startswith("f_rest_") and p[7:].lstrip("+").isdigit()),
+                  key=lambda t: t[0])
+    stride = len(props)
+    if (len(data) - off) // (4 * stride) < count:
+        raise ValueError(f"failed to read vertex {(len(data) - off) // (4 * stride)} of {count}")
+    rows = np.frombuffer(data, "<f4", count * stride, off).reshape(count, stride)
+    f = np.float32
+    x, y, z, w = (rows[:, col[k]] for k in ("rot_1", "rot_2", "rot_3", "rot_0"))
+    ln = np.sqrt((x * x + z * z) + (y * y + w * w)).astype(f)
+    attrs = np.stack([rows[:, col["x"]], rows[:, col["y"]], rows[:, col["z"]],
+                      w / ln, x / ln, y / ln, z / ln, rows[:, col["scale_0"]],
+                      rows[:, col["scale_1"]], rows[:, col["scale_2"]],
+                      rows[:, col["opacity"]]]).astype(f)
+    per = len(rest) // 3
+    sh = np.zeros((len(rest) + 3, count), f)
+    for c in range(3):
+        sh[c] = rows[:, col[f"f_dc_{c}"]]
+        for j in range(per):
+            sh[(j + 1) * 3 + c] = rows[:, rest[c * per + j][1]]
+    return attrs.reshape(-1), sh.reshape(-1)
+
+
+def write_ply(attrs, sh, n):
+    """CpuSplats::write_ply (src/ply.rs:114-170): canonical INRIA property
+    order, zero normals, f_rest channel-major."""
+    attrs = _f32(attrs).reshape(11, n); sh = _f32(sh).reshape(-1, n)
+    k = sh.shape[0] // 3
+    rest = (k - 1) * 3
+    props = ["x", "y", "z", "nx", "ny", "nz", "f_dc_0", "f_dc_1", "f_dc_2"]
+    props += [f"f_rest_{j}" for j in range(rest)]
+    props += ["opacity", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3"]
+    head = "ply\nformat binary_little_endian 1.0\nelement vertex %d\n" % n
+    head += "".join(f"property float {p}\n" for p in props) + "end_header\n"
+    cols = [attrs[0], attrs[1], attrs[2], np.zeros(n), np.zeros(n), np.zeros(n), sh[0], sh[1],
+            sh[2]]
+    per = rest // 3
+    cols += [sh[(j % per + 1) * 3 + j // per] for j in range(rest)]
+    cols += [attrs[10], attrs[7], attrs[8], attrs[9], attrs[3], attrs[4], attrs[5], attrs[6]]
+    rows = np.stack(cols, axis=1).astype("<f4")
+    return head.encode() + rows.tobytes()
+
+
 def render_posterior(view, attrs, sh, n, sh_per_ch, w, h, a, b, tile_w=16, nthreads=1):
     """Splats::render_posterior (src/seg/paint.rs:22-47): prepare, paint each
     projected row with m = a/(a+b), RGB raster."""

@@ -33,6 +33,12 @@ class TileIsects(C.Structure):
     ]
 
 
+class PlyLayout(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("data_offset", C.c_uint64), ("stride", u32),
+                ("n_rest", u32), ("sh_planes", u32), ("sh_per_ch", u32),
+                ("base", C.c_int32 * 14), ("rest", C.c_int32 * 45)]
+
+
 class RenderBuffers(C.Structure):
     _fields_ = [
         ("depth_keys", P), ("depth_order", P), ("projected", P), ("tile_counts", P),
@@ -112,6 +118,9 @@ SIGNATURES = {
     "sfk_localize": (i32, [P, P, P, P, P, P, P]),
     "sfk_render_posterior": (i32, [P, P, P, P, P, u32, C.POINTER(Camera), u32, u32, P, P]),
     "sfk_tint": (i32, [P, P, P, P, u32, P]),
+    "sfk_ply_parse_header": (i32, [P, C.c_size_t, C.POINTER(PlyLayout), C.c_char_p, i32]),
+    "sfk_ply_decode": (i32, [P, C.POINTER(PlyLayout), P, P, P]),
+    "sfk_splats_from_ply": (i32, [P, P, C.c_size_t, PP]),
     "sfk_save_colors": (i32, [P, P, P]),
     "sfk_restore_colors": (i32, [P, P, P]),
     "sfk_project": (i32, [P, C.POINTER(View), P, P, u32, u32, u32, P, P, P, P, P]),

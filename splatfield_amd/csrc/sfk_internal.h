@@ -323,6 +323,9 @@ hipError_t launch_beta_update(hipStream_t s, float* a, float* b, const uint32_t*
 hipError_t launch_tint_colors(hipStream_t s, float* sh, const float* a, const float* b,
                               uint32_t n, float r, float g, float bl);
 hipError_t launch_copy_f32(hipStream_t s, float* dst, const float* src, size_t count);
+// PLY vertex rows (l.stride floats each, as in the file) -> field-major planes.
+hipError_t launch_ply_decode(hipStream_t s, const sfk_ply_layout& l, const float* rows,
+                             float* attrs, float* sh);
 // views::localize on the device; partials: 4 * localize_blocks(n) doubles;
 // out (device): [cx, cy, cz, sum of weights, radius].
 uint32_t localize_blocks(uint32_t n);

@@ -67,6 +67,56 @@ extern "C" int sfk_splats_create(sfk_ctx* c, const float* attrs, const float* sh
     return SFK_OK;
 }
 
+extern "C" int sfk_splats_from_ply(sfk_ctx* c, const void* bytes, size_t len,
+                                   sfk_splats** out) {
+    if (!c || !bytes || !out) return SFK_E_ARG;
+    *out = nullptr;
+    sfk_ply_layout l;
+    char err[256];
+    int rc = sfk_ply_parse_header(bytes, len, &l, err, (int)sizeof err);
+    if (rc) return fail(c, rc, "%s", err);
+    if (!(l.sh_planes == 3 || l.sh_planes == 12 || l.sh_planes == 27 || l.sh_planes == 48))
+        return fail(c, SFK_E_ARG, "PLY with %u f_rest properties: SH degree 0-3 only", l.n_rest);
+    if (l.n > 0xffffffffull) return fail(c, SFK_E_ARG, "PLY with %llu vertices",
+                                         (unsigned long long)l.n);
+    const uint32_t n = (uint32_t)l.n;
+    const size_t row_bytes = (size_t)l.stride * 4 * n;
+    auto* s = new (std::nothrow) sfk_splats();
+    if (!s) return SFK_E_NOMEM;
+    s->ctx = c;
+    s->n = n;
+    s->sh_per_ch = l.sh_planes / 3;
+    float* rows = nullptr;
+    const char* data = static_cast<const char*>(bytes) + l.data_offset;
+    hipError_t e = hipMalloc(&s->attrs, (size_t)n * SFK_ATTR_PLANES * 4);
+    if (e == hipSuccess) e = hipMalloc(&s->sh, (size_t)n * l.sh_planes * 4);
+    if (e == hipSuccess) e = hipMalloc(&rows, row_bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(rows, data, row_bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_ply_decode(c->stream, l, rows, s->attrs, s->sh);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (rows) hipFree(rows);
+    if (e != hipSuccess) {
+        if (s->attrs) hipFree(s->attrs);
+        if (s->sh) hipFree(s->sh);
+        delete s;
+        return hip_fail(c, e, "PLY upload");
+    }
+    // Scene AABB (render.rs:253-259) straight from the rows' x, y, z.
+    for (int k = 0; k < 3; ++k) {
+        s->bmin[k] = 3.40282347e38f;
+        s->bmax[k] = -3.40282347e38f;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            float v;
+            memcpy(&v, data + ((size_t)i * l.stride + l.base[k]) * 4, 4);
+            s->bmin[k] = std::min(s->bmin[k], v);
+            s->bmax[k] = std::max(s->bmax[k], v);
+        }
+    *out = s;
+    return SFK_OK;
+}
+
 extern "C" void sfk_splats_destroy(sfk_splats* s) {
     if (!s) return;
     if (s->attrs) hipFree(s->attrs);

@@ -56,6 +56,17 @@ class Camera:
         return v
 
 
+def parse_ply_header(data):
+    """sfk_ply_parse_header (src/ply.rs:13-75): the header's layout, or
+    ValueError with the reference's message."""
+    buf = np.frombuffer(bytes(data), np.uint8) if not isinstance(data, np.ndarray) else data
+    lay = _lib.PlyLayout()
+    err = C.create_string_buffer(256)
+    if lib().sfk_ply_parse_header(buf.ctypes.data, buf.size, C.byref(lay), err, 256) != 0:
+        raise ValueError(err.value.decode())
+    return lay
+
+
 def view_array(v):
     return np.frombuffer(bytes(v), dtype=np.float32).copy()
 
@@ -80,6 +91,25 @@ class Splats:
         xyz = attributes[:3 * n].reshape(3, n)
         self.bounds = (xyz.min(axis=1), xyz.max(axis=1))
         self.positions = xyz.copy()
+
+    @classmethod
+    def from_ply(cls, ctx, data):
+        """parse_ply (src/ply.rs:9-112) + Splats::new: header parsed on the
+        host, vertex rows decoded into the field-major planes on the device.
+        data: the file's bytes.  Errors carry the reference's messages."""
+        buf = np.frombuffer(bytes(data), np.uint8)
+        out = C.c_void_p()
+        ctx.check(lib().sfk_splats_from_ply(ctx.ptr, buf.ctypes.data, buf.size, C.byref(out)),
+                  "parse_ply")
+        self = cls.__new__(cls)
+        self.ctx, self.ptr = ctx, out.value
+        lay = parse_ply_header(buf)
+        self.n, self.sh_per_ch = int(lay.n), int(lay.sh_per_ch)
+        d_attrs, _ = self.device_ptrs()
+        xyz = GpuTensor(ctx, d_attrs, (3, self.n), np.float32, False, self).read()
+        self.bounds = (xyz.min(axis=1), xyz.max(axis=1))
+        self.positions = xyz
+        return self
 
     def render_posterior(self, scratch, a, b, camera, w, h):
         """Splats::render_posterior (src/seg/paint.rs:22-47): the frame with

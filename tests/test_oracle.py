@@ -491,3 +491,94 @@ def test_tint_overwrites_foreground_dc_only():
         assert np.all(dc[c, fg] == np.float32(v))
         np.testing.assert_array_equal(dc[c, ~fg], sh[c * n:(c + 1) * n][~fg])
     np.testing.assert_array_equal(out[3 * n:], sh[3 * n:])
+
+
+# -------------------------------------------------------------------- PLY
+
+def _synthetic_ply(n, extra):
+    """ply.rs tests' synthetic_ply: the 14 base properties then extra, value
+    (i*31 + j)*0.25 - 8 at row i, column j."""
+    base = ["x", "y", "z", "rot_0", "rot_1", "rot_2", "rot_3", "scale_0", "scale_1", "scale_2",
+            "opacity", "f_dc_0", "f_dc_1", "f_dc_2"]
+    props = base + list(extra)
+    head = "ply\nformat binary_little_endian 1.0\nelement vertex %d\n" % n
+    head += "".join(f"property float {p}\n" for p in props) + "end_header\n"
+    st = len(props)
+    rows = np.array([[(i * 31 + j) * 0.25 - 8.0 for j in range(st)] for i in range(n)], "<f4")
+    return head.encode() + rows.tobytes(), rows
+
+
+def _both_parsers(data):
+    """(oracle result or error text, C header parser error text or layout)."""
+    from splatfield_amd import render as R
+    try:
+        o = O.parse_ply(data)
+    except ValueError as e:
+        o = str(e)
+    try:
+        c = R.parse_ply_header(data)
+    except ValueError as e:
+        c = str(e)
+    return o, c
+
+
+def test_ply_layout_kat():
+    """ply.rs test_parse_ply_layout: field-major planes, DC planes, f_rest
+    sorted by number and moved channel-major -> interleaved; the rotation
+    within an ulp of the f64 normalisation."""
+    data, rows = _synthetic_ply(3, ["f_rest_0", "f_rest_2", "f_rest_1"])
+    (attrs, sh), lay = _both_parsers(data)
+    assert attrs.size == 33 and sh.size == 18
+    assert (lay.n, lay.stride, lay.n_rest, lay.sh_planes) == (3, 17, 3, 6)
+    assert list(lay.rest)[:3] == [14, 16, 15]
+    assert lay.data_offset == data.index(b"end_header") + 11
+    a = attrs.reshape(11, 3)
+    for k, col in ((0, 0), (1, 1), (2, 2), (7, 7), (8, 8), (9, 9), (10, 10)):
+        np.testing.assert_array_equal(a[k], rows[:, col])
+    s = sh.reshape(6, 3)
+    for c in range(3):
+        np.testing.assert_array_equal(s[c], rows[:, 11 + c])
+    np.testing.assert_array_equal(s[3], rows[:, 14])  # f_rest_0 -> coefficient 1, red
+    np.testing.assert_array_equal(s[4], rows[:, 16])  # f_rest_1 -> green
+    np.testing.assert_array_equal(s[5], rows[:, 15])  # f_rest_2 -> blue
+    q = rows[:, [3, 4, 5, 6]].astype(np.float64)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    assert np.max(ulp_dist(a[3:7].T, q.astype(np.float32))) <= 1
+
+
+@pytest.mark.parametrize("case,msg", [
+    ("missing", "Missing property: f_dc_2"), ("empty", "PLY contains no vertices"),
+    ("ascii", "Unsupported PLY format: ascii"),
+    ("double", "Unsupported property type double: f_rest_0"),
+    ("short", "failed to read vertex 2 of 3")])
+def test_ply_rejections(case, msg):
+    """ply.rs tests: missing property, empty, other formats, non-float
+    properties; and a file cut short (read_exact).  Oracle and the C header
+    parser give the same message."""
+    data, _ = _synthetic_ply(3, [])
+    if case == "missing":
+        data = data.replace(b"property float f_dc_2\n", b"")
+    elif case == "empty":
+        data = data.replace(b"element vertex 3", b"element vertex 0")
+    elif case == "ascii":
+        data = data.replace(b"binary_little_endian", b"ascii")
+    elif case == "double":
+        data = data.replace(b"end_header", b"property double f_rest_0\nend_header")
+    else:
+        data = data[:-10]
+    o, c = _both_parsers(data)
+    assert o == msg and c == msg, (o, c)
+
+
+@pytest.mark.parametrize("k_per_ch", [1, 3, 16])
+def test_ply_write_parse_roundtrip(k_per_ch):
+    """ply.rs test_write_ply_roundtrips_parse: identity rotations, every
+    plane back exactly."""
+    n = 3
+    attrs = np.array([((i * 7 % 23) - 11.0) / 8.0 for i in range(11 * n)], np.float32)
+    attrs[3 * n:4 * n] = 1.0
+    attrs[4 * n:7 * n] = 0.0
+    sh = (np.arange(n * k_per_ch * 3, dtype=np.float32) * 0.25 - 9.0)
+    a2, s2 = O.parse_ply(O.write_ply(attrs, sh, n))
+    np.testing.assert_array_equal(a2, attrs)
+    np.testing.assert_array_equal(s2, sh)
