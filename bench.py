@@ -247,6 +247,36 @@ def bench_paint(ctx, splats, scratch, cam, w, h, n, reps=20):
     return out
 
 
+def bench_ply(ctx, attrs, sh, n, reps=10):
+    """PLY ingestion (src/ply.rs:9-112) of the frame's scene: the device
+    transpose of the file's vertex rows (inputs resident), its algorithmic
+    bytes the rows read plus the planes written."""
+    import ctypes as C
+    from splatfield_amd import render as R
+    from splatfield_amd import scene as S
+    from splatfield_amd._lib import lib
+    from splatfield_amd.runtime import GpuTensor
+    data = S.write_ply(attrs, sh, n)
+    lay = R.parse_ply_header(data)
+    rows = np.frombuffer(data, np.uint8, lay.n * lay.stride * 4, lay.data_offset)
+    d_rows = GpuTensor.from_numpy(ctx, rows)
+    d_attrs = GpuTensor.empty(ctx, 11 * n, np.float32)
+    d_sh = GpuTensor.empty(ctx, lay.sh_planes * n, np.float32)
+
+    def dec():
+        ctx.check(lib().sfk_ply_decode(ctx.ptr, C.byref(lay), d_rows.ptr, d_attrs.ptr,
+                                       d_sh.ptr), "ply_decode")
+    dec()
+    ctx.timer_start()
+    for _ in range(reps):
+        dec()
+    ms = ctx.timer_stop_ms() / reps
+    algo = n * lay.stride * 4 + n * (11 + lay.sh_planes) * 4
+    return {"splats": n, "row_floats": int(lay.stride), "ms": round(ms, 4),
+            "achieved_gbs": round(algo / (ms * 1e-3) / 1e9, 1),
+            "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def kernel_bytes(n, k, vis, ni, px, pairs):
     """Algorithmic HBM bytes per launch of each step of the frame: the reads
     and writes the implemented pipeline must make (not counting cache
@@ -383,6 +413,8 @@ def main():
         extra["eig_round_c4"] = bench_c4(ctx, args)
         stage("paint: posterior-mean render at the frame's scene")
         extra["paint"] = bench_paint(ctx, splats, scratch, cam, w, h, n)
+        stage("ply: decode the frame's scene")
+        extra["ply_decode"] = bench_ply(ctx, attrs, sh, n)
 
     cpu = None
     if not args.no_cpu:

@@ -37,11 +37,17 @@ struct PlyArgs {
 enum { kX, kY, kZ, kS0, kS1, kS2, kOp, kR0, kR1, kR2, kR3, kDc0, kDc1, kDc2 };
 
 __global__ __launch_bounds__(256) void k_ply_decode(PlyArgs a) {
-    __shared__ float rows[kPlyRows * SFK_PLY_MAX_STRIDE];
+    __shared__ __align__(16) float rows[kPlyRows * SFK_PLY_MAX_STRIDE];
     const uint64_t v0 = (uint64_t)blockIdx.x * kPlyRows;
     const uint32_t nv = (uint32_t)min((uint64_t)kPlyRows, a.n - v0);
+    // The block's rows start 64 * stride floats apart: 16-byte aligned, so
+    // they load as float4 (a ragged last block finishes with scalars).
     const float* src = a.rows + v0 * a.stride;
-    for (uint32_t e = threadIdx.x; e < nv * a.stride; e += 256) rows[e] = src[e];
+    const uint32_t total = nv * a.stride, n4 = total / 4;
+    const float4* src4 = reinterpret_cast<const float4*>(src);
+    float4* rows4 = reinterpret_cast<float4*>(rows);
+    for (uint32_t e = threadIdx.x; e < n4; e += 256) rows4[e] = src4[e];
+    for (uint32_t e = 4 * n4 + threadIdx.x; e < total; e += 256) rows[e] = src[e];
     __syncthreads();
     const uint32_t per_ch = a.n_rest / 3;
     const uint32_t planes = SFK_ATTR_PLANES + a.sh_planes;

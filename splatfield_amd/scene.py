@@ -57,3 +57,30 @@ def fibonacci_directions(count):
         out.append((np.float32(r * np.float32(math.cos(theta))),
                     np.float32(r * np.float32(math.sin(theta))), np.float32(z)))
     return np.array(out, np.float32)
+
+
+def write_ply(attrs, sh, n):
+    """CpuSplats::write_ply (src/ply.rs:114-170): binary little-endian 3DGS
+    PLY with the canonical INRIA property order (x y z, zero normals, f_dc,
+    f_rest channel-major, opacity, scales, rot_0 = w then x y z), the exact
+    inverse of the device parse for normalised rotations."""
+    attrs = np.ascontiguousarray(attrs, np.float32).reshape(11, n)
+    sh = np.ascontiguousarray(sh, np.float32).reshape(-1, n)
+    if n == 0 or sh.shape[0] % 3:
+        raise ValueError("no splats to write")
+    per = sh.shape[0] // 3 - 1
+    names = ["x", "y", "z", "nx", "ny", "nz", "f_dc_0", "f_dc_1", "f_dc_2"]
+    names += ["f_rest_%d" % j for j in range(3 * per)]
+    names += ["opacity", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3"]
+    header = ("ply\nformat binary_little_endian 1.0\nelement vertex %d\n" % n +
+              "".join("property float %s\n" % p for p in names) + "end_header\n")
+    out = np.zeros((n, len(names)), "<f4")
+    out[:, 0:3] = attrs[0:3].T
+    out[:, 6:9] = sh[0:3].T
+    for c in range(3):  # file column 9 + c*per + j holds coefficient j+1, channel c
+        out[:, 9 + c * per:9 + (c + 1) * per] = sh[3 + c::3][:per].T
+    t = 9 + 3 * per
+    out[:, t] = attrs[10]
+    out[:, t + 1:t + 4] = attrs[7:10].T
+    out[:, t + 4:t + 8] = attrs[3:7].T
+    return header.encode() + out.tobytes()

@@ -582,3 +582,18 @@ def test_ply_write_parse_roundtrip(k_per_ch):
     a2, s2 = O.parse_ply(O.write_ply(attrs, sh, n))
     np.testing.assert_array_equal(a2, attrs)
     np.testing.assert_array_equal(s2, sh)
+
+
+@pytest.mark.parametrize("deg", [0, 1, 3])
+def test_product_ply_writer_matches_oracle(deg):
+    """scene.write_ply (the library's CpuSplats::write_ply) writes the
+    oracle writer's bytes, and the oracle parses them back exactly."""
+    from splatfield_amd import scene as S
+    n = 257
+    attrs, sh, _ = S.synthetic_scene(n, deg, seed=9)
+    data = S.write_ply(attrs, sh, n)
+    assert data == O.write_ply(attrs, sh, n)
+    a2, s2 = O.parse_ply(data)
+    np.testing.assert_array_equal(s2, sh)
+    np.testing.assert_array_equal(a2[:3 * n], attrs[:3 * n])
+    np.testing.assert_array_equal(a2[7 * n:], attrs[7 * n:])
