@@ -311,6 +311,33 @@ int sfk_ply_decode(sfk_ctx* ctx, const sfk_ply_layout* layout, const float* rows
  * header on the host, rows decoded on the device.  SH degree 0-3. */
 int sfk_splats_from_ply(sfk_ctx* ctx, const void* bytes, size_t len, sfk_splats** out);
 
+/* SOG archives (src/sog.rs:97-198).  The caller unzips meta.json and decodes
+ * the WebP sheets to RGBA8 (one 4-byte texel per splat, row-major, at least
+ * n texels each; the centroid sheet any size); the device decodes the
+ * splats. */
+typedef struct {
+    uint64_t n;             /* meta.count */
+    uint64_t centroid_px;   /* texels in the shN centroid sheet (0 without shN) */
+    float mins[3], maxs[3]; /* meta.means */
+    uint32_t sh_per_ch;     /* 1, or 4 / 9 / 16 for shN.bands 1 / 2 / 3 */
+    uint32_t centroid_w;    /* centroid sheet width in texels */
+    uint32_t palette_count; /* centroid_px / (sh_per_ch - 1) */
+} sfk_sog_meta;
+/* The decode loop of parse_sog (sog.rs:108-195) into field-major planes
+ * (11 x n attrs, 3 sh_per_ch x n sh).  codebooks: 768 device floats (scales,
+ * sh0, shN, 256 each).  status (2 device words): the first splat with a
+ * rotation tag below 252, and the first whose shN label is >= palette_count
+ * or whose palette leaves the sheet; 0xffffffff when none (the reference
+ * fails the whole parse on either). */
+int sfk_sog_decode(sfk_ctx* ctx, const sfk_sog_meta* meta, const uint8_t* means_lo,
+                   const uint8_t* means_hi, const uint8_t* scales, const uint8_t* quats,
+                   const uint8_t* sh0, const uint8_t* labels, const uint8_t* centroids,
+                   const float* codebooks, float* attrs, float* sh, uint32_t* status);
+/* Splats::new over planes already in device memory (copied; the caller's
+ * buffers stay the caller's). */
+int sfk_splats_create_device(sfk_ctx* ctx, const float* d_attrs, const float* d_sh, uint32_t n,
+                             uint32_t sh_per_ch, sfk_splats** out);
+
 /* ------------------------------------------------ paint (src/seg/paint.rs) */
 
 /* Splats::render_posterior (src/seg/paint.rs:22-47): render_with with every

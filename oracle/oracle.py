@@ -318,6 +318,65 @@ def parse_ply(data):
     return attrs.reshape(-1), sh.reshape(-1)
 
 
+def sog_unpack_quat(px, py, pz, tag):
+    """unpack_quat (src/sog.rs:80-91) in f32: [w, x, y, z]."""
+    f = np.float32
+    u = lambda v: (f(v) / f(255.0) * f(2.0) - f(1.0)) / f(1.41421356)
+    a, b, c = u(px), u(py), u(pz)
+    d = np.sqrt(np.maximum(f(1.0) - a * a - b * b - c * c, f(0.0)))
+    return {252: (d, a, b, c), 253: (a, d, b, c), 254: (a, b, d, c)}.get(int(tag), (a, b, c, d))
+
+
+def sog_palette_offset(label, cw, cpc):
+    """palette_offset (src/sog.rs:201-205): (column in texels, row)."""
+    per_row = cw // cpc
+    return label % per_row * cpc, label // per_row
+
+
+def sog_decode(info, sheets, cw, codebooks):
+    """The decode loop of parse_sog (src/sog.rs:108-195) in numpy over RGBA8
+    sheets; exp_m1 and ln evaluated in f64 and rounded to f32 (the
+    reference's f32 libm calls, correctly rounded but for rare ties).
+    Returns (attrs, sh) or raises ValueError with the reference's message."""
+    f = np.float32
+    n, per = info["n"], info["sh_per_ch"]
+    px = {k: v[:4 * n].reshape(n, 4) for k, v in sheets.items() if k != "centroids"}
+    mins = np.asarray(info["mins"], f)
+    spans = np.asarray(info["maxs"], f) - mins
+    cb = codebooks.astype(f)
+    attrs = np.zeros((11, n), f)
+    for k in range(3):
+        t = (px["lo"][:, k].astype(np.uint32) | px["hi"][:, k].astype(np.uint32) << 8)
+        v = mins[k] + spans[k] * (t.astype(f) / f(65535.0))
+        sgn = np.where(np.signbit(v), f(-1.0), f(1.0))
+        attrs[k] = sgn * np.expm1(np.abs(v).astype(np.float64)).astype(f)
+        attrs[7 + k] = cb[px["scales"][:, k]]
+    tags = px["quats"][:, 3]
+    if np.any(tags < 252):
+        raise ValueError(f"quats: invalid rotation tag {tags[np.argmax(tags < 252)]}")
+    for i in range(n):
+        attrs[3:7, i] = sog_unpack_quat(*px["quats"][i])
+    sh = np.zeros((3 * per, n), f)
+    for k in range(3):
+        sh[k] = cb[256 + px["sh0"][:, k].astype(np.int64)]
+    e = np.clip(px["sh0"][:, 3].astype(f) / f(255.0), f(1e-6), f(1.0) - f(1e-6))
+    attrs[10] = np.log((e / (f(1.0) - e)).astype(np.float64)).astype(f)
+    if per > 1:
+        cpc = per - 1
+        cen = sheets["centroids"]
+        pcount = cen.size // 4 // cpc
+        for i in range(n):
+            label = int(px["labels"][i, 0]) | int(px["labels"][i, 1]) << 8
+            if label >= pcount:
+                raise ValueError(f"shN label {label} >= palette size {pcount}")
+            bx, by = sog_palette_offset(label, cw, cpc)
+            for j in range(cpc):
+                p = (by * cw + bx + j) * 4
+                for k in range(3):
+                    sh[(j + 1) * 3 + k, i] = cb[512 + int(cen[p + k])]
+    return attrs.reshape(-1), sh.reshape(-1)
+
+
 def write_ply(attrs, sh, n):
     """CpuSplats::write_ply (src/ply.rs:114-170): canonical INRIA property
     order, zero normals, f_rest channel-major."""

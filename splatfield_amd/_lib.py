@@ -39,6 +39,12 @@ class PlyLayout(C.Structure):
                 ("base", C.c_int32 * 14), ("rest", C.c_int32 * 45)]
 
 
+class SogMeta(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("centroid_px", C.c_uint64), ("mins", f32 * 3),
+                ("maxs", f32 * 3), ("sh_per_ch", u32), ("centroid_w", u32),
+                ("palette_count", u32)]
+
+
 class RenderBuffers(C.Structure):
     _fields_ = [
         ("depth_keys", P), ("depth_order", P), ("projected", P), ("tile_counts", P),
@@ -121,6 +127,8 @@ SIGNATURES = {
     "sfk_ply_parse_header": (i32, [P, C.c_size_t, C.POINTER(PlyLayout), C.c_char_p, i32]),
     "sfk_ply_decode": (i32, [P, C.POINTER(PlyLayout), P, P, P]),
     "sfk_splats_from_ply": (i32, [P, P, C.c_size_t, PP]),
+    "sfk_sog_decode": (i32, [P, C.POINTER(SogMeta), P, P, P, P, P, P, P, P, P, P, P]),
+    "sfk_splats_create_device": (i32, [P, P, P, u32, u32, PP]),
     "sfk_save_colors": (i32, [P, P, P]),
     "sfk_restore_colors": (i32, [P, P, P]),
     "sfk_project": (i32, [P, C.POINTER(View), P, P, u32, u32, u32, P, P, P, P, P]),

@@ -67,6 +67,47 @@ extern "C" int sfk_splats_create(sfk_ctx* c, const float* attrs, const float* sh
     return SFK_OK;
 }
 
+extern "C" int sfk_splats_create_device(sfk_ctx* c, const float* d_attrs, const float* d_sh,
+                                        uint32_t n, uint32_t sh_per_ch, sfk_splats** out) {
+    if (!c || !out) return SFK_E_ARG;
+    *out = nullptr;
+    if (n == 0) return fail(c, SFK_E_ARG, "Splats::new: zero splats");
+    if (!d_attrs || !d_sh || !(sh_per_ch == 1 || sh_per_ch == 4 || sh_per_ch == 9 || sh_per_ch == 16))
+        return fail(c, SFK_E_ARG, "splats: bad attributes/sh (sh_per_ch=%u)", sh_per_ch);
+    auto* s = new (std::nothrow) sfk_splats();
+    if (!s) return SFK_E_NOMEM;
+    s->ctx = c;
+    s->n = n;
+    s->sh_per_ch = sh_per_ch;
+    const size_t ab = (size_t)n * SFK_ATTR_PLANES * 4, sb = (size_t)n * sh_per_ch * 3 * 4;
+    std::vector<float> xyz((size_t)3 * n);
+    hipError_t e = hipMalloc(&s->attrs, ab);
+    if (e == hipSuccess) e = hipMalloc(&s->sh, sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->attrs, d_attrs, ab, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->sh, d_sh, sb, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(xyz.data(), d_attrs, xyz.size() * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        if (s->attrs) hipFree(s->attrs);
+        if (s->sh) hipFree(s->sh);
+        delete s;
+        return hip_fail(c, e, "splats copy");
+    }
+    for (int k = 0; k < 3; ++k) {
+        s->bmin[k] = 3.40282347e38f;
+        s->bmax[k] = -3.40282347e38f;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const float v = xyz[(size_t)k * n + i];
+            s->bmin[k] = std::min(s->bmin[k], v);
+            s->bmax[k] = std::max(s->bmax[k], v);
+        }
+    *out = s;
+    return SFK_OK;
+}
+
 extern "C" int sfk_splats_from_ply(sfk_ctx* c, const void* bytes, size_t len,
                                    sfk_splats** out) {
     if (!c || !bytes || !out) return SFK_E_ARG;

@@ -111,6 +111,49 @@ class Splats:
         self.positions = xyz
         return self
 
+    @classmethod
+    def from_sog(cls, ctx, data):
+        """parse_sog (src/sog.rs:97-198) + Splats::new: sheets unzipped and
+        WebP-decoded on the host, every splat decoded on the device."""
+        from . import scene as S
+        info, sheets, cw, cbs = S.read_sog(data)
+        n, per = info["n"], info["sh_per_ch"]
+        m = _lib.SogMeta()
+        m.n = n
+        m.mins[:] = info["mins"]
+        m.maxs[:] = info["maxs"]
+        m.sh_per_ch = per
+        if per > 1:
+            m.centroid_w = cw
+            m.centroid_px = sheets["centroids"].size // 4
+            m.palette_count = m.centroid_px // (per - 1)
+        dev = {k: GpuTensor.from_numpy(ctx, v) for k, v in sheets.items()}
+        d_cb = GpuTensor.from_numpy(ctx, cbs)
+        d_attrs = GpuTensor.empty(ctx, 11 * n, np.float32)
+        d_sh = GpuTensor.empty(ctx, 3 * per * n, np.float32)
+        status = GpuTensor.empty(ctx, 2, np.uint32)
+        ptr = lambda k: dev[k].ptr if k in dev else None
+        ctx.check(lib().sfk_sog_decode(ctx.ptr, C.byref(m), ptr("lo"), ptr("hi"), ptr("scales"),
+                                       ptr("quats"), ptr("sh0"), ptr("labels"),
+                                       ptr("centroids"), d_cb.ptr, d_attrs.ptr, d_sh.ptr,
+                                       status.ptr), "parse_sog")
+        bad_tag, bad_label = (int(v) for v in status.read())
+        if bad_tag != 0xFFFFFFFF:
+            raise ValueError(f"quats: invalid rotation tag {sheets['quats'][4 * bad_tag + 3]}")
+        if bad_label != 0xFFFFFFFF:
+            lb = sheets["labels"]
+            label = int(lb[4 * bad_label]) | int(lb[4 * bad_label + 1]) << 8
+            raise ValueError(f"shN label {label} >= palette size {m.palette_count}")
+        out = C.c_void_p()
+        ctx.check(lib().sfk_splats_create_device(ctx.ptr, d_attrs.ptr, d_sh.ptr, n, per,
+                                                 C.byref(out)), "Splats::new")
+        self = cls.__new__(cls)
+        self.ctx, self.ptr, self.n, self.sh_per_ch = ctx, out.value, n, per
+        xyz = d_attrs.read()[:3 * n].reshape(3, n)
+        self.bounds = (xyz.min(axis=1), xyz.max(axis=1))
+        self.positions = xyz
+        return self
+
     def render_posterior(self, scratch, a, b, camera, w, h):
         """Splats::render_posterior (src/seg/paint.rs:22-47): the frame with
         every splat painted by its posterior mean a/(a+b) as grayscale; a

@@ -84,3 +84,74 @@ def write_ply(attrs, sh, n):
     out[:, t + 1:t + 4] = attrs[7:10].T
     out[:, t + 4:t + 8] = attrs[3:7].T
     return header.encode() + out.tobytes()
+
+
+def read_sog(data):
+    """The host half of parse_sog (src/sog.rs:44-64, 97-175): unzip, read
+    meta.json, decode the WebP sheets to RGBA8, with the reference's checks
+    and messages.  Returns (meta dict, sheets dict of flat uint8 RGBA
+    arrays, centroid sheet width, codebooks float32[768])."""
+    import io
+    import json
+    import zipfile
+    try:
+        z = zipfile.ZipFile(io.BytesIO(bytes(data)))
+    except zipfile.BadZipFile:
+        raise ValueError("not a SOG archive") from None
+    names = set(z.namelist())
+    if "meta.json" not in names:
+        raise ValueError("missing meta.json")
+    try:
+        meta = json.loads(z.read("meta.json"))
+        n = int(meta["count"])
+        means, scales, quats, sh0 = meta["means"], meta["scales"], meta["quats"], meta["sh0"]
+        shn = meta.get("shN")
+        files = dict(lo=means["files"][0], hi=means["files"][1], scales=scales["files"][0],
+                     quats=quats["files"][0], sh0=sh0["files"][0])
+        if shn is not None:
+            files.update(centroids=shn["files"][0], labels=shn["files"][1])
+    except (ValueError, KeyError, TypeError, IndexError):
+        raise ValueError("invalid meta.json") from None
+    if n == 0:
+        raise ValueError("SOG contains no splats")
+
+    def rgba(name, min_px):
+        if name not in names:
+            raise ValueError(f"missing {name}")
+        from PIL import Image
+        try:
+            img = Image.open(io.BytesIO(z.read(name)), formats=["WEBP"]).convert("RGBA")
+        except Exception:
+            raise ValueError(f"decode {name}") from None
+        w, h = img.size
+        if w * h < min_px:
+            raise ValueError(f"{name}: {w}x{h} < {min_px} pixels")
+        return np.frombuffer(img.tobytes(), np.uint8).copy(), w
+
+    def codebook(cb, what):
+        if len(cb) != 256:
+            raise ValueError(f"{what} codebook: 256 floats expected, got {len(cb)}")
+        return np.asarray(cb, np.float32)
+
+    sheets = {}
+    for k in ("lo", "hi", "scales"):
+        sheets[k], _ = rgba(files[k], n)
+    cbs = [codebook(scales["codebook"], "scales")]
+    sheets["quats"], _ = rgba(files["quats"], n)
+    sh_per_ch, cw = 1, 0
+    if shn is not None:
+        bands = int(shn["bands"])
+        if bands not in (1, 2, 3):
+            raise ValueError(f"shN.bands {bands} outside 1..=3")
+        sh_per_ch = (bands + 1) ** 2
+    sheets["sh0"], _ = rgba(files["sh0"], n)
+    cbs.append(codebook(sh0["codebook"], "sh0"))
+    if shn is not None:
+        sheets["centroids"], cw = rgba(files["centroids"], 1)
+        sheets["labels"], _ = rgba(files["labels"], n)
+        cbs.append(codebook(shn["codebook"], "shN"))
+    else:
+        cbs.append(np.zeros(256, np.float32))
+    info = dict(n=n, mins=[float(v) for v in means["mins"]],
+                maxs=[float(v) for v in means["maxs"]], sh_per_ch=sh_per_ch)
+    return info, sheets, cw, np.concatenate(cbs)

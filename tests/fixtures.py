@@ -45,3 +45,55 @@ def jitter(i):
 # produced by the reference's kernel on its CI device.
 GOLDEN_EPS = [411225574, 64983936, 17882252, 5200864, 773093, 333408, 108187, 61572, 12526,
               2462, 1466, 1461, 10] + [0] * 27
+
+
+def make_sog(n, bands, seed, width=64, bad_tag_at=None, bad_label_at=None):
+    """A SOG archive (meta.json + lossless WebP sheets, the layout
+    src/sog.rs:8-42 reads) of n random splats; sheets are width texels wide
+    with rows padded past n.  Returns (zip bytes, the RGBA sheets)."""
+    import io
+    import json
+    import zipfile
+    from PIL import Image
+    rng = np.random.default_rng(seed)
+    rows = (n + width - 1) // width
+    px = width * rows
+
+    def sheet():
+        return rng.integers(0, 256, (px, 4), dtype=np.uint8)
+    sheets = {k: sheet() for k in ("lo", "hi", "scales", "quats", "sh0")}
+    sheets["quats"][:, 3] = rng.integers(252, 256, px)
+    if bad_tag_at is not None:
+        sheets["quats"][bad_tag_at, 3] = 17
+    cpc = (bands + 1) ** 2 - 1
+    meta = {"count": n,
+            "means": {"mins": [-1.5, -0.5, 0.25], "maxs": [1.25, 2.0, 3.0],
+                      "files": ["means_l.webp", "means_u.webp"]},
+            "scales": {"codebook": rng.normal(-4, 1, 256).tolist(), "files": ["scales.webp"]},
+            "quats": {"files": ["quats.webp"]},
+            "sh0": {"codebook": rng.normal(0, 0.5, 256).tolist(), "files": ["sh0.webp"]}}
+    if bands:
+        cw = 16 * cpc + 3  # a ragged row: the last 3 texels hold no palette
+        crow = 7
+        palettes = (cw // cpc) * crow
+        sheets["centroids"] = rng.integers(0, 256, (cw * crow, 4), dtype=np.uint8)
+        lab = rng.integers(0, palettes, px)
+        if bad_label_at is not None:
+            lab[bad_label_at] = palettes + 5
+        sheets["labels"] = np.stack([lab & 255, lab >> 8, np.zeros(px), np.full(px, 255)],
+                                    1).astype(np.uint8)
+        meta["shN"] = {"bands": bands, "codebook": rng.normal(0, 0.1, 256).tolist(),
+                       "files": ["shN_centroids.webp", "shN_labels.webp"]}
+    names = {"lo": "means_l.webp", "hi": "means_u.webp", "scales": "scales.webp",
+             "quats": "quats.webp", "sh0": "sh0.webp", "centroids": "shN_centroids.webp",
+             "labels": "shN_labels.webp"}
+    buf = io.BytesIO()
+    with zipfile.ZipFile(buf, "w") as z:
+        z.writestr("meta.json", json.dumps(meta))
+        for k, v in sheets.items():
+            w = cw if k == "centroids" else width
+            img = Image.frombytes("RGBA", (w, v.shape[0] // w), v.tobytes())
+            out = io.BytesIO()
+            img.save(out, "WEBP", lossless=True, exact=True)
+            z.writestr(names[k], out.getvalue())
+    return buf.getvalue(), {k: v.reshape(-1) for k, v in sheets.items()}

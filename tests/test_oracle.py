@@ -597,3 +597,64 @@ def test_product_ply_writer_matches_oracle(deg):
     np.testing.assert_array_equal(s2, sh)
     np.testing.assert_array_equal(a2[:3 * n], attrs[:3 * n])
     np.testing.assert_array_equal(a2[7 * n:], attrs[7 * n:])
+
+
+# -------------------------------------------------------------------- SOG
+
+def test_sog_reference_unit_tests():
+    """sog.rs tests: palette_offset row width, unit quaternions for every
+    tag, inv_log / logit round trip."""
+    assert O.sog_palette_offset(20, 128, 8) == (32, 1)
+    for tag in range(252, 256):
+        q = np.array(O.sog_unpack_quat(200, 10, 77, tag), np.float32)
+        assert abs(float((q * q).sum()) - 1.0) < 1e-5
+        assert q[tag - 252] == q.max() or tag == 255  # the rebuilt component sits at tag - 252
+    info = dict(n=5, mins=[-3, -3, -3], maxs=[3, 3, 3], sh_per_ch=1)
+    for v in (-3.0, -0.5, 0.0, 0.5, 3.0):
+        y = np.float32(1.0) / (np.float32(1.0) + np.exp(np.float32(-v)))
+        e = np.clip(np.float32(y), np.float32(1e-6), np.float32(1) - np.float32(1e-6))
+        assert abs(float(np.log(np.float64(e / (np.float32(1) - e)))) - v) < 1e-4
+
+
+def test_sog_host_reader_and_oracle():
+    """WebP sheets survive the archive exactly; the oracle decode's means hit
+    the closed forms at q16 = 0 and 65535, and every rejection carries the
+    reference's message."""
+    from splatfield_amd import scene as S
+    from .fixtures import make_sog
+    data, sheets = make_sog(300, 2, seed=1)
+    info, got, cw, cbs = S.read_sog(data)
+    for k, v in sheets.items():
+        np.testing.assert_array_equal(got[k], v)
+    assert info["sh_per_ch"] == 9 and cw == 16 * 8 + 3 and cbs.size == 768
+    got["lo"][:4] = 0; got["hi"][:4] = 0          # splat 0: t = 0 on x, y, z
+    got["lo"][4:8] = 255; got["hi"][4:8] = 255    # splat 1: t = 1
+    attrs, sh = O.sog_decode(info, got, cw, cbs)
+    a = attrs.reshape(11, 300)
+    for k in range(3):
+        assert a[k, 0] == np.float32(np.sign(info["mins"][k]) * np.expm1(abs(info["mins"][k])))
+        assert a[k, 1] == np.float32(np.sign(info["maxs"][k]) * np.expm1(abs(info["maxs"][k])))
+    q = a[3:7].astype(np.float64)
+    slot = got["quats"].reshape(-1, 4)[:300, 3].astype(int) - 252
+    rebuilt = q[slot, np.arange(300)]
+    norm = (q ** 2).sum(0)
+    # Unit norm wherever the three stored components fit in the unit ball;
+    # random texels past it rebuild the fourth as 0 (the max(0) clamp).
+    inside = rebuilt > 0
+    assert inside.mean() > 0.8
+    np.testing.assert_allclose(norm[inside], 1.0, atol=1e-5)
+    assert np.all(rebuilt[~inside] == 0) and np.all(norm[~inside] >= 1.0 - 1e-6)
+    assert np.all(np.isfinite(sh))
+    bad, _ = make_sog(64, 1, seed=2, bad_tag_at=9)
+    with pytest.raises(ValueError, match="quats: invalid rotation tag 17"):
+        O.sog_decode(*S.read_sog(bad))
+    with pytest.raises(ValueError, match="not a SOG archive"):
+        S.read_sog(b"not a zip")
+    import io, json, zipfile
+    buf = io.BytesIO()
+    with zipfile.ZipFile(buf, "w") as z:
+        z.writestr("meta.json", json.dumps({"count": 0, "means": {"files": ["a", "b"]},
+                                            "scales": {"files": ["c"]}, "quats": {"files": ["d"]},
+                                            "sh0": {"files": ["e"]}}))
+    with pytest.raises(ValueError, match="SOG contains no splats"):
+        S.read_sog(buf.getvalue())
