@@ -277,6 +277,51 @@ def bench_ply(ctx, attrs, sh, n, reps=10):
             "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def bench_sog(ctx, n, reps=10):
+    """SOG decode (src/sog.rs:108-195) of n SH3 splats from device-resident
+    RGBA8 sheets (synthetic texels, 1024-texel rows, a 4096-palette centroid
+    sheet): 24 bytes of texels read and 236 bytes of planes written per
+    splat (palette gathers hit cache)."""
+    import ctypes as C
+    from splatfield_amd import _lib
+    from splatfield_amd._lib import lib
+    from splatfield_amd.runtime import GpuTensor
+    rng = np.random.default_rng(3)
+    sheets = {k: rng.integers(0, 256, 4 * n, dtype=np.uint8)
+              for k in ("lo", "hi", "scales", "quats", "sh0", "labels")}
+    sheets["quats"][3::4] = rng.integers(252, 256, n)
+    sheets["labels"][1::4] = rng.integers(0, 16, n)  # labels < 4096
+    cw, cpc = 1020, 15
+    sheets["centroids"] = rng.integers(0, 256, 4 * cw * 61, dtype=np.uint8)
+    dev = {k: GpuTensor.from_numpy(ctx, v) for k, v in sheets.items()}
+    cb = GpuTensor.from_numpy(ctx, rng.normal(0, 1, 768).astype(np.float32))
+    m = _lib.SogMeta()
+    m.n, m.sh_per_ch, m.centroid_w = n, 16, cw
+    m.centroid_px = cw * 61
+    m.palette_count = m.centroid_px // cpc
+    m.mins[:] = [-2.0, -2.0, -2.0]
+    m.maxs[:] = [2.0, 2.0, 2.0]
+    attrs = GpuTensor.empty(ctx, 11 * n, np.float32)
+    sh = GpuTensor.empty(ctx, 48 * n, np.float32)
+    status = GpuTensor.empty(ctx, 2, np.uint32)
+
+    def dec():
+        ctx.check(lib().sfk_sog_decode(ctx.ptr, C.byref(m), dev["lo"].ptr, dev["hi"].ptr,
+                                       dev["scales"].ptr, dev["quats"].ptr, dev["sh0"].ptr,
+                                       dev["labels"].ptr, dev["centroids"].ptr, cb.ptr,
+                                       attrs.ptr, sh.ptr, status.ptr), "sog_decode")
+    dec()
+    ctx.timer_start()
+    for _ in range(reps):
+        dec()
+    ms = ctx.timer_stop_ms() / reps
+    assert list(status.read()) == [0xFFFFFFFF, 0xFFFFFFFF]
+    algo = n * (24 + 44 + 192)
+    return {"splats": n, "ms": round(ms, 4),
+            "achieved_gbs": round(algo / (ms * 1e-3) / 1e9, 1),
+            "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def kernel_bytes(n, k, vis, ni, px, pairs):
     """Algorithmic HBM bytes per launch of each step of the frame: the reads
     and writes the implemented pipeline must make (not counting cache
@@ -415,6 +460,8 @@ def main():
         extra["paint"] = bench_paint(ctx, splats, scratch, cam, w, h, n)
         stage("ply: decode the frame's scene")
         extra["ply_decode"] = bench_ply(ctx, attrs, sh, n)
+        stage("sog: decode 1M SH3 splats")
+        extra["sog_decode"] = bench_sog(ctx, n)
 
     cpu = None
     if not args.no_cpu:
