@@ -76,7 +76,7 @@ typedef struct {
     float rotation[4];
 } sfk_camera;
 
-/* Kernel view of a camera (CameraView, src/project.rs:121-157). */
+/* Kernel view of a camera (CameraView, src/project.rs:60-74). */
 typedef struct {
     float rot0[3], rot1[3], rot2[3]; /* rows of the world->camera rotation */
     float trans[3];
@@ -86,7 +86,7 @@ typedef struct {
     float tile_bounds[2];
 } sfk_view;
 
-/* CameraViewLaunch::for_camera (src/project.rs:137-157). */
+/* CameraViewLaunch::for_camera (src/project.rs:76-96). */
 int sfk_camera_view(const sfk_camera* cam, uint32_t w, uint32_t h, uint32_t tile_w,
                     sfk_view* out);
 /* Camera::look_at rotation (src/camera.rs:57-63). */
@@ -259,14 +259,14 @@ int sfk_accumulate_prepared(sfk_ctx* ctx, sfk_render_scratch* scratch,
                             uint32_t words_per_row, uint32_t* resp, uint32_t* fg, uint32_t* bg,
                             float* scale_out);
 
-/* BetaState::eig (src/seg/beta.rs:454-482) with the entropy in f64 on the
+/* BetaState::eig (src/seg/beta.rs:240-268) with the entropy in f64 on the
  * device instead of the 445x445 table.  SYNC (one f64 readback). */
 int sfk_eig(sfk_ctx* ctx, const float* a, const float* b, const uint32_t* resp, uint32_t n,
             float scale, double* out);
 /* The same, leaving the f64 result in device memory (no sync). */
 int sfk_eig_device(sfk_ctx* ctx, const float* a, const float* b, const uint32_t* resp,
                    uint32_t n, float scale, double* d_out);
-/* BetaState::update (src/seg/beta.rs:484-500): a += fg/scale, b += bg/scale. */
+/* BetaState::update (src/seg/beta.rs:270-286): a += fg/scale, b += bg/scale. */
 int sfk_beta_update(sfk_ctx* ctx, float* a, float* b, const uint32_t* fg, const uint32_t* bg,
                     uint32_t n, float scale);
 /* views::localize (src/seg/views.rs:32-67) on the device: centre = sum(m p) /
@@ -427,7 +427,7 @@ int sfk_render_backward_l2(sfk_ctx* ctx, const sfk_splats* splats, sfk_render_sc
 
 /* ------------------------------------------------------ kernel-level calls */
 
-/* project_splats (src/project.rs:295-388) over all n splats.  Writes per
+/* project_splats (src/project.rs:234-327) over all n splats.  Writes per
  * splat i: depth_keys[i], projected row i, tile_counts[i], bbox[2i..2i+1];
  * adds (isects, visible) of each workgroup to one of 64 slot pairs:
  * counters[2s] += isects, counters[2s+1] += visible (128 u32, not cleared;

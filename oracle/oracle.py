@@ -121,7 +121,7 @@ def fractional_bits(w, h):
 
 
 def project(view, attrs, sh, n, sh_per_ch, tile_w=16, nthreads=1):
-    """project_splats (src/project.rs:295-388) -> dict of per-splat outputs."""
+    """project_splats (src/project.rs:234-388) -> dict of per-splat outputs."""
     attrs = _f32(attrs); sh = _f32(sh)
     out = dict(
         visible=np.zeros(n, np.uint8), depth_keys=np.zeros(n, np.uint32),

@@ -9,7 +9,7 @@
  *     correctly rounded; inverseSqrt(x) -> 1/sqrt(x); recip(x) -> 1/x.
  *   - visible splats are ranked by (depth key, splat index): the stable depth
  *     sort of the reference fed with index-ordered compaction, one of the
- *     orders the reference itself can produce (src/project.rs:109-118).
+ *     orders the reference itself can produce (src/project.rs:50-54, 281-283).
  */
 #include "sfo.h"
 #include "sfo_math.h"
@@ -20,7 +20,7 @@
 
 #define SFO_PROJ_FLOATS 9 /* src/layout.rs:60-61 */
 
-/* src/project.rs:74 — 10/255 evaluated in f32. */
+/* src/project.rs:13 — 10/255 evaluated in f32. */
 static const float SFO_ALPHA_CUTOFF = 10.0f / 255.0f;
 
 /* ---------------------------------------------------------------- camera */
@@ -42,7 +42,7 @@ static float dot3(const float a[3], const float b[3]) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-/* CameraView::for_camera (src/project.rs:137-157) with w2c from
+/* CameraView::for_camera (src/project.rs:76-96) with w2c from
  * src/camera.rs:42-44 taken as the exact rigid inverse (R^T, -R^T p). */
 void sfo_camera_view(const float fov[2], const float pos[3], const float q[4], uint32_t w,
                      uint32_t h, uint32_t tile_w, sfo_view* out) {
@@ -104,7 +104,7 @@ uint32_t sfo_fractional_bits(uint32_t w, uint32_t h) {
 
 /* --------------------------------------------------------------- project */
 
-static float sigmoidf_(float x) { return 1.0f / (1.0f + sfo_expf(-x)); } /* project.rs:93-96 */
+static float sigmoidf_(float x) { return 1.0f / (1.0f + sfo_expf(-x)); } /* project.rs:32-35 */
 
 #define SH_C0 0.2820948f
 #define SH_C1 0.48860252f
@@ -113,7 +113,7 @@ static float sigmoidf_(float x) { return 1.0f / (1.0f + sfo_expf(-x)); } /* proj
 #define SH_C2_2 -1.0925485f
 #define SH_C2_3 0.54627424f
 
-/* src/project.rs:403-449, including its band-3 basis as written. */
+/* src/project.rs:342-388, including its band-3 basis as written. */
 static void sh_to_rgb(uint32_t chs, const float dir[3], uint32_t s, uint32_t n,
                       const float* shs, float rgb[3]) {
     size_t st = n;
@@ -173,7 +173,7 @@ typedef struct {
     uint64_t isects, vis;
 } project_job;
 
-/* One splat of project_splats (src/project.rs:295-388). */
+/* One splat of project_splats (src/project.rs:234-388). */
 static void project_one(project_job* J, uint32_t i) {
     const sfo_view* v = J->v;
     size_t n = J->n;
@@ -193,7 +193,7 @@ static void project_one(project_job* J, uint32_t i) {
     if (cam[2] <= 0.1f) return;
     float inv_z = 1.0f / cam[2];
 
-    /* compute_cov2d (project.rs:236-282) */
+    /* compute_cov2d (project.rs:175-221) */
     float x2 = qx * qx, y2 = qy * qy, z2 = qz * qz;
     float xy = qx * qy, xz = qx * qz, yz = qy * qz;
     float wx = qw * qx, wy = qw * qy, wz = qw * qz;
@@ -223,12 +223,12 @@ static void project_one(project_job* J, uint32_t i) {
     float ca = dot3(j0, j0) + 0.3f;
     float cb = dot3(j0, j1);
     float cc = dot3(j1, j1) + 0.3f;
-    /* compute_conic (project.rs:284-293) */
+    /* compute_conic (project.rs:223-232) */
     float det = fmaxf(ca * cc - cb * cb, 1e-6f);
     float inv_det = 1.0f / det;
 
     J->visible[i] = 1;
-    J->depth_keys[i] = sfo_f2b(cam[2]); /* project.rs:116-119 */
+    J->depth_keys[i] = sfo_f2b(cam[2]); /* project.rs:56-58, 283 */
 
     float dir[3] = {mean[0] - v->world_pos[0], mean[1] - v->world_pos[1], mean[2] - v->world_pos[2]};
     float inv = 1.0f / sqrtf(dot3(dir, dir));
@@ -252,7 +252,7 @@ static void project_one(project_job* J, uint32_t i) {
     float cutoff = sfo_logf(opacity / SFO_ALPHA_CUTOFF);
     float ex = sqrtf(2.0f * cutoff * ca);
     float ey = sqrtf(2.0f * cutoff * cc);
-    /* tile_bbox (project.rs:98-107) */
+    /* tile_bbox (project.rs:37-46) */
     float inv_tile = 1.0f / (float)J->tile_w;
     float bx = v->tile_bounds[0], by = v->tile_bounds[1];
     uint32_t min_x = sfo_f2u(sfo_clampf((mx - ex) * inv_tile, 0.0f, bx));
@@ -540,7 +540,7 @@ void sfo_rasterize(int mode, uint32_t w, uint32_t h, uint32_t tile_w, uint32_t r
 
 /* ------------------------------------------------------------------- seg */
 
-/* ln Gamma, Lanczos g = 7 with 9 terms (src/seg/beta.rs:55-70). */
+/* ln Gamma, Lanczos g = 7 with 9 terms (src/seg/beta.rs:52-72). */
 double sfo_lgamma(double x) {
     static const double c[8] = {676.5203681218851,     -1259.1392167224028,
                                 771.3234287776531,     -176.6150291621406,
@@ -554,7 +554,7 @@ double sfo_lgamma(double x) {
 }
 
 /* digamma: recurrence to x >= 6, then the asymptotic series
- * (src/seg/beta.rs:75-86). */
+ * (src/seg/beta.rs:74-86). */
 double sfo_digamma(double x) {
     double r = 0.0;
     while (x < 6.0) {
@@ -574,7 +574,7 @@ double sfo_beta_entropy(double a, double b) {
 }
 
 /* EIG of one candidate: sum_i max(0, H(a,b) - H(a k, b k)), k = 1 + eps/(a+b),
- * eps = f32(q)/scale, skipped at eps = 0 (src/seg/beta.rs:146-201, 241-268),
+ * eps = f32(q)/scale, skipped at eps = 0 (src/seg/beta.rs:146-201, 240-268),
  * with H evaluated in f64 instead of the f32 table. */
 double sfo_eig(const float* a, const float* b, const uint32_t* resp, uint32_t n, float scale) {
     double total = 0.0;

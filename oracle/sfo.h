@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-/* Kernel view of a camera (src/project.rs:121-157, CameraView). */
+/* Kernel view of a camera (src/project.rs:60-74, CameraView). */
 typedef struct {
     float rot0[3], rot1[3], rot2[3]; /* rows of the world->camera rotation */
     float trans[3];                  /* world->camera translation */

@@ -3,10 +3,10 @@
  *
  * The reference has no backward pass (docs/00_pipeline.md:1); the forward
  * function being differentiated is the reference's: project_splats
- * (src/project.rs:236-388), the compositor (src/raster.rs:138-229) and an L2
+ * (src/project.rs:234-388), the compositor (src/raster.rs:138-229) and an L2
  * loss over the f32 RGBA image (r, g, b, 1 - T).  The function is piecewise
- * smooth: culls, the u/v clamp (project.rs:253-256), the det clamp
- * (project.rs:286), alpha >= 1/255, the 0.999 alpha cap and the T < 1/255
+ * smooth: culls, the u/v clamp (project.rs:192-195), the det clamp
+ * (project.rs:225), alpha >= 1/255, the 0.999 alpha cap and the T < 1/255
  * stop are branch decisions.  sfo_bwd_loss(record = 1) evaluates in f64 and
  * records every decision; record = 0 replays them, so central finite
  * differences measure the derivative inside the recorded piece — the
@@ -59,7 +59,7 @@ static double d_sigmoid(double x) { return 1.0 / (1.0 + exp(-x)); }
 #define C23 0.54627424
 #define K9 -0.5900436
 
-/* SH basis values of project.rs:403-449 in coefficient order (k = 0..15):
+/* SH basis values of project.rs:342-388 in coefficient order (k = 0..15):
  * k -> planes (3k + c). */
 static void sh_basis(double x, double y, double z, int chs, double B[16]) {
     memset(B, 0, 16 * sizeof(double));
@@ -261,7 +261,7 @@ double sfo_bwd_loss(sfo_bwd_ctx* B, const double* A, const double* sh, const flo
     return loss;
 }
 
-/* Projection adjoint of splat i (project.rs:236-388) given its screen-space
+/* Projection adjoint of splat i (project.rs:234-388) given its screen-space
  * gradient G2 = d/d(mx, my, conic a, b, c, opacity, r, g, b); B->pj[i] and
  * B->pf[i] hold its f64 forward state. */
 static void proj_adjoint(sfo_bwd_ctx* B, uint32_t i, const double* G2, const double* A,

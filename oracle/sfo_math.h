@@ -2,7 +2,7 @@
  * restatement of the splat render path.
  *
  * The reference evaluates exp/ln through its shader compiler's builtins
- * (src/project.rs:94-96, 324-327, 374; src/raster.rs:187), whose last-ulp
+ * (src/project.rs:34, 105, 264-266, 313-316; src/raster.rs:187), whose last-ulp
  * behaviour is device dependent.  To make CPU and GPU results comparable bit
  * for bit, both sides evaluate one fixed algorithm: Cody-Waite range
  * reduction followed by the Cephes single-precision minimax polynomials

@@ -208,7 +208,7 @@ static void quat_cols(const float q[4], float c0[3], float c1[3], float c2[3]) {
 
 static float hdot3(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
-// CameraViewLaunch::for_camera (src/project.rs:137-157); w2c = (R^T, -R^T p),
+// CameraViewLaunch::for_camera (src/project.rs:76-96); w2c = (R^T, -R^T p),
 // the exact rigid inverse of camera.rs:42-44.
 extern "C" int sfk_camera_view(const sfk_camera* cam, uint32_t w, uint32_t h, uint32_t tile_w,
                                sfk_view* out) {

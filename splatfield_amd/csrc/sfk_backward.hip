@@ -1,6 +1,6 @@
 // Backward (adjoint) pass of the splat render for gfx950.  The reference is
 // forward-only (docs/00_pipeline.md:1); this differentiates its forward
-// function — project_splats (src/project.rs:236-388) and the front-to-back
+// function — project_splats (src/project.rs:234-388) and the front-to-back
 // compositor (src/raster.rs:138-229) — with respect to means, log-scales,
 // quaternions (raw, as the kernel uses them), opacity logits and SH.
 //

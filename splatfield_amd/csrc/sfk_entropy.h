@@ -1,4 +1,4 @@
-// Beta-distribution entropy in f64 on the device (src/seg/beta.rs:52-96):
+// Beta-distribution entropy in f64 on the device (src/seg/beta.rs:52-98):
 // the reference evaluates it on the host to fill a 445x445 f32 table
 // (beta.rs:100-144); here it is evaluated per splat in the EIG kernels.
 #pragma once
@@ -154,7 +154,7 @@ __device__ __forceinline__ double d_beta_entropy(double a, double b) {
 }
 
 // max(0, H0 - H(a k, b k)), k = 1 + eps/(a+b), eps = f32(q)/scale; 0 when
-// eps = 0 (beta.rs:365-415).  H0 = H(a, b).
+// eps = 0 (beta.rs:172-186).  H0 = H(a, b).
 __device__ __forceinline__ double d_entropy_drop(float a, float b, double h0, uint32_t q,
                                                  float scale) {
     float eps = (float)q / scale;

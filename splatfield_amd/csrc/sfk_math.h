@@ -2,7 +2,7 @@
 // splat render path.
 //
 // The reference evaluates exp/ln with its shader compiler's builtins
-// (src/project.rs:94-96, 324-327, 374; src/raster.rs:187), whose last-ulp
+// (src/project.rs:34, 105, 264-266, 313-316; src/raster.rs:187), whose last-ulp
 // behaviour depends on the device.  Here one fixed algorithm is used so that
 // results are reproducible on any host and device: Cody-Waite range reduction
 // and the Cephes single-precision minimax polynomials (S. L. Moshier,

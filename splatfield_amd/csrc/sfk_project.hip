@@ -1,4 +1,4 @@
-// project_splats for gfx950 (src/project.rs:295-388, docs/01_project.md).
+// project_splats for gfx950 (src/project.rs:234-388, docs/01_project.md).
 //
 // One thread per splat over coalesced field-major planes.  Differences from
 // the reference kernel, all deliberate:
@@ -15,7 +15,7 @@ namespace sfk {
 
 namespace {
 
-constexpr float kShC0 = 0.2820948f;  // project.rs:390-401
+constexpr float kShC0 = 0.2820948f;  // project.rs:329
 constexpr float kShC1 = 0.48860252f;
 constexpr float kShC20 = 0.9461747f;
 constexpr float kShC21 = 0.31539157f;
@@ -31,7 +31,7 @@ __device__ __forceinline__ void sh_load(const float* __restrict__ shs, uint32_t 
     for (int k = 0; k < 3 * CHS; ++k) c[k] = __builtin_nontemporal_load(shs + (size_t)k * n + s);
 }
 
-// sh_to_rgb (project.rs:403-449) with its band-3 basis as written.
+// sh_to_rgb (project.rs:342-388) with its band-3 basis as written.
 template <int CHS>
 __device__ __forceinline__ void sh_to_rgb(float x, float y, float z, const float (&SH)[3 * CHS],
                                           float& r, float& g, float& b) {
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_project(sfk_view v, const float* __rest
             }
             float4* row = proj + (size_t)i * 3;
             row[0] = make_float4(o.px, o.py, o.qa, o.qb);
-            constexpr float kOff = CHS < 0 ? 0.0f : 0.5f;  // project.rs:308-310
+            constexpr float kOff = CHS < 0 ? 0.0f : 0.5f;  // project.rs:306-310
             row[1] = make_float4(o.qc, s.opacity, cr + kOff, cg + kOff);
             row[2] = make_float4(cbl + kOff, 0.0f, 0.0f, 0.0f);
             bbox[i] = make_uint2(o.bb_lo, o.bb_hi);

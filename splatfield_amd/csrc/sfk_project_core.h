@@ -1,4 +1,4 @@
-// Per-splat projection math of project_splats (src/project.rs:236-388),
+// Per-splat projection math of project_splats (src/project.rs:234-388),
 // shared by the single-view and the batched multi-view kernels so that both
 // produce the same bits.  Operation order follows the reference; the library
 // is compiled with -ffp-contract=off.
@@ -10,7 +10,7 @@
 
 namespace sfk {
 
-constexpr float kAlphaCutoff = 10.0f / 255.0f;  // project.rs:74
+constexpr float kAlphaCutoff = 10.0f / 255.0f;  // project.rs:13
 
 struct SplatIn {
     float mx, my, mz, qw, qx, qy, qz, sx, sy, sz, opacity;
@@ -29,7 +29,7 @@ __device__ __forceinline__ float dot3f(float ax, float ay, float az, float bx, f
 }
 
 // Loads splat i of the field-major planes, applying exp/sigmoid as the
-// reference does before its culls (project.rs:313-333).
+// reference does before its culls (project.rs:245-271).
 // The 11 loads are issued together and their values pinned here: the
 // scales are only used past the culls, and left to itself the compiler sinks
 // their loads below the cull branches, one memory round trip each.
@@ -67,7 +67,7 @@ __device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s
     float cz = dot3f(v.rot2[0], v.rot2[1], v.rot2[2], s.mx, s.my, s.mz) + v.trans[2];
     if (cz <= 0.1f) return false;
     float iz = 1.0f / cz;
-    // compute_cov2d (project.rs:236-282)
+    // compute_cov2d (project.rs:175-221)
     float x2 = s.qx * s.qx, y2 = s.qy * s.qy, z2 = s.qz * s.qz;
     float xy = s.qx * s.qy, xz = s.qx * s.qz, yz = s.qy * s.qz;
     float wx = s.qw * s.qx, wy = s.qw * s.qy, wz = s.qw * s.qz;
@@ -100,7 +100,7 @@ __device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s
     float ca = dot3f(j00, j01, j02, j00, j01, j02) + 0.3f;
     float cb = dot3f(j00, j01, j02, j10, j11, j12);
     float cc = dot3f(j10, j11, j12, j10, j11, j12) + 0.3f;
-    // compute_conic (project.rs:284-293)
+    // compute_conic (project.rs:223-232)
     float det = fmaxf(ca * cc - cb * cb, 1e-6f);
     float inv_det = 1.0f / det;
     o.qa = cc * inv_det;
@@ -112,7 +112,7 @@ __device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s
     float cutoff = sfk_logf(s.opacity / kAlphaCutoff);
     float ex = sqrtf(2.0f * cutoff * ca);
     float ey = sqrtf(2.0f * cutoff * cc);
-    // tile_bbox (project.rs:98-107)
+    // tile_bbox (project.rs:37-46)
     float bx = v.tile_bounds[0], by = v.tile_bounds[1];
     uint32_t min_x = sfk_f2u(sfk_clampf((o.px - ex) * inv_tile, 0.0f, bx));
     uint32_t min_y = sfk_f2u(sfk_clampf((o.py - ey) * inv_tile, 0.0f, by));
@@ -129,7 +129,7 @@ __device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s
 // splat (zero tiles).  It rides through the depth sort as the payload of
 // the identity values, so the emission reads it in depth order.
 // Derivatives D[k][j] of the SH basis functions of sh_to_rgb
-// (project.rs:403-449, band 3 as written there) with respect to the unit
+// (project.rs:342-388, band 3 as written there) with respect to the unit
 // view direction component j (x, y, z), coefficients k < CHS; the rest 0.
 template <int CHS>
 __device__ __forceinline__ void sh_basis_dir_grad(float x, float y, float z, float (&D)[16][3]) {

@@ -1,7 +1,7 @@
 // Render and seg orchestration behind the C ABI: Splats, RenderScratch,
 // prepare_isects / finalize / render_with (src/render.rs:114-460) and the
 // responsibility, EIG and Beta-update entry points (src/seg/mod.rs:146-210,
-// src/seg/beta.rs:440-500).
+// src/seg/beta.rs:240-286).
 #include <math.h>
 #include <string.h>
 

@@ -40,7 +40,7 @@ constexpr uint32_t kEigEpt = 4;
 constexpr uint32_t kEigBlock = kEigThreads * kEigEpt;
 
 // Per-splat dH = max(0, H(a,b) - H(a k, b k)), k = 1 + eps/(a+b),
-// eps = f32(q)/scale, skipped at eps = 0 (beta.rs:365-415); per-block f64 sums.
+// eps = f32(q)/scale, skipped at eps = 0 (beta.rs:172-186); per-block f64 sums.
 __global__ __launch_bounds__(kEigThreads) void k_eig_partial(const float* __restrict__ a,
                                                              const float* __restrict__ b,
                                                              const uint32_t* __restrict__ resp,
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_eig_final(const double* __restrict__ pa
     if (threadIdx.x == 0) *out = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
 }
 
-// beta_update_kernel (beta.rs:417-426).
+// beta_update_kernel (beta.rs:203-212).
 __global__ __launch_bounds__(256) void k_beta_update(float* __restrict__ a, float* __restrict__ b,
                                                      const uint32_t* __restrict__ fg,
                                                      const uint32_t* __restrict__ bg, uint32_t n,

@@ -47,7 +47,7 @@ class Camera:
         return Camera(tuple(fov), tuple(p.tolist()), tuple(q.tolist()))
 
     def view(self, w, h, tile_w=16):
-        """CameraViewLaunch::for_camera (project.rs:137-157)."""
+        """CameraViewLaunch::for_camera (project.rs:76-96)."""
         v = _lib.View()
         c = self.to_c()
         rc = lib().sfk_camera_view(C.byref(c), w, h, tile_w, C.byref(v))

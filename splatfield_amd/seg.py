@@ -77,7 +77,7 @@ def accumulate_prepared(scratch, isects, acc, mask=None):
 
 
 class BetaState:
-    """Per-splat Beta posteriors (beta.rs:428-501)."""
+    """Per-splat Beta posteriors (beta.rs:214-287)."""
 
     def __init__(self, ctx, a, b):
         self.ctx = ctx
@@ -106,7 +106,7 @@ class BetaState:
 
 
 def map_labels(a, b):
-    """MAP foreground iff a > b, ties background (beta.rs:503-511)."""
+    """MAP foreground iff a > b, ties background (beta.rs:289-297)."""
     a = np.asarray(a)
     b = np.asarray(b)
     if a.shape != b.shape:

@@ -110,7 +110,7 @@ def check_eps(ctx, scratch, isects, ref, n, w, h, tile_w, mask_bytes=None):
 
 
 def test_project_kat(ctx):
-    """project.rs:460-507: one splat, 64x64, covers all 16 tiles."""
+    """project.rs:399-446: one splat, 64x64, covers all 16 tiles."""
     attrs = np.array([0, 0, 1, 1, 0, 0, 0, 0, 0, 0, 5], np.float32)
     sh = np.zeros(3, np.float32)
     splats, scratch, view, isects, ref = run_both(ctx, attrs, sh, 1, 1, R.Camera(), 64, 64)

@@ -157,7 +157,7 @@ def _entropy_exact(a, b):
 
 
 def test_eig_reference_cases(ctx):
-    """beta.rs:758-825: monotone and diminishing, invisible -> 0, fresh splat
+    """beta.rs:548-612: monotone and diminishing, invisible -> 0, fresh splat
     dH = H(1,1) - H(1.25,1.25)."""
     scale = 4096.0
 

@@ -92,7 +92,7 @@ def test_fractional_bits():
 # ---------------------------------------------------------------- project
 
 def test_project_kat():
-    """project.rs:460-507."""
+    """project.rs:399-446."""
     attrs = np.array([0, 0, 1, 1, 0, 0, 0, 0, 0, 0, 5], np.float32)
     pr = O.project(default_view(64, 64), attrs, np.zeros(3, np.float32), 1, 1)
     np.testing.assert_array_equal(pr["counters"], [16, 1])
@@ -123,7 +123,7 @@ def test_project_closed_form_isotropic():
 
 
 def test_project_culls():
-    """Opacity below 10/255 and depth <= 0.1 are culled (project.rs:329-337)."""
+    """Opacity below 10/255 and depth <= 0.1 are culled (project.rs:267-276)."""
     a = sample_opaque_attributes(3, lambda i: [(0, 0, 1), (0, 0, 0.05), (0, 0, -2)][i])
     a = a.reshape(11, 3)
     a[10, 0] = -4.0  # sigmoid(-4) = 0.018 < 10/255
@@ -154,7 +154,7 @@ def test_project_invariants_random_scene():
 def _real_sh_basis(x, y, z):
     """Textbook real SH basis (orthonormal, Condon-Shortley) in the INRIA
     3DGS ordering for bands 0-2, with the sign conventions of
-    project.rs:412-428 (band 1: -y, z, -x)."""
+    project.rs:351-355 (band 1: -y, z, -x)."""
     c0 = 0.28209479177387814
     c1 = 0.4886025119029199
     c2 = [1.0925484305920792, -1.0925484305920792, 0.31539156525252005,
@@ -165,7 +165,7 @@ def _real_sh_basis(x, y, z):
 
 
 def test_sh_bands_0_to_2_match_textbook_basis():
-    """project.rs:403-428 vs the textbook real SH basis: each coefficient in
+    """project.rs:342-367 vs the textbook real SH basis: each coefficient in
     turn set to 1 must reproduce its basis function at the view direction."""
     rng = np.random.default_rng(7)
     for _ in range(5):
@@ -182,6 +182,78 @@ def test_sh_bands_0_to_2_match_textbook_basis():
             v = O.camera_view((1.0, 1.0), (0, 0, -3), q, 32, 32)
             pr = O.project(v, a, sh, 1, 9)
             assert abs((pr["projected"][5] - 0.5) - basis[k]) < 2e-6, k
+
+
+# Band-3 normalisations of the real SH basis, from their closed forms.
+_K3_33 = math.sqrt(35.0 / (2.0 * math.pi)) / 4.0    # x(x^2-3y^2), y(3x^2-y^2)
+_K3_32 = math.sqrt(105.0 / math.pi) / 2.0           # xyz
+_K3_31 = math.sqrt(21.0 / (2.0 * math.pi)) / 4.0    # x(5z^2-1), y(5z^2-1)
+_K3_30 = math.sqrt(7.0 / math.pi) / 4.0             # z(5z^2-3)
+_K3_2z = math.sqrt(105.0 / math.pi) / 4.0           # z(x^2-y^2)
+
+
+def _band3_as_written(x, y, z):
+    """Band 3 of project.rs:362-384 by coefficient slot 9..15, derived by
+    hand.  Slots 11..15 are the textbook real SH functions (INRIA sign
+    convention) at permuted slots: y(5z^2-1), x(5z^2-1), z(5z^2-3), xyz,
+    z(x^2-y^2).  Slots 9 and 10 are not: with Rust precedence
+    b9 = (-c*x)*(2xy) + y*(x^2-y^2) and b10 = (-c*x)*(x^2-y^2) - y*(2xy),
+    which expand to b9 = (1-2c)x^2y - y^3 and b10 = -c x^3 + (c-2)xy^2."""
+    c = _K3_33
+    return {9: (1 - 2 * c) * x * x * y - y ** 3,
+            10: -c * x ** 3 + (c - 2) * x * y * y,
+            11: -_K3_31 * y * (5 * z * z - 1),
+            12: -_K3_31 * x * (5 * z * z - 1),
+            13: _K3_30 * z * (5 * z * z - 3),
+            14: _K3_32 * x * y * z,
+            15: _K3_2z * z * (x * x - y * y)}
+
+
+def _sh3_red(direction, k):
+    """Red colour - 0.5 of one splat seen along `direction` whose only
+    non-zero SH coefficient is coefficient k's red channel (slot 3k)."""
+    d = np.asarray(direction, np.float64)
+    pos = (2.0 * d / np.linalg.norm(d)).astype(np.float32)
+    q = O.look_at((0, 0, 0), pos)
+    v = O.camera_view((1.0, 1.0), (0, 0, 0), q, 32, 32)
+    a = sample_opaque_attributes(1, lambda i: pos)
+    sh = np.zeros(48, np.float32)
+    sh[3 * k] = 1.0
+    pr = O.project(v, a, sh, 1, 16)
+    assert pr["counters"][1] == 1
+    return float(pr["projected"][5]) - 0.5
+
+
+def test_sh_band3_matches_textbook_basis_and_hand_values():
+    """project.rs:362-384 (band 3): b11..b15 against the textbook real SH
+    functions with normalisations from their closed forms, at random unit
+    directions; b9/b10 on the axes, where they reduce to -y^3 (b9 = -1 along
+    +y, 0 along +-x) and -c x^3 (b10 = -c along +x, 0 along +-y), and at
+    random directions against their hand expansion.  A slot swap, a sign or
+    the bracketed -c*(...) form of b9/b10 fails it."""
+    rng = np.random.default_rng(11)
+    dirs = [rng.normal(size=3) for _ in range(6)]
+    dirs = [d / np.linalg.norm(d) for d in dirs]
+    for d in dirs:
+        want = _band3_as_written(*d)
+        for k in range(9, 16):
+            got = _sh3_red(d, k)
+            assert abs(got - want[k]) < 3e-6, (k, d, got, want[k])
+    c = np.float32(_K3_33)
+    axis = {(1, 0, 0): {9: 0.0, 10: -c}, (-1, 0, 0): {9: 0.0, 10: c},
+            (0, 1, 0): {9: -1.0, 10: 0.0}, (0, -1, 0): {9: 1.0, 10: 0.0}}
+    for d, vals in axis.items():
+        for k, want in vals.items():
+            assert abs(_sh3_red(d, k) - float(want)) < 1e-6, (d, k)
+    # the test tells the as-written b9/b10 from the bracketed form
+    # -c*(x*(2xy) + y*(x^2-y^2)) and -c*(x*(x^2-y^2) - y*(2xy)) at these
+    # directions, so a precedence slip in the oracle could not pass it
+    for d in dirs:
+        x, y, _ = d
+        bracket9 = -_K3_33 * (x * 2 * x * y + y * (x * x - y * y))
+        bracket10 = -_K3_33 * (x * (x * x - y * y) - y * 2 * x * y)
+        w = _band3_as_written(*d)
+        assert max(abs(bracket9 - w[9]), abs(bracket10 - w[10])) > 1e-3
 
 
 # ------------------------------------------------------------- sort / scan
@@ -368,7 +440,7 @@ def test_entropy_vs_scipy():
 
 
 def test_eig_mirror_and_fresh_splat():
-    """beta.rs:714-756, 800-825: EIG equals the sum of per-splat entropy drops;
+    """beta.rs:501-547, 587-612: EIG equals the sum of per-splat entropy drops;
     a fresh splat seen at eps = 0.5 gains H(1,1) - H(1.25,1.25)."""
     rng = np.random.default_rng(6)
     n = 1000
