@@ -266,6 +266,17 @@ int sfk_eig(sfk_ctx* ctx, const float* a, const float* b, const uint32_t* resp, 
 /* The same, leaving the f64 result in device memory (no sync). */
 int sfk_eig_device(sfk_ctx* ctx, const float* a, const float* b, const uint32_t* resp,
                    uint32_t n, float scale, double* d_out);
+/* sfk_eig_device without any allocation: no counterpart in the reference,
+ * for callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops).  The per-block partials go to a fixed file-scope
+ * device array of the library (one per device, 512 KiB, held until the
+ * process ends) instead of the context's grown workspace; n above
+ * SFK_EIG_NOALLOC_MAX_SPLATS is SFK_E_CAPACITY.  All calls on one device
+ * share that array: calls that overlap on one device (two contexts or
+ * streams) race. */
+#define SFK_EIG_NOALLOC_MAX_SPLATS (1u << 26)
+int sfk_eig_device_noalloc(sfk_ctx* ctx, const float* a, const float* b, const uint32_t* resp,
+                           uint32_t n, float scale, double* d_out);
 /* BetaState::update (src/seg/beta.rs:270-286): a += fg/scale, b += bg/scale. */
 int sfk_beta_update(sfk_ctx* ctx, float* a, float* b, const uint32_t* fg, const uint32_t* bg,
                     uint32_t n, float scale);

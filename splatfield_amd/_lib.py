@@ -109,6 +109,7 @@ SIGNATURES = {
     "sfk_accumulate_prepared": (i32, [P, P, C.POINTER(TileIsects), P, u32, P, P, P, P]),
     "sfk_eig": (i32, [P, P, P, P, u32, f32, P]),
     "sfk_eig_device": (i32, [P, P, P, P, u32, f32, P]),
+    "sfk_eig_device_noalloc": (i32, [P, P, P, P, u32, f32, P]),
     "sfk_beta_update": (i32, [P, P, P, P, P, u32, f32]),
     "sfk_eig_round_create": (i32, [P, P, P, P, P, u32, u32, u32, PP]),
     "sfk_eig_round_destroy": (None, [P]),

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <new>
 
 #include "sfk_internal.h"
@@ -87,6 +88,34 @@ extern "C" int sfk_prof_report(sfk_ctx* c, char* buf, int len) {
 
 // ----------------------------------------------------------------- context
 
+// Fixed reduction workspace of sfk_eig_device_noalloc: one f64 partial per
+// EIG block of the largest admitted splat count.  Each device holds its own
+// copy (512 KiB) for the life of the process; host code reaches it only
+// through the address hipGetSymbolAddress gives for that device.
+static __device__ __align__(256) double scratch[SFK_EIG_NOALLOC_MAX_SPLATS / 1024];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_fixed_mu;
+double* g_fixed_ptr[kMaxDevices] = {};
+
+// This device's address of `scratch`, looked up once per device with the
+// device current (the caller has set it).
+hipError_t fixed_scratch(int device, double** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_fixed_mu);
+    if (!g_fixed_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_fixed_ptr[device] = static_cast<double*>(p);
+    }
+    *out = g_fixed_ptr[device];
+    return hipSuccess;
+}
+}  // namespace
+
 extern "C" int sfk_ctx_create(int device, sfk_ctx** out) {
     if (!out) return SFK_E_ARG;
     *out = nullptr;
@@ -103,7 +132,27 @@ extern "C" int sfk_ctx_create(int device, sfk_ctx** out) {
         delete c;
         return SFK_E_HIP;
     }
+    // A failed lookup only disables sfk_eig_device_noalloc, which reports it.
+    c->eig_fixed_err = fixed_scratch(device, &c->eig_fixed);
     *out = c;
+    return SFK_OK;
+}
+
+extern "C" int sfk_eig_device_noalloc(sfk_ctx* c, const float* a, const float* b,
+                                      const uint32_t* resp, uint32_t n, float scale,
+                                      double* d_out) {
+    if (!c || !d_out || (n && (!a || !b || !resp))) return SFK_E_ARG;
+    if (n > SFK_EIG_NOALLOC_MAX_SPLATS)
+        return fail(c, SFK_E_CAPACITY, "eig (no alloc): %u splats, at most %u", n,
+                    (uint32_t)SFK_EIG_NOALLOC_MAX_SPLATS);
+    constexpr uint32_t kFixedPartials = sizeof(scratch) / sizeof(double);
+    if (eig_num_partials(n) > kFixedPartials)
+        return fail(c, SFK_E_CAPACITY, "eig (no alloc): %u partials, room for %u",
+                    eig_num_partials(n), kFixedPartials);
+    if (!c->eig_fixed) return hip_fail(c, c->eig_fixed_err, "eig (no alloc): workspace address");
+    // Every partial the final sum reads is written by this call's first
+    // kernel, so nothing needs clearing between calls.
+    SFK_HIP(c, launch_eig(c->stream, a, b, resp, n, scale, c->eig_fixed, d_out));
     return SFK_OK;
 }
 

@@ -23,6 +23,11 @@ struct sfk_ctx {
     double* eig_partials = nullptr;
     uint32_t eig_partials_cap = 0;
     double* eig_out = nullptr;
+    // Device address of the file-scope EIG partials array that
+    // sfk_eig_device_noalloc reduces into (this device's copy), resolved at
+    // context creation; null with the lookup's error kept when it failed.
+    double* eig_fixed = nullptr;
+    hipError_t eig_fixed_err = hipSuccess;
     // Per-launch event timing (sfk_prof_enable).
     bool prof = false;
     std::vector<sfk_prof_rec> prof_recs;
