@@ -46,6 +46,23 @@ class Camera:
         lib().sfk_look_at(p.ctypes.data, t.ctypes.data, q.ctypes.data)
         return Camera(tuple(fov), tuple(p.tolist()), tuple(q.tolist()))
 
+    def forward(self):
+        """The viewing direction, rotation * +Z (f64)."""
+        x, y, z, w = (float(v) for v in self.rotation)
+        return np.array([2.0 * (x * z + w * y), 2.0 * (y * z - w * x), 1.0 - 2.0 * (x * x + y * y)])
+
+    def frame_bounds(self, bounds):
+        """Camera::frame_bounds (camera.rs:46-52): centre on the bounds and
+        pull back along -Y by twice their largest extent, pitched -90 degrees
+        about X (looking along +Y).  Returns that distance."""
+        f = np.float32
+        lo, hi = (np.asarray(b, np.float32) for b in bounds)
+        d = f((hi - lo).max() * f(2.0))
+        self.position = tuple(((lo + hi) * f(0.5) - np.array([0, d, 0], np.float32)).tolist())
+        half = f(-math.pi / 2) * f(0.5)
+        self.rotation = (float(f(math.sin(half))), 0.0, 0.0, float(f(math.cos(half))))
+        return float(d)
+
     def view(self, w, h, tile_w=16):
         """CameraViewLaunch::for_camera (project.rs:76-96)."""
         v = _lib.View()

@@ -3,6 +3,7 @@ fixed-point accumulators, mask packing, responsibility renders, analytic EIG
 with f64 entropy on the device, and the Beta update."""
 import ctypes as C
 import math
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -248,6 +249,54 @@ def sphere_cameras(count, radius=3.0, center=(0.0, 0.0, 0.0), fov=CANDIDATE_FOV)
     return [Camera.look_at(c + np.float32(radius) * d, c, fov=(fov, fov)) for d in dirs]
 
 
+def footprint_mask(ctx, attrs, sh, n, sh_per_ch, camera, w, h, seed=0, radius=0.3,
+                   threshold=0.5):
+    """A synthetic 2D oracle's mask for one view: the rendered footprint of
+    the splats within `radius` of a seeded centre (uniform in [-0.5, 0.5]^3).
+    Only those splats are rendered; a pixel is foreground where their alpha
+    1 - T reaches `threshold`.  Returns (one byte per pixel, the subset's
+    boolean selection)."""
+    from .render import RenderScratch, Splats, finalize_rgb, prepare_isects
+    A = np.asarray(attrs, np.float32).reshape(-1, n)
+    Sh = np.asarray(sh, np.float32).reshape(-1, n)
+    centre = np.random.default_rng(seed).uniform(-0.5, 0.5, 3).astype(np.float32)
+    sel = np.sqrt(((A[0:3].T - centre) ** 2).sum(axis=1)) <= np.float32(radius)
+    m = int(sel.sum())
+    if m == 0:
+        return np.zeros(w * h, np.uint8), sel
+    sub = Splats(ctx, np.ascontiguousarray(A[:, sel]).reshape(-1),
+                 np.ascontiguousarray(Sh[:, sel]).reshape(-1))
+    scratch = RenderScratch(ctx, m, w, h)
+    isects = prepare_isects(sub, scratch, camera, w, h)
+    finalize_rgb(scratch, isects)
+    alpha = 1.0 - scratch.read("final_t").reshape(h, w)
+    return (alpha >= threshold).astype(np.uint8).reshape(-1), sel
+
+
+def observe(splats, scratch, acc, state, camera, w, h, mask_bytes):
+    """The observation half of Segmenter::step (active.rs:150-176): render
+    the view, lift the mask to fg/bg evidence over the render's own isects
+    and fold it into the posterior.  Returns the mask's foreground pixels."""
+    from .render import finalize_rgb, prepare_isects
+    isects = prepare_isects(splats, scratch, camera, w, h)
+    finalize_rgb(scratch, isects)
+    mask = Mask(splats.ctx, w, h, mask_bytes)
+    accumulate_prepared(scratch, isects, acc, mask)
+    state.update(acc)
+    return int(mask.fg_pixels)
+
+
+def observe_bootstrap(splats, camera, res, mask_bytes):
+    """A Beta(1,1) posterior after one observation of `mask_bytes` from
+    `camera` at res x res (the round-0 step of the loop)."""
+    from .render import RenderScratch
+    n = splats.n
+    state = BetaState.new_uniform(splats.ctx, n)
+    observe(splats, RenderScratch(splats.ctx, n, res, res), Accumulators(splats.ctx, n), state,
+            camera, res, res, mask_bytes)
+    return state
+
+
 def bitmap_to_rgb(bitmap_words, width, height, row_stride):
     """active.rs:240-256: the RGBA8 bitmap (row stride in words) unpacked to
     tightly packed RGB bytes (alpha dropped), row-major."""
@@ -255,37 +304,62 @@ def bitmap_to_rgb(bitmap_words, width, height, row_stride):
     return np.ascontiguousarray(w.view(np.uint8).reshape(height, width, 4)[:, :, :3]).reshape(-1)
 
 
+@dataclass(frozen=True)
+class Config:
+    """Loop parameters (active.rs:31-55): square render resolution, candidate
+    views per iteration (N_cand) and active iterations (T), with the shipped
+    defaults."""
+    resolution: int = 512
+    candidates: int = 20
+    iterations: int = 20
+
+
+@dataclass
+class Iteration:
+    """One oracle observation as the loop took it (active.rs:57-67): the
+    camera, the EIG it scored (0 for the blind round-0 camera) and the mask's
+    foreground pixels."""
+    camera: Camera
+    eig: float
+    fg_pixels: int
+
+
 class Segmenter:
-    """Segmenter (active.rs:60-183) over the device pipeline: each step
+    """Segmenter (active.rs:69-238) over the device pipeline.  Each step
     localizes the posterior foreground on the device, scores candidate views
-    (batched round or the plain select_view loop), renders the best view,
+    (batched round, or the plain select_view loop), renders the best view,
     hands its RGB bytes to the caller's 2D oracle, lifts the returned mask to
     per-splat evidence over the same intersections and updates the Beta
-    posterior."""
+    posterior.  The config is validated before any device allocation
+    (active.rs:86-95); errors are ValueError with the reference's messages."""
 
-    def __init__(self, splats, attributes, camera, resolution=512, num_candidates=20,
-                 batched=True, tile_w=16):
+    def __init__(self, splats, cfg=Config(), camera=None, batched=True, tile_w=16):
+        if cfg.iterations < 1:
+            raise ValueError("iterations must be >= 1")
+        if cfg.resolution < 16:
+            raise ValueError("resolution must be >= 16")
+        if cfg.candidates < 1:
+            raise ValueError("candidates must be >= 1: zero would panic select_view's "
+                             "cameras[0] index")
         from .render import RenderScratch
+        self.cfg = cfg
         self.splats = splats
         self.ctx = splats.ctx
-        self.camera = camera
-        self.res = int(resolution)
-        self.num_candidates = int(num_candidates)
+        self.camera = camera if camera is not None else Camera()
         self.batched = bool(batched)
         self.tile_w = tile_w
         n = splats.n
-        A = np.asarray(attributes, np.float32).reshape(-1)
-        p = np.stack([A[0:n], A[n:2 * n], A[2 * n:3 * n]], 1)
-        self.bounds = (p.min(0), p.max(0))  # render.rs:242-269
-        self.scratch = RenderScratch(self.ctx, n, self.res, self.res, tile_w=tile_w)
+        res = cfg.resolution
+        self.scratch = RenderScratch(self.ctx, n, res, res, tile_w=tile_w)
         self.acc = Accumulators(self.ctx, n)
         self.state = BetaState.new_uniform(self.ctx, n)
         self.bootstrapped = False
 
     def select_view(self, center, radius):
-        """active.rs:113-129: the EIG-best of the candidate cameras."""
-        cams = candidates(center, radius, self.num_candidates)
-        sizes = [(self.res, self.res)] * len(cams)
+        """active.rs:110-129: the EIG-best of the candidate cameras (the first
+        of equal scores, as the strict > of the reference's loop)."""
+        cams = candidates(center, radius, self.cfg.candidates)
+        sizes = [(self.cfg.resolution, self.cfg.resolution)] * len(cams)
         if self.batched:
             rnd = EigRound(self.splats, cams, sizes, tile_w=self.tile_w)
             eig, best = rnd.run(self.state, use_graph=False)
@@ -296,8 +370,10 @@ class Segmenter:
         return cams[best], float(eig[best])
 
     def step(self, oracle):
-        """active.rs:133-183.  oracle(rgb_bytes, (w, h), camera) -> one mask
-        byte per pixel.  Returns (camera, eig, foreground pixels)."""
+        """One observation round (active.rs:131-183).  oracle(rgb_bytes,
+        (w, h), camera) returns one mask byte per pixel (nonzero =
+        foreground); its exceptions surface as RuntimeError("oracle
+        failed")."""
         from .render import finalize_rgb, prepare_isects
         loc = localize(self.splats, self.state)
         if loc is not None:
@@ -305,15 +381,46 @@ class Segmenter:
         elif not self.bootstrapped:
             camera, eig = self.camera, 0.0
         else:
-            lo, hi = self.bounds
-            camera, eig = self.select_view((lo + hi) * 0.5, float((hi - lo).max()) * 0.5)
+            # nothing foreground yet: survey the whole scene
+            lo, hi = (np.asarray(b, np.float32) for b in self.splats.bounds)
+            camera, eig = self.select_view((lo + hi) * np.float32(0.5),
+                                           float((hi - lo).max() * np.float32(0.5)))
         self.bootstrapped = True
-        w = h = self.res
+        w = h = self.cfg.resolution
         isects = prepare_isects(self.splats, self.scratch, camera, w, h)
         finalize_rgb(self.scratch, isects)
         b = self.scratch.buffers()
         rgb = bitmap_to_rgb(self.scratch.read("bitmap"), w, h, b.row_stride)
-        mask = Mask(self.ctx, w, h, oracle(rgb, (w, h), camera))
+        try:
+            mask_bytes = oracle(rgb, (w, h), camera)
+        except Exception as e:
+            raise RuntimeError("oracle failed") from e
+        mask = Mask(self.ctx, w, h, mask_bytes)
         accumulate_prepared(self.scratch, isects, self.acc, mask)
         self.state.update(self.acc)
-        return camera, eig, mask.fg_pixels
+        return Iteration(camera, float(eig), int(mask.fg_pixels))
+
+    def run_with(self, oracle, on_round=None):
+        """The loop (active.rs:185-225): up to cfg.iterations observations;
+        on_round(iteration, state) returning False stops after the round just
+        delivered; an empty mask ends the run while no round has seen
+        foreground.  Returns the rounds taken."""
+        any_fg = False
+        rounds = []
+        for _ in range(self.cfg.iterations):
+            r = self.step(oracle)
+            any_fg |= r.fg_pixels > 0
+            stop = on_round is not None and not on_round(r, self.state)
+            rounds.append(r)
+            if stop:
+                break
+            if r.fg_pixels == 0 and not any_fg:
+                break
+        return rounds
+
+    def posteriors(self):
+        return self.state.posteriors()
+
+    def segmentation(self):
+        """The 3D segmentation so far, {i : a_i > b_i} (active.rs:227-232)."""
+        return map_labels(*self.posteriors())

@@ -97,3 +97,56 @@ def make_sog(n, bands, seed, width=64, bad_tag_at=None, bad_label_at=None):
             img.save(out, "WEBP", lossless=True, exact=True)
             z.writestr(names[k], out.getvalue())
     return buf.getvalue(), {k: v.reshape(-1) for k, v in sheets.items()}
+
+
+PER_CLUSTER = 40  # views.rs:109-111
+
+
+def cluster_scene():
+    """views.rs:113-131: three well-separated spherical clusters of 40 opaque
+    splats each; cluster 0 is the segmentation target.  Returns (attributes,
+    [(centre, radius)])."""
+    f = np.float32
+    centers = [(np.array([1.2, 0.0, 0.0], f), f(0.30)),
+               (np.array([-1.6, 0.6, 0.4], f), f(0.45)),
+               (np.array([0.1, -0.4, 1.9], f), f(0.40))]
+
+    def pos(i):
+        c, r = centers[i // PER_CLUSTER]
+        return c + (f(1.4) * r) * jitter(i)
+    return sample_opaque_attributes(len(centers) * PER_CLUSTER, pos), centers
+
+
+def _quat_matrix(q):
+    x, y, z, w = (float(v) for v in q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def sphere_oracle(spheres):
+    """views.rs:133-170: a perfect 2D oracle whose object is a set of
+    spheres; a pixel is foreground iff its camera ray (the renderer's pixel
+    convention, screen = focal * cam.xy / cam.z + size / 2) hits a sphere."""
+    def oracle(rgb, size, camera):
+        w, h = size
+        mask = np.zeros((h, w), np.uint8)
+        if not spheres:
+            return mask.reshape(-1)
+        fx, fy = camera.focal(w, h)
+        xs = (np.arange(w) + 0.5 - w * 0.5) / float(fx)
+        ys = (np.arange(h) + 0.5 - h * 0.5) / float(fy)
+        X, Y = np.meshgrid(xs, ys)
+        d = np.stack([X, Y, np.ones_like(X)], -1)
+        d /= np.linalg.norm(d, axis=-1, keepdims=True)
+        d = d @ _quat_matrix(camera.rotation).T
+        pos = np.asarray(camera.position, np.float64)
+        hit = np.zeros((h, w), bool)
+        for c, r in spheres:
+            oc = np.asarray(c, np.float64) - pos
+            t = d @ oc
+            l2 = float(oc @ oc)
+            d2 = np.where(t < 0, l2, l2 - t * t)
+            hit |= d2 <= float(r) * float(r)
+        return hit.astype(np.uint8).reshape(-1)
+    return oracle

@@ -331,3 +331,30 @@ def test_depth_sort_key_window(ctx, zspan):
     splats, scratch, view, isects, ref = run_both(ctx, attrs, sh, n, k, cam, 320, 240)
     check_prepare(scratch, isects, ref, n)
     check_rgb(scratch, isects, ref, n, 320, 240, 16)
+
+
+def test_c2_full_frame_against_oracle(ctx):
+    """The full-size forward frame (1,048,576 SH3 splats, 1920x1080, seed 0,
+    the benchmark's camera): visibility, keys, rows, counts, bboxes,
+    counters, depth order, per-rank offsets, sorted tile and splat ids and
+    every tile range equal to the oracle's (~9.8M isects, 13 tile bits,
+    24-bit rebased depth sort, binned emission); the image within 1/255."""
+    n, w, h = 1 << 20, 1920, 1080
+    attrs, sh, k = S.synthetic_scene(n, 3, seed=0)
+    fy = np.float32(np.pi / 3)
+    fx = np.float32(S.fov_for_aspect(w, h, fy))
+    cam = R.Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), fov=(fx, fy))
+    splats = R.Splats(ctx, attrs, sh)
+    scratch = R.RenderScratch(ctx, n, w, h)
+    scratch.reserve_isects(16 << 20)
+    view = cam.view(w, h)
+    isects = R.prepare_isects(splats, scratch, view=view, w=w, h=h)
+    ref = O.prepare(oview(view), attrs, sh, n, k, w, h, 16, nthreads=16)
+    assert ref["num_isects"] > 8_000_000
+    ni = check_prepare(scratch, isects, ref, n)
+    assert ni == ref["num_isects_raw"]
+    R.finalize_rgb(scratch, isects)
+    got = scratch.read("rgba")
+    ras = O.rasterize(O.MODE_RGB, w, h, ref["ranges"], ref["gaussian_ids"], ref["projected"], n,
+                      16, nthreads=16)
+    assert np.max(np.abs(got - ras["rgba"])) <= 1.0 / 255.0

@@ -235,9 +235,10 @@ def test_segmenter_steps(ctx):
         seen.append(camera)
         return disk.reshape(-1)
 
-    seg = G.Segmenter(sp, attrs, cam0, resolution=res, num_candidates=8, batched=True)
-    cam, eig, fg = seg.step(oracle)
-    assert cam is cam0 and eig == 0.0 and fg == int(disk.sum())
+    seg = G.Segmenter(sp, G.Config(resolution=res, candidates=8, iterations=2), cam0,
+                      batched=True)
+    r0 = seg.step(oracle)
+    assert r0.camera is cam0 and r0.eig == 0.0 and r0.fg_pixels == int(disk.sum())
     a, b = seg.state.posteriors()
     assert (a >= 1).all() and (b >= 1).all() and (a > 1).any() and (b > 1).any()
     loc = G.localize(sp, seg.state)
@@ -247,6 +248,6 @@ def test_segmenter_steps(ctx):
     cp, ep = seg.select_view(*loc)
     assert cb.position == cp.position and abs(eb - ep) <= 1e-6 * max(1.0, abs(ep))
     seg.batched = True
-    cam2, eig2, _ = seg.step(oracle)
-    assert cam2.position == cb.position and eig2 > 0.0
+    r1 = seg.step(oracle)
+    assert r1.camera.position == cb.position and r1.eig > 0.0
     assert len(seen) == 2
