@@ -64,68 +64,108 @@ def stage(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def eig_round_bytes(n, views, batched):
+    """Algorithmic bytes of an EIG round over 512^2 views at 16-px tiles
+    (10 tile bits), per view: attributes 44N (once when batched), projected
+    rows 48V, depth sort 4 x 16V, emission read 16V, isects 8I, tile sort
+    16 x 2 x I, raster 28I, responsibility clear + write 8N, EIG read 12N.
+    `views` holds (isects, visible) per view."""
+    total = 44 * n if batched else 0
+    for ni, vis in views:
+        total += (0 if batched else 44 * n) + 48 * vis + 64 * vis + 16 * vis + 8 * ni \
+            + 16 * 2 * ni + 28 * ni + 8 * n + 12 * n
+    return total
+
+
 def bench_eig_round(ctx, args):
-    """EIG-round latency: 20 candidate views at 512^2 around a 1M-splat SH3
-    scene, plain (the reference's select_view loop: per view a
-    responsibility render and a synchronous EIG, at its fastest tile width)
-    vs the batched round (one hipGraph replay, autotuned tile width x digit
-    width), with the last view's u32 evidence compared."""
+    """The B3-Seg evidence round on the 1M-splat SH3 scene: a Beta(1,1)
+    prior updated by one observation (the footprint of the splats within 0.3
+    of a seeded centre, seen from (0, 0, 3) at 512^2), then 20 Fibonacci
+    candidate views at 512^2 scored by analytic f64 EIG.  Plain = the
+    reference's select_view loop (per view a responsibility render and a
+    synchronous EIG) at each tile width; batched = one hipGraph replay at the
+    autotuned tile width x digit width.  Ratios compare equal tile widths
+    (the evidence is then bit-identical)."""
     from splatfield_amd import render as R
     from splatfield_amd import scene as S
     from splatfield_amd import seg as G
     from splatfield_amd.runtime import GpuTensor
-    n = args.n
+    n, res, nv = args.n, args.eig_res, args.eig_views
     attrs, sh, k = S.synthetic_scene(n, 3, seed=0)
     splats = R.Splats(ctx, attrs, sh)
-    cams = G.sphere_cameras(args.eig_views, radius=3.0)
-    sizes = [(args.eig_res, args.eig_res)] * args.eig_views
-    rng = np.random.default_rng(2)
-    state = G.BetaState(ctx, (1 + rng.random(n) * 4).astype(np.float32),
-                        (1 + rng.random(n) * 4).astype(np.float32))
+    cam0 = R.Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0))
+    mask, subset = G.footprint_mask(ctx, attrs, sh, n, k, cam0, res, res, seed=0)
+    state = G.observe_bootstrap(splats, cam0, res, mask)
+    cams = G.sphere_cameras(nv, radius=3.0)
+    sizes = [(res, res)] * nv
     resp = GpuTensor.empty(ctx, n)
-    # Plain: the fastest tile width for the single-view path.
     plain = {}
     for tw in (8, 16, 32):
         stage(f"eig round: plain loop, tile width {tw}")
-        scratch = R.RenderScratch(ctx, n, args.eig_res, args.eig_res, tile_w=tw)
+        scratch = R.RenderScratch(ctx, n, res, res, tile_w=tw)
         scratch.reserve_isects(1 << 24)
-        eig_p, best_p = G.eig_round_plain(splats, scratch, cams, sizes, state, resp)  # warm
+        G.eig_round_plain(splats, scratch, cams, sizes, state, resp)  # warm
         t = []
         for _ in range(args.eig_reps):
             t0 = time.perf_counter()
             eig_p, best_p = G.eig_round_plain(splats, scratch, cams, sizes, state, resp)
             t.append(time.perf_counter() - t0)
-        plain[tw] = (1e3 * float(np.median(t)), eig_p, best_p)
+        plain[tw] = (1e3 * float(np.median(t)), eig_p, best_p, resp.read())
         del scratch
-    plain_tw = min(plain, key=lambda k_: plain[k_][0])
-    plain_ms, eig_p, best_p = plain[plain_tw]
-    # Batched: the autotuner's choice of tile width x digit width.
+
+    def timed_round(tw, db):
+        rnd = G.EigRound(splats, cams, sizes, tile_w=tw, digit_bits=db)
+        rnd.run(state, use_graph=True)
+        tb = []
+        for _ in range(args.eig_reps):
+            t0 = time.perf_counter()
+            eig_b, best_b = rnd.run(state, use_graph=True)
+            tb.append(time.perf_counter() - t0)
+        return rnd, 1e3 * float(np.median(tb)), eig_b, best_b
+
     stage("eig round: autotune")
     (tw_b, db_b), table = G.autotune_round(splats, cams, sizes, state, reps=3)
-    rnd = G.EigRound(splats, cams, sizes, tile_w=tw_b, digit_bits=db_b)
-    rnd.run(state, use_graph=True)
-    tb = []
-    for _ in range(args.eig_reps):
-        t0 = time.perf_counter()
-        eig_b, best_b = rnd.run(state, use_graph=True)
-        tb.append(time.perf_counter() - t0)
-    batch_ms = 1e3 * float(np.median(tb))
-    # evidence of the last view against a TW-matched plain render
-    sc = R.RenderScratch(ctx, n, args.eig_res, args.eig_res, tile_w=tw_b)
-    sc.reserve_isects(1 << 24)
-    acc = G.Accumulators(ctx, n)
-    G.render_responsibility(splats, sc, acc, cams[-1], args.eig_res, args.eig_res)
-    exact = bool(np.array_equal(rnd.resp(args.eig_views - 1)[0], acc.bits.read()))
-    return {"views": args.eig_views, "res": args.eig_res, "plain_ms": round(plain_ms, 3),
-            "plain_tile_w": plain_tw,
-            "plain_ms_by_tile_w": {str(k_): round(v_[0], 3) for k_, v_ in plain.items()},
-            "batched_ms": round(batch_ms, 3), "batched_tile_w": tw_b, "batched_digit_bits": db_b,
-            "autotune_ms": {f"tile{t_}_digit{d_}": round(v_, 3) for (t_, d_), v_ in table.items()},
-            "ratio": round(batch_ms / plain_ms, 4),
-            "views_per_s": round(args.eig_views / (batch_ms * 1e-3), 1),
-            "best_plain": best_p, "best_batched": best_b,
-            "eig_max_abs_diff": float(np.max(np.abs(eig_b - eig_p))),
-            "last_view_evidence_bit_exact": exact, "isects_total": int(rnd.counters()[-1])}
+    rnd, batch_ms, eig_b, best_b = timed_round(tw_b, db_b)
+    p_ms, eig_p, best_p, last_resp = plain[tw_b]
+    exact = bool(np.array_equal(rnd.resp(nv - 1)[0], last_resp))
+    cnt = rnd.counters()
+    per_view = [(int(cnt[2 * v]), int(cnt[2 * v + 1])) for v in range(nv)]
+    del rnd
+    out = {"workload": "Beta(1,1) prior + one footprint-mask update, 20 Fibonacci views "
+                       "(radius 3, fov 0.8) at 512^2, 1M SH3 splats",
+           "views": nv, "res": res, "mask_fg_pixels": int(mask.sum()),
+           "target_splats": int(subset.sum()),
+           "plain_ms_by_tile_w": {str(t_): round(v_[0], 3) for t_, v_ in plain.items()},
+           "batched_ms": round(batch_ms, 3), "batched_tile_w": tw_b, "batched_digit_bits": db_b,
+           "autotune_ms": {f"tile{t_}_digit{d_}": round(v_, 3)
+                           for (t_, d_), v_ in table.items()},
+           "ratio": round(batch_ms / p_ms, 4),
+           "ratio_vs_fastest_plain": round(batch_ms / min(v_[0] for v_ in plain.values()), 4),
+           "views_per_s": round(nv / (batch_ms * 1e-3), 1),
+           "best_plain": best_p, "best_batched": best_b,
+           "eig_max_abs_diff": float(np.max(np.abs(eig_b - eig_p))),
+           "last_view_evidence_bit_exact": exact,
+           "isects_per_view": [v_[0] for v_ in per_view],
+           "visible_per_view": [v_[1] for v_ in per_view]}
+    # The reference's 16-px tiles on both sides, and the round's HBM
+    # fraction on the algorithmic bytes of that configuration.
+    if tw_b == 16:
+        ms16, db16, views16 = batch_ms, db_b, per_view
+    else:
+        r16, ms16, _, _ = timed_round(16, 8)
+        c16 = r16.counters()
+        views16 = [(int(c16[2 * v]), int(c16[2 * v + 1])) for v in range(nv)]
+        db16 = 8
+        del r16
+    b16 = eig_round_bytes(n, views16, True)
+    p16 = eig_round_bytes(n, views16, False)
+    out["tile16"] = {"batched_ms": round(ms16, 3), "digit_bits": db16,
+                     "plain_ms": round(plain[16][0], 3), "ratio": round(ms16 / plain[16][0], 4),
+                     "batched_algorithmic_bytes": b16, "plain_algorithmic_bytes": p16,
+                     "batched_gbs": round(b16 / (ms16 * 1e-3) / 1e9, 1),
+                     "batched_frac": round(b16 / (ms16 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "plain_gbs": round(p16 / (plain[16][0] * 1e-3) / 1e9, 1)}
+    return out
 
 
 def bench_c4(ctx, args):
@@ -445,10 +485,21 @@ def main():
     dom_gbs = kb[dom] / (dom_ms * 1e-3) / 1e9 if kb.get(dom) else None
     traffic = measured_traffic(dom)
 
+    # The frame's algorithmic bytes as the reference's two-stage algorithm
+    # moves them (8-bit digits, 4 depth passes, ceil(tile_bits / 8) tile
+    # passes), whatever this pipeline does instead.
+    tile_bits = max(1, math.ceil(math.log2(isects.c.tiles_x * isects.c.tiles_y)))
+    b_fwd_ref = (n * (44 + 4 * 3 * k) + 56 * vis + 16 * vis + 8 * ni + 4 * 16 * vis +
+                 math.ceil(tile_bits / 8) * 16 * ni + 40 * ni + 16 * px)
+    b_bwd_ref = 40 * ni + 32 * px + 36 * vis + 2 * (n * (44 + 4 * 3 * k)) + 36 * vis
+    frame_gbs = (b_fwd_ref + b_bwd_ref) / (ms_per_step * 1e-3) / 1e9
+    # The same for the reads and writes this pipeline must make.
     b_fwd = (kb["project"] + depth_passes * 28 * n + 28 * pairs + kb["bin_scan"] +
              kb["bin_count"] + kb["bin_place"] + kb["raster_rgb"])
     b_bwd = 40 * ni + 32 * px + 36 * vis + n * (44 + 4 * 3 * k) + 36 * vis + n * (44 + 4 * 3 * k)
     step_gbs = (b_fwd + b_bwd) / (ms_per_step * 1e-3) / 1e9
+    stage("copy ceiling: 256 MiB device copy")
+    copy_gbs = copy_ceiling(ctx)
 
     extra = {}
     if not args.no_extra:
@@ -465,8 +516,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(attrs, sh, n, k, view, w, h, args.cpu_threads, args.cpu_seconds,
-                           sc_target=None)
+        cpu = cpu_baseline(attrs, sh, pa, ps, n, k, view, w, h, args.cpu_threads,
+                           args.cpu_seconds)
 
     line = {
         "metric": "frames/s at 1M Gaussians 1080p fwd+bwd",
@@ -489,8 +540,13 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 4) if dom_gbs else None,
                      "traffic": traffic, "kernel": dom,
-                     "frame_achieved": round(step_gbs, 1),
-                     "frame_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+                     "frame_bytes": b_fwd_ref + b_bwd_ref,
+                     "frame_achieved": round(frame_gbs, 1),
+                     "frame_frac": round(frame_gbs / HBM_PEAK_GBS, 4),
+                     "frame_bytes_implemented": b_fwd + b_bwd,
+                     "frame_frac_implemented": round(step_gbs / HBM_PEAK_GBS, 4),
+                     "copy_ceiling_gbs": copy_gbs,
+                     "frame_frac_of_copy": round(frame_gbs / copy_gbs, 4)},
         "kernels": per_kernel,
         "extra": extra,
         "cpu_baseline": cpu,
@@ -514,29 +570,65 @@ def measured_traffic(scope):
         return None
 
 
-def cpu_baseline(attrs, sh, n, k, view, w, h, threads, seconds, sc_target=None):
-    """The oracle's C restatement of the same fwd+bwd frame, timed on the host
-    cores: f32 forward (project, sorts, map, composite) and the CPU reference
-    gradient, repeated for about `seconds`."""
+def host_cores():
+    """Cores this process may use, at most 16 (a GPU box's host share)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_baseline(attrs, sh, pa, ps, n, k, view, w, h, threads, seconds):
+    """The oracle's C restatement of the same fwd+bwd frame (f32 forward:
+    project, sorts, map, composite; the CPU reference gradient of the same L2
+    loss against the same perturbed-scene target), timed on the host: all
+    cores for about `seconds`, then one thread for about the same.  The
+    target render is outside the timed samples."""
     from oracle import oracle as O
-    cores = threads or min(16, os.cpu_count() or 1)
+    cores = threads or host_cores()
     ov = O.View.from_buffer_copy(bytes(view))
-    target = None
-    frames = 0
-    t0 = time.perf_counter()
-    while True:
-        r = O.render(ov, attrs, sh, n, k, w, h, nthreads=cores)
-        if target is None:
-            target = np.clip(r["rgba"] + np.float32(0.01), 0, None).astype(np.float32)
-        d = (2.0 * (r["rgba"] - target)).astype(np.float32)
-        O.backward(ov, attrs, sh, n, k, w, h, r, r, d, nthreads=cores)
-        frames += 1
-        dt = time.perf_counter() - t0
-        if dt >= seconds:
-            break
-    return {"value": round(frames / dt, 4), "unit": "frames/s", "cores": cores, "kind": "port",
-            "sample": f"{frames} full 1080p fwd+bwd frames of the same 1M-splat scene by the "
-                      f"oracle's C restatement (f32 forward + CPU reference gradient), {dt:.1f} s"}
+    target = O.render(ov, pa, ps, n, k, w, h, nthreads=cores)["rgba"]
+
+    def sample(nthreads, budget):
+        frames = 0
+        t0 = time.perf_counter()
+        while True:
+            r = O.render(ov, attrs, sh, n, k, w, h, nthreads=nthreads)
+            d = (2.0 * (r["rgba"] - target)).astype(np.float32)
+            O.backward(ov, attrs, sh, n, k, w, h, r, r, d, nthreads=nthreads)
+            frames += 1
+            dt = time.perf_counter() - t0
+            if dt >= budget:
+                return frames, dt
+    stage(f"cpu baseline: {cores} threads")
+    fa, ta = sample(cores, seconds)
+    stage("cpu baseline: 1 thread")
+    f1, t1 = sample(1, seconds)
+    return {"value": round(fa / ta, 4), "unit": "frames/s", "cores": cores, "kind": "port",
+            "single_thread": round(f1 / t1, 4),
+            "sample": f"the same 1080p fwd+bwd frame (1M SH3 splats, L2 loss against the "
+                      f"perturbed-scene target) by the oracle's C restatement (f32 forward + "
+                      f"CPU reference gradient): {fa} frames in {ta:.1f} s on {cores} threads; "
+                      f"{f1} frames in {t1:.1f} s on 1 thread"}
+
+
+def copy_ceiling(ctx, reps=5):
+    """Measured device-copy bandwidth (256 MiB read + 256 MiB written per
+    copy), GB/s: the practical ceiling under the 8 TB/s vendor peak."""
+    from splatfield_amd.runtime import GpuTensor
+    from splatfield_amd._lib import lib
+    n = 64 << 20
+    src = GpuTensor.empty(ctx, n)
+    dst = GpuTensor.empty(ctx, n)
+    src.fill_u32(7)
+    ctx.check(lib().sfk_copy_u32(ctx.ptr, dst.ptr, src.ptr, n), "copy")
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(reps):
+        ctx.check(lib().sfk_copy_u32(ctx.ptr, dst.ptr, src.ptr, n), "copy")
+    ms = ctx.timer_stop_ms() / reps
+    return round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)
 
 
 if __name__ == "__main__":
