@@ -157,12 +157,15 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
             uint32_t g = a.gids[idx];
             const float4* row = J.proj + (size_t)g * J.rows4;
             float4 geo = row[0], aux = row[1];
-            s_geo[t] = geo;
-            s_aux[t] = aux;
             s_box[t] = reach_box(geo, aux, &s_tau[t]);
             if constexpr (MODE == kRgb) {
+                // Conic a and c staged halved, as the adjoint stages them.
+                s_geo[t] = make_float4(geo.x, geo.y, 0.5f * geo.z, geo.w);
+                s_aux[t] = make_float4(0.5f * aux.x, aux.y, aux.z, aux.w);
                 s_blue[t] = row[2].x;
             } else {
+                s_geo[t] = geo;
+                s_aux[t] = aux;
                 s_gid[t] = g;
                 s_q0[t] = 0;
                 if constexpr (MODE == kEvidence) s_q1[t] = 0;
@@ -171,7 +174,10 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
         // Staging fence and whole-tile early exit (raster.rs:166-171).
         if (__syncthreads_count(!done) == 0) break;
         const uint32_t cnt = (end - chunk) < (uint32_t)NT ? (end - chunk) : (uint32_t)NT;
-        const uint32_t nl = __ballot(!done) ? wave_reach_list(s_box, s_tau, s_geo, s_aux, cnt, bx0, by0, list) : 0u;
+        constexpr int CH = MODE == kRgb ? 2 : 1;
+        const uint32_t nl =
+            __ballot(!done) ? wave_reach_list<CH>(s_box, s_tau, s_geo, s_aux, cnt, bx0, by0, list)
+                            : 0u;
         // Listed splats in groups of 8 (entries past the list end repeat the
         // last entry and are masked off).  Each member is evaluated straight
         // through for the whole wave once any lane may be reached; lanes that
@@ -209,17 +215,32 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
                     g4 = s_geo[jj[s]]; x4 = s_aux[jj[s]];
                 }
                 const float dx = g4.x - pxf, dy = g4.y - pyf;
-                const float power = 0.5f * (g4.z * dx * dx + x4.x * dy * dy) + g4.w * dx * dy;
+                float power;
+                if constexpr (MODE == kRgb) {
+                    // The adjoint's power and exp, operation for operation,
+                    // so both passes take the same alpha and contributor
+                    // decisions; the image is held to 1/255 of the exact
+                    // evaluation, not to its bits.
+                    power = fmaf(dx, fmaf(g4.z, dx, g4.w * dy), x4.x * dy * dy);
+                } else {
+                    power = 0.5f * (g4.z * dx * dx + x4.x * dy * dy) + g4.w * dx * dy;
+                }
                 const bool in = !done && i0 + s < nl && !(power > kPowerSkip);
                 if (!__ballot(in)) continue;
-                const float alpha = fminf(x4.y * sfk_expf_neg_in(power), 0.999f);
+                // RGB: exp(-power) by the hardware exp2 (argument capped as in
+                // the adjoint); the responsibility modes keep the exact exp
+                // that their bit-exact u32 weights are specified with.
+                const float G = MODE == kRgb
+                                    ? __builtin_amdgcn_exp2f(fminf(-power * 1.44269504f, 126.0f))
+                                    : sfk_expf_neg_in(power);
+                const float alpha = fminf(x4.y * G, 0.999f);
                 const bool con = in && alpha >= kInv255;
                 const float vis = alpha * T;
                 if constexpr (MODE == kRgb) {
                     if constexpr (!PF) bl = s_blue[jj[s]];
-                    cr = con ? cr + x4.z * vis : cr;
-                    cg = con ? cg + x4.w * vis : cg;
-                    cb = con ? cb + bl * vis : cb;
+                    cr = con ? fmaf(x4.z, vis, cr) : cr;
+                    cg = con ? fmaf(x4.w, vis, cg) : cg;
+                    cb = con ? fmaf(bl, vis, cb) : cb;
                 } else {
                     q[s] = con ? sfk_f2u(vis * J.scale) : 0u;
                     any |= q[s] != 0;

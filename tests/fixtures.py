@@ -150,3 +150,22 @@ def sphere_oracle(spheres):
             hit |= d2 <= float(r) * float(r)
         return hit.astype(np.uint8).reshape(-1)
     return oracle
+
+
+def assert_image_close(rgba, final_t, bitmap, ref, n_contrib=None):
+    """The RGB render contract against the oracle's exact evaluation: every
+    f32 channel within 1/255 (the device takes exp(-power) from the hardware
+    exp2), the mean error far below that (no systematic drift), the RGBA8
+    bytes within one step, and the per-pixel last contributor equal on all
+    but a sliver of pixels (a decision may flip only where alpha is within
+    rounding of 1/255)."""
+    d = np.abs(np.asarray(rgba, np.float64) - ref["rgba"])
+    assert d.max() <= 1.0 / 255.0, d.max()
+    assert d.mean() <= 1e-5, d.mean()
+    dt = np.abs(np.asarray(final_t, np.float64) - ref["final_t"])
+    assert dt.max() <= 1.0 / 255.0, dt.max()
+    gb = np.asarray(bitmap, np.uint32).view(np.uint8).astype(np.int16)
+    rb = np.asarray(ref["bitmap"], np.uint32).view(np.uint8).astype(np.int16)
+    assert np.abs(gb - rb).max() <= 1
+    if n_contrib is not None:
+        assert np.mean(np.asarray(n_contrib) != ref["n_contrib"]) <= 1e-3

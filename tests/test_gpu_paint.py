@@ -1,5 +1,6 @@
 """GPU parity of the GUI paint ops (src/seg/paint.rs) against the oracle:
-the posterior-mean render is bit-exact with the oracle's paint + RGB raster,
+the posterior-mean render meets the render contract (1/255) against the
+oracle's paint + RGB raster,
 the tint writes exactly the oracle's DC planes, and a save / tint / restore
 round trip gives back the original frame word for word."""
 import numpy as np
@@ -9,6 +10,8 @@ from oracle import oracle as O
 from splatfield_amd import render as R
 from splatfield_amd import scene as S
 from splatfield_amd.runtime import GpuTensor
+
+from .fixtures import assert_image_close
 
 pytestmark = pytest.mark.gpu
 
@@ -31,8 +34,8 @@ def posterior(n, seed):
 @pytest.mark.parametrize("per_deg", [0, 3])
 def test_render_posterior_matches_oracle(ctx, per_deg):
     """render_posterior (paint.rs:22-47): projected colours replaced by
-    a/(a+b) before blending; rgba bits, T and the RGBA8 bitmap equal the
-    oracle's."""
+    a/(a+b) before blending; the image, T and the RGBA8 bitmap within the
+    render contract of the oracle's."""
     n = 16384
     attrs, sh, k = S.synthetic_scene(n, per_deg, seed=31 + per_deg)
     a, b = posterior(n, 5)
@@ -44,9 +47,8 @@ def test_render_posterior_matches_oracle(ctx, per_deg):
     ref = O.render_posterior(O.View.from_buffer_copy(bytes(cam.view(W, H))), attrs, sh, n, k, W,
                              H, a, b)
     got = scratch.read("rgba")
-    np.testing.assert_array_equal(got.view(np.uint32), ref["rgba"].view(np.uint32), "rgba bits")
-    np.testing.assert_array_equal(scratch.read("final_t"), ref["final_t"], "final T")
-    np.testing.assert_array_equal(bm.reshape(-1), ref["bitmap"], "rgba8 bitmap")
+    assert_image_close(got, scratch.read("final_t"), bm.reshape(-1), ref,
+                       scratch.read("n_contrib"))
     px = got.reshape(-1, 4)
     assert np.all(px[:, 0] == px[:, 1]) and np.all(px[:, 1] == px[:, 2]), "grayscale"
     assert px[:, 3].max() > 0.5

@@ -2,9 +2,10 @@
 tile sort -> ranges -> raster) against the oracle, on the same inputs.
 
 Integer outputs (visibility, depth keys, tile counts, bboxes, counters, sort
-order, offsets, tile ranges, u32 epsilon) must be bit-exact.  The f32 image is
-compared bit-exact too (both sides evaluate the same operations and the same
-exp/ln); the contract tolerance is 1/255 per channel."""
+order, offsets, tile ranges, u32 epsilon) must be bit-exact.  The RGB image
+is held to the contract's 1/255 per channel (the device's RGB compositor
+takes exp(-power) from the hardware exp2; the responsibility modes keep the
+exact exp their u32 words are specified with)."""
 import numpy as np
 import pytest
 
@@ -14,7 +15,8 @@ from splatfield_amd import scene as S
 from splatfield_amd import seg as G
 from splatfield_amd.runtime import GpuTensor
 
-from .fixtures import GOLDEN_EPS, overdraw_attributes, sample_opaque_attributes
+from .fixtures import (GOLDEN_EPS, assert_image_close, overdraw_attributes,
+                       sample_opaque_attributes)
 
 pytestmark = pytest.mark.gpu
 
@@ -75,15 +77,10 @@ def check_prepare(scratch, isects, ref, n):
 
 def check_rgb(scratch, isects, ref, n, w, h, tile_w):
     R.finalize_rgb(scratch, isects)
-    got = scratch.read("rgba")
     ras = O.rasterize(O.MODE_RGB, w, h, ref["ranges"], ref["gaussian_ids"], ref["projected"], n,
                       tile_w)
-    assert np.max(np.abs(got - ras["rgba"])) <= 1.0 / 255.0
-    np.testing.assert_array_equal(got.view(np.uint32), ras["rgba"].view(np.uint32), "rgba bits")
-    np.testing.assert_array_equal(scratch.read("final_t"), ras["final_t"], "final T")
-    np.testing.assert_array_equal(scratch.read("n_contrib"), ras["n_contrib"], "n_contrib")
-    bm = scratch.read("bitmap")
-    np.testing.assert_array_equal(bm, ras["bitmap"], "rgba8 bitmap")
+    assert_image_close(scratch.read("rgba"), scratch.read("final_t"), scratch.read("bitmap"), ras,
+                       scratch.read("n_contrib"))
     return ras
 
 
@@ -354,7 +351,7 @@ def test_c2_full_frame_against_oracle(ctx):
     ni = check_prepare(scratch, isects, ref, n)
     assert ni == ref["num_isects_raw"]
     R.finalize_rgb(scratch, isects)
-    got = scratch.read("rgba")
     ras = O.rasterize(O.MODE_RGB, w, h, ref["ranges"], ref["gaussian_ids"], ref["projected"], n,
                       16, nthreads=16)
-    assert np.max(np.abs(got - ras["rgba"])) <= 1.0 / 255.0
+    assert_image_close(scratch.read("rgba"), scratch.read("final_t"), scratch.read("bitmap"), ras,
+                       scratch.read("n_contrib"))
