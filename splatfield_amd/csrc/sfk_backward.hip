@@ -56,6 +56,7 @@ constexpr int kBwdMembers = 4;
 struct BwdArgs {
     uint32_t w, h, tiles_x, tiles_y, n;
     const uint32_t* tile_starts;
+    const uint32_t* tile_order;  // optional launch order of the tiles (heaviest first)
     const uint32_t* gids;
     const float* proj;
     const float* final_t;
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
     __shared__ double s_w[NW];
 
     const uint32_t ntiles = a.tiles_x * a.tiles_y;
-    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
+    const uint32_t tile = a.tile_order ? a.tile_order[blockIdx.x] : xcd_remap(blockIdx.x, ntiles);
     const uint32_t tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const uint32_t t = threadIdx.x;
     uint32_t lx, ly;
@@ -585,7 +586,7 @@ hipError_t launch_raster_bwd(hipStream_t s, const RasterArgs& ra, const float* f
                              float* grad2d, const BwdLoss* loss) {
     BwdArgs a;
     a.w = ra.w; a.h = ra.h; a.tiles_x = ra.tiles_x; a.tiles_y = ra.tiles_y; a.n = n;
-    a.tile_starts = ra.tile_starts; a.gids = ra.gids; a.proj = ra.proj;
+    a.tile_starts = ra.tile_starts; a.tile_order = ra.tile_order; a.gids = ra.gids; a.proj = ra.proj;
     a.final_t = final_t; a.n_contrib = n_contrib; a.d_img = d_img; a.grad2d = grad2d;
     a.img = a.target = nullptr;
     a.loss_partials = nullptr;

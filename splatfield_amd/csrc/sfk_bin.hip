@@ -379,6 +379,8 @@ struct ChunkArgs {
     const uint32_t* tile_starts;   // place pass
     uint32_t* gids;
     uint32_t* tile_ids;
+    uint32_t* tile_order;          // place pass, optional: tiles heaviest first
+    uint32_t order_tiles;
 };
 
 // Bits (ty - oy) * 8 + (tx - ox) of the supertile at tile (ox, oy) that the
@@ -432,13 +434,87 @@ __device__ __forceinline__ uint32_t round_totals(const uint32_t (&cnt)[N]) {
 }
 constexpr uint32_t kRoundLane = 64 / kRounds;
 
+// Work class of a tile with c isects: 0 for none, else 1 + 2 log2(c) + the
+// bit below the leading one (classes a factor of ~1.4 apart), at most 64.
+__device__ __forceinline__ uint32_t work_class(uint32_t c) {
+    if (!c) return 0u;
+    const uint32_t p = 31u - (uint32_t)__clz(c);
+    const uint32_t sub = p ? (c >> (p - 1u)) & 1u : 0u;
+    return 1u + 2u * p + sub;
+}
+
+// Lanes of the wave holding the same work class as this lane (7 ballots),
+// restricted to valid lanes.
+__device__ __forceinline__ uint64_t class_peers(uint32_t k, bool valid) {
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (uint32_t bit = 0; bit < 7; ++bit) {
+        const bool set = (k >> bit) & 1u;
+        const uint64_t bal = __ballot(set);
+        peers &= set ? bal : ~bal;
+    }
+    return peers;
+}
+
+// The rasterizers' launch order: the n tiles of `starts` (exclusive
+// prefixes, total at [n]) heaviest work class first, by a counting sort on
+// work_class of their isect counts (order inside a class unspecified).  One
+// 256-thread workgroup; hist holds 65 words of LDS.
+__device__ void tile_order_block(const uint32_t* __restrict__ starts, uint32_t n,
+                                 uint32_t* __restrict__ order, uint32_t* hist) {
+    for (uint32_t i = threadIdx.x; i < 65; i += 256) hist[i] = 0;
+    __syncthreads();
+    for (uint32_t t0 = 0; t0 < n; t0 += 256) {
+        const uint32_t t = t0 + threadIdx.x;
+        const bool valid = t < n;
+        const uint32_t k = valid ? work_class(starts[t + 1] - starts[t]) : 0u;
+        const uint64_t peers = class_peers(k, valid);
+        if (valid && mask_rank(peers) == 0) atomicAdd(&hist[k], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int k = 64; k >= 0; --k) {
+            const uint32_t c = hist[k];
+            hist[k] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t t0 = 0; t0 < n; t0 += 256) {
+        const uint32_t t = t0 + threadIdx.x;
+        const bool valid = t < n;
+        const uint32_t k = valid ? work_class(starts[t + 1] - starts[t]) : 0u;
+        const uint64_t peers = class_peers(k, valid);
+        const uint32_t rank = mask_rank(peers);
+        uint32_t base = 0;
+        if (valid && rank == 0) base = atomicAdd(&hist[k], (uint32_t)__popcll(peers));
+        const int leader = valid ? __ffsll((unsigned long long)peers) - 1 : 0;
+        base = __shfl(base, leader, 64);
+        if (valid) order[base + rank] = t;
+    }
+}
+
+// With a.tile_order (place pass), workgroup 0 writes the tile order, off
+// the critical path, while the others place isects.
 template <bool PLACE>
 __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
     constexpr uint32_t SW = PLACE ? 4 : 1;
     __shared__ uint16_t s_st[SW][PLACE ? kStage : 1];
     __shared__ uint32_t s_gid[SW][PLACE ? kBinChunk : 1];
+    uint32_t blk = blockIdx.x, nblk = gridDim.x;
+    if constexpr (PLACE) {
+        if (a.tile_order) {
+            if (blockIdx.x == 0) {
+                tile_order_block(a.tile_starts, a.order_tiles, a.tile_order, &s_gid[0][0]);
+                return;
+            }
+            blk -= 1;
+            nblk -= 1;
+        }
+    }
     const uint32_t lane = lane_id(), w = threadIdx.x / 64u;
-    const uint32_t wave = blockIdx.x * 4u + w, nwaves = gridDim.x * 4u;
+    const uint32_t wave = blk * 4u + w, nwaves = nblk * 4u;
     const uint32_t slots = a.sstart[a.supers] / kBinChunk + a.supers;
     const TransposeLanes tl = transpose_lanes();
     for (uint32_t c = wave; c < slots; c += nwaves) {
@@ -726,6 +802,8 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     ca.tile_starts = j.tile_starts;
     ca.gids = j.gids;
     ca.tile_ids = j.tile_ids;
+    ca.tile_order = nullptr;
+    ca.order_tiles = 0;
     const uint32_t grid = std::min(div_up(bin_chunk_slots(j.cap, j.supers), 4), 2048u);
     {
         SFK_PROF(c, "bin_count");
@@ -748,7 +826,10 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     }
     {
         SFK_PROF(c, "bin_place");
-        k_bin_chunks<true><<<grid, 256, 0, st>>>(ca);
+        const bool order = j.tile_order && j.out_tiles <= kTileOrderMax;
+        ca.tile_order = order ? j.tile_order : nullptr;
+        ca.order_tiles = j.out_tiles;
+        k_bin_chunks<true><<<grid + (order ? 1u : 0u), 256, 0, st>>>(ca);
         SFK_LAUNCHED(c, "bin place");
     }
     return SFK_OK;

@@ -226,6 +226,10 @@ inline uint32_t bin_supers(uint32_t tiles_x, uint32_t tiles_y) {
 inline uint32_t bin_chunk_slots(uint32_t cap, uint32_t supers) {
     return cap / kBinChunk + supers + 1;
 }
+// Largest tile count for which the binned emission writes a heavy-first
+// tile order (the rasterizers' launch order).
+constexpr uint32_t kTileOrderMax = 32768;
+
 struct BinJob {
     const uint2* ordered;   // (in-view splat, compact bbox) per depth rank
     uint32_t n;             // ranks (all views)
@@ -249,6 +253,8 @@ struct BinJob {
     uint32_t* chunk_counts; // 64 per chunk slot
     uint32_t* scan_sums;    // div_up(out_tiles, kScanBlock) words
     uint32_t* tile_starts;  // out_tiles + 1
+    uint32_t* tile_order;   // optional, out_tiles: tiles heaviest first (written only
+                            // when out_tiles <= kTileOrderMax)
     uint32_t* gids;         // cap: splat per isect
     uint32_t* tile_ids;     // cap or nullptr: tile per isect
     uint32_t* counters;     // [0] raw isect count, [1] pair count
@@ -277,6 +283,7 @@ struct RasterArgs {
     uint32_t w, h, tiles_x, tiles_y, tile_w, row_stride;
     float scale;
     const uint32_t* tile_starts;
+    const uint32_t* tile_order;  // optional launch order of the tiles (heaviest first)
     const uint32_t* gids;
     const float* proj;
     const uint32_t* mask;

@@ -308,7 +308,7 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
 template <int MODE, int TW>
 __global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
     const uint32_t ntiles = a.tiles_x * a.tiles_y;
-    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
+    const uint32_t tile = a.tile_order ? a.tile_order[blockIdx.x] : xcd_remap(blockIdx.x, ntiles);
     TileJob J;
     J.w = a.w; J.h = a.h;
     J.tx = tile % a.tiles_x;

@@ -192,6 +192,10 @@ struct sfk_render_scratch {
     uint32_t* tile_ids = nullptr;
     uint32_t* gaussian_ids = nullptr;
     uint32_t* tile_starts = nullptr;
+    // Tiles heaviest first, written by the binned emission of grids of at
+    // most kTileOrderMax tiles; order_valid while the last prepare wrote it.
+    uint32_t* tile_order = nullptr;
+    bool order_valid = false;
     uint32_t* bitmap = nullptr;
     float* rgba = nullptr;
     float* final_t = nullptr;
@@ -241,13 +245,15 @@ static void scratch_free_frame(sfk_render_scratch* s) {
                     s->counters, s->tile_starts, s->bitmap, s->rgba, s->final_t, s->n_contrib,
                     s->tile_ids, s->gaussian_ids, s->grad2d, s->emit_status, s->cbbox,
                     s->order_buf, s->bin_sstart, s->bin_map, s->bin_counts, s->bin_offs,
-                    s->dcol};
+                    s->dcol, s->tile_order};
     for (void* p : bufs)
         if (p) hipFree(p);
     s->depth_keys = nullptr; s->projected = nullptr; s->tile_counts = nullptr; s->bbox = nullptr;
     s->offsets = nullptr; s->counters = nullptr; s->tile_starts = nullptr; s->bitmap = nullptr;
     s->rgba = nullptr; s->final_t = nullptr; s->n_contrib = nullptr; s->tile_ids = nullptr;
     s->gaussian_ids = nullptr;
+    s->tile_order = nullptr;
+    s->order_valid = false;
     s->grad2d = nullptr;
     s->grad2d_dirty = true;
     s->emit_status = nullptr;
@@ -304,6 +310,7 @@ static int scratch_build(sfk_render_scratch* s, uint32_t n, uint32_t w, uint32_t
     SFK_HIP(c, hipMalloc(&s->offsets, nn * 4));
     SFK_HIP(c, hipMalloc(&s->counters, kCounterWords * 4));
     SFK_HIP(c, hipMalloc(&s->tile_starts, (size_t)(tiles + 1) * 4));
+    if (tiles <= kTileOrderMax) SFK_HIP(c, hipMalloc(&s->tile_order, (size_t)tiles * 4));
     SFK_HIP(c, hipMalloc(&s->bitmap, (size_t)h * s->row_stride * 4));
     SFK_HIP(c, hipMalloc(&s->rgba, px * 16));
     SFK_HIP(c, hipMalloc(&s->final_t, px * 4));
@@ -471,6 +478,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
     const uint32_t n = sp->n;
     const uint32_t tiles = s->tiles_x * s->tiles_y;
     const bool compact = compact_grid(s);
+    s->order_valid = false;
 
     // Counters (and, for the binned emission, its look-back words) start at 0.
     const bool binned = compact && s->binned && s->capacity >= 2 && s->capacity <= kBinMaxCap;
@@ -524,6 +532,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
             j.chunk_map = s->bin_map;
             j.chunk_counts = s->bin_counts;
             j.tile_starts = s->tile_starts;
+            j.tile_order = s->tile_order;
             j.gids = s->gaussian_ids;
             j.tile_ids = s->tile_ids;
             j.counters = s->counters;
@@ -532,6 +541,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
             j.host_total = s->host_counters_dev;  // the raw count, for next frame's capacity
             rc = bin_emit(c, j);
             if (rc) return rc;
+            s->order_valid = s->tile_order != nullptr;
             SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));
             s->counters_pending = true;
             s->depth_order = s->order_buf;
@@ -612,6 +622,7 @@ static RasterArgs raster_args(sfk_render_scratch* s, const sfk_tile_isects* t) {
     a.w = t->w; a.h = t->h; a.tiles_x = t->tiles_x; a.tiles_y = t->tiles_y;
     a.tile_w = t->tile_w; a.row_stride = t->row_stride;
     a.tile_starts = t->tile_starts;
+    a.tile_order = s->order_valid ? s->tile_order : nullptr;
     a.gids = t->gaussian_ids;
     a.proj = s->projected;
     return a;
