@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--eig-views", type=int, default=20)
     p.add_argument("--eig-res", type=int, default=512)
     p.add_argument("--eig-reps", type=int, default=5)
+    p.add_argument("--leg", choices=["eig", "sort", "c4"], default=None,
+                   help="run only this extra measurement (no frame), print its JSON")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a 256 MiB device copy (known traffic, for counter scaling)")
     return p.parse_args()
@@ -397,6 +399,12 @@ def main():
     if args.calibrate_copy:
         ctx = Context(0)
         calibrate_copy(ctx)
+        ctx.close()
+        return
+    if args.leg:
+        ctx = Context(0)
+        fn = {"eig": bench_eig_round, "sort": bench_sort, "c4": bench_c4}[args.leg]
+        print(json.dumps({args.leg: fn(ctx, args)}))
         ctx.close()
         return
     n, w, h = args.n, args.width, args.height

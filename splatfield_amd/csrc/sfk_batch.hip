@@ -189,6 +189,9 @@ struct sfk_eig_round {
     uint32_t* bin_map = nullptr;
     uint32_t* bin_counts = nullptr;
     uint32_t* scan_sums = nullptr;
+    // Tile-start slots heaviest first (the batched raster's launch order),
+    // written by the binned emission when the slot table is small enough.
+    uint32_t* tile_order = nullptr;
     // Depth keys rebased per view to 24 bits when every view's AABB depth
     // span allows it (depth_rebase).
     uint32_t depth_bits = 32;
@@ -279,6 +282,7 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
             j.chunk_counts = r->bin_counts;
             j.scan_sums = r->scan_sums;
             j.tile_starts = r->starts;
+            j.tile_order = r->tile_order;
             j.gids = r->gids;
             j.counters = r->counters;
             j.status = r->emit_status;
@@ -334,6 +338,8 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
         ra.nviews = V;
         ra.batch_tiles = r->total_tiles;
         ra.seg_len = n;
+        // (slot order of the binned emission; the sorted path writes none)
+        ra.tile_order = r->sorted_tiles == nullptr ? r->tile_order : nullptr;
         SFK_HIP(c, launch_rasterize_batch(c->stream, ra));
     }
     {
@@ -455,6 +461,8 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
         RALLOC(r->offsets, vn * 4);
         RALLOC(r->bin_sstart, ((size_t)r->supers + 1) * 4);
         RALLOC(r->scan_sums, (size_t)div_up((uint64_t)r->max_tiles * nviews, kScanBlock) * 4);
+        if ((uint64_t)r->max_tiles * nviews <= kTileOrderMax)
+            RALLOC(r->tile_order, (size_t)r->max_tiles * nviews * 4);
     } else {
         RALLOC(r->counts, vn * 4);
         RALLOC(r->bbox, vn * 8);
@@ -500,7 +508,7 @@ extern "C" void sfk_eig_round_destroy(sfk_eig_round* r) {
     void* bufs[] = {r->d_views, r->d_bviews, r->proj, r->keys, r->counts, r->bbox, r->cbbox,
                     r->counters, r->starts, r->resp, r->h0, r->partials, r->eig, r->best,
                     r->emit_status, r->d_binviews, r->offsets, r->bin_sstart, r->scan_sums,
-                    r->d_rebase};
+                    r->d_rebase, r->tile_order};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (r->host_out) hipHostFree(r->host_out);

@@ -462,14 +462,24 @@ __device__ __forceinline__ uint64_t class_peers(uint32_t k, bool valid) {
 // 256-thread workgroup; hist holds 65 words of LDS.
 __device__ void tile_order_block(const uint32_t* __restrict__ starts, uint32_t n,
                                  uint32_t* __restrict__ order, uint32_t* hist) {
+    constexpr uint32_t U = 8;  // tiles per thread whose counts load together
     for (uint32_t i = threadIdx.x; i < 65; i += 256) hist[i] = 0;
     __syncthreads();
-    for (uint32_t t0 = 0; t0 < n; t0 += 256) {
-        const uint32_t t = t0 + threadIdx.x;
-        const bool valid = t < n;
-        const uint32_t k = valid ? work_class(starts[t + 1] - starts[t]) : 0u;
-        const uint64_t peers = class_peers(k, valid);
-        if (valid && mask_rank(peers) == 0) atomicAdd(&hist[k], (uint32_t)__popcll(peers));
+    for (uint32_t t0 = 0; t0 < n; t0 += 256 * U) {
+        uint32_t k[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t t = t0 + u * 256 + threadIdx.x;
+            k[u] = t < n ? starts[t + 1] - starts[t] : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t t = t0 + u * 256 + threadIdx.x;
+            const bool valid = t < n;
+            const uint32_t kc = work_class(k[u]);
+            const uint64_t peers = class_peers(kc, valid);
+            if (valid && mask_rank(peers) == 0) atomicAdd(&hist[kc], (uint32_t)__popcll(peers));
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -481,17 +491,26 @@ __device__ void tile_order_block(const uint32_t* __restrict__ starts, uint32_t n
         }
     }
     __syncthreads();
-    for (uint32_t t0 = 0; t0 < n; t0 += 256) {
-        const uint32_t t = t0 + threadIdx.x;
-        const bool valid = t < n;
-        const uint32_t k = valid ? work_class(starts[t + 1] - starts[t]) : 0u;
-        const uint64_t peers = class_peers(k, valid);
-        const uint32_t rank = mask_rank(peers);
-        uint32_t base = 0;
-        if (valid && rank == 0) base = atomicAdd(&hist[k], (uint32_t)__popcll(peers));
-        const int leader = valid ? __ffsll((unsigned long long)peers) - 1 : 0;
-        base = __shfl(base, leader, 64);
-        if (valid) order[base + rank] = t;
+    for (uint32_t t0 = 0; t0 < n; t0 += 256 * U) {
+        uint32_t k[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t t = t0 + u * 256 + threadIdx.x;
+            k[u] = t < n ? starts[t + 1] - starts[t] : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t t = t0 + u * 256 + threadIdx.x;
+            const bool valid = t < n;
+            const uint32_t kc = work_class(k[u]);
+            const uint64_t peers = class_peers(kc, valid);
+            const uint32_t rank = mask_rank(peers);
+            uint32_t base = 0;
+            if (valid && rank == 0) base = atomicAdd(&hist[kc], (uint32_t)__popcll(peers));
+            const int leader = valid ? __ffsll((unsigned long long)peers) - 1 : 0;
+            base = __shfl(base, leader, 64);
+            if (valid) order[base + rank] = t;
+        }
     }
 }
 

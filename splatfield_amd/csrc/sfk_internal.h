@@ -228,7 +228,7 @@ inline uint32_t bin_chunk_slots(uint32_t cap, uint32_t supers) {
 }
 // Largest tile count for which the binned emission writes a heavy-first
 // tile order (the rasterizers' launch order).
-constexpr uint32_t kTileOrderMax = 32768;
+constexpr uint32_t kTileOrderMax = 65536;
 
 struct BinJob {
     const uint2* ordered;   // (in-view splat, compact bbox) per depth rank

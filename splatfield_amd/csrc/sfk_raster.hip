@@ -327,16 +327,27 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
 template <int TW>
 __global__ __launch_bounds__(TW * TW) void k_rasterize_batch(RasterArgs a) {
     const uint32_t total = a.batch_tiles;
-    const uint32_t tile = xcd_remap(blockIdx.x, total);
-    // view of this tile: last view whose tile base <= tile
-    uint32_t lo = 0, hi = a.nviews;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) / 2;
-        if (a.views[mid].tile_base <= tile) lo = mid;
-        else hi = mid;
+    uint32_t lo, local;
+    if (a.tile_order) {
+        // A start-table slot local * nviews + view, heaviest first; slots
+        // past a smaller view's grid have empty ranges and are skipped.
+        const uint32_t slot = a.tile_order[blockIdx.x];
+        lo = slot % a.nviews;
+        local = slot / a.nviews;
+        if (local >= a.views[lo].tiles_x * a.views[lo].tiles_y) return;
+    } else {
+        const uint32_t tile = xcd_remap(blockIdx.x, total);
+        // view of this tile: last view whose tile base <= tile
+        lo = 0;
+        uint32_t hi = a.nviews;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) / 2;
+            if (a.views[mid].tile_base <= tile) lo = mid;
+            else hi = mid;
+        }
+        local = tile - a.views[lo].tile_base;
     }
     const BatchView bv = a.views[lo];
-    const uint32_t local = tile - bv.tile_base;
     TileJob J;
     J.w = bv.w; J.h = bv.h;
     J.tx = local % bv.tiles_x;
