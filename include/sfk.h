@@ -315,7 +315,9 @@ int sfk_ply_parse_header(const void* bytes, size_t len, sfk_ply_layout* out, cha
 /* parse_ply's row loop (ply.rs:78-106) on the device: rows (n x stride
  * floats, as in the file) to the 11 attribute planes (rotation normalised)
  * and sh_planes SH planes (DC, then f_rest_k channel-major in the file to
- * coefficient-interleaved planes). */
+ * coefficient-interleaved planes).  rows must be 4-byte aligned (a device
+ * copy of the file plus data_offset may not be: SFK_E_ARG); 16-byte aligned
+ * rows take vector loads, others scalar ones. */
 int sfk_ply_decode(sfk_ctx* ctx, const sfk_ply_layout* layout, const float* rows, float* attrs,
                    float* sh);
 /* Splats from a PLY file image in host memory (lib.rs:45-50 + Splats::new):

@@ -28,6 +28,7 @@ struct PlyArgs {
     float* attrs;
     float* sh;
     uint64_t n;
+    uint32_t vec4;  // rows 16-byte aligned: stage with float4 loads
     uint32_t stride, n_rest, sh_planes;
     int32_t base[14];
     int32_t rest[SFK_PLY_MAX_REST];
@@ -40,10 +41,11 @@ __global__ __launch_bounds__(256) void k_ply_decode(PlyArgs a) {
     __shared__ __align__(16) float rows[kPlyRows * SFK_PLY_MAX_STRIDE];
     const uint64_t v0 = (uint64_t)blockIdx.x * kPlyRows;
     const uint32_t nv = (uint32_t)min((uint64_t)kPlyRows, a.n - v0);
-    // The block's rows start 64 * stride floats apart: 16-byte aligned, so
-    // they load as float4 (a ragged last block finishes with scalars).
+    // The block's rows start 64 * stride floats apart: with a 16-byte
+    // aligned base they load as float4 (a ragged last block finishes with
+    // scalars); rows at an arbitrary float offset of the file load singly.
     const float* src = a.rows + v0 * a.stride;
-    const uint32_t total = nv * a.stride, n4 = total / 4;
+    const uint32_t total = nv * a.stride, n4 = a.vec4 ? total / 4 : 0u;
     const float4* src4 = reinterpret_cast<const float4*>(src);
     float4* rows4 = reinterpret_cast<float4*>(rows);
     for (uint32_t e = threadIdx.x; e < n4; e += 256) rows4[e] = src4[e];
@@ -124,6 +126,7 @@ hipError_t launch_ply_decode(hipStream_t s, const sfk_ply_layout& l, const float
     a.stride = l.stride;
     a.n_rest = l.n_rest;
     a.sh_planes = l.sh_planes;
+    a.vec4 = (reinterpret_cast<uintptr_t>(rows) & 15u) == 0;
     memcpy(a.base, l.base, sizeof a.base);
     memcpy(a.rest, l.rest, sizeof a.rest);
     const uint64_t blocks = (l.n + kPlyRows - 1) / kPlyRows;
@@ -227,6 +230,12 @@ extern "C" int sfk_ply_decode(sfk_ctx* c, const sfk_ply_layout* l, const float* 
     for (uint32_t k = 0; k < l->n_rest; ++k)
         if (l->rest[k] < 0 || (uint32_t)l->rest[k] >= l->stride)
             return fail(c, SFK_E_ARG, "ply_decode: column %d out of the row", l->rest[k]);
+    // A device copy of the file plus data_offset lands wherever the header
+    // ended: float loads need 4-byte alignment (16 for the vector path,
+    // chosen per call).
+    if (reinterpret_cast<uintptr_t>(rows) & 3u)
+        return fail(c, SFK_E_ARG,
+                    "ply_decode: rows at a misaligned address (copy them to a 4-byte boundary)");
     SFK_PROF(c, "ply_decode");
     SFK_HIP(c, launch_ply_decode(c->stream, *l, rows, attrs, sh));
     return SFK_OK;

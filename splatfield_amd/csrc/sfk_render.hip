@@ -216,7 +216,8 @@ struct sfk_render_scratch {
     const uint32_t* sorted_gids = nullptr;
     sfk_view view;  // view of the last prepare
     bool has_view = false;
-    bool has_colors = false;  // the last prepare projected SH colours
+    bool has_colors = false;  // the last prepare wrote colours (SH or posterior)
+    bool sh_colors = false;   // ... and they are this frame's SH colours
     float* grad2d = nullptr;  // 9 planes x n screen-space gradients (backward)
     bool grad2d_dirty = true; // grad2d not known to be all zero
     double* loss_partials = nullptr;  // one per tile (fused L2 backward)
@@ -262,6 +263,8 @@ static void scratch_free_frame(sfk_render_scratch* s) {
     s->dcol = nullptr;
     s->has_dcol = false;
     s->has_view = false;
+    s->has_colors = false;
+    s->sh_colors = false;
     sfk_radix_scratch_destroy(s->sort_depth);
     sfk_radix_scratch_destroy(s->sort_tile);
     s->sort_depth = s->sort_tile = nullptr;
@@ -492,6 +495,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
                                   s->projected, s->tile_counts, s->bbox, s->counters + 2,
                                   compact ? s->cbbox : nullptr, dcol, post_a, post_b));
         s->has_colors = colors || post_a;
+        s->sh_colors = colors && !post_a;
         s->has_dcol = dcol != nullptr;
     }
 
@@ -896,6 +900,11 @@ static int render_backward_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scr
     if (!sp || !grad_attrs || !grad_sh || (!d_rgba && (!rgba || !target || !d_loss)))
         return fail(c, SFK_E_ARG, "backward: null buffer");
     if (sp->n != s->n || !s->has_view) return fail(c, SFK_E_ARG, "backward: scratch was not prepared for these splats");
+    // The adjoint differentiates the SH colour; rows that hold posterior
+    // means (render_posterior) or no colour (a responsibility-only prepare)
+    // would give silently wrong gradients.
+    if (!s->sh_colors)
+        return fail(c, SFK_E_ARG, "backward: the last prepare did not project SH colours");
     if (!s->grad2d) SFK_HIP(c, hipMalloc(&s->grad2d, (size_t)s->n * 9 * 4));
     // The projection adjoint clears what it reads, so only a fresh buffer (or
     // one left by an interrupted call) needs the fill.

@@ -86,11 +86,60 @@ def write_ply(attrs, sh, n):
     return header.encode() + out.tobytes()
 
 
+def _sog_meta_schema(meta):
+    """The shape serde demands of meta.json (sog.rs:8-42): count a
+    non-negative integer; means {mins, maxs: 3 numbers, files: 2 names};
+    scales and sh0 {codebook: numbers, files: names}; quats {files: 1 name};
+    shN absent/null or {bands: non-negative integer, codebook: numbers,
+    files: 2 names}.  Returns (count, files by sheet, shN or None)."""
+    def uint(v):
+        if isinstance(v, bool) or not isinstance(v, int) or v < 0:
+            raise TypeError("not a non-negative integer")
+        return v
+
+    def nums(v, k=None):
+        if not isinstance(v, list) or (k is not None and len(v) != k) or \
+                not all(isinstance(x, (int, float)) and not isinstance(x, bool) for x in v):
+            raise TypeError("not a number array")
+        return v
+
+    def names(v, k=None):
+        if not isinstance(v, list) or (k is not None and len(v) != k) or not v or \
+                not all(isinstance(x, str) for x in v):
+            raise TypeError("not a file-name array")
+        return v
+
+    if not isinstance(meta, dict):
+        raise TypeError("not an object")
+    n = uint(meta["count"])
+    means, scales, quats, sh0 = meta["means"], meta["scales"], meta["quats"], meta["sh0"]
+    nums(means["mins"], 3)
+    nums(means["maxs"], 3)
+    mf = names(means["files"], 2)
+    nums(scales["codebook"])
+    sf = names(scales["files"])
+    qf = names(quats["files"], 1)
+    nums(sh0["codebook"])
+    cf = names(sh0["files"])
+    files = dict(lo=mf[0], hi=mf[1], scales=sf[0], quats=qf[0], sh0=cf[0])
+    shn = meta.get("shN")
+    if shn is not None:
+        uint(shn["bands"])
+        nums(shn["codebook"])
+        nf = names(shn["files"], 2)
+        files.update(centroids=nf[0], labels=nf[1])
+    return n, files, shn
+
+
 def read_sog(data):
-    """The host half of parse_sog (src/sog.rs:44-64, 97-175): unzip, read
+    """The host half of parse_sog (src/sog.rs:44-64, 97-198): unzip, read
     meta.json, decode the WebP sheets to RGBA8, with the reference's checks
-    and messages.  Returns (meta dict, sheets dict of flat uint8 RGBA
-    arrays, centroid sheet width, codebooks float32[768])."""
+    and messages in its order (meta schema, count, means / scales sheets,
+    quats sheet and rotation tags, shN bands, sh0, shN sheets and labels).
+    Returns (meta dict, sheets dict of flat uint8 RGBA arrays, centroid
+    sheet width, codebooks float32[768]).  A label inside the palette count
+    whose palette runs off the centroid sheet (the reference indexes past
+    the sheet there) is refused with its own message."""
     import io
     import json
     import zipfile
@@ -103,17 +152,12 @@ def read_sog(data):
         raise ValueError("missing meta.json")
     try:
         meta = json.loads(z.read("meta.json"))
-        n = int(meta["count"])
-        means, scales, quats, sh0 = meta["means"], meta["scales"], meta["quats"], meta["sh0"]
-        shn = meta.get("shN")
-        files = dict(lo=means["files"][0], hi=means["files"][1], scales=scales["files"][0],
-                     quats=quats["files"][0], sh0=sh0["files"][0])
-        if shn is not None:
-            files.update(centroids=shn["files"][0], labels=shn["files"][1])
+        n, files, shn = _sog_meta_schema(meta)
     except (ValueError, KeyError, TypeError, IndexError):
         raise ValueError("invalid meta.json") from None
     if n == 0:
         raise ValueError("SOG contains no splats")
+    means, scales, sh0 = meta["means"], meta["scales"], meta["sh0"]
 
     def rgba(name, min_px):
         if name not in names:
@@ -138,9 +182,13 @@ def read_sog(data):
         sheets[k], _ = rgba(files[k], n)
     cbs = [codebook(scales["codebook"], "scales")]
     sheets["quats"], _ = rgba(files["quats"], n)
+    tags = sheets["quats"].reshape(-1, 4)[:n, 3]
+    bad = np.flatnonzero(tags < 252)
+    if bad.size:
+        raise ValueError(f"quats: invalid rotation tag {int(tags[bad[0]])}")
     sh_per_ch, cw = 1, 0
     if shn is not None:
-        bands = int(shn["bands"])
+        bands = shn["bands"]
         if bands not in (1, 2, 3):
             raise ValueError(f"shN.bands {bands} outside 1..=3")
         sh_per_ch = (bands + 1) ** 2
@@ -150,6 +198,23 @@ def read_sog(data):
         sheets["centroids"], cw = rgba(files["centroids"], 1)
         sheets["labels"], _ = rgba(files["labels"], n)
         cbs.append(codebook(shn["codebook"], "shN"))
+        cpc = sh_per_ch - 1
+        px = sheets["centroids"].size // 4
+        count = px // cpc
+        lb = sheets["labels"].reshape(-1, 4)[:n].astype(np.int64)
+        label = lb[:, 0] | lb[:, 1] << 8
+        per_row = cw // cpc
+        if per_row:
+            last = (label // per_row) * cw + (label % per_row) * cpc + cpc - 1
+        else:
+            last = np.full(n, px, np.int64)  # not one palette fits in a row
+        bad = np.flatnonzero((label >= count) | (last >= px))
+        if bad.size:
+            i = bad[0]
+            if label[i] >= count:
+                raise ValueError(f"shN label {int(label[i])} >= palette size {count}")
+            raise ValueError(f"shN label {int(label[i])}: its palette runs off the "
+                             f"{cw}x{px // cw} centroid sheet")
     else:
         cbs.append(np.zeros(256, np.float32))
     info = dict(n=n, mins=[float(v) for v in means["mins"]],

@@ -182,3 +182,37 @@ def test_backward_colour_jacobian_frames(ctx):
         for p in range(11):
             assert group_rel(g[p], ra[p]) <= REL, (frame, p)
         assert group_rel(gs.read(), rs) <= REL
+
+
+def test_backward_refuses_rows_without_sh_colours(ctx):
+    """The adjoint differentiates the SH colour of the projected rows: after a
+    posterior-mean render or a responsibility-only prepare the rows hold
+    other values, and the backward is refused (SFK_E_ARG) instead of giving
+    wrong gradients; a colour prepare makes it legal again."""
+    from splatfield_amd import seg as G
+    n, w, h = 2048, 64, 64
+    attrs, sh, k = S.synthetic_scene(n, 3, seed=5)
+    cam = R.Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0))
+    splats = R.Splats(ctx, attrs, sh)
+    scratch = R.RenderScratch(ctx, n, w, h)
+    d_rgba = GpuTensor.empty(ctx, w * h * 4, np.float32)
+    d_rgba.fill_u32(0)
+    ga = GpuTensor.empty(ctx, n * 11, np.float32)
+    gs = GpuTensor.empty(ctx, n * 3 * k, np.float32)
+    ab = np.ones(n, np.float32)
+    a, b = GpuTensor.from_numpy(ctx, ab), GpuTensor.from_numpy(ctx, 2 * ab)
+    # posterior-mean rows
+    splats.render_posterior(scratch, a, b, cam, w, h)
+    isects = R.prepare_isects(splats, scratch, cam, w, h)  # colour prepare ...
+    acc = G.Accumulators(ctx, n)
+    G.render_responsibility(splats, scratch, acc, cam, w, h)  # ... then geometry only
+    with pytest.raises(Exception, match="did not project SH colours"):
+        R.render_backward(splats, scratch, isects, d_rgba.ptr, ga, gs)
+    splats.render_posterior(scratch, a, b, cam, w, h)
+    with pytest.raises(Exception, match="did not project SH colours"):
+        R.render_backward(splats, scratch, isects, d_rgba.ptr, ga, gs)
+    isects = R.prepare_isects(splats, scratch, cam, w, h)
+    R.finalize_rgb(scratch, isects)
+    R.render_backward(splats, scratch, isects, d_rgba.ptr, ga, gs)
+    ctx.sync()
+    assert not np.any(ga.read())  # dL/dC = 0: zero gradients

@@ -67,3 +67,33 @@ def test_splats_from_ply_errors(ctx):
     data, _ = _synthetic_ply(3, ["f_rest_0", "f_rest_1", "f_rest_2"])
     with pytest.raises(Exception, match="SH degree 0-3"):
         R.Splats.from_ply(ctx, data)
+
+
+def test_ply_decode_from_file_image_at_any_offset(ctx):
+    """sfk_ply_decode over a device copy of the whole file (rows at the
+    header's data_offset): rows 4-byte but not 16-byte aligned take the
+    scalar staging and give the oracle's planes; rows at a byte offset that
+    breaks float alignment are refused (SFK_E_ARG), not read."""
+    n, deg = 1000, 3
+    attrs, sh, k = S.synthetic_scene(n, deg, seed=61)
+    data = O.write_ply(attrs, sh, n)
+    lay = R.parse_ply_header(data)
+    ra, rs = O.parse_ply(data)
+    d_attrs = GpuTensor.empty(ctx, 11 * n, np.float32)
+    d_sh = GpuTensor.empty(ctx, lay.sh_planes * n, np.float32)
+    buf = np.frombuffer(data, np.uint8)
+    for mis in (4, 8, 12, 0):
+        # place the file so that its rows sit `mis` bytes past a 16-byte boundary
+        pad = (mis - lay.data_offset) % 16
+        img = np.zeros(pad + buf.size + 16, np.uint8)
+        img[pad:pad + buf.size] = buf
+        dev = GpuTensor.from_numpy(ctx, img)
+        rows = dev.ptr + pad + lay.data_offset
+        assert rows % 16 == mis
+        ctx.check(lib().sfk_ply_decode(ctx.ptr, C.byref(lay), rows, d_attrs.ptr, d_sh.ptr),
+                  "ply_decode")
+        np.testing.assert_array_equal(d_attrs.read(), ra, f"rows at +{mis}")
+        np.testing.assert_array_equal(d_sh.read(), rs, f"rows at +{mis}")
+    for mis in (1, 2, 3):
+        assert lib().sfk_ply_decode(ctx.ptr, C.byref(lay), dev.ptr + 16 + mis, d_attrs.ptr,
+                                    d_sh.ptr) == -1

@@ -722,11 +722,102 @@ def test_sog_host_reader_and_oracle():
         O.sog_decode(*S.read_sog(bad))
     with pytest.raises(ValueError, match="not a SOG archive"):
         S.read_sog(b"not a zip")
-    import io, json, zipfile
+    with pytest.raises(ValueError, match="SOG contains no splats"):
+        S.read_sog(_sog_with_meta(dict(_SOG_META, count=0)))
+
+
+_SOG_META = {"count": 4, "means": {"mins": [0, 0, 0], "maxs": [1, 1, 1], "files": ["a", "b"]},
+             "scales": {"codebook": [0.0] * 256, "files": ["c"]}, "quats": {"files": ["d"]},
+             "sh0": {"codebook": [0.0] * 256, "files": ["e"]}}
+
+
+def _sog_with_meta(meta):
+    import io
+    import json
+    import zipfile
     buf = io.BytesIO()
     with zipfile.ZipFile(buf, "w") as z:
-        z.writestr("meta.json", json.dumps({"count": 0, "means": {"files": ["a", "b"]},
-                                            "scales": {"files": ["c"]}, "quats": {"files": ["d"]},
-                                            "sh0": {"files": ["e"]}}))
-    with pytest.raises(ValueError, match="SOG contains no splats"):
+        z.writestr("meta.json", json.dumps(meta))
+    return buf.getvalue()
+
+
+def _meta_edit(path, value):
+    import copy
+    m = copy.deepcopy(_SOG_META)
+    d = m
+    for k in path[:-1]:
+        d = d[k]
+    if value is KeyError:
+        del d[path[-1]]
+    else:
+        d[path[-1]] = value
+    return m
+
+
+@pytest.mark.parametrize("path,value", [
+    (("count",), -1), (("count",), 2.5), (("count",), True), (("count",), "4"),
+    (("means", "mins"), [0, 0]), (("means", "maxs"), KeyError), (("means", "files"), ["a"]),
+    (("scales", "codebook"), KeyError), (("scales", "files"), "c"), (("quats", "files"), ["d", "x"]),
+    (("sh0", "codebook"), [0.0, "x"]), (("shN",), {"bands": 1, "codebook": [0.0] * 256}),
+    (("shN",), {"bands": -1, "codebook": [], "files": ["f", "g"]}),
+])
+def test_sog_meta_schema_rejections(path, value):
+    """sog.rs:8-42 + 99-101: every departure from the serde schema (missing
+    or mistyped fields, a negative, fractional or boolean count, arrays of
+    the wrong length) is 'invalid meta.json', before any sheet is read."""
+    from splatfield_amd import scene as S
+    with pytest.raises(ValueError, match="^invalid meta.json$"):
+        S.read_sog(_sog_with_meta(_meta_edit(path, value)))
+    # the unedited schema passes the meta stage (it fails later on a sheet)
+    with pytest.raises(ValueError, match="missing a"):
+        S.read_sog(_sog_with_meta(_SOG_META))
+    # null shN is the absent one
+    with pytest.raises(ValueError, match="missing a"):
+        S.read_sog(_sog_with_meta(dict(_SOG_META, shN=None)))
+
+
+def test_sog_check_order_and_label_messages():
+    """sog.rs:97-198 order: a bad rotation tag is reported before a bad
+    bands value (the quats sheet is read first); a label past the palette
+    count and a label whose palette leaves the centroid sheet get their own
+    messages."""
+    import io
+    import json
+    import zipfile
+    from splatfield_amd import scene as S
+    from .fixtures import make_sog
+    data, _ = make_sog(64, 1, seed=3, bad_tag_at=5)
+    z = zipfile.ZipFile(io.BytesIO(data))
+    meta = json.loads(z.read("meta.json"))
+    meta["shN"]["bands"] = 7
+    buf = io.BytesIO()
+    with zipfile.ZipFile(buf, "w") as out:
+        for name in z.namelist():
+            out.writestr(name, json.dumps(meta) if name == "meta.json" else z.read(name))
+    with pytest.raises(ValueError, match="quats: invalid rotation tag 17"):
+        S.read_sog(buf.getvalue())
+    data, _ = make_sog(64, 1, seed=4, bad_label_at=7)
+    with pytest.raises(ValueError, match=r"shN label \d+ >= palette size \d+"):
+        S.read_sog(data)
+    # Off the sheet: with two bands (8 coefficients a palette) make_sog's
+    # centroid rows are 16 * 8 + 3 texels wide, 7 rows: 114 palettes by
+    # count, 112 on the sheet, so labels 112 and 113 lie past the last row.
+    data, sheets = make_sog(64, 2, seed=5)
+    info, got, cw, _ = S.read_sog(data)
+    cpc, rows = 8, got["centroids"].size // 4 // cw
+    count = got["centroids"].size // 4 // cpc
+    per_row = cw // cpc
+    off = next(l for l in range(count) if l // per_row >= rows)
+    z = zipfile.ZipFile(io.BytesIO(data))
+    from PIL import Image
+    lab = np.asarray(Image.open(io.BytesIO(z.read("shN_labels.webp"))).convert("RGBA")).copy()
+    flat = lab.reshape(-1, 4)
+    flat[3, 0], flat[3, 1] = off & 255, off >> 8
+    img = io.BytesIO()
+    Image.fromarray(lab, "RGBA").save(img, "WEBP", lossless=True, exact=True)
+    buf = io.BytesIO()
+    with zipfile.ZipFile(buf, "w") as out:
+        for name in z.namelist():
+            out.writestr(name, img.getvalue() if name == "shN_labels.webp" else z.read(name))
+    with pytest.raises(ValueError, match=f"shN label {off}: its palette runs off"):
         S.read_sog(buf.getvalue())
