@@ -133,11 +133,14 @@ SFK_HD float sfk_logf(float x) {
     return r;
 }
 
-// `f32 as u32` of the reference (Rust): saturating, NaN -> 0.
+// `f32 as u32` of the reference (Rust): saturating, NaN -> 0.  Branch-free:
+// for v > 0 the clamps keep every v < 2^32 (the largest such f32 is
+// 4294967040) and the conversion truncates; the selects give 0 for NaN and
+// v <= 0 (whatever the clamped conversion made of them) and saturate
+// v >= 2^32, infinity included.
 SFK_HD uint32_t sfk_f2u(float v) {
-    if (!(v > 0.0f)) return 0u;
-    if (v >= 4294967296.0f) return 0xffffffffu;
-    return (uint32_t)v;
+    const uint32_t t = (uint32_t)fminf(fmaxf(v, 0.0f), 4294967040.0f);
+    return v > 0.0f ? (v >= 4294967296.0f ? 0xffffffffu : t) : 0u;
 }
 
 // f32::clamp of the reference: NaN passes through.

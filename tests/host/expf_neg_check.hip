@@ -1,7 +1,8 @@
 // Host-side check that the rasterizers' sfk_expf_neg(p) returns the same bits
 // as sfk_expf(-p) (the oracle's exp) over every float p in [-88, 6] visited
 // with the given stride, that NaN propagates, and that arguments past the
-// clamp still saturate alpha = min(o * G, 0.999) for o >= 10/255.
+// clamp still saturate alpha = min(o * G, 0.999) for o >= 10/255; and that
+// the branch-free sfk_f2u is Rust's saturating `f32 as u32`.
 #include "../../splatfield_amd/csrc/sfk_math.h"
 
 #include <cstdio>
@@ -45,6 +46,28 @@ int main(int argc, char** argv) {
         const float g = sfk_expf_neg(__builtin_inff());
         if (!(g >= 0.0f && g < 1.0f / 255.0f)) ++bad;
     }
+    // sfk_f2u (branch-free) against Rust's saturating `f32 as u32` spelled
+    // with branches, on every float pattern visited with the same stride
+    // plus the boundaries.
+    auto f2u_ref = [](float v) -> uint32_t {
+        if (!(v > 0.0f)) return 0u;
+        if (v >= 4294967296.0f) return 0xffffffffu;
+        return (uint32_t)v;
+    };
+    auto f2u_one = [&](uint32_t u) {
+        float v;
+        memcpy(&v, &u, 4);
+        if (sfk_f2u(v) != f2u_ref(v)) {
+            if (bad < 5) printf("f2u v=%a got=%u want=%u\n", v, sfk_f2u(v), f2u_ref(v));
+            ++bad;
+        }
+        ++checked;
+    };
+    for (uint64_t u = 0; u <= 0xffffffffull; u += stride) f2u_one((uint32_t)u);
+    const uint32_t edges[] = {0x00000000u, 0x80000000u, 0x3f800000u, 0x4f7fffffu, 0x4f800000u,
+                              0x7f7fffffu, 0x7f800000u, 0xff800000u, 0x7fc00000u, 0x7f800001u,
+                              0xffc00000u, 0xff800001u, 0x00000001u, 0xbf800000u};
+    for (uint32_t u : edges) f2u_one(u);
     printf("checked %llu mismatches %llu\n", checked, bad);
     return bad != 0;
 }

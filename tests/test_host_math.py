@@ -1,6 +1,7 @@
 """The rasterizers' branch-free exp (sfk_expf_neg) against the library's
-full exp (the oracle's algorithm), bit for bit, on the host: a small HIP
-host program built with hipcc from the product header."""
+full exp (the oracle's algorithm), bit for bit, and the branch-free
+saturating float -> u32 conversion against its branchy definition, on the
+host: a small HIP host program built with hipcc from the product header."""
 import os
 import shutil
 import subprocess
