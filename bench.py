@@ -431,6 +431,8 @@ def main():
 
     splats = R.Splats(ctx, attrs, sh)
     scratch = R.RenderScratch(ctx, n, w, h)
+    # The frame reads tile ranges and splat ids only: no per-isect tile ids.
+    scratch.set_tile_ids(False)
     loss = GpuTensor.empty(ctx, 1, np.float64)
     g_attrs = GpuTensor.empty(ctx, n * 11, np.float32)
     g_sh = GpuTensor.empty(ctx, n * 3 * k, np.float32)

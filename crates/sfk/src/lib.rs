@@ -248,6 +248,12 @@ impl<'c> RenderScratch<'c> {
         self.ctx.check(unsafe { sys::sfk_render_scratch_set_tile_width(self.raw, tile_w) })
     }
 
+    /// Whether the binned emission writes per-isect tile ids (default on); a
+    /// training loop that only rasterizes switches them off.
+    pub fn set_tile_ids(&mut self, on: bool) -> Result<()> {
+        self.ctx.check(unsafe { sys::sfk_render_scratch_set_tile_ids(self.raw, on as i32) })
+    }
+
     /// Grow the isect capacity now so the first frame renders untruncated.
     pub fn reserve_isects(&mut self, isects: u32) -> Result<()> {
         self.ctx.check(unsafe { sys::sfk_render_scratch_reserve_isects(self.raw, isects) })

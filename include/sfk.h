@@ -175,6 +175,13 @@ int sfk_render_scratch_set_digit_bits(sfk_render_scratch* s, uint32_t digit_bits
  * give the same arrays word for word (a tuning and test knob, no counterpart
  * in the reference). */
 int sfk_render_scratch_set_binned(sfk_render_scratch* s, int on);
+/* Whether the binned emission also writes the per-isect tile ids (on, the
+ * default).  The rasterizers read only tile_starts and gaussian_ids, so a
+ * caller that never reads tile_ids (a training loop) switches them off and
+ * saves one 4-byte store per isect; sfk_tile_isects.tile_ids is then NULL.
+ * The sorting emission always has them (they are its sort keys).  No
+ * counterpart in the reference, whose emission sorts by tile id. */
+int sfk_render_scratch_set_tile_ids(sfk_render_scratch* s, int on);
 
 /* TileIsects (src/render.rs:463-471) plus the device-side counters. */
 typedef struct {

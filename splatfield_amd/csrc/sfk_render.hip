@@ -225,6 +225,7 @@ struct sfk_render_scratch {
     // Binned emission (compact grids): supertile ranges, chunk map, per-chunk
     // tile counts (the last two sized by the isect capacity).
     bool binned = true;
+    bool want_tile_ids = true;  // the binned emission also writes tile_ids
     uint32_t* bin_sstart = nullptr;
     uint32_t* bin_map = nullptr;
     uint32_t* bin_counts = nullptr;
@@ -379,6 +380,12 @@ extern "C" int sfk_render_scratch_set_tile_width(sfk_render_scratch* s, uint32_t
 extern "C" int sfk_render_scratch_set_binned(sfk_render_scratch* s, int on) {
     if (!s) return SFK_E_ARG;
     s->binned = on != 0;
+    return SFK_OK;
+}
+
+extern "C" int sfk_render_scratch_set_tile_ids(sfk_render_scratch* s, int on) {
+    if (!s) return SFK_E_ARG;
+    s->want_tile_ids = on != 0;
     return SFK_OK;
 }
 
@@ -538,7 +545,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
             j.tile_starts = s->tile_starts;
             j.tile_order = s->tile_order;
             j.gids = s->gaussian_ids;
-            j.tile_ids = s->tile_ids;
+            j.tile_ids = s->want_tile_ids ? s->tile_ids : nullptr;
             j.counters = s->counters;
             j.status = s->emit_status;
             j.status_zeroed = true;
@@ -549,7 +556,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
             SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));
             s->counters_pending = true;
             s->depth_order = s->order_buf;
-            s->sorted_tiles = s->tile_ids;
+            s->sorted_tiles = j.tile_ids;
             s->sorted_gids = s->gaussian_ids;
             s->view = *view;
             s->has_view = true;

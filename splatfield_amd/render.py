@@ -242,6 +242,8 @@ class TileIsects:
         return int(out[0]), int(out[1]), int(out[2])
 
     def tile_ids(self, n):
+        if not self.c.tile_ids:
+            raise ValueError("tile ids were not written (RenderScratch.set_tile_ids(False))")
         return GpuTensor(self.scratch.ctx, self.c.tile_ids, (n,), np.uint32, False,
                          self.scratch).read()
 
@@ -279,6 +281,12 @@ class RenderScratch:
         """Binned isect emission (default) or emission + stable tile argsort;
         both give the same isect arrays on grids of at most 255 x 255 tiles."""
         self.ctx.check(lib().sfk_render_scratch_set_binned(self.ptr, 1 if on else 0), "binned")
+
+    def set_tile_ids(self, on):
+        """Whether the binned emission writes per-isect tile ids (default on);
+        off, TileIsects.tile_ids is NULL and the rasterizers run unchanged."""
+        self.ctx.check(lib().sfk_render_scratch_set_tile_ids(self.ptr, 1 if on else 0),
+                       "tile ids")
 
     def reserve_isects(self, n):
         self.ctx.check(lib().sfk_render_scratch_reserve_isects(self.ptr, n), "reserve")

@@ -283,6 +283,36 @@ def test_binned_matches_sorted(ctx, w, h, tile_w, n, boost):
         np.testing.assert_array_equal(a[key], b[key], key)
 
 
+def test_binned_without_tile_ids(ctx):
+    """With tile ids switched off the binned emission writes the same splat
+    ids and tile ranges, the RGB render is unchanged bit for bit, and the
+    tile ids are reported absent; switched back on, they are written again."""
+    n, w, h = 16384, 320, 180
+    attrs, sh, k = S.synthetic_scene(n, 1, seed=12)
+    cam = R.Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0))
+    view = cam.view(w, h)
+    splats = R.Splats(ctx, attrs, sh)
+    scratch = R.RenderScratch(ctx, n, w, h)
+    scratch.reserve_isects(1 << 22)
+    isects = R.prepare_isects(splats, scratch, view=view, w=w, h=h)
+    full = _isect_arrays(scratch, isects)
+    R.finalize_rgb(scratch, isects)
+    rgba = scratch.read("rgba").copy()
+    scratch.set_tile_ids(False)
+    isects = R.prepare_isects(splats, scratch, view=view, w=w, h=h)
+    ni = isects.counts()[2]
+    assert ni == full["counts"][2] > 0
+    np.testing.assert_array_equal(isects.gaussian_ids(ni), full["gids"])
+    np.testing.assert_array_equal(isects.tile_starts(), full["starts"])
+    with pytest.raises(ValueError):
+        isects.tile_ids(ni)
+    R.finalize_rgb(scratch, isects)
+    np.testing.assert_array_equal(scratch.read("rgba"), rgba)
+    scratch.set_tile_ids(True)
+    isects = R.prepare_isects(splats, scratch, view=view, w=w, h=h)
+    np.testing.assert_array_equal(isects.tile_ids(ni), full["tile_ids"])
+
+
 def test_truncated_first_frame(ctx):
     """A fresh scratch holds min(tiles * n, 2^22) isects (render.rs:143-167);
     a first frame emitting more keeps exactly the isects whose emission slot
