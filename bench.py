@@ -376,8 +376,9 @@ def kernel_bytes(n, k, vis, ni, px, pairs):
         # (supertile, splat, bbox) pairs written
         "bin_scan": 16 * n + 12 * pairs,
         "bin_count": 8 * pairs + 256 * (pairs // 512),
-        # pairs read again; splat id and tile id written per isect
-        "bin_place": 8 * pairs + 8 * ni,
+        # pairs read again; splat id written per isect (the frame's scratch
+        # writes no tile ids)
+        "bin_place": 8 * pairs + 4 * ni,
         "raster_rgb": 40 * ni + 16 * px,
         "loss_l2": 48 * px,
         "raster_bwd": 40 * ni + 32 * px + 36 * vis,

@@ -334,39 +334,6 @@ __global__ __launch_bounds__(256) void k_bin_chunk_map(const uint32_t* __restric
     }
 }
 
-// Single view, pairs sorted in one radix pass: the pass's digit totals are
-// the supertiles' pair counts, so one workgroup scans them into the
-// supertile ranges and writes the chunk map (k_tile_starts over the sorted
-// keys and k_bin_chunk_map in one launch).
-__global__ __launch_bounds__(256) void k_bin_ranges1(const uint32_t* __restrict__ totals,
-                                                     uint32_t supers,
-                                                     uint32_t* __restrict__ sstart,
-                                                     uint4* __restrict__ map) {
-    __shared__ uint32_t tmp[256 / 64 + 1];
-    __shared__ uint32_t s_start[257];
-    const uint32_t t = threadIdx.x;
-    const uint32_t c = t < supers ? totals[t] : 0u;
-    uint32_t total;
-    const uint32_t ex = block_excl_scan_u32<256>(c, tmp, &total);
-    if (t < supers) {
-        s_start[t] = ex;
-        sstart[t] = ex;
-    }
-    if (t == 0) {
-        s_start[supers] = total;
-        sstart[supers] = total;
-    }
-    __syncthreads();
-    if (t < supers) {
-        const uint32_t st = ex, en = s_start[t + 1];
-        const uint32_t c0 = st / kBinChunk + t, c1 = en / kBinChunk + t + 1;
-        for (uint32_t k = c0; k < c1; ++k) {
-            const uint32_t p0 = st + (k - c0) * kBinChunk;
-            map[k] = make_uint4(t, p0, min(p0 + kBinChunk, en), 0u);
-        }
-    }
-}
-
 struct ChunkArgs {
     const uint2* pvals;     // sorted pair values: (splat | may-truncate << 31, bbox)
     const uint32_t* sstart;
@@ -381,7 +348,63 @@ struct ChunkArgs {
     uint32_t* tile_ids;
     uint32_t* tile_order;          // place pass, optional: tiles heaviest first
     uint32_t order_tiles;
+    // Count pass, one view of at most 256 supertiles: the pair sort's digit
+    // totals, from which every workgroup derives the supertile ranges and
+    // chunk slots itself; workgroup 0 writes them out (sstart, map) for the
+    // later passes.  Null: sstart and map are read.
+    const uint32_t* totals;
+    uint32_t* sstart_out;
+    uint4* map_out;
+    // Count pass, optional: two word ranges it zeroes (the emission scan's
+    // look-back words, done with by then, and other counters), so that the
+    // next emission needs no clearing launch.
+    uint32_t* zero_a;
+    uint32_t zero_a_n;
+    uint32_t* zero_b;
+    uint32_t zero_b_n;
 };
+
+// Supertile starts (exclusive prefixes of the pair counts, total at
+// [supers]) into LDS by one 256-thread workgroup, supers <= 256; with `out`,
+// also into device memory with the chunk map (the records of k_bin_chunk_map).
+__device__ void supertile_ranges(const uint32_t* __restrict__ totals, uint32_t supers,
+                                 uint32_t* s_start, uint32_t* tmp, uint32_t* __restrict__ sstart,
+                                 uint4* __restrict__ map, bool out) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t c = t < supers ? totals[t] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan_u32<256>(c, tmp, &total);
+    if (t < supers) s_start[t] = ex;
+    if (t == 0) s_start[supers] = total;
+    __syncthreads();
+    if (out) {
+        if (t < supers) sstart[t] = ex;
+        if (t == 0) sstart[supers] = total;
+        if (t < supers) {
+            const uint32_t st = ex, en = s_start[t + 1];
+            const uint32_t c0 = st / kBinChunk + t, c1 = en / kBinChunk + t + 1;
+            for (uint32_t k = c0; k < c1; ++k) {
+                const uint32_t p0 = st + (k - c0) * kBinChunk;
+                map[k] = make_uint4(t, p0, min(p0 + kBinChunk, en), 0u);
+            }
+        }
+    }
+}
+
+// Chunk slot c from the supertile starts in LDS: (supertile, first pair,
+// end pair), first >= end for a gap slot.  Slot c0(s) = start(s) / kBinChunk
+// + s is supertile s's first; the last s with c0(s) <= c owns slot c.
+__device__ __forceinline__ uint4 chunk_slot(const uint32_t* s_start, uint32_t supers, uint32_t c) {
+    uint32_t lo = 0, hi = supers;  // c0(lo) <= c < c0(hi), c0(supers) = infinity
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (s_start[mid] / kBinChunk + mid <= c) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t st = s_start[lo], en = s_start[lo + 1];
+    const uint32_t p0 = st + (c - (st / kBinChunk + lo)) * kBinChunk;
+    return make_uint4(lo, p0, min(p0 + kBinChunk, en), 0u);
+}
 
 // Bits (ty - oy) * 8 + (tx - ox) of the supertile at tile (ox, oy) that the
 // compact bbox covers and the truncation keeps.  Isect (tx, ty) of a rank
@@ -521,7 +544,20 @@ __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
     constexpr uint32_t SW = PLACE ? 4 : 1;
     __shared__ uint16_t s_st[SW][PLACE ? kStage : 1];
     __shared__ uint32_t s_gid[SW][PLACE ? kBinChunk : 1];
+    __shared__ uint32_t s_start[PLACE ? 1 : 257];
+    __shared__ uint32_t s_tmp[PLACE ? 1 : 5];
     uint32_t blk = blockIdx.x, nblk = gridDim.x;
+    const bool own_ranges = !PLACE && a.totals;
+    if constexpr (!PLACE) {
+        for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < a.zero_a_n + a.zero_b_n;
+             i += gridDim.x * 256u) {
+            if (i < a.zero_a_n) a.zero_a[i] = 0u;
+            else a.zero_b[i - a.zero_a_n] = 0u;
+        }
+        if (own_ranges)
+            supertile_ranges(a.totals, a.supers, s_start, s_tmp, a.sstart_out, a.map_out,
+                             blockIdx.x == 0);
+    }
     if constexpr (PLACE) {
         if (a.tile_order) {
             if (blockIdx.x == 0) {
@@ -534,10 +570,10 @@ __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
     }
     const uint32_t lane = lane_id(), w = threadIdx.x / 64u;
     const uint32_t wave = blk * 4u + w, nwaves = nblk * 4u;
-    const uint32_t slots = a.sstart[a.supers] / kBinChunk + a.supers;
+    const uint32_t slots = (own_ranges ? s_start[a.supers] : a.sstart[a.supers]) / kBinChunk + a.supers;
     const TransposeLanes tl = transpose_lanes();
     for (uint32_t c = wave; c < slots; c += nwaves) {
-        const uint4 d = a.map[c];
+        const uint4 d = own_ranges ? chunk_slot(s_start, a.supers, c) : a.map[c];
         const uint32_t p0 = d.y, pe = d.z;
         if (p0 >= pe) continue;  // a gap slot
         const uint32_t s = d.x & 0xffffffu;
@@ -651,6 +687,54 @@ __global__ __launch_bounds__(256) void k_bin_chunks(ChunkArgs a) {
     }
 }
 
+// Exclusive scan of buf[0, n) in place with the total in buf[n] by one
+// workgroup of NT threads, 16 consecutive words per thread per sweep.
+constexpr uint32_t kSmallScanEpt = 16;
+constexpr uint32_t kSmallScanMax = 1u << 16;
+template <uint32_t NT>
+__device__ void block_scan_small(uint32_t* __restrict__ buf, uint32_t n) {
+    __shared__ uint32_t tmp[NT / 64 + 1];
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += NT * kSmallScanEpt) {
+        const uint32_t first = c0 + threadIdx.x * kSmallScanEpt;
+        const bool full = first + kSmallScanEpt <= n;
+        uint32_t v[kSmallScanEpt];
+        if (full) {
+#pragma unroll
+            for (uint32_t q = 0; q < kSmallScanEpt / 4; ++q) {
+                const uint4 x = reinterpret_cast<const uint4*>(buf + first)[q];
+                v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t e = 0; e < kSmallScanEpt; ++e) v[e] = first + e < n ? buf[first + e] : 0u;
+        }
+        uint32_t s = 0;
+#pragma unroll
+        for (uint32_t e = 0; e < kSmallScanEpt; ++e) s += v[e];
+        uint32_t total;
+        uint32_t run = carry + block_excl_scan_u32<NT>(s, tmp, &total);
+        uint32_t o[kSmallScanEpt];
+#pragma unroll
+        for (uint32_t e = 0; e < kSmallScanEpt; ++e) {
+            o[e] = run;
+            run += v[e];
+        }
+        if (full) {
+#pragma unroll
+            for (uint32_t q = 0; q < kSmallScanEpt / 4; ++q)
+                reinterpret_cast<uint4*>(buf + first)[q] =
+                    make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+        } else {
+#pragma unroll
+            for (uint32_t e = 0; e < kSmallScanEpt; ++e)
+                if (first + e < n) buf[first + e] = o[e];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) buf[n] = carry;
+}
+
 // One workgroup per supertile: per tile (lane), the exclusive prefix of the
 // chunk counts over the supertile's chunks (in place), and the tile total
 // into tile_starts.  The four waves split the chunks; their partial sums meet
@@ -698,51 +782,11 @@ __global__ __launch_bounds__(256) void k_bin_chunk_scan(const uint32_t* __restri
 }
 
 // Exclusive scan of buf[0, n) in place with the total in buf[n]: one
-// workgroup, 16 consecutive words per thread per sweep.
-constexpr uint32_t kSmallScanThreads = 1024, kSmallScanEpt = 16;
-constexpr uint32_t kSmallScanMax = 1u << 16;
+// workgroup.
+constexpr uint32_t kSmallScanThreads = 1024;
 __global__ __launch_bounds__(kSmallScanThreads) void k_scan_small(uint32_t* __restrict__ buf,
                                                                    uint32_t n) {
-    __shared__ uint32_t tmp[kSmallScanThreads / 64 + 1];
-    uint32_t carry = 0;
-    for (uint32_t c0 = 0; c0 < n; c0 += kSmallScanThreads * kSmallScanEpt) {
-        const uint32_t first = c0 + threadIdx.x * kSmallScanEpt;
-        const bool full = first + kSmallScanEpt <= n;
-        uint32_t v[kSmallScanEpt];
-        if (full) {
-#pragma unroll
-            for (uint32_t q = 0; q < kSmallScanEpt / 4; ++q) {
-                const uint4 x = reinterpret_cast<const uint4*>(buf + first)[q];
-                v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
-            }
-        } else {
-#pragma unroll
-            for (uint32_t e = 0; e < kSmallScanEpt; ++e) v[e] = first + e < n ? buf[first + e] : 0u;
-        }
-        uint32_t s = 0;
-#pragma unroll
-        for (uint32_t e = 0; e < kSmallScanEpt; ++e) s += v[e];
-        uint32_t total;
-        uint32_t run = carry + block_excl_scan_u32<kSmallScanThreads>(s, tmp, &total);
-        uint32_t o[kSmallScanEpt];
-#pragma unroll
-        for (uint32_t e = 0; e < kSmallScanEpt; ++e) {
-            o[e] = run;
-            run += v[e];
-        }
-        if (full) {
-#pragma unroll
-            for (uint32_t q = 0; q < kSmallScanEpt / 4; ++q)
-                reinterpret_cast<uint4*>(buf + first)[q] =
-                    make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
-        } else {
-#pragma unroll
-            for (uint32_t e = 0; e < kSmallScanEpt; ++e)
-                if (first + e < n) buf[first + e] = o[e];
-        }
-        carry += total;
-    }
-    if (threadIdx.x == 0) buf[n] = carry;
+    block_scan_small<kSmallScanThreads>(buf, n);
 }
 
 }  // namespace
@@ -791,15 +835,15 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     int rc = argsort_u32_vals8(c, j.sort, pair_keys, pair_vals, j.cap, j.counters + 1,
                                j.view_shift, &sk, &sv);
     if (rc) return rc;
-    {
+    // One view of at most 256 supertiles sorted in one pass: the pass's digit
+    // totals are the supertiles' pair counts, and the count pass derives the
+    // supertile ranges and chunk slots from them (no ranges launch).
+    uint32_t db = 0;
+    const uint32_t* totals = radix_scratch_totals(j.sort, &db);
+    const bool ranges_in_count = j.nviews == 1 && j.view_shift <= db && j.supers <= 256;
+    if (!ranges_in_count) {
         SFK_PROF(c, "bin_ranges");
-        uint32_t db = 0;
-        const uint32_t* totals = radix_scratch_totals(j.sort, &db);
-        if (j.nviews == 1 && j.view_shift <= db && j.supers <= 256) {
-            k_bin_ranges1<<<1, 256, 0, st>>>(totals, j.supers, j.sstart,
-                                             reinterpret_cast<uint4*>(j.chunk_map));
-            SFK_LAUNCHED(c, "bin ranges");
-        } else {
+        {
             SFK_HIP(c, launch_tile_starts(st, sk, j.cap, j.counters + 1, j.supers, j.sstart,
                                           j.nviews > 1 ? j.nviews : 0u, j.view_shift));
             k_bin_chunk_map<<<j.supers, 256, 0, st>>>(
@@ -823,12 +867,24 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     ca.tile_ids = j.tile_ids;
     ca.tile_order = nullptr;
     ca.order_tiles = 0;
+    ca.totals = ranges_in_count ? totals : nullptr;
+    ca.sstart_out = j.sstart;
+    ca.map_out = reinterpret_cast<uint4*>(j.chunk_map);
+    // With self_clean the count pass clears the emission scan's ticket and
+    // look-back words (2 + 2 per workgroup of kRanks) and j.clear.
+    ca.zero_a = j.self_clean ? j.status : nullptr;
+    ca.zero_a_n = j.self_clean ? 2u + 2u * div_up(j.n, kRanks) : 0u;
+    ca.zero_b = j.self_clean ? j.clear : nullptr;
+    ca.zero_b_n = j.self_clean && j.clear ? j.clear_n : 0u;
     const uint32_t grid = std::min(div_up(bin_chunk_slots(j.cap, j.supers), 4), 2048u);
     {
         SFK_PROF(c, "bin_count");
         k_bin_chunks<false><<<grid, 256, 0, st>>>(ca);
         SFK_LAUNCHED(c, "bin count");
     }
+    ca.totals = nullptr;
+    ca.zero_a = ca.zero_b = nullptr;
+    ca.zero_a_n = ca.zero_b_n = 0u;
     {
         SFK_PROF(c, "bin_tile_scan");
         if (j.zero_starts) SFK_HIP(c, launch_fill_u32(st, j.tile_starts, 0u, j.out_tiles));

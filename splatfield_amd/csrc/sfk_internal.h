@@ -260,6 +260,11 @@ struct BinJob {
     uint32_t* counters;     // [0] raw isect count, [1] pair count
     uint32_t* status;       // scan_emit_status_words(n)
     bool status_zeroed;     // the caller cleared `status` (else bin_emit does)
+    bool self_clean;        // leave `status` clear once the emission scan is done with
+                            // it (the next job then needs no clearing), and zero
+                            // clear[0, clear_n) with it
+    uint32_t* clear;
+    uint32_t clear_n;
     uint32_t* host_total;   // optional device view of mapped pinned host memory:
                             // receives the raw isect count as well
 };

@@ -203,6 +203,13 @@ struct sfk_render_scratch {
     sfk_radix_scratch* sort_depth = nullptr;
     sfk_radix_scratch* sort_tile = nullptr;
     uint32_t* emit_status = nullptr;  // look-back words of the fused scan/emission
+    // Two banks of kCounterWords: a prepare counts into bank `bank`, and
+    // the binned emission's scan clears the other bank's count slots and
+    // its own look-back words for the next prepare (emit_clean), so a
+    // binned frame starts with no clearing launch.
+    uint32_t bank = 0;
+    uint32_t* cur_counters = nullptr;  // the bank of the last prepare
+    bool emit_clean = false;
     uint32_t* cbbox = nullptr;        // compact tile bbox per splat (depth-sort payload)
     uint32_t* order_buf = nullptr;    // depth order written by the emission
     // Previous frame's raw emission, read back asynchronously for growth.
@@ -312,7 +319,10 @@ static int scratch_build(sfk_render_scratch* s, uint32_t n, uint32_t w, uint32_t
     SFK_HIP(c, hipMalloc(&s->tile_counts, nn * 4));
     SFK_HIP(c, hipMalloc(&s->bbox, nn * 8));
     SFK_HIP(c, hipMalloc(&s->offsets, nn * 4));
-    SFK_HIP(c, hipMalloc(&s->counters, kCounterWords * 4));
+    SFK_HIP(c, hipMalloc(&s->counters, 2 * kCounterWords * 4));
+    s->cur_counters = s->counters;
+    s->bank = 0;
+    s->emit_clean = false;
     SFK_HIP(c, hipMalloc(&s->tile_starts, (size_t)(tiles + 1) * 4));
     if (tiles <= kTileOrderMax) SFK_HIP(c, hipMalloc(&s->tile_order, (size_t)tiles * 4));
     SFK_HIP(c, hipMalloc(&s->bitmap, (size_t)h * s->row_stride * 4));
@@ -416,7 +426,7 @@ extern "C" int sfk_render_scratch_buffers(const sfk_render_scratch* s, sfk_rende
     o->tile_counts = s->tile_counts;
     o->bbox = s->bbox;
     o->offsets = s->offsets;
-    o->counters = s->counters;
+    o->counters = s->cur_counters;
     o->bitmap = s->bitmap;
     o->rgba = s->rgba;
     o->final_t = s->final_t;
@@ -447,7 +457,7 @@ static void fill_isects(const sfk_render_scratch* s, sfk_tile_isects* o) {
     o->tile_ids = s->sorted_tiles;
     o->gaussian_ids = s->sorted_gids;
     o->tile_starts = s->tile_starts;
-    o->d_counters = s->counters;
+    o->d_counters = s->cur_counters;
     o->capacity = s->capacity;
     o->tiles_x = s->tiles_x;
     o->tiles_y = s->tiles_y;
@@ -490,16 +500,23 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
     const bool compact = compact_grid(s);
     s->order_valid = false;
 
-    // Counters (and, for the binned emission, its look-back words) start at 0.
+    // Counters (and, for the binned emission, its look-back words) start at
+    // 0: cleared here, or left clear by the last binned prepare.
     const bool binned = compact && s->binned && s->capacity >= 2 && s->capacity <= kBinMaxCap;
-    SFK_HIP(c, launch_zero2_u32(c->stream, s->counters, kCounterWords, s->emit_status,
-                                binned ? scan_emit_status_words(n) : 0u));
+    const uint32_t bank = s->bank;
+    uint32_t* ctr = s->counters + (size_t)bank * kCounterWords;
+    s->cur_counters = ctr;
+    if (!(binned && s->emit_clean)) {
+        SFK_HIP(c, launch_zero2_u32(c->stream, s->counters, 2 * kCounterWords, s->emit_status,
+                                    binned ? scan_emit_status_words(n) : 0u));
+    }
+    s->emit_clean = false;
     {
         SFK_PROF(c, "project");
         float* dcol = colors && s->want_dcol && sp->sh_per_ch > 1 ? s->dcol : nullptr;
         SFK_HIP(c, launch_project(c->stream, *view, sp->attrs, sp->sh, n,
                                   colors ? sp->sh_per_ch : 0u, s->tile_w, s->depth_keys,
-                                  s->projected, s->tile_counts, s->bbox, s->counters + 2,
+                                  s->projected, s->tile_counts, s->bbox, ctr + 2,
                                   compact ? s->cbbox : nullptr, dcol, post_a, post_b));
         s->has_colors = colors || post_a;
         s->sh_colors = colors && !post_a;
@@ -546,12 +563,17 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
             j.tile_order = s->tile_order;
             j.gids = s->gaussian_ids;
             j.tile_ids = s->want_tile_ids ? s->tile_ids : nullptr;
-            j.counters = s->counters;
+            j.counters = ctr;
             j.status = s->emit_status;
             j.status_zeroed = true;
+            j.self_clean = true;
+            j.clear = s->counters + (size_t)(bank ^ 1u) * kCounterWords + 2;
+            j.clear_n = 2 * kCountSlots;
             j.host_total = s->host_counters_dev;  // the raw count, for next frame's capacity
             rc = bin_emit(c, j);
             if (rc) return rc;
+            s->bank = bank ^ 1u;
+            s->emit_clean = true;
             s->order_valid = s->tile_order != nullptr;
             SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));
             s->counters_pending = true;
@@ -581,21 +603,21 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
         // pass; the grand total is the raw isect count (counters[0]).
         SFK_PROF(c, "scan_emit");
         SFK_HIP(c, launch_scan_emit(c->stream, src, n, s->tiles_x, s->capacity, s->tile_ids,
-                                    s->gaussian_ids, s->offsets, s->counters, s->emit_status));
+                                    s->gaussian_ids, s->offsets, ctr, s->emit_status));
     }
-    SFK_HIP(c, hipMemcpyAsync(s->host_counters, s->counters, 4, hipMemcpyDeviceToHost, c->stream));
+    SFK_HIP(c, hipMemcpyAsync(s->host_counters, ctr, 4, hipMemcpyDeviceToHost, c->stream));
     SFK_HIP(c, hipEventRecord(s->counters_ready, c->stream));
     s->counters_pending = true;
     uint32_t* st = nullptr;
     uint32_t* sg = nullptr;
-    rc = sfk_argsort_u32(c, s->sort_tile, s->tile_ids, s->gaussian_ids, s->capacity, s->counters,
+    rc = sfk_argsort_u32(c, s->sort_tile, s->tile_ids, s->gaussian_ids, s->capacity, ctr,
                          sfk_bits_for(tiles), 1, &st, &sg);
     if (rc) return rc;
     s->sorted_tiles = st;
     s->sorted_gids = sg;
     {
         SFK_PROF(c, "tile_starts");
-        SFK_HIP(c, launch_tile_starts(c->stream, st, s->capacity, s->counters, tiles,
+        SFK_HIP(c, launch_tile_starts(c->stream, st, s->capacity, ctr, tiles,
                                       s->tile_starts));
     }
     s->view = *view;
