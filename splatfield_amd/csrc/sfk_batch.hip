@@ -87,7 +87,11 @@ __global__ __launch_bounds__(256) void k_entropy_base(const float* __restrict__ 
 constexpr uint32_t kEigBlock = 1024;
 
 // grid (blocks per view, views): per-block f64 partial sums of the entropy
-// drops of one view.
+// drops of one view.  Only splats with a nonzero responsibility have a drop,
+// so each wavefront first compacts its 256 elements to the nonzero ones
+// (ballot order, in LDS) and then evaluates them 64 at a time with every
+// lane busy, instead of leaving the lanes of zero weights idle through the
+// f64 entropy.  The summation order is fixed by the compacted order.
 __global__ __launch_bounds__(256) void k_eig_batch(const float* __restrict__ a,
                                                    const float* __restrict__ b,
                                                    const double* __restrict__ h0,
@@ -95,18 +99,35 @@ __global__ __launch_bounds__(256) void k_eig_batch(const float* __restrict__ a,
                                                    const BatchView* __restrict__ views,
                                                    double* __restrict__ partials) {
     __shared__ double s_w[4];
+    __shared__ uint32_t s_i[4][256], s_q[4][256];
     const uint32_t v = blockIdx.y;
     const float scale = views[v].scale;
     const uint32_t* r = resp + (size_t)v * n;
-    double sum = 0.0;
-    const uint32_t base = blockIdx.x * kEigBlock + threadIdx.x;
+    const uint32_t w = threadIdx.x / 64, lane = lane_id();
+    // Wave w owns elements base + [w * 256, (w + 1) * 256).
+    const uint32_t base = blockIdx.x * kEigBlock + w * 256u;
+    uint32_t q[4];
 #pragma unroll
     for (uint32_t e = 0; e < 4; ++e) {
-        uint32_t i = base + e * 256;
-        if (i < n) {
-            uint32_t q = r[i];
-            if (q) sum += d_entropy_drop(a[i], b[i], h0[i], q, scale);
+        const uint32_t i = base + e * 64 + lane;
+        q[e] = i < n ? r[i] : 0u;
+    }
+    uint32_t cnt = 0;
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e) {
+        const uint64_t m = __ballot(q[e] != 0u);
+        if (q[e]) {
+            const uint32_t p = cnt + mask_rank(m);
+            s_i[w][p] = base + e * 64 + lane;
+            s_q[w][p] = q[e];
         }
+        cnt += (uint32_t)__popcll(m);
+    }
+    __builtin_amdgcn_wave_barrier();
+    double sum = 0.0;
+    for (uint32_t j = lane; j < cnt; j += 64) {
+        const uint32_t i = s_i[w][j];
+        sum += d_entropy_drop(a[i], b[i], h0[i], s_q[w][j], scale);
     }
     sum = wave_sum_f64(sum);
     if (lane_id() == 0) s_w[threadIdx.x / 64] = sum;
