@@ -70,14 +70,19 @@ SFK_HD float sfk_expf_neg(float p) {
     return a == a ? y : a;
 }
 
-// sfk_expf_neg for the lanes a forward rasterizer uses: the same bits for
-// every p <= 87 and for NaN, without the lower clamp of the argument (it only
-// binds past 87; there the value is unspecified, and the rasterizers discard
-// it: a lane uses G only when its power is at most 5.6 or NaN).  Dropping the
-// clamp lets the negation fold into the remaining min.
+// exp(-p) for the lanes a responsibility rasterizer uses, whose G only
+// enters alpha = min(o * G, 0.999) with o >= 1/255 (culled splats never
+// reach a rasterizer), and only where the power is at most 5.6 or NaN:
+//  * p <= 87: the same bits as sfk_expf(-p) (the scale 2^n is applied by
+//    an exact ldexp, as sfk_expf's multiply by the normal 2^n is exact up to
+//    the same final rounding);
+//  * p < -88 and NaN p: exp(88) = 1.65e38, so o * G >= 0.999 and alpha is
+//    0.999, as with the true value and as min(NaN, 0.999) gives for NaN;
+//  * p > 87: unspecified (such lanes are discarded).
+// Dropping sfk_expf_neg's lower clamp and NaN select saves their operations
+// in the rasterizers' innermost loop.
 SFK_HD float sfk_expf_neg_in(float p) {
-    const float a = -p;
-    const float x = fminf(a, 88.0f);
+    const float x = fminf(-p, 88.0f);  // NaN -> 88
     const float n = floorf(x * 1.44269504088896341f + 0.5f);
     float r = fmaf(-n, 0.693359375f, x);
     r = fmaf(-n, -2.12194440e-4f, r);
@@ -90,8 +95,7 @@ SFK_HD float sfk_expf_neg_in(float p) {
     q = fmaf(q, r, 5.0000001201e-1f);
     q = fmaf(q, z, r);
     q = q + 1.0f;
-    const float y = q * sfk_pow2i((int)n);
-    return a == a ? y : a;
+    return ldexpf(q, (int)n);
 }
 
 SFK_HD float sfk_logf(float x) {
@@ -133,14 +137,12 @@ SFK_HD float sfk_logf(float x) {
     return r;
 }
 
-// `f32 as u32` of the reference (Rust): saturating, NaN -> 0.  Branch-free:
-// for v > 0 the clamps keep every v < 2^32 (the largest such f32 is
-// 4294967040) and the conversion truncates; the selects give 0 for NaN and
-// v <= 0 (whatever the clamped conversion made of them) and saturate
-// v >= 2^32, infinity included.
+// `f32 as u32` of the reference (Rust): saturating, NaN -> 0.  The
+// conversion is only taken for 0 < v < 2^32, where it truncates exactly;
+// written as selects, it compiles without branches (two compares, the
+// hardware conversion, two selects).
 SFK_HD uint32_t sfk_f2u(float v) {
-    const uint32_t t = (uint32_t)fminf(fmaxf(v, 0.0f), 4294967040.0f);
-    return v > 0.0f ? (v >= 4294967296.0f ? 0xffffffffu : t) : 0u;
+    return v > 0.0f ? (v >= 4294967296.0f ? 0xffffffffu : (uint32_t)v) : 0u;
 }
 
 // f32::clamp of the reference: NaN passes through.

@@ -1,8 +1,10 @@
 // Host-side check that the rasterizers' sfk_expf_neg(p) returns the same bits
 // as sfk_expf(-p) (the oracle's exp) over every float p in [-88, 6] visited
 // with the given stride, that NaN propagates, and that arguments past the
-// clamp still saturate alpha = min(o * G, 0.999) for o >= 10/255; and that
-// the branch-free sfk_f2u is Rust's saturating `f32 as u32`.
+// clamp still saturate alpha = min(o * G, 0.999) for o >= 10/255; that the
+// rasterizers' sfk_expf_neg_in has the same bits below p = 87 and gives
+// alpha 0.999 for NaN; and that the branch-free sfk_f2u is Rust's
+// saturating `f32 as u32`.
 #include "../../splatfield_amd/csrc/sfk_math.h"
 
 #include <cstdio>
@@ -27,8 +29,13 @@ int main(int argc, char** argv) {
     const float nan = __builtin_nanf("");
     const float r = sfk_expf_neg(nan);
     if (r == r) ++bad;
+    // sfk_expf_neg_in(NaN) is exp(88): alpha = min(o * G, 0.999) is 0.999 for
+    // every opacity a rasterized splat can have (o >= 1/255), as min(NaN,
+    // 0.999) is.
     const float r_in = sfk_expf_neg_in(nan);
-    if (r_in == r_in) ++bad;
+    const float e88 = sfk_expf(88.0f);
+    if (memcmp(&r_in, &e88, 4) != 0) ++bad;
+    if (!(fminf((1.0f / 255.0f) * r_in, 0.999f) == 0.999f)) ++bad;
     // sfk_expf_neg_in: the same bits as sfk_expf_neg below 87 and at the
     // upper clamp (p < -88) too.
     for (float p = -1e30f; p < 87.0f; p = p < -1.0f ? p / 1.7f : (p < 1.0f ? p + 0.37f : p * 1.3f)) {
