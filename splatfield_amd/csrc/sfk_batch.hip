@@ -33,13 +33,17 @@ namespace {
 constexpr uint32_t kMaxViews = 64;
 
 // One thread per splat, looping over the views: the 44-byte attribute row is
-// read once for all views.
+// read once for all views.  The projected rows are split: per (view, splat)
+// proj = (mean x, mean y, conic a, conic b) and proj_c = conic c; the
+// opacity, the same in every view, once per splat (opac).
 __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restrict__ views,
                                                        uint32_t nviews,
                                                        const float* __restrict__ A, uint32_t n,
                                                        float inv_tile,
                                                        uint32_t* __restrict__ depth_keys,
                                                        float4* __restrict__ proj,
+                                                       float* __restrict__ proj_c,
+                                                       float* __restrict__ opac,
                                                        uint32_t* __restrict__ tile_counts,
                                                        uint2* __restrict__ bbox,
                                                        uint32_t* __restrict__ cbbox,
@@ -51,7 +55,10 @@ __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restric
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     const bool live = i < n;
     SplatIn s;
-    if (live) s = load_splat(A, i, n);
+    if (live) {
+        s = load_splat(A, i, n);
+        opac[i] = s.opacity;
+    }
     for (uint32_t v = 0; v < nviews; ++v) {
         bool vis = false;
         uint32_t cnt = 0;
@@ -63,8 +70,8 @@ __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restric
                 vis = true;
                 key = __builtin_bit_cast(uint32_t, o.cz);
                 cnt = o.count;
-                proj[gi * 2] = make_float4(o.px, o.py, o.qa, o.qb);
-                proj[gi * 2 + 1] = make_float4(o.qc, s.opacity, 0.0f, 0.0f);
+                proj[gi] = make_float4(o.px, o.py, o.qa, o.qb);
+                proj_c[gi] = o.qc;
                 if (!cbbox) bbox[gi] = make_uint2(o.bb_lo, o.bb_hi);
             }
             depth_keys[gi] = key;
@@ -180,7 +187,9 @@ struct sfk_eig_round {
     std::vector<BatchView> bviews;
     sfk_view* d_views = nullptr;
     BatchView* d_bviews = nullptr;
-    float4* proj = nullptr;
+    float4* proj = nullptr;    // per (view, splat): mean x, y, conic a, b
+    float* proj_c = nullptr;   // per (view, splat): conic c
+    float* opac = nullptr;     // per splat: opacity
     uint32_t* keys = nullptr;
     uint32_t* counts = nullptr;
     uint2* bbox = nullptr;
@@ -266,7 +275,8 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
     {
         SFK_PROF(c, "batch_project");
         k_project_batch<<<div_up(n, 256), 256, 0, c->stream>>>(
-            r->d_views, V, attrs, n, 1.0f / (float)r->tile_w, r->keys, r->proj, r->counts,
+            r->d_views, V, attrs, n, 1.0f / (float)r->tile_w, r->keys, r->proj, r->proj_c,
+            r->opac, r->counts,
             r->bbox, r->compact ? r->cbbox : nullptr, r->counters);
         SFK_LAUNCHED(c, "batch project");
     }
@@ -354,6 +364,8 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
         ra.tile_starts = r->starts;
         ra.gids = r->sorted_gids;
         ra.proj = reinterpret_cast<const float*>(r->proj);
+        ra.proj_c = r->proj_c;
+        ra.opac = r->opac;
         ra.resp = r->resp;
         ra.views = r->d_bviews;
         ra.nviews = V;
@@ -462,7 +474,9 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     } while (0)
     RALLOC(r->d_views, nviews * sizeof(sfk_view));
     RALLOC(r->d_bviews, nviews * sizeof(BatchView));
-    RALLOC(r->proj, vn * 32);
+    RALLOC(r->proj, vn * 16);
+    RALLOC(r->proj_c, vn * 4);
+    RALLOC(r->opac, (size_t)n * 4);
     RALLOC(r->keys, vn * 4);
     std::vector<uint32_t> rebase(nviews);
     bool depth_fit = true;
@@ -526,7 +540,7 @@ extern "C" void sfk_eig_round_destroy(sfk_eig_round* r) {
     if (r->ctx) hipStreamSynchronize(r->ctx->stream);
     if (r->exec) hipGraphExecDestroy(r->exec);
     if (r->graph) hipGraphDestroy(r->graph);
-    void* bufs[] = {r->d_views, r->d_bviews, r->proj, r->keys, r->counts, r->bbox, r->cbbox,
+    void* bufs[] = {r->d_views, r->d_bviews, r->proj, r->proj_c, r->opac, r->keys, r->counts, r->bbox, r->cbbox,
                     r->counters, r->starts, r->resp, r->h0, r->partials, r->eig, r->best,
                     r->emit_status, r->d_binviews, r->offsets, r->bin_sstart, r->scan_sums,
                     r->d_rebase, r->tile_order};

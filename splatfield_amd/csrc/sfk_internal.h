@@ -300,9 +300,13 @@ struct RasterArgs {
     uint32_t* resp;
     uint32_t* fg;
     uint32_t* bg;
-    // batched responsibility (launch_rasterize_batch)
+    // batched responsibility (launch_rasterize_batch): proj holds one float4
+    // per (view, splat) (mean x, y, conic a, b), proj_c conic c per (view,
+    // splat) and opac the opacity per splat.
     const BatchView* views;
     uint32_t nviews, batch_tiles, seg_len;
+    const float* proj_c;
+    const float* opac;
 };
 hipError_t launch_rasterize(hipStream_t s, int mode, const RasterArgs& a);
 hipError_t launch_rasterize_batch(hipStream_t s, const RasterArgs& a);

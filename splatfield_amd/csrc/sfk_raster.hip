@@ -100,6 +100,11 @@ struct TileJob {
     float scale;
     const float4* proj;  // row base of this view
     uint32_t rows4;      // float4s per projected row (3 with colour, 2 without)
+    // Split rows (batched): rows4 == 1, conic c from proj_c[g] and the
+    // opacity from opac[g - obase].
+    const float* proj_c;
+    const float* opac;
+    uint32_t obase;
     uint32_t* resp;
     uint32_t* fg;
     uint32_t* bg;
@@ -156,7 +161,14 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
         if (idx < end) {
             uint32_t g = a.gids[idx];
             const float4* row = J.proj + (size_t)g * J.rows4;
-            float4 geo = row[0], aux = row[1];
+            float4 geo, aux;
+            if (J.rows4 == 1) {
+                geo = row[0];
+                aux = make_float4(J.proj_c[g], J.opac[g - J.obase], 0.0f, 0.0f);
+            } else {
+                geo = row[0];
+                aux = row[1];
+            }
             s_box[t] = reach_box(geo, aux, &s_tau[t]);
             if constexpr (MODE == kRgb) {
                 // Conic a and c staged halved, as the adjoint stages them.
@@ -319,6 +331,9 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
     J.scale = a.scale;
     J.proj = reinterpret_cast<const float4*>(a.proj);
     J.rows4 = 3;
+    J.proj_c = nullptr;
+    J.opac = nullptr;
+    J.obase = 0;
     J.resp = a.resp; J.fg = a.fg; J.bg = a.bg;
     raster_tile<MODE, TW, MODE != kRgb>(J, a);
 }
@@ -360,7 +375,10 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize_batch(RasterArgs a) {
     J.row_stride = 0;
     J.scale = bv.scale;
     J.proj = reinterpret_cast<const float4*>(a.proj);
-    J.rows4 = 2;
+    J.rows4 = 1;
+    J.proj_c = a.proj_c;
+    J.opac = a.opac;
+    J.obase = lo * a.seg_len;  // ids are view-major: view lo's splat i is lo * n + i
     J.resp = a.resp;
     J.fg = nullptr; J.bg = nullptr;
     RasterArgs b = a;
