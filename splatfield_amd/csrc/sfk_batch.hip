@@ -49,15 +49,24 @@ __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restric
                                                        uint32_t* __restrict__ cbbox,
                                                        uint32_t* __restrict__ counters) {
     __shared__ sfk_view s_views[kMaxViews];
+    __shared__ float2 s_lims[kMaxViews];
     for (uint32_t k = threadIdx.x; k < nviews * (sizeof(sfk_view) / 4); k += 256)
         reinterpret_cast<float*>(s_views)[k] = reinterpret_cast<const float*>(views)[k];
+    __syncthreads();
+    // Per view once, not per splat and view: the clamp limits.
+    for (uint32_t k = threadIdx.x; k < nviews; k += 256) s_lims[k] = view_lims(s_views[k]);
     __syncthreads();
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     const bool live = i < n;
     SplatIn s;
+    float m[9];
+    float cutoff = 0.0f;
     if (live) {
         s = load_splat(A, i, n);
         opac[i] = s.opacity;
+        // Per splat once, not per view: R S and the extent cutoff.
+        splat_rs(s, m);
+        cutoff = splat_cutoff(s);
     }
     for (uint32_t v = 0; v < nviews; ++v) {
         bool vis = false;
@@ -66,7 +75,7 @@ __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restric
             ProjGeom o;
             const size_t gi = (size_t)v * n + i;
             uint32_t key = 0xffffffffu;
-            if (project_geom(s_views[v], s, inv_tile, o)) {
+            if (project_geom_pre(s_views[v], s, m, cutoff, s_lims[v], inv_tile, o)) {
                 vis = true;
                 key = __builtin_bit_cast(uint32_t, o.cz);
                 cnt = o.count;

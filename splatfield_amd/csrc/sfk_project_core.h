@@ -57,10 +57,38 @@ __device__ __forceinline__ bool passes_culls(const sfk_view& v, const SplatIn& s
     return !(cz <= 0.1f);
 }
 
-// Culls, EWA covariance, conic, mean2d, extent and tile bbox for one view.
-// Returns false when the splat is culled.
-__device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s, float inv_tile,
-                                             ProjGeom& o) {
+// The view-independent part of compute_cov2d (project.rs:175-221): the
+// rotation of the quaternion times the scales, M = R S, row-major.
+__device__ __forceinline__ void splat_rs(const SplatIn& s, float (&m)[9]) {
+    float x2 = s.qx * s.qx, y2 = s.qy * s.qy, z2 = s.qz * s.qz;
+    float xy = s.qx * s.qy, xz = s.qx * s.qz, yz = s.qy * s.qz;
+    float wx = s.qw * s.qx, wy = s.qw * s.qy, wz = s.qw * s.qz;
+    float r00 = 1.0f - 2.0f * (y2 + z2), r01 = 2.0f * (xy - wz), r02 = 2.0f * (xz + wy);
+    float r10 = 2.0f * (xy + wz), r11 = 1.0f - 2.0f * (x2 + z2), r12 = 2.0f * (yz - wx);
+    float r20 = 2.0f * (xz - wy), r21 = 2.0f * (yz + wx), r22 = 1.0f - 2.0f * (x2 + y2);
+    m[0] = r00 * s.sx; m[1] = r01 * s.sy; m[2] = r02 * s.sz;
+    m[3] = r10 * s.sx; m[4] = r11 * s.sy; m[5] = r12 * s.sz;
+    m[6] = r20 * s.sx; m[7] = r21 * s.sy; m[8] = r22 * s.sz;
+}
+
+// The extent cutoff ln(opacity / (1/255)) of project.rs (view-independent).
+__device__ __forceinline__ float splat_cutoff(const SplatIn& s) {
+    return sfk_logf(s.opacity / kAlphaCutoff);
+}
+
+// The clamp limits of the projected mean's tangent, per view.
+__device__ __forceinline__ float2 view_lims(const sfk_view& v) {
+    return make_float2(1.3f * v.img[0] / (2.0f * v.focal[0]), 1.3f * v.img[1] / (2.0f * v.focal[1]));
+}
+
+// Culls, EWA covariance, conic, mean2d, extent and tile bbox for one view,
+// from the splat's view-independent parts (splat_rs, splat_cutoff) and the
+// view's clamp limits (view_lims): the same operations as computing them
+// inline, so a caller that projects one splat into many views computes those
+// once.  Returns false when the splat is culled.
+__device__ __forceinline__ bool project_geom_pre(const sfk_view& v, const SplatIn& s,
+                                                 const float (&m)[9], float cutoff, float2 lim,
+                                                 float inv_tile, ProjGeom& o) {
     if (s.opacity < kAlphaCutoff) return false;
     float cx = dot3f(v.rot0[0], v.rot0[1], v.rot0[2], s.mx, s.my, s.mz) + v.trans[0];
     float cy = dot3f(v.rot1[0], v.rot1[1], v.rot1[2], s.mx, s.my, s.mz) + v.trans[1];
@@ -68,17 +96,10 @@ __device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s
     if (cz <= 0.1f) return false;
     float iz = 1.0f / cz;
     // compute_cov2d (project.rs:175-221)
-    float x2 = s.qx * s.qx, y2 = s.qy * s.qy, z2 = s.qz * s.qz;
-    float xy = s.qx * s.qy, xz = s.qx * s.qz, yz = s.qy * s.qz;
-    float wx = s.qw * s.qx, wy = s.qw * s.qy, wz = s.qw * s.qz;
-    float r00 = 1.0f - 2.0f * (y2 + z2), r01 = 2.0f * (xy - wz), r02 = 2.0f * (xz + wy);
-    float r10 = 2.0f * (xy + wz), r11 = 1.0f - 2.0f * (x2 + z2), r12 = 2.0f * (yz - wx);
-    float r20 = 2.0f * (xz - wy), r21 = 2.0f * (yz + wx), r22 = 1.0f - 2.0f * (x2 + y2);
-    float m00 = r00 * s.sx, m01 = r01 * s.sy, m02 = r02 * s.sz;
-    float m10 = r10 * s.sx, m11 = r11 * s.sy, m12 = r12 * s.sz;
-    float m20 = r20 * s.sx, m21 = r21 * s.sy, m22 = r22 * s.sz;
-    float lim_x = 1.3f * v.img[0] / (2.0f * v.focal[0]);
-    float lim_y = 1.3f * v.img[1] / (2.0f * v.focal[1]);
+    const float m00 = m[0], m01 = m[1], m02 = m[2];
+    const float m10 = m[3], m11 = m[4], m12 = m[5];
+    const float m20 = m[6], m21 = m[7], m22 = m[8];
+    const float lim_x = lim.x, lim_y = lim.y;
     float u = sfk_clampf(cx * iz, -lim_x, lim_x);
     float vv = sfk_clampf(cy * iz, -lim_y, lim_y);
     float fx_iz = v.focal[0] * iz;
@@ -109,7 +130,6 @@ __device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s
     o.cz = cz;
     o.px = v.focal[0] * cx * iz + v.img[0] * 0.5f;
     o.py = v.focal[1] * cy * iz + v.img[1] * 0.5f;
-    float cutoff = sfk_logf(s.opacity / kAlphaCutoff);
     float ex = sqrtf(2.0f * cutoff * ca);
     float ey = sqrtf(2.0f * cutoff * cc);
     // tile_bbox (project.rs:37-46)
@@ -122,6 +142,15 @@ __device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s
     o.bb_lo = min_x | (min_y << 16);
     o.bb_hi = max_x | (max_y << 16);
     return true;
+}
+
+// project_geom_pre with every part computed here (one view).
+__device__ __forceinline__ bool project_geom(const sfk_view& v, const SplatIn& s, float inv_tile,
+                                             ProjGeom& o) {
+    if (s.opacity < kAlphaCutoff) return false;
+    float m[9];
+    splat_rs(s, m);
+    return project_geom_pre(v, s, m, splat_cutoff(s), view_lims(v), inv_tile, o);
 }
 
 // Tile bbox in one word, 8 bits per bound (min_x, min_y, max_x, max_y from
