@@ -182,6 +182,7 @@ unsafe extern "C" {
     ) -> c_int;
     pub fn sfk_radix_scratch_destroy(s: *mut sfk_radix_scratch);
     pub fn sfk_radix_set_digit_bits(s: *mut sfk_radix_scratch, digit_bits: u32) -> c_int;
+    pub fn sfk_radix_set_bucket_finish(s: *mut sfk_radix_scratch, on: c_int) -> c_int;
     pub fn sfk_argsort_u32(
         ctx: *mut sfk_ctx,
         s: *mut sfk_radix_scratch,

@@ -450,6 +450,12 @@ impl<'c> RadixScratch<'c> {
                              _scratch: PhantomData }))
     }
 
+    /// u64 sorts: two passes on the top 16 bits, then every group sorted on
+    /// chip when all fit (default on; a tuning knob, same results).
+    pub fn set_bucket_finish(&mut self, on: bool) -> Result<()> {
+        self.ctx.check(unsafe { sys::sfk_radix_set_bucket_finish(self.raw, on as i32) })
+    }
+
     /// The same over u64 keys (a scratch made by `new_u64`).
     pub fn argsort_u64<'s>(&'s mut self, keys: &GpuTensor<'c, u64>,
                            vals: Option<&GpuTensor<'c, u32>>, n: u32, bits: u32, write_keys: bool)

@@ -107,6 +107,14 @@ void sfk_radix_scratch_destroy(sfk_radix_scratch* s);
 /* Digit width per pass, 4..8 bits (default 8).  The reference fixes 6
  * (sort.rs:11-17); here it is a tunable. */
 int sfk_radix_set_digit_bits(sfk_radix_scratch* s, uint32_t digit_bits);
+/* Strategy of u64 sorts (a tuning and test knob, no counterpart in the
+ * reference; on by default): a sort of at least 2^20 keys of >= 24 bits
+ * with a host count runs two stable passes on the keys' top 16 bits, reads
+ * back the largest resulting group (one synchronisation of the context
+ * stream), and if every group holds at most 512 keys sorts each group by
+ * the remaining bits on chip in one launch; otherwise it runs the plain
+ * passes.  Same results either way. */
+int sfk_radix_set_bucket_finish(sfk_radix_scratch* s, int on);
 
 /* RadixScratch::argsort (sort.rs:302-381): stable argsort of keys[0..n) on
  * their low `bits` bits (keys must be < 2^bits), carrying vals.  vals may be

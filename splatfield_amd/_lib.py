@@ -77,6 +77,7 @@ SIGNATURES = {
     "sfk_bits_for": (u32, [u32]),
     "sfk_radix_scratch_create": (i32, [P, u32, u32, PP]),
     "sfk_radix_scratch_destroy": (None, [P]),
+    "sfk_radix_set_bucket_finish": (i32, [P, i32]),
     "sfk_radix_set_digit_bits": (i32, [P, u32]),
     "sfk_argsort_u32": (i32, [P, P, P, P, u32, P, u32, i32, PP, PP]),
     "sfk_argsort_u64": (i32, [P, P, P, P, u32, P, u32, i32, PP, PP]),

@@ -315,6 +315,12 @@ struct sfk_radix_scratch {
     uint32_t* vals[2] = {nullptr, nullptr};
     uint32_t* hist = nullptr;
     uint32_t* totals = nullptr;
+    // u64 bucket finish (launch_bucket_split / launch_bucket_sort): on unless
+    // switched off; buffers allocated for u64 scratches only.
+    bool bucket_finish = true;
+    uint32_t* bucket_first = nullptr;  // 65536
+    uint32_t* bucket_size = nullptr;   // 65536 + 1 (the largest size last)
+    uint32_t* bucket_max_host = nullptr;  // pinned
 };
 
 static void radix_release(sfk_radix_scratch* s) {
@@ -327,7 +333,11 @@ static void radix_release(sfk_radix_scratch* s) {
     }
     if (s->hist) hipFree(s->hist);
     if (s->totals) hipFree(s->totals);
+    if (s->bucket_first) hipFree(s->bucket_first);
+    if (s->bucket_size) hipFree(s->bucket_size);
+    if (s->bucket_max_host) hipHostFree(s->bucket_max_host);
     s->hist = s->totals = nullptr;
+    s->bucket_first = s->bucket_size = s->bucket_max_host = nullptr;
 }
 
 static int radix_alloc(sfk_ctx* c, sfk_radix_scratch* s, uint32_t max_elems) {
@@ -341,6 +351,11 @@ static int radix_alloc(sfk_ctx* c, sfk_radix_scratch* s, uint32_t max_elems) {
     // Room for up to 256 segments (each may round its last block up).
     SFK_HIP(c, hipMalloc(&s->hist, ((size_t)s->max_blocks + 256) * 256 * 4));
     SFK_HIP(c, hipMalloc(&s->totals, 256 * 256 * 4));
+    if (s->key_bytes == 8 && max_elems >= kBucketMinKeys) {
+        SFK_HIP(c, hipMalloc(&s->bucket_first, 65536 * 4));
+        SFK_HIP(c, hipMalloc(&s->bucket_size, (65536 + 1) * 4));
+        SFK_HIP(c, hipHostMalloc(&s->bucket_max_host, 4, hipHostMallocDefault));
+    }
     return SFK_OK;
 }
 
@@ -391,6 +406,12 @@ int sfk::radix_scratch_spare_u32x2(sfk_radix_scratch* s, uint32_t** keys, uint2*
 }
 
 
+extern "C" int sfk_radix_set_bucket_finish(sfk_radix_scratch* s, int on) {
+    if (!s) return SFK_E_ARG;
+    s->bucket_finish = on != 0;
+    return SFK_OK;
+}
+
 extern "C" int sfk_radix_set_digit_bits(sfk_radix_scratch* s, uint32_t bits) {
     if (!s || bits < 4 || bits > 8) return SFK_E_ARG;
     s->digit_bits = bits;
@@ -426,6 +447,53 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const V
     // Without values the identity permutation is wanted: a zero-width pass
     // is a stable copy that materializes it.
     if (n <= 1 || bits == 0) bits = 0;
+    if (sizeof(K) == 8 && s->bucket_finish && s->bucket_first && !d_n && nseg == 1 &&
+        bits >= 24 && n >= kBucketMinKeys) {
+        // Two stable passes on the top 16 bits group the keys by them; if
+        // every group fits on chip, one launch sorts each by the rest.
+        SortArgs a;
+        memset(&a, 0, sizeof a);
+        a.n_max = n;
+        a.nseg = 1;
+        a.hist = s->hist;
+        a.kpb = radix_keys_per_block(n);
+        a.stride = s->max_blocks;
+        a.totals = s->totals;
+        a.width = 8;
+        for (uint32_t p = 0; p < 2; ++p) {
+            a.keys_in = p == 0 ? static_cast<const void*>(keys) : s->keys[0];
+            a.vals_in = p == 0 ? static_cast<const void*>(vals) : s->vals[0];
+            a.keys_out = s->keys[p];
+            a.vals_out = s->vals[p];
+            a.shift = bits - 16 + 8 * p;
+            SFK_PROF(c, "radix_pass_u64");
+            hipError_t e = launch_radix_pass(c->stream, a, 8, sizeof(V), 8);
+            if (e != hipSuccess) return hip_fail(c, e, "radix pass");
+        }
+        const uint64_t* grouped = static_cast<const uint64_t*>(s->keys[1]);
+        {
+            SFK_PROF(c, "radix_bucket_split");
+            hipError_t e = launch_bucket_split(c->stream, grouped, n, bits - 16, s->bucket_first,
+                                               s->bucket_size, s->bucket_size + 65536);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(s->bucket_max_host, s->bucket_size + 65536, 4,
+                                   hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) return hip_fail(c, e, "radix bucket split");
+        }
+        if (*s->bucket_max_host <= bucket_cap()) {
+            SFK_PROF(c, "radix_bucket_sort");
+            hipError_t e = launch_bucket_sort(
+                c->stream, grouped, s->vals[1],
+                write_keys ? static_cast<uint64_t*>(s->keys[0]) : nullptr, s->vals[0],
+                s->bucket_first, s->bucket_size, bits - 16);
+            if (e != hipSuccess) return hip_fail(c, e, "radix bucket sort");
+            if (out_keys) *out_keys = static_cast<K*>(s->keys[0]);
+            *out_vals = reinterpret_cast<V*>(s->vals[0]);
+            return SFK_OK;
+        }
+        // A bucket too large for one wavefront: the plain passes below.
+    }
     const K* cur_k = keys;
     const V* cur_v = vals;
     int dst = 0;

@@ -109,6 +109,8 @@ hipError_t launch_project(hipStream_t s, const sfk_view& v, const float* attrs, 
 
 // Radix sort: one pass = upsweep + digit scan + scatter.
 constexpr uint32_t kSortThreads = 512;
+// Smallest u64 sort (host count) that tries the bucket finish.
+constexpr uint32_t kBucketMinKeys = 1u << 20;
 constexpr uint64_t kSortSmallMax = 1u << 22;  // up to this many keys: 2048-key blocks
 uint32_t radix_keys_per_block(uint64_t total_keys);   // 2048 or 4096
 uint32_t radix_max_blocks(uint32_t max_elems);
@@ -147,6 +149,17 @@ struct KeyRebase {
 bool depth_rebase(const sfk_view& v, const float bmin[3], const float bmax[3], uint32_t* base);
 hipError_t launch_radix_pass(hipStream_t s, const SortArgs& a, uint32_t key_bytes,
                              uint32_t val_bytes, uint32_t digit_bits);
+// Bucket finish of u64 sorts (sfk_sort.hip): keys grouped by their 16 bits
+// at `shift` (two stable passes) -> first[65536] / size[65536] of every
+// bucket and the largest size (*max_size, device); then every bucket of at
+// most bucket_cap() keys sorted by its bits below `shift` (lbits of them).
+hipError_t launch_bucket_split(hipStream_t s, const uint64_t* grouped_keys, uint32_t n,
+                               uint32_t shift, uint32_t* first, uint32_t* size,
+                               uint32_t* max_size);
+uint32_t bucket_cap();
+hipError_t launch_bucket_sort(hipStream_t s, const uint64_t* keys_in, const uint32_t* vals_in,
+                              uint64_t* keys_out, uint32_t* vals_out, const uint32_t* first,
+                              const uint32_t* size, uint32_t lbits);
 // Radix scratch with 8-byte values (u32 keys only), and the argsort of u32
 // keys whose identity values carry a u32 payload: out[r] = (index, payload[index])
 // of the element of rank r within its segment.

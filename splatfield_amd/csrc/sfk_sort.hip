@@ -30,6 +30,7 @@
 #include "sfk_internal.h"
 
 #include <algorithm>
+#include <cstring>
 
 namespace sfk {
 
@@ -393,7 +394,233 @@ hipError_t pass_impl(hipStream_t st, const SortArgs& a) {
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ bucket finish
+//
+// A u64 sort of `bits` >= 24 bits whose keys spread over their top 16 bits:
+// two stable passes on those 16 bits (the three-launch passes above) leave
+// the keys grouped by their top 16 bits, each group (bucket) in input order;
+// sorting every bucket on its own by the remaining bits, stably, completes
+// the stable sort by all `bits`.  k_bucket_first marks where each bucket
+// starts, k_bucket_sizes turns that into sizes and their maximum (the host
+// reads it: every bucket must fit kBucketCap, else the plain passes run
+// instead), and k_bucket_sort sorts each bucket with one wavefront on chip:
+// keys and 16-bit slots through LDS, LSD passes of at most 8 bits ranked by
+// 64-lane ballot match, values gathered once by slot.
+
+// first[t] = index of the first key of bucket t in the grouped keys (set
+// only for nonempty buckets; the caller fills first with 0xffffffff).
+__global__ __launch_bounds__(256) void k_bucket_first(const uint64_t* __restrict__ keys, uint32_t n,
+                                                      uint32_t shift, uint32_t* __restrict__ first) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t t = (uint32_t)(keys[i] >> shift) & 0xffffu;
+    if (i == 0 || ((uint32_t)(keys[i - 1] >> shift) & 0xffffu) != t) first[t] = i;
+}
+
+// size[t] of every bucket from first[] (empty buckets 0), and the largest
+// into *max_size: one workgroup, each thread a run of 64 buckets; the next
+// nonempty start after a run comes from a suffix minimum over the runs.
+constexpr uint32_t kBucketSizeThreads = 1024;
+__global__ __launch_bounds__(kBucketSizeThreads) void k_bucket_sizes(const uint32_t* __restrict__ first,
+                                                                     uint32_t n,
+                                                                     uint32_t* __restrict__ size,
+                                                                     uint32_t* __restrict__ max_size) {
+    constexpr uint32_t R = 65536 / kBucketSizeThreads;  // buckets per thread
+    __shared__ uint32_t s_min[kBucketSizeThreads + 1];
+    __shared__ uint32_t s_max[kBucketSizeThreads / 64];
+    const uint32_t t = threadIdx.x, t0 = t * R;
+    uint32_t f[R];
+    uint32_t mn = 0xffffffffu;
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        f[r] = first[t0 + r];
+        mn = min(mn, f[r]);
+    }
+    s_min[t] = mn;
+    if (t == 0) s_min[kBucketSizeThreads] = n;  // past the last bucket: the end
+    __syncthreads();
+    // Suffix minimum over the runs after this one (a serial scan by one
+    // wave would do; log steps over LDS keep it short).
+    for (uint32_t o = 1; o < kBucketSizeThreads; o <<= 1) {
+        const uint32_t v = t + o <= kBucketSizeThreads ? s_min[t + o] : 0xffffffffu;
+        __syncthreads();
+        s_min[t] = min(s_min[t], v);
+        __syncthreads();
+    }
+    uint32_t next = t + 1 < kBucketSizeThreads ? s_min[t + 1] : n;  // next nonempty start after the run
+    next = min(next, n);
+    uint32_t mx = 0;
+#pragma unroll
+    for (int r = (int)R - 1; r >= 0; --r) {
+        const uint32_t sz = f[r] == 0xffffffffu ? 0u : next - f[r];
+        size[t0 + r] = sz;
+        mx = max(mx, sz);
+        if (f[r] != 0xffffffffu) next = f[r];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o, 64));
+    if (lane_id() == 0) s_max[t / 64] = mx;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t m = 0;
+        for (uint32_t k = 0; k < kBucketSizeThreads / 64; ++k) m = max(m, s_max[k]);
+        *max_size = m;
+    }
+}
+
+// One wavefront per bucket (four per workgroup): the bucket's keys (u64,
+// at most kBucketCap) sorted by bits [0, lbits) in LSD passes of width[p]
+// bits, stably; keys_in / vals_in the grouped arrays, keys_out / vals_out
+// the result (keys_out may be null).
+struct BucketArgs {
+    const uint64_t* keys_in;
+    const uint32_t* vals_in;
+    uint64_t* keys_out;
+    uint32_t* vals_out;
+    const uint32_t* first;
+    const uint32_t* size;
+    uint32_t passes;
+    uint32_t width[7];
+};
+constexpr uint32_t kBucketEpt = 8;
+constexpr uint32_t kBucketCap = 64 * kBucketEpt;
+static_assert(kBucketCap <= 65536, "16-bit slots");
+
+__global__ __launch_bounds__(256) void k_bucket_sort(BucketArgs a) {
+    __shared__ uint64_t s_key[4][kBucketCap];
+    __shared__ uint16_t s_idx[4][kBucketCap];
+    __shared__ uint32_t s_cnt[4][256];
+    const uint32_t w = threadIdx.x / 64, lane = lane_id();
+    const uint32_t b = blockIdx.x * 4u + w;
+    const uint32_t m = a.size[b];
+    if (m == 0 || m > kBucketCap) return;  // (sizes above the cap never reach here)
+    const size_t start = a.first[b];
+    const uint32_t ept = (m + 63) / 64;  // wave-uniform, 1..kBucketEpt
+    uint64_t key[kBucketEpt];
+    uint32_t pos[kBucketEpt], loc[kBucketEpt], dg[kBucketEpt];
+    bool valid[kBucketEpt];
+#pragma unroll
+    for (uint32_t e = 0; e < kBucketEpt; ++e) {
+        const uint32_t i = e * 64 + lane;  // item e of lane l: element e * 64 + l
+        valid[e] = e < ept && i < m;
+        key[e] = valid[e] ? a.keys_in[start + i] : 0ull;
+        pos[e] = i;
+    }
+    uint32_t* cnt = s_cnt[w];
+    uint32_t sh = 0;
+    for (uint32_t p = 0; p < a.passes; ++p) {
+        const uint32_t width = a.width[p], mask = (1u << width) - 1u;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) cnt[lane + 64 * k] = 0u;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (uint32_t e = 0; e < kBucketEpt; ++e) {
+            if (e >= ept) break;
+            const uint32_t d = (uint32_t)(key[e] >> sh) & mask;
+            dg[e] = d;
+            uint64_t peers = __ballot(valid[e]);
+#pragma unroll
+            for (uint32_t bit = 0; bit < 8; ++bit) {
+                if (bit < width) {
+                    const bool set = (d >> bit) & 1u;
+                    const uint64_t bal = __ballot(set);
+                    peers &= set ? bal : ~bal;
+                }
+            }
+            const uint32_t rank = mask_rank(peers);
+            uint32_t base = 0;
+            if (valid[e]) base = cnt[d];
+            if (valid[e] && rank == 0) cnt[d] = base + (uint32_t)__popcll(peers);
+            loc[e] = base + rank;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // Exclusive scan of the 256 digit counts: lane l holds digits
+        // 4l .. 4l + 3.
+        {
+            uint32_t c[4], sum = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                c[k] = cnt[4 * lane + k];
+                sum += c[k];
+            }
+            uint32_t run = wave_incl_scan_u32(sum) - sum;
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                cnt[4 * lane + k] = run;
+                run += c[k];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (uint32_t e = 0; e < kBucketEpt; ++e) {
+            if (e >= ept) break;
+            if (valid[e]) {
+                const uint32_t q = cnt[dg[e]] + loc[e];
+                s_key[w][q] = key[e];
+                s_idx[w][q] = (uint16_t)pos[e];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        sh += width;
+#pragma unroll
+        for (uint32_t e = 0; e < kBucketEpt; ++e) {
+            if (e >= ept) break;
+            const uint32_t i = e * 64 + lane;
+            if (valid[e]) {
+                key[e] = s_key[w][i];
+                pos[e] = s_idx[w][i];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    // Every gather issued before any store.
+    uint32_t v[kBucketEpt];
+#pragma unroll
+    for (uint32_t e = 0; e < kBucketEpt; ++e)
+        if (valid[e]) v[e] = a.vals_in[start + pos[e]];
+#pragma unroll
+    for (uint32_t e = 0; e < kBucketEpt; ++e) {
+        if (valid[e]) {
+            const size_t o = start + e * 64 + lane;
+            if (a.keys_out) a.keys_out[o] = key[e];
+            a.vals_out[o] = v[e];
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t launch_bucket_split(hipStream_t st, const uint64_t* grouped_keys, uint32_t n,
+                               uint32_t shift, uint32_t* first, uint32_t* size,
+                               uint32_t* max_size) {
+    hipError_t e = launch_fill_u32(st, first, 0xffffffffu, 65536);
+    if (e != hipSuccess) return e;
+    k_bucket_first<<<div_up(n, 256), 256, 0, st>>>(grouped_keys, n, shift, first);
+    k_bucket_sizes<<<1, kBucketSizeThreads, 0, st>>>(first, n, size, max_size);
+    return hipGetLastError();
+}
+
+uint32_t bucket_cap() { return kBucketCap; }
+
+hipError_t launch_bucket_sort(hipStream_t st, const uint64_t* keys_in, const uint32_t* vals_in,
+                              uint64_t* keys_out, uint32_t* vals_out, const uint32_t* first,
+                              const uint32_t* size, uint32_t lbits) {
+    BucketArgs a;
+    memset(&a, 0, sizeof a);
+    a.keys_in = keys_in;
+    a.vals_in = vals_in;
+    a.keys_out = keys_out;
+    a.vals_out = vals_out;
+    a.first = first;
+    a.size = size;
+    a.passes = (lbits + 7) / 8;
+    if (a.passes == 0 || a.passes > 7) return hipErrorInvalidValue;
+    for (uint32_t p = 0; p < a.passes; ++p)
+        a.width[p] = lbits / a.passes + (p < lbits % a.passes ? 1u : 0u);
+    k_bucket_sort<<<65536 / 4, 256, 0, st>>>(a);
+    return hipGetLastError();
+}
 
 uint32_t radix_keys_per_block(uint64_t total) {
     return total <= kSortSmallMax ? NT * kEptSmall : NT * kEptBig;

@@ -30,6 +30,12 @@ class RadixScratch:
     def set_digit_bits(self, bits):
         self.ctx.check(lib().sfk_radix_set_digit_bits(self.ptr, bits), "digit bits")
 
+    def set_bucket_finish(self, on):
+        """u64 sorts of >= 2^20 keys: two passes on the top 16 bits, then each
+        group sorted on chip when all fit (default on); same results."""
+        self.ctx.check(lib().sfk_radix_set_bucket_finish(self.ptr, 1 if on else 0),
+                       "bucket finish")
+
     def argsort(self, keys, vals, n, bits, write_keys=True, d_n=None):
         """Stable argsort of keys[0..n) on their low ``bits`` (sort.rs:302-381).
         ``vals`` may be None (identity).  Returns (keys, vals) GpuTensors that

@@ -117,6 +117,47 @@ def test_u64_16m_standalone(ctx):
     np.testing.assert_array_equal(ov2, ref)
 
 
+def _bucket_keys(kind, n, rng):
+    if kind == "uniform64":
+        return rng.integers(0, 2**64 - 1, n, dtype=np.uint64), 64
+    if kind == "ties":  # 64 keys a bucket, 4 values each: stability inside a bucket
+        hi = rng.integers(0, 1 << 16, n, dtype=np.uint64) << np.uint64(48)
+        return hi | rng.integers(0, 4, n, dtype=np.uint64), 64
+    if kind == "bits40":  # keys < 2^40 sorted on 40 bits: groups on bits 24..39
+        return rng.integers(0, 1 << 40, n, dtype=np.uint64), 40
+    if kind == "bits24":  # one 8-bit pass inside each group
+        return rng.integers(0, 1 << 24, n, dtype=np.uint64), 24
+    if kind == "narrow":  # keys < 2^40 sorted on 64 bits: one group -> the plain passes
+        return rng.integers(0, 1 << 40, n, dtype=np.uint64), 64
+    if kind == "one_heavy":  # 1% of keys in one group: past the on-chip cap -> plain
+        k = rng.integers(0, 2**64 - 1, n, dtype=np.uint64)
+        k[rng.random(n) < 0.01] = np.uint64(0x1234) << np.uint64(48) | np.uint64(7)
+        return k, 64
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["uniform64", "ties", "bits40", "bits24", "narrow", "one_heavy"])
+def test_u64_bucket_finish_matches_stable_argsort(ctx, kind):
+    """Two passes on the top 16 bits then an on-chip sort of every group
+    (or, when a group exceeds the on-chip capacity, the plain passes): the
+    stable argsort of the keys' low `bits`, keys written or not, identity or
+    given values; the same as with the strategy switched off."""
+    n = 1 << 21
+    rng = np.random.default_rng(len(kind))
+    keys, bits = _bucket_keys(kind, n, rng)
+    ref = np.argsort(keys, kind="stable")
+    vals = rng.integers(0, 2**32 - 1, n, dtype=np.uint64).astype(np.uint32)
+    s = SO.RadixScratch(ctx, n, 8)
+    ok, ov, _ = argsort_gpu(ctx, keys, vals, bits, True, key_bytes=8, scratch=s)
+    np.testing.assert_array_equal(ov, vals[ref])
+    np.testing.assert_array_equal(ok, keys[ref])
+    _, ov, _ = argsort_gpu(ctx, keys, None, bits, False, key_bytes=8, scratch=s)
+    np.testing.assert_array_equal(ov, ref)
+    s.set_bucket_finish(False)
+    _, ov_plain, _ = argsort_gpu(ctx, keys, None, bits, False, key_bytes=8, scratch=s)
+    np.testing.assert_array_equal(ov_plain, ref)
+
+
 @pytest.mark.parametrize("n", [1, 5, 1024, 1025, 70000, 100000, 2_000_003])
 def test_scan_gather_matches_cpu(ctx, n):
     """scan.rs:283-311 (gather) and 313-331 (deterministic counts = i % 4)."""
