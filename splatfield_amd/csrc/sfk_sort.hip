@@ -469,9 +469,17 @@ __global__ __launch_bounds__(kBucketSizeThreads) void k_bucket_sizes(const uint3
 }
 
 // One wavefront per bucket (four per workgroup): the bucket's keys (u64,
-// at most kBucketCap) sorted by bits [0, lbits) in LSD passes of width[p]
-// bits, stably; keys_in / vals_in the grouped arrays, keys_out / vals_out
-// the result (keys_out may be null).
+// at most kBucketCap) sorted stably by bits [0, lbits); keys_in / vals_in the
+// grouped arrays, keys_out / vals_out the result (keys_out may be null).
+//  * lbits <= 16: LSD passes of width[p] bits over all of them.
+//  * lbits > 16: two passes on the top 16 of them leave the bucket sorted
+//    by those, equal 16-bit prefixes in input order; a run of equal
+//    prefixes is then almost always one or two keys (a bucket of ~256
+//    random keys shares a 16-bit prefix pair about once), and a pair is put
+//    in order by one compare and swap on the remaining bits (equal keys
+//    keep their order).  A bucket with a run of three or more (ties,
+//    clustered keys) is sorted again from its input order by the full LSD
+//    passes instead.
 struct BucketArgs {
     const uint64_t* keys_in;
     const uint32_t* vals_in;
@@ -479,12 +487,87 @@ struct BucketArgs {
     uint32_t* vals_out;
     const uint32_t* first;
     const uint32_t* size;
+    uint32_t lbits;
     uint32_t passes;
     uint32_t width[7];
 };
-constexpr uint32_t kBucketEpt = 8;
+constexpr uint32_t kBucketEpt = 6;
 constexpr uint32_t kBucketCap = 64 * kBucketEpt;
 static_assert(kBucketCap <= 65536, "16-bit slots");
+
+// One stable LSD pass of the wave's items (item e of lane l is element
+// e * 64 + l, e < ept) on the `width`-bit digit at `sh`: ranks by 64-lane
+// ballot match, digit starts from a 256-entry count, keys and slots through
+// LDS back into the same item order.
+__device__ __forceinline__ void bucket_pass(uint64_t (&key)[kBucketEpt], uint32_t (&pos)[kBucketEpt],
+                                            const bool (&valid)[kBucketEpt], uint32_t ept,
+                                            uint32_t sh, uint32_t width, uint32_t* cnt,
+                                            uint64_t* s_key, uint16_t* s_idx) {
+    const uint32_t lane = lane_id();
+    const uint32_t mask = (1u << width) - 1u;
+    uint32_t loc[kBucketEpt], dg[kBucketEpt];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) cnt[lane + 64 * k] = 0u;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (uint32_t e = 0; e < kBucketEpt; ++e) {
+        if (e >= ept) break;
+        const uint32_t d = (uint32_t)(key[e] >> sh) & mask;
+        dg[e] = d;
+        uint64_t peers = __ballot(valid[e]);
+#pragma unroll
+        for (uint32_t bit = 0; bit < 8; ++bit) {
+            if (bit < width) {
+                const bool set = (d >> bit) & 1u;
+                const uint64_t bal = __ballot(set);
+                peers &= set ? bal : ~bal;
+            }
+        }
+        const uint32_t rank = mask_rank(peers);
+        uint32_t base = 0;
+        if (valid[e]) base = cnt[d];
+        if (valid[e] && rank == 0) cnt[d] = base + (uint32_t)__popcll(peers);
+        loc[e] = base + rank;
+    }
+    __builtin_amdgcn_wave_barrier();
+    {
+        // Exclusive scan of the 256 counts: lane l holds digits 4l .. 4l + 3.
+        uint32_t c[4], sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            c[k] = cnt[4 * lane + k];
+            sum += c[k];
+        }
+        uint32_t run = wave_incl_scan_u32(sum) - sum;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            cnt[4 * lane + k] = run;
+            run += c[k];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (uint32_t e = 0; e < kBucketEpt; ++e) {
+        if (e >= ept) break;
+        if (valid[e]) {
+            const uint32_t q = cnt[dg[e]] + loc[e];
+            s_key[q] = key[e];
+            s_idx[q] = (uint16_t)pos[e];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (uint32_t e = 0; e < kBucketEpt; ++e) {
+        if (e >= ept) break;
+        const uint32_t i = e * 64 + lane;
+        if (valid[e]) {
+            key[e] = s_key[i];
+            pos[e] = s_idx[i];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
 
 __global__ __launch_bounds__(256) void k_bucket_sort(BucketArgs a) {
     __shared__ uint64_t s_key[4][kBucketCap];
@@ -497,82 +580,75 @@ __global__ __launch_bounds__(256) void k_bucket_sort(BucketArgs a) {
     const size_t start = a.first[b];
     const uint32_t ept = (m + 63) / 64;  // wave-uniform, 1..kBucketEpt
     uint64_t key[kBucketEpt];
-    uint32_t pos[kBucketEpt], loc[kBucketEpt], dg[kBucketEpt];
+    uint32_t pos[kBucketEpt];
     bool valid[kBucketEpt];
-#pragma unroll
-    for (uint32_t e = 0; e < kBucketEpt; ++e) {
-        const uint32_t i = e * 64 + lane;  // item e of lane l: element e * 64 + l
-        valid[e] = e < ept && i < m;
-        key[e] = valid[e] ? a.keys_in[start + i] : 0ull;
-        pos[e] = i;
-    }
-    uint32_t* cnt = s_cnt[w];
-    uint32_t sh = 0;
-    for (uint32_t p = 0; p < a.passes; ++p) {
-        const uint32_t width = a.width[p], mask = (1u << width) - 1u;
-#pragma unroll
-        for (uint32_t k = 0; k < 4; ++k) cnt[lane + 64 * k] = 0u;
-        __builtin_amdgcn_wave_barrier();
+    auto load = [&]() {
 #pragma unroll
         for (uint32_t e = 0; e < kBucketEpt; ++e) {
-            if (e >= ept) break;
-            const uint32_t d = (uint32_t)(key[e] >> sh) & mask;
-            dg[e] = d;
-            uint64_t peers = __ballot(valid[e]);
-#pragma unroll
-            for (uint32_t bit = 0; bit < 8; ++bit) {
-                if (bit < width) {
-                    const bool set = (d >> bit) & 1u;
-                    const uint64_t bal = __ballot(set);
-                    peers &= set ? bal : ~bal;
-                }
-            }
-            const uint32_t rank = mask_rank(peers);
-            uint32_t base = 0;
-            if (valid[e]) base = cnt[d];
-            if (valid[e] && rank == 0) cnt[d] = base + (uint32_t)__popcll(peers);
-            loc[e] = base + rank;
+            const uint32_t i = e * 64 + lane;
+            valid[e] = e < ept && i < m;
+            key[e] = valid[e] ? a.keys_in[start + i] : 0ull;
+            pos[e] = i;
         }
-        __builtin_amdgcn_wave_barrier();
-        // Exclusive scan of the 256 digit counts: lane l holds digits
-        // 4l .. 4l + 3.
-        {
-            uint32_t c[4], sum = 0;
-#pragma unroll
-            for (uint32_t k = 0; k < 4; ++k) {
-                c[k] = cnt[4 * lane + k];
-                sum += c[k];
-            }
-            uint32_t run = wave_incl_scan_u32(sum) - sum;
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (uint32_t k = 0; k < 4; ++k) {
-                cnt[4 * lane + k] = run;
-                run += c[k];
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (uint32_t e = 0; e < kBucketEpt; ++e) {
-            if (e >= ept) break;
-            if (valid[e]) {
-                const uint32_t q = cnt[dg[e]] + loc[e];
-                s_key[w][q] = key[e];
-                s_idx[w][q] = (uint16_t)pos[e];
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        sh += width;
+    };
+    load();
+    bool need_full = a.lbits <= 16;
+    if (!need_full) {
+        const uint32_t hs = a.lbits - 16;
+        for (uint32_t p = 0; p < 2; ++p)
+            bucket_pass(key, pos, valid, ept, hs + 8 * p, 8, s_cnt[w], s_key[w], s_idx[w]);
+        // Runs of equal 16-bit prefixes (s_key holds the sorted keys).
+        const uint64_t low = (1ull << hs) - 1ull;
+        uint32_t pair_at = 0;  // bit e: item e starts an out-of-order pair
+        bool long_run = false;
 #pragma unroll
         for (uint32_t e = 0; e < kBucketEpt; ++e) {
             if (e >= ept) break;
             const uint32_t i = e * 64 + lane;
-            if (valid[e]) {
-                key[e] = s_key[w][i];
-                pos[e] = s_idx[w][i];
+            if (!valid[e]) continue;
+            const uint32_t pfx = (uint32_t)(key[e] >> hs) & 0xffffu;
+            const bool tie_next = i + 1 < m && ((uint32_t)(s_key[w][i + 1] >> hs) & 0xffffu) == pfx;
+            const bool tie_prev = i > 0 && ((uint32_t)(s_key[w][i - 1] >> hs) & 0xffffu) == pfx;
+            long_run |= tie_next && tie_prev;
+            if (tie_next && (key[e] & low) > (s_key[w][i + 1] & low)) pair_at |= 1u << e;
+        }
+        if (__ballot(long_run)) {
+            need_full = true;
+            load();
+        } else {
+            // Disjoint pairs: swap each out-of-order one (both halves
+            // written by the lane holding its first item).
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (uint32_t e = 0; e < kBucketEpt; ++e) {
+                if ((pair_at >> e) & 1u) {
+                    const uint32_t i = e * 64 + lane;
+                    const uint64_t k1 = s_key[w][i + 1];
+                    const uint16_t p1 = s_idx[w][i + 1];
+                    s_key[w][i + 1] = key[e];
+                    s_idx[w][i + 1] = (uint16_t)pos[e];
+                    s_key[w][i] = k1;
+                    s_idx[w][i] = p1;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (uint32_t e = 0; e < kBucketEpt; ++e) {
+                if (e >= ept) break;
+                const uint32_t i = e * 64 + lane;
+                if (valid[e]) {
+                    key[e] = s_key[w][i];
+                    pos[e] = s_idx[w][i];
+                }
             }
         }
-        __builtin_amdgcn_wave_barrier();
+    }
+    if (need_full) {
+        uint32_t sh = 0;
+        for (uint32_t p = 0; p < a.passes; ++p) {
+            bucket_pass(key, pos, valid, ept, sh, a.width[p], s_cnt[w], s_key[w], s_idx[w]);
+            sh += a.width[p];
+        }
     }
     // Every gather issued before any store.
     uint32_t v[kBucketEpt];
@@ -614,6 +690,7 @@ hipError_t launch_bucket_sort(hipStream_t st, const uint64_t* keys_in, const uin
     a.vals_out = vals_out;
     a.first = first;
     a.size = size;
+    a.lbits = lbits;
     a.passes = (lbits + 7) / 8;
     if (a.passes == 0 || a.passes > 7) return hipErrorInvalidValue;
     for (uint32_t p = 0; p < a.passes; ++p)
