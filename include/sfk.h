@@ -109,11 +109,11 @@ void sfk_radix_scratch_destroy(sfk_radix_scratch* s);
 int sfk_radix_set_digit_bits(sfk_radix_scratch* s, uint32_t digit_bits);
 /* Strategy of u64 sorts (a tuning and test knob, no counterpart in the
  * reference; on by default): a sort of at least 2^20 keys of >= 24 bits
- * with a host count runs two stable passes on the keys' top 16 bits, reads
- * back the largest resulting group (one synchronisation of the context
- * stream), and if every group holds at most 512 keys sorts each group by
- * the remaining bits on chip in one launch; otherwise it runs the plain
- * passes.  Same results either way. */
+ * with a host count runs two stable passes on the keys' top 16 bits, then
+ * sorts every resulting group of at most 384 keys by the remaining bits on
+ * chip in one launch, and reads back the largest group size (the call
+ * synchronises the context stream once); if a group was larger, it runs the
+ * plain passes instead.  Same results either way. */
 int sfk_radix_set_bucket_finish(sfk_radix_scratch* s, int on);
 
 /* RadixScratch::argsort (sort.rs:302-381): stable argsort of keys[0..n) on

@@ -319,8 +319,9 @@ struct sfk_radix_scratch {
     // switched off; buffers allocated for u64 scratches only.
     bool bucket_finish = true;
     uint32_t* bucket_first = nullptr;  // 65536
-    uint32_t* bucket_size = nullptr;   // 65536 + 1 (the largest size last)
-    uint32_t* bucket_max_host = nullptr;  // pinned
+    uint32_t* bucket_last = nullptr;   // 65536
+    uint32_t* bucket_size = nullptr;   // 65536, then bucket_size_groups() group maxima
+    uint32_t* bucket_max_host = nullptr;  // pinned, bucket_size_groups()
 };
 
 static void radix_release(sfk_radix_scratch* s) {
@@ -334,10 +335,11 @@ static void radix_release(sfk_radix_scratch* s) {
     if (s->hist) hipFree(s->hist);
     if (s->totals) hipFree(s->totals);
     if (s->bucket_first) hipFree(s->bucket_first);
+    if (s->bucket_last) hipFree(s->bucket_last);
     if (s->bucket_size) hipFree(s->bucket_size);
     if (s->bucket_max_host) hipHostFree(s->bucket_max_host);
     s->hist = s->totals = nullptr;
-    s->bucket_first = s->bucket_size = s->bucket_max_host = nullptr;
+    s->bucket_first = s->bucket_last = s->bucket_size = s->bucket_max_host = nullptr;
 }
 
 static int radix_alloc(sfk_ctx* c, sfk_radix_scratch* s, uint32_t max_elems) {
@@ -353,8 +355,10 @@ static int radix_alloc(sfk_ctx* c, sfk_radix_scratch* s, uint32_t max_elems) {
     SFK_HIP(c, hipMalloc(&s->totals, 256 * 256 * 4));
     if (s->key_bytes == 8 && max_elems >= kBucketMinKeys) {
         SFK_HIP(c, hipMalloc(&s->bucket_first, 65536 * 4));
-        SFK_HIP(c, hipMalloc(&s->bucket_size, (65536 + 1) * 4));
-        SFK_HIP(c, hipHostMalloc(&s->bucket_max_host, 4, hipHostMallocDefault));
+        SFK_HIP(c, hipMalloc(&s->bucket_last, 65536 * 4));
+        SFK_HIP(c, hipMalloc(&s->bucket_size, (65536 + (size_t)bucket_size_groups()) * 4));
+        SFK_HIP(c, hipHostMalloc(&s->bucket_max_host, (size_t)bucket_size_groups() * 4,
+                                 hipHostMallocDefault));
     }
     return SFK_OK;
 }
@@ -471,28 +475,37 @@ static int argsort_impl(sfk_ctx* c, sfk_radix_scratch* s, const K* keys, const V
             if (e != hipSuccess) return hip_fail(c, e, "radix pass");
         }
         const uint64_t* grouped = static_cast<const uint64_t*>(s->keys[1]);
+        const uint32_t ng = bucket_size_groups();
         {
             SFK_PROF(c, "radix_bucket_split");
             hipError_t e = launch_bucket_split(c->stream, grouped, n, bits - 16, s->bucket_first,
-                                               s->bucket_size, s->bucket_size + 65536);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(s->bucket_max_host, s->bucket_size + 65536, 4,
-                                   hipMemcpyDeviceToHost, c->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+                                               s->bucket_last, s->bucket_size,
+                                               s->bucket_size + 65536);
             if (e != hipSuccess) return hip_fail(c, e, "radix bucket split");
         }
-        if (*s->bucket_max_host <= bucket_cap()) {
+        {
+            // Queued before the size check is read back: in the usual case
+            // the host waits once, for the finished sort.
             SFK_PROF(c, "radix_bucket_sort");
             hipError_t e = launch_bucket_sort(
                 c->stream, grouped, s->vals[1],
                 write_keys ? static_cast<uint64_t*>(s->keys[0]) : nullptr, s->vals[0],
                 s->bucket_first, s->bucket_size, bits - 16);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(s->bucket_max_host, s->bucket_size + 65536, (size_t)ng * 4,
+                                   hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
             if (e != hipSuccess) return hip_fail(c, e, "radix bucket sort");
+        }
+        uint32_t mx = 0;
+        for (uint32_t g = 0; g < ng; ++g) mx = std::max(mx, s->bucket_max_host[g]);
+        if (mx <= bucket_cap()) {
             if (out_keys) *out_keys = static_cast<K*>(s->keys[0]);
             *out_vals = reinterpret_cast<V*>(s->vals[0]);
             return SFK_OK;
         }
-        // A bucket too large for one wavefront: the plain passes below.
+        // A bucket too large for one wavefront (the bucket sort left it
+        // alone): the plain passes below, from the input again.
     }
     const K* cur_k = keys;
     const V* cur_v = vals;

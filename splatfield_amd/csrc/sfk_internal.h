@@ -150,12 +150,15 @@ bool depth_rebase(const sfk_view& v, const float bmin[3], const float bmax[3], u
 hipError_t launch_radix_pass(hipStream_t s, const SortArgs& a, uint32_t key_bytes,
                              uint32_t val_bytes, uint32_t digit_bits);
 // Bucket finish of u64 sorts (sfk_sort.hip): keys grouped by their 16 bits
-// at `shift` (two stable passes) -> first[65536] / size[65536] of every
-// bucket and the largest size (*max_size, device); then every bucket of at
-// most bucket_cap() keys sorted by its bits below `shift` (lbits of them).
+// at `shift` (two stable passes) -> first / last / size[65536] of every
+// bucket and the largest size per group of 256 buckets
+// (wg_max[bucket_size_groups()], device); then every bucket of at most
+// bucket_cap() keys sorted by its bits below `shift` (lbits of them; larger
+// buckets are left alone).
 hipError_t launch_bucket_split(hipStream_t s, const uint64_t* grouped_keys, uint32_t n,
-                               uint32_t shift, uint32_t* first, uint32_t* size,
-                               uint32_t* max_size);
+                               uint32_t shift, uint32_t* first, uint32_t* last, uint32_t* size,
+                               uint32_t* wg_max);
+uint32_t bucket_size_groups();
 uint32_t bucket_cap();
 hipError_t launch_bucket_sort(hipStream_t s, const uint64_t* keys_in, const uint32_t* vals_in,
                               uint64_t* keys_out, uint32_t* vals_out, const uint32_t* first,

@@ -407,65 +407,44 @@ hipError_t pass_impl(hipStream_t st, const SortArgs& a) {
 // keys and 16-bit slots through LDS, LSD passes of at most 8 bits ranked by
 // 64-lane ballot match, values gathered once by slot.
 
-// first[t] = index of the first key of bucket t in the grouped keys (set
-// only for nonempty buckets; the caller fills first with 0xffffffff).
-__global__ __launch_bounds__(256) void k_bucket_first(const uint64_t* __restrict__ keys, uint32_t n,
-                                                      uint32_t shift, uint32_t* __restrict__ first) {
+// first[t] / last[t] = indices of the first and last key of bucket t in the
+// grouped keys (written only for nonempty buckets, each by the one thread
+// at that boundary; the caller fills first with 0xffffffff).
+// One key per lane; the neighbours' buckets come from the adjacent lanes,
+// and only a wavefront's two edge lanes load a second key.
+__global__ __launch_bounds__(256) void k_bucket_bounds(const uint64_t* __restrict__ keys, uint32_t n,
+                                                       uint32_t shift, uint32_t* __restrict__ first,
+                                                       uint32_t* __restrict__ last) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t lane = lane_id();
+    const uint32_t t = i < n ? (uint32_t)(keys[i] >> shift) & 0xffffu : 0xffffffffu;
+    uint32_t tp = (uint32_t)__shfl_up((int)t, 1, 64);
+    uint32_t tn = (uint32_t)__shfl_down((int)t, 1, 64);
+    if (lane == 0 && i > 0 && i - 1 < n) tp = (uint32_t)(keys[i - 1] >> shift) & 0xffffu;
+    if (lane == 63 && i + 1 < n) tn = (uint32_t)(keys[i + 1] >> shift) & 0xffffu;
     if (i >= n) return;
-    const uint32_t t = (uint32_t)(keys[i] >> shift) & 0xffffu;
-    if (i == 0 || ((uint32_t)(keys[i - 1] >> shift) & 0xffffu) != t) first[t] = i;
+    if (i == 0 || tp != t) first[t] = i;
+    if (i + 1 == n || tn != t) last[t] = i;
 }
 
-// size[t] of every bucket from first[] (empty buckets 0), and the largest
-// into *max_size: one workgroup, each thread a run of 64 buckets; the next
-// nonempty start after a run comes from a suffix minimum over the runs.
-constexpr uint32_t kBucketSizeThreads = 1024;
-__global__ __launch_bounds__(kBucketSizeThreads) void k_bucket_sizes(const uint32_t* __restrict__ first,
-                                                                     uint32_t n,
-                                                                     uint32_t* __restrict__ size,
-                                                                     uint32_t* __restrict__ max_size) {
-    constexpr uint32_t R = 65536 / kBucketSizeThreads;  // buckets per thread
-    __shared__ uint32_t s_min[kBucketSizeThreads + 1];
-    __shared__ uint32_t s_max[kBucketSizeThreads / 64];
-    const uint32_t t = threadIdx.x, t0 = t * R;
-    uint32_t f[R];
-    uint32_t mn = 0xffffffffu;
-#pragma unroll
-    for (uint32_t r = 0; r < R; ++r) {
-        f[r] = first[t0 + r];
-        mn = min(mn, f[r]);
-    }
-    s_min[t] = mn;
-    if (t == 0) s_min[kBucketSizeThreads] = n;  // past the last bucket: the end
-    __syncthreads();
-    // Suffix minimum over the runs after this one (a serial scan by one
-    // wave would do; log steps over LDS keep it short).
-    for (uint32_t o = 1; o < kBucketSizeThreads; o <<= 1) {
-        const uint32_t v = t + o <= kBucketSizeThreads ? s_min[t + o] : 0xffffffffu;
-        __syncthreads();
-        s_min[t] = min(s_min[t], v);
-        __syncthreads();
-    }
-    uint32_t next = t + 1 < kBucketSizeThreads ? s_min[t + 1] : n;  // next nonempty start after the run
-    next = min(next, n);
-    uint32_t mx = 0;
-#pragma unroll
-    for (int r = (int)R - 1; r >= 0; --r) {
-        const uint32_t sz = f[r] == 0xffffffffu ? 0u : next - f[r];
-        size[t0 + r] = sz;
-        mx = max(mx, sz);
-        if (f[r] != 0xffffffffu) next = f[r];
-    }
+// size[t] of every bucket (empty: 0) and, per workgroup of 256 buckets, the
+// largest (wg_max[blockIdx.x]; the host takes the maximum of the 256).
+constexpr uint32_t kBucketSizeGroups = 65536 / 256;
+__global__ __launch_bounds__(256) void k_bucket_sizes(const uint32_t* __restrict__ first,
+                                                      const uint32_t* __restrict__ last,
+                                                      uint32_t* __restrict__ size,
+                                                      uint32_t* __restrict__ wg_max) {
+    __shared__ uint32_t s_max[4];
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t f = first[t];
+    const uint32_t sz = f == 0xffffffffu ? 0u : last[t] - f + 1u;
+    size[t] = sz;
+    uint32_t mx = sz;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o, 64));
-    if (lane_id() == 0) s_max[t / 64] = mx;
+    if (lane_id() == 0) s_max[threadIdx.x / 64] = mx;
     __syncthreads();
-    if (t == 0) {
-        uint32_t m = 0;
-        for (uint32_t k = 0; k < kBucketSizeThreads / 64; ++k) m = max(m, s_max[k]);
-        *max_size = m;
-    }
+    if (threadIdx.x == 0) wg_max[blockIdx.x] = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
 }
 
 // One wavefront per bucket (four per workgroup): the bucket's keys (u64,
@@ -668,14 +647,16 @@ __global__ __launch_bounds__(256) void k_bucket_sort(BucketArgs a) {
 }  // namespace
 
 hipError_t launch_bucket_split(hipStream_t st, const uint64_t* grouped_keys, uint32_t n,
-                               uint32_t shift, uint32_t* first, uint32_t* size,
-                               uint32_t* max_size) {
+                               uint32_t shift, uint32_t* first, uint32_t* last, uint32_t* size,
+                               uint32_t* wg_max) {
     hipError_t e = launch_fill_u32(st, first, 0xffffffffu, 65536);
     if (e != hipSuccess) return e;
-    k_bucket_first<<<div_up(n, 256), 256, 0, st>>>(grouped_keys, n, shift, first);
-    k_bucket_sizes<<<1, kBucketSizeThreads, 0, st>>>(first, n, size, max_size);
+    k_bucket_bounds<<<div_up(n, 256), 256, 0, st>>>(grouped_keys, n, shift, first, last);
+    k_bucket_sizes<<<kBucketSizeGroups, 256, 0, st>>>(first, last, size, wg_max);
     return hipGetLastError();
 }
+
+uint32_t bucket_size_groups() { return kBucketSizeGroups; }
 
 uint32_t bucket_cap() { return kBucketCap; }
 
