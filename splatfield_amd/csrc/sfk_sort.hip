@@ -410,21 +410,43 @@ hipError_t pass_impl(hipStream_t st, const SortArgs& a) {
 // first[t] / last[t] = indices of the first and last key of bucket t in the
 // grouped keys (written only for nonempty buckets, each by the one thread
 // at that boundary; the caller fills first with 0xffffffff).
-// One key per lane; the neighbours' buckets come from the adjacent lanes,
-// and only a wavefront's two edge lanes load a second key.
+// A wavefront takes 64 * kBoundsKpt consecutive keys, step j lane l key
+// 64j + l, and also loads the 64 keys on either side of its range (steps
+// -1 and kBoundsKpt; neighbouring wavefronts read them too, so they mostly
+// hit in L2).  Every load is issued before any compare (indices clamped, no
+// branch around a load).  A key's neighbours come from the adjacent lanes,
+// across steps from lane 63 of the step before / lane 0 of the step after.
+constexpr uint32_t kBoundsKpt = 16;
 __global__ __launch_bounds__(256) void k_bucket_bounds(const uint64_t* __restrict__ keys, uint32_t n,
                                                        uint32_t shift, uint32_t* __restrict__ first,
                                                        uint32_t* __restrict__ last) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    constexpr uint32_t kSteps = kBoundsKpt + 2;
     const uint32_t lane = lane_id();
-    const uint32_t t = i < n ? (uint32_t)(keys[i] >> shift) & 0xffffu : 0xffffffffu;
-    uint32_t tp = (uint32_t)__shfl_up((int)t, 1, 64);
-    uint32_t tn = (uint32_t)__shfl_down((int)t, 1, 64);
-    if (lane == 0 && i > 0 && i - 1 < n) tp = (uint32_t)(keys[i - 1] >> shift) & 0xffffu;
-    if (lane == 63 && i + 1 < n) tn = (uint32_t)(keys[i + 1] >> shift) & 0xffffu;
-    if (i >= n) return;
-    if (i == 0 || tp != t) first[t] = i;
-    if (i + 1 == n || tn != t) last[t] = i;
+    const int64_t wbase = ((int64_t)blockIdx.x * 4 + threadIdx.x / 64) * (64 * kBoundsKpt);
+    if (wbase >= (int64_t)n) return;
+    const int64_t top = (int64_t)n - 1;
+    constexpr uint32_t kNone = 0xffffffffu;
+    uint32_t t[kSteps];  // t[j + 1] = bucket of key wbase + 64j + lane
+#pragma unroll
+    for (uint32_t s = 0; s < kSteps; ++s) {
+        const int64_t i = wbase + 64 * ((int64_t)s - 1) + lane;
+        const uint64_t k = keys[i < 0 ? 0 : (i < top ? i : top)];
+        t[s] = i >= 0 && i <= top ? (uint32_t)(k >> shift) & 0xffffu : kNone;
+    }
+#pragma unroll
+    for (uint32_t s = 1; s <= kBoundsKpt; ++s) {
+        uint32_t tp = (uint32_t)__shfl_up((int)t[s], 1, 64);
+        uint32_t tn = (uint32_t)__shfl_down((int)t[s], 1, 64);
+        const uint32_t before = (uint32_t)__builtin_amdgcn_readlane((int)t[s - 1], 63);
+        const uint32_t after = (uint32_t)__builtin_amdgcn_readlane((int)t[s + 1], 0);
+        if (lane == 0) tp = before;
+        if (lane == 63) tn = after;
+        const int64_t i = wbase + 64 * ((int64_t)s - 1) + lane;
+        if (i <= top) {
+            if (tp != t[s]) first[t[s]] = (uint32_t)i;
+            if (tn != t[s]) last[t[s]] = (uint32_t)i;
+        }
+    }
 }
 
 // size[t] of every bucket (empty: 0) and, per workgroup of 256 buckets, the
@@ -451,14 +473,17 @@ __global__ __launch_bounds__(256) void k_bucket_sizes(const uint32_t* __restrict
 // at most kBucketCap) sorted stably by bits [0, lbits); keys_in / vals_in the
 // grouped arrays, keys_out / vals_out the result (keys_out may be null).
 //  * lbits <= 16: LSD passes of width[p] bits over all of them.
-//  * lbits > 16: two passes on the top 16 of them leave the bucket sorted
-//    by those, equal 16-bit prefixes in input order; a run of equal
-//    prefixes is then almost always one or two keys (a bucket of ~256
-//    random keys shares a 16-bit prefix pair about once), and a pair is put
-//    in order by one compare and swap on the remaining bits (equal keys
-//    keep their order).  A bucket with a run of three or more (ties,
-//    clustered keys) is sorted again from its input order by the full LSD
-//    passes instead.
+//  * lbits > 16: rank by the top kPfxBits of them through a presence
+//    bitmap in LDS (one bit per prefix value): a key's rank is the number of
+//    set bits below its prefix, a popcount over its 128-bit group plus the
+//    group's prefix count.  When every prefix is distinct (about a third of
+//    random buckets of ~256 keys), those ranks are the final places.  Else a
+//    count per distinct prefix gives each of its keys a place (in arbitrary
+//    order within the prefix); when no prefix is shared by more than two
+//    keys, one compare and swap per shared pair on (remaining bits, input
+//    position) restores the stable order.  A bucket where three or more
+//    keys share a prefix (ties, clustered keys) is sorted from its input
+//    order by the full LSD passes instead.
 struct BucketArgs {
     const uint64_t* keys_in;
     const uint32_t* vals_in;
@@ -548,97 +573,202 @@ __device__ __forceinline__ void bucket_pass(uint64_t (&key)[kBucketEpt], uint32_
     __builtin_amdgcn_wave_barrier();
 }
 
+// Wave-synchronous LDS hand-off: the wave's earlier LDS accesses are seen
+// by its later ones (LDS serves one wave's instructions in order; the fences
+// keep the compiler from moving accesses across).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr uint32_t kPfxBits = 15;
+constexpr uint32_t kMapWords = (1u << kPfxBits) / 32;  // presence bitmap
+constexpr uint32_t kMapGroups = kMapWords / 4;         // 128-bit groups
+static_assert(kMapGroups == 4 * 64, "four bitmap groups per lane");
+// Per-wave LDS region, used twice:
+//   ranking: bitmap [0, kMapWords*4), group prefix counts (u16) after it;
+//   placing: prefix-class counts (u32) [0, kCntBytes), sorted keys (u64) at
+//            kKeyOff, sorted input positions (u16) at kIdxOff; the full
+//            passes' digit counts (256 u32) reuse [0, 1024).
+constexpr uint32_t kCntBytes = kBucketCap * 4;
+constexpr uint32_t kKeyOff = kCntBytes;
+constexpr uint32_t kIdxOff = kKeyOff + kBucketCap * 8;
+constexpr uint32_t kRegionBytes =
+    std::max(kIdxOff + kBucketCap * 2, kMapWords * 4 + kMapGroups * 2);
+static_assert(kRegionBytes % 16 == 0 && kCntBytes >= 1024, "region layout");
+
 __global__ __launch_bounds__(256) void k_bucket_sort(BucketArgs a) {
-    __shared__ uint64_t s_key[4][kBucketCap];
-    __shared__ uint16_t s_idx[4][kBucketCap];
-    __shared__ uint32_t s_cnt[4][256];
+    __shared__ __align__(16) unsigned char s_mem[4][kRegionBytes];
     const uint32_t w = threadIdx.x / 64, lane = lane_id();
     const uint32_t b = blockIdx.x * 4u + w;
     const uint32_t m = a.size[b];
     if (m == 0 || m > kBucketCap) return;  // (sizes above the cap never reach here)
+    unsigned char* reg = s_mem[w];
+    uint32_t* map = reinterpret_cast<uint32_t*>(reg);
+    uint16_t* gpre = reinterpret_cast<uint16_t*>(reg + kMapWords * 4);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(reg);
+    uint64_t* s_key = reinterpret_cast<uint64_t*>(reg + kKeyOff);
+    uint16_t* s_idx = reinterpret_cast<uint16_t*>(reg + kIdxOff);
     const size_t start = a.first[b];
     const uint32_t ept = (m + 63) / 64;  // wave-uniform, 1..kBucketEpt
     uint64_t key[kBucketEpt];
-    uint32_t pos[kBucketEpt];
     bool valid[kBucketEpt];
-    auto load = [&]() {
 #pragma unroll
-        for (uint32_t e = 0; e < kBucketEpt; ++e) {
-            const uint32_t i = e * 64 + lane;
-            valid[e] = e < ept && i < m;
-            key[e] = valid[e] ? a.keys_in[start + i] : 0ull;
-            pos[e] = i;
-        }
-    };
-    load();
+    for (uint32_t e = 0; e < kBucketEpt; ++e) {
+        const uint32_t i = e * 64 + lane;
+        valid[e] = e < ept && i < m;
+        key[e] = valid[e] ? a.keys_in[start + i] : 0ull;
+    }
     bool need_full = a.lbits <= 16;
     if (!need_full) {
-        const uint32_t hs = a.lbits - 16;
-        for (uint32_t p = 0; p < 2; ++p)
-            bucket_pass(key, pos, valid, ept, hs + 8 * p, 8, s_cnt[w], s_key[w], s_idx[w]);
-        // Runs of equal 16-bit prefixes (s_key holds the sorted keys).
-        const uint64_t low = (1ull << hs) - 1ull;
-        uint32_t pair_at = 0;  // bit e: item e starts an out-of-order pair
-        bool long_run = false;
+        const uint32_t hs = a.lbits - kPfxBits;
+        uint32_t pf[kBucketEpt], q[kBucketEpt], sub[kBucketEpt];
+        // Presence bitmap of the prefixes.
+#pragma unroll
+        for (uint32_t j = 0; j < kMapWords / 256; ++j)
+            reinterpret_cast<uint4*>(map)[lane + 64 * j] = make_uint4(0u, 0u, 0u, 0u);
+        wave_lds_sync();
+#pragma unroll
+        for (uint32_t e = 0; e < kBucketEpt; ++e) {
+            pf[e] = (uint32_t)(key[e] >> hs) & ((1u << kPfxBits) - 1u);
+            if (valid[e]) atomicOr(&map[pf[e] >> 5], 1u << (pf[e] & 31u));
+        }
+        wave_lds_sync();
+        // Set bits per 128-bit group (lane l: groups l, l+64, l+128, l+192),
+        // then the exclusive count of set bits before every group; two groups
+        // per 16-bit field of one scan (counts stay below 2^16).
+        uint32_t c[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint4 v = reinterpret_cast<const uint4*>(map)[lane + 64 * j];
+            c[j] = __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+        }
+        const uint32_t x0 = c[0] | (c[1] << 16), x1 = c[2] | (c[3] << 16);
+        const uint32_t i0 = wave_incl_scan_u32(x0), i1 = wave_incl_scan_u32(x1);
+        const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)i0, 63);
+        const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)i1, 63);
+        const uint32_t r0 = t0 & 0xffffu, r1 = t0 >> 16, r2 = t1 & 0xffffu, r3 = t1 >> 16;
+        const uint32_t e0 = i0 - x0, e1 = i1 - x1;
+        gpre[lane] = (uint16_t)(e0 & 0xffffu);
+        gpre[lane + 64] = (uint16_t)(r0 + (e0 >> 16));
+        gpre[lane + 128] = (uint16_t)(r0 + r1 + (e1 & 0xffffu));
+        gpre[lane + 192] = (uint16_t)(r0 + r1 + r2 + (e1 >> 16));
+        const uint32_t distinct = r0 + r1 + r2 + r3;
+        wave_lds_sync();
+        // Rank of each key's prefix among the distinct prefixes.
 #pragma unroll
         for (uint32_t e = 0; e < kBucketEpt; ++e) {
             if (e >= ept) break;
-            const uint32_t i = e * 64 + lane;
-            if (!valid[e]) continue;
-            const uint32_t pfx = (uint32_t)(key[e] >> hs) & 0xffffu;
-            const bool tie_next = i + 1 < m && ((uint32_t)(s_key[w][i + 1] >> hs) & 0xffffu) == pfx;
-            const bool tie_prev = i > 0 && ((uint32_t)(s_key[w][i - 1] >> hs) & 0xffffu) == pfx;
-            long_run |= tie_next && tie_prev;
-            if (tie_next && (key[e] & low) > (s_key[w][i + 1] & low)) pair_at |= 1u << e;
+            const uint32_t g = pf[e] >> 7, sb = pf[e] & 127u, k = sb >> 5;
+            const uint4 v = reinterpret_cast<const uint4*>(map)[g];
+            const uint32_t wk = k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+            uint32_t r = gpre[g] + __popc(wk & ((1u << (sb & 31u)) - 1u));
+            r += k > 0 ? __popc(v.x) : 0u;
+            r += k > 1 ? __popc(v.y) : 0u;
+            r += k > 2 ? __popc(v.z) : 0u;
+            q[e] = r;
+            sub[e] = 0;
         }
-        if (__ballot(long_run)) {
-            need_full = true;
-            load();
-        } else {
-            // Disjoint pairs: swap each out-of-order one (both halves
-            // written by the lane holding its first item).
-            __builtin_amdgcn_wave_barrier();
+        wave_lds_sync();
+        const bool shared = distinct != m;  // some prefix held by two or more keys
+        if (shared) {
+            for (uint32_t i = lane; i < distinct; i += 64) cnt[i] = 0u;
+            wave_lds_sync();
+            bool many = false;
 #pragma unroll
             for (uint32_t e = 0; e < kBucketEpt; ++e) {
-                if ((pair_at >> e) & 1u) {
-                    const uint32_t i = e * 64 + lane;
-                    const uint64_t k1 = s_key[w][i + 1];
-                    const uint16_t p1 = s_idx[w][i + 1];
-                    s_key[w][i + 1] = key[e];
-                    s_idx[w][i + 1] = (uint16_t)pos[e];
-                    s_key[w][i] = k1;
-                    s_idx[w][i] = p1;
+                if (valid[e]) {
+                    sub[e] = atomicAdd(&cnt[q[e]], 1u);
+                    many |= sub[e] >= 2;
                 }
             }
-            __builtin_amdgcn_wave_barrier();
+            wave_lds_sync();
+            if (__ballot(many)) {
+                need_full = true;
+            } else {
+                // Exclusive scan of the class counts: lane l, classes 6l .. 6l+5.
+                constexpr uint32_t kPer = kBucketCap / 64;
+                uint32_t cc[kPer], sum = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < kPer; ++k) {
+                    const uint32_t i = kPer * lane + k;
+                    cc[k] = i < distinct ? cnt[i] : 0u;
+                    sum += cc[k];
+                }
+                uint32_t run = wave_incl_scan_u32(sum) - sum;
+                wave_lds_sync();
+#pragma unroll
+                for (uint32_t k = 0; k < kPer; ++k) {
+                    const uint32_t i = kPer * lane + k;
+                    if (i < distinct) cnt[i] = run;
+                    run += cc[k];
+                }
+                wave_lds_sync();
+#pragma unroll
+                for (uint32_t e = 0; e < kBucketEpt; ++e)
+                    if (valid[e]) q[e] = cnt[q[e]] + sub[e];
+            }
+        }
+        if (!need_full) {
 #pragma unroll
             for (uint32_t e = 0; e < kBucketEpt; ++e) {
-                if (e >= ept) break;
-                const uint32_t i = e * 64 + lane;
                 if (valid[e]) {
-                    key[e] = s_key[w][i];
-                    pos[e] = s_idx[w][i];
+                    s_key[q[e]] = key[e];
+                    s_idx[q[e]] = (uint16_t)(e * 64 + lane);
                 }
+            }
+            wave_lds_sync();
+            if (shared) {
+                // The second key of a shared prefix (sub 1) sits right after
+                // the first: order the pair by (remaining bits, position).
+                const uint64_t low = (1ull << hs) - 1ull;
+#pragma unroll
+                for (uint32_t e = 0; e < kBucketEpt; ++e) {
+                    if (valid[e] && sub[e] == 1u) {
+                        const uint32_t qa = q[e] - 1u, pos = e * 64 + lane;
+                        const uint64_t ka = s_key[qa];
+                        const uint32_t pa = s_idx[qa];
+                        const uint64_t lm = key[e] & low, la = ka & low;
+                        if (lm < la || (lm == la && pos < pa)) {
+                            s_key[qa] = key[e];
+                            s_idx[qa] = (uint16_t)pos;
+                            s_key[q[e]] = ka;
+                            s_idx[q[e]] = (uint16_t)pa;
+                        }
+                    }
+                }
+                wave_lds_sync();
             }
         }
     }
     if (need_full) {
+        uint32_t pos[kBucketEpt];
+#pragma unroll
+        for (uint32_t e = 0; e < kBucketEpt; ++e) pos[e] = e * 64 + lane;
         uint32_t sh = 0;
         for (uint32_t p = 0; p < a.passes; ++p) {
-            bucket_pass(key, pos, valid, ept, sh, a.width[p], s_cnt[w], s_key[w], s_idx[w]);
+            bucket_pass(key, pos, valid, ept, sh, a.width[p], cnt, s_key, s_idx);
             sh += a.width[p];
         }
+        // (bucket_pass leaves the sorted keys and positions in s_key / s_idx.)
     }
     // Every gather issued before any store.
+    uint64_t ko[kBucketEpt];
     uint32_t v[kBucketEpt];
 #pragma unroll
-    for (uint32_t e = 0; e < kBucketEpt; ++e)
-        if (valid[e]) v[e] = a.vals_in[start + pos[e]];
+    for (uint32_t e = 0; e < kBucketEpt; ++e) {
+        if (valid[e]) {
+            const uint32_t i = e * 64 + lane;
+            ko[e] = s_key[i];
+            v[e] = a.vals_in[start + s_idx[i]];
+        }
+    }
 #pragma unroll
     for (uint32_t e = 0; e < kBucketEpt; ++e) {
         if (valid[e]) {
             const size_t o = start + e * 64 + lane;
-            if (a.keys_out) a.keys_out[o] = key[e];
+            if (a.keys_out) a.keys_out[o] = ko[e];
             a.vals_out[o] = v[e];
         }
     }
@@ -651,7 +781,7 @@ hipError_t launch_bucket_split(hipStream_t st, const uint64_t* grouped_keys, uin
                                uint32_t* wg_max) {
     hipError_t e = launch_fill_u32(st, first, 0xffffffffu, 65536);
     if (e != hipSuccess) return e;
-    k_bucket_bounds<<<div_up(n, 256), 256, 0, st>>>(grouped_keys, n, shift, first, last);
+    k_bucket_bounds<<<div_up(n, 256u * kBoundsKpt), 256, 0, st>>>(grouped_keys, n, shift, first, last);
     k_bucket_sizes<<<kBucketSizeGroups, 256, 0, st>>>(first, last, size, wg_max);
     return hipGetLastError();
 }
