@@ -207,15 +207,20 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t row, const Transpo
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Inclusive prefix sum across the 64 lanes of a wave.
+// Inclusive prefix sum across the 64 lanes of a wave with DPP moves: a
+// Hillis-Steele scan inside each 16-lane row (row_shr 1, 2, 4, 8; lanes
+// shifted in from outside the row read 0), then row_bcast15 adds row 0's
+// total into row 1 and row 2's into row 3, and row_bcast31 adds rows 0-1's
+// total into rows 2 and 3.
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
-    uint32_t lane = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o, 64);
-        if (lane >= (uint32_t)o) v += t;
-    }
-    return v;
+    int x = (int)v;
+    x += SFK_DPP_I(x, 0x111, 0xf);  // row_shr:1
+    x += SFK_DPP_I(x, 0x112, 0xf);  // row_shr:2
+    x += SFK_DPP_I(x, 0x114, 0xf);  // row_shr:4
+    x += SFK_DPP_I(x, 0x118, 0xf);  // row_shr:8
+    x += SFK_DPP_I(x, 0x142, 0xa);  // row_bcast15 -> rows 1, 3
+    x += SFK_DPP_I(x, 0x143, 0xc);  // row_bcast31 -> rows 2, 3
+    return (uint32_t)x;
 }
 
 // Exclusive prefix sum over a workgroup of NT threads (NT a multiple of 64,

@@ -588,7 +588,8 @@ constexpr uint32_t kMapGroups = kMapWords / 4;         // 128-bit groups
 static_assert(kMapGroups == 4 * 64, "four bitmap groups per lane");
 // Per-wave LDS region, used twice:
 //   ranking: bitmap [0, kMapWords*4), group prefix counts (u16) after it;
-//   placing: prefix-class counts (u32) [0, kCntBytes), sorted keys (u64) at
+//   placing: prefix-class counts (u32) [0, kCntBytes), then the values in
+//            sorted order (u32) in the same bytes; sorted keys (u64) at
 //            kKeyOff, sorted input positions (u16) at kIdxOff; the full
 //            passes' digit counts (256 u32) reuse [0, 1024).
 constexpr uint32_t kCntBytes = kBucketCap * 4;
@@ -608,17 +609,20 @@ __global__ __launch_bounds__(256) void k_bucket_sort(BucketArgs a) {
     uint32_t* map = reinterpret_cast<uint32_t*>(reg);
     uint16_t* gpre = reinterpret_cast<uint16_t*>(reg + kMapWords * 4);
     uint32_t* cnt = reinterpret_cast<uint32_t*>(reg);
+    uint32_t* s_val = reinterpret_cast<uint32_t*>(reg);
     uint64_t* s_key = reinterpret_cast<uint64_t*>(reg + kKeyOff);
     uint16_t* s_idx = reinterpret_cast<uint16_t*>(reg + kIdxOff);
     const size_t start = a.first[b];
     const uint32_t ept = (m + 63) / 64;  // wave-uniform, 1..kBucketEpt
     uint64_t key[kBucketEpt];
+    uint32_t val[kBucketEpt];
     bool valid[kBucketEpt];
 #pragma unroll
     for (uint32_t e = 0; e < kBucketEpt; ++e) {
         const uint32_t i = e * 64 + lane;
         valid[e] = e < ept && i < m;
         key[e] = valid[e] ? a.keys_in[start + i] : 0ull;
+        val[e] = valid[e] ? a.vals_in[start + i] : 0u;
     }
     bool need_full = a.lbits <= 16;
     if (!need_full) {
@@ -711,11 +715,13 @@ __global__ __launch_bounds__(256) void k_bucket_sort(BucketArgs a) {
             }
         }
         if (!need_full) {
+            wave_lds_sync();  // (the class counts are dead: s_val takes their bytes)
 #pragma unroll
             for (uint32_t e = 0; e < kBucketEpt; ++e) {
                 if (valid[e]) {
                     s_key[q[e]] = key[e];
                     s_idx[q[e]] = (uint16_t)(e * 64 + lane);
+                    s_val[q[e]] = val[e];
                 }
             }
             wave_lds_sync();
@@ -731,10 +737,13 @@ __global__ __launch_bounds__(256) void k_bucket_sort(BucketArgs a) {
                         const uint32_t pa = s_idx[qa];
                         const uint64_t lm = key[e] & low, la = ka & low;
                         if (lm < la || (lm == la && pos < pa)) {
+                            const uint32_t va = s_val[qa];
                             s_key[qa] = key[e];
                             s_idx[qa] = (uint16_t)pos;
+                            s_val[qa] = val[e];
                             s_key[q[e]] = ka;
                             s_idx[q[e]] = (uint16_t)pa;
+                            s_val[q[e]] = va;
                         }
                     }
                 }
@@ -753,7 +762,8 @@ __global__ __launch_bounds__(256) void k_bucket_sort(BucketArgs a) {
         }
         // (bucket_pass leaves the sorted keys and positions in s_key / s_idx.)
     }
-    // Every gather issued before any store.
+    // The full passes carry positions only: their values are gathered (all
+    // gathers issued before any store).
     uint64_t ko[kBucketEpt];
     uint32_t v[kBucketEpt];
 #pragma unroll
@@ -761,7 +771,7 @@ __global__ __launch_bounds__(256) void k_bucket_sort(BucketArgs a) {
         if (valid[e]) {
             const uint32_t i = e * 64 + lane;
             ko[e] = s_key[i];
-            v[e] = a.vals_in[start + s_idx[i]];
+            v[e] = need_full ? a.vals_in[start + s_idx[i]] : s_val[i];
         }
     }
 #pragma unroll
