@@ -469,14 +469,7 @@ __device__ __forceinline__ uint32_t work_class(uint32_t c) {
 // Lanes of the wave holding the same work class as this lane (7 ballots),
 // restricted to valid lanes.
 __device__ __forceinline__ uint64_t class_peers(uint32_t k, bool valid) {
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (uint32_t bit = 0; bit < 7; ++bit) {
-        const bool set = (k >> bit) & 1u;
-        const uint64_t bal = __ballot(set);
-        peers &= set ? bal : ~bal;
-    }
-    return peers;
+    return match_peers<7>(k, valid);
 }
 
 // The rasterizers' launch order: the n tiles of `starts` (exclusive

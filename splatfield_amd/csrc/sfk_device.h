@@ -207,6 +207,25 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t row, const Transpo
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Lanes of the wave (of those with `live` set) whose digit d agrees with this
+// lane's in bits [0, W) (match-any): one ballot per bit.  With the lane's bit
+// spread to all-ones or all-zeros m, each half of the mask updates as
+// peers &= ~(ballot ^ m), which gfx950 does in one v_bitop3.  Digit bits at
+// or above a digit's real width are zero in every lane and match trivially.
+template <uint32_t W>
+__device__ __forceinline__ uint64_t match_peers(uint32_t d, bool live) {
+    const uint64_t l = __ballot(live);
+    uint32_t lo = (uint32_t)l, hi = (uint32_t)(l >> 32);
+#pragma unroll
+    for (uint32_t b = 0; b < W; ++b) {
+        const uint32_t m = (uint32_t)((int32_t)(d << (31u - b)) >> 31);
+        const uint64_t bal = __ballot(m != 0u);
+        lo &= ~((uint32_t)bal ^ m);
+        hi &= ~((uint32_t)(bal >> 32) ^ m);
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Inclusive prefix sum across the 64 lanes of a wave with DPP moves: a
 // Hillis-Steele scan inside each 16-lane row (row_shr 1, 2, 4, 8; lanes
 // shifted in from outside the row read 0), then row_bcast15 adds row 0's

@@ -229,8 +229,16 @@ __device__ __forceinline__ uint2 identity_val<uint2>(uint32_t li, const uint32_t
 template <typename K, typename V, int RADIX, uint32_t EPT>
 __device__ __forceinline__ void scatter_block(const SortArgs& a, const SegPos& p) {
     constexpr uint32_t KPB = NT * EPT;
-    __shared__ K s_keys[KPB];
-    __shared__ V s_vals[KPB];
+    // The digit width is log2(RADIX) (RADIX = 16 also serves widths below 4:
+    // the digit's upper bits are then zero in every lane).
+    constexpr uint32_t kDigitBits = RADIX >= 256 ? 8 : RADIX >= 128 ? 7 : RADIX >= 64 ? 6 : RADIX >= 32 ? 5 : 4;
+    // Keys, then values, staged in sorted order through the same bytes: a
+    // block holds the larger of the two instead of both (more workgroups per
+    // CU for 4096-key blocks of u64 keys or 8-byte values).
+    constexpr size_t kStage = KPB * (sizeof(K) > sizeof(V) ? sizeof(K) : sizeof(V));
+    __shared__ __align__(16) unsigned char s_stage[kStage];
+    K* s_keys = reinterpret_cast<K*>(s_stage);
+    V* s_vals = reinterpret_cast<V*>(s_stage);
     __shared__ uint32_t s_wh[NW][RADIX];  // per-wave digit counters, then bases
     __shared__ uint32_t s_gofs[RADIX];    // global position minus local position
     __shared__ uint32_t s_tmp[NW + 1];
@@ -297,15 +305,7 @@ __device__ __forceinline__ void scatter_block(const SortArgs& a, const SegPos& p
     for (uint32_t e = 0; e < EPT; ++e) {
         bool valid = wbase + e * 64 < p.seg_n;
         uint32_t d = digit_of(key[e], shift, mask);
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (uint32_t bit = 0; bit < 8; ++bit) {
-            if (bit < width) {
-                bool set = (d >> bit) & 1u;
-                uint64_t bal = __ballot(set);
-                peers &= set ? bal : ~bal;
-            }
-        }
+        const uint64_t peers = match_peers<kDigitBits>(d, valid);
         uint32_t rank = mask_rank(peers);
         uint32_t base = 0;
         if (valid) base = s_wh[w][d];
@@ -336,27 +336,37 @@ __device__ __forceinline__ void scatter_block(const SortArgs& a, const SegPos& p
     }
     __syncthreads();
 
+    // loc[e] becomes the key's place in the block's sorted order.
 #pragma unroll
     for (uint32_t e = 0; e < EPT; ++e) {
         if (wbase + e * 64 < p.seg_n) {
             uint32_t d = digit_of(key[e], shift, mask);
-            uint32_t pos = s_wh[w][d] + loc[e];
-            s_keys[pos] = key[e];
-            s_vals[pos] = val[e];
+            loc[e] += s_wh[w][d];
+            s_keys[loc[e]] = key[e];
         }
     }
     __syncthreads();
 
+    uint32_t dst[EPT];  // segment-global position of sorted slot t + e * NT
 #pragma unroll
     for (uint32_t e = 0; e < EPT; ++e) {
         uint32_t s = t + e * NT;
         if (s < live) {
             K k = s_keys[s];
             uint32_t d = digit_of(k, shift, mask);
-            size_t dst = (size_t)(s_gofs[d] + s);
-            if (keys_out) keys_out[dst] = k;
-            vals_out[dst] = s_vals[s];
+            dst[e] = s_gofs[d] + s;
+            if (keys_out) keys_out[dst[e]] = k;
         }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t e = 0; e < EPT; ++e)
+        if (wbase + e * 64 < p.seg_n) s_vals[loc[e]] = val[e];
+    __syncthreads();
+#pragma unroll
+    for (uint32_t e = 0; e < EPT; ++e) {
+        uint32_t s = t + e * NT;
+        if (s < live) vals_out[dst[e]] = s_vals[s];
     }
 }
 
@@ -518,15 +528,7 @@ __device__ __forceinline__ void bucket_pass(uint64_t (&key)[kBucketEpt], uint32_
         if (e >= ept) break;
         const uint32_t d = (uint32_t)(key[e] >> sh) & mask;
         dg[e] = d;
-        uint64_t peers = __ballot(valid[e]);
-#pragma unroll
-        for (uint32_t bit = 0; bit < 8; ++bit) {
-            if (bit < width) {
-                const bool set = (d >> bit) & 1u;
-                const uint64_t bal = __ballot(set);
-                peers &= set ? bal : ~bal;
-            }
-        }
+        const uint64_t peers = match_peers<8>(d, valid[e]);
         const uint32_t rank = mask_rank(peers);
         uint32_t base = 0;
         if (valid[e]) base = cnt[d];
