@@ -171,6 +171,9 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
     uint32_t wave_last = last;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) wave_last = max(wave_last, (uint32_t)__shfl_xor(wave_last, o, 64));
+    // Equal in every lane: made uniform, the list length and the member loop
+    // below are scalar (no per-lane loop control or list addressing).
+    wave_last = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_last);
     float acc_r = 0.f, acc_g = 0.f, acc_b = 0.f;
     // d loss / d alpha through the final transmittance: Tfin dC.w / (1 - alpha).
     const float tfin_da = Tfin * dC.w;
@@ -213,6 +216,8 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
         }
         __syncthreads();
         const uint32_t lim = wave_last > cbeg ? min(cnt, wave_last - cbeg) : 0u;
+        // This lane's contributors in the chunk: staged rows j < mine.
+        const uint32_t mine = last > cbeg ? last - cbeg : 0u;
         const uint32_t nl = wave_reach_list<2>(s_box, s_tau, s_geo, s_aux, lim, bx0, by0, list);
         // Listed splats in groups of 8, back to front: member s of the group
         // whose top is `itop` is list entry itop - 1 - s.  The 9 x 8 per-pixel
@@ -233,7 +238,7 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
                 // difference can only move the kPowerSkip test where alpha
                 // is below 1/255 anyway.
                 const float power = fmaf(dx, fmaf(geo.z, dx, geo.w * dy), aux.x * dy * dy);
-                const bool in = member && cbeg + j < last && !(power > kPowerSkip);
+                const bool in = member && j < mine && !(power > kPowerSkip);
                 if (!__ballot(in)) {
 #pragma unroll
                     for (int k = 0; k < 9; ++k) g[k][s] = 0.0f;

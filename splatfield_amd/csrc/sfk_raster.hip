@@ -217,8 +217,10 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
             uint32_t q[8];
             bool any = false;
 #pragma unroll
+            for (int s = 0; s < 8; ++s) q[s] = 0;
+#pragma unroll
             for (int s = 0; s < 8; ++s) {
-                q[s] = 0;
+                if (i0 + s >= nl) break;  // past the list end (wave-uniform)
                 float4 g4, x4;
                 float bl = 0.0f;
                 if constexpr (PF) {
@@ -237,8 +239,15 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
                 } else {
                     power = 0.5f * (g4.z * dx * dx + x4.x * dy * dy) + g4.w * dx * dy;
                 }
-                const bool in = !done && i0 + s < nl && !(power > kPowerSkip);
-                if (!__ballot(in)) continue;
+                const bool in = !done && !(power > kPowerSkip);
+                // Every listed splat is in reach of some pixel of the block, so
+                // only lanes that are done can leave a member unused: RGB
+                // evaluates it straight through (the wave-wide test would cost
+                // more than it saves); the responsibility modes, with the
+                // exact exp, skip it when no lane takes it.
+                if constexpr (MODE != kRgb) {
+                    if (!__ballot(in)) continue;
+                }
                 // RGB: exp(-power) by the hardware exp2 (argument capped as in
                 // the adjoint); the responsibility modes keep the exact exp
                 // that their bit-exact u32 weights are specified with.
