@@ -82,7 +82,9 @@ SFK_HD float sfk_expf_neg(float p) {
 // Dropping sfk_expf_neg's lower clamp and NaN select saves their operations
 // in the rasterizers' innermost loop.
 SFK_HD float sfk_expf_neg_in(float p) {
-    const float x = fminf(-p, 88.0f);  // NaN -> 88
+    // min(-p, 88) written as -max(p, -88) (the same value, NaN -> 88), so
+    // that the negation folds into the operands that use x.
+    const float x = -fmaxf(p, -88.0f);
     const float n = floorf(x * 1.44269504088896341f + 0.5f);
     float r = fmaf(-n, 0.693359375f, x);
     r = fmaf(-n, -2.12194440e-4f, r);

@@ -215,12 +215,18 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
                 }
             }
             uint32_t q[8];
-            bool any = false;
 #pragma unroll
             for (int s = 0; s < 8; ++s) q[s] = 0;
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
-                if (i0 + s >= nl) break;  // past the list end (wave-uniform)
+                // RGB stops at the list end (wave-uniform); the responsibility
+                // modes evaluate all 8 members straight through and mask the
+                // ones past the end, so that the 8 weights need no merge of
+                // control paths before their wave sum.
+                if constexpr (MODE == kRgb) {
+                    if (i0 + s >= nl) break;
+                }
+                const bool valid = MODE == kRgb || i0 + s < nl;
                 float4 g4, x4;
                 float bl = 0.0f;
                 if constexpr (PF) {
@@ -239,15 +245,11 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
                 } else {
                     power = 0.5f * (g4.z * dx * dx + x4.x * dy * dy) + g4.w * dx * dy;
                 }
-                const bool in = !done && !(power > kPowerSkip);
                 // Every listed splat is in reach of some pixel of the block, so
-                // only lanes that are done can leave a member unused: RGB
-                // evaluates it straight through (the wave-wide test would cost
-                // more than it saves); the responsibility modes, with the
-                // exact exp, skip it when no lane takes it.
-                if constexpr (MODE != kRgb) {
-                    if (!__ballot(in)) continue;
-                }
+                // only lanes that are done can leave a member unused: each
+                // member is evaluated straight through (a wave-wide test
+                // costs more than it saves).
+                const bool in = valid && !done && !(power > kPowerSkip);
                 // RGB: exp(-power) by the hardware exp2 (argument capped as in
                 // the adjoint); the responsibility modes keep the exact exp
                 // that their bit-exact u32 weights are specified with.
@@ -263,8 +265,11 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
                     cg = con ? fmaf(x4.w, vis, cg) : cg;
                     cb = con ? fmaf(bl, vis, cb) : cb;
                 } else {
-                    q[s] = con ? sfk_f2u(vis * J.scale) : 0u;
-                    any |= q[s] != 0;
+                    // `f32 as u32` of the weight: on a contributing lane
+                    // 0 < vis <= 0.999 and scale <= 2^32, so the product lies
+                    // in (0, 2^32) and truncates exactly as the saturating
+                    // conversion would.
+                    q[s] = con ? (uint32_t)(vis * J.scale) : 0u;
                 }
                 T = con ? T * (1.0f - alpha) : T;
                 last = con ? chunk + jj[s] + 1 : last;
@@ -273,7 +278,8 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
             if constexpr (MODE != kRgb) {
                 // The u32 weights of the 8 members are summed over the wave
                 // in one transposed pass, lane l ending with member l >> 3.
-                if (!__ballot(any)) continue;
+                const uint32_t qor = (q[0] | q[1] | q[2] | q[3]) | (q[4] | q[5] | q[6] | q[7]);
+                if (!__ballot(qor != 0u)) continue;
                 const bool writer = (lane & 7u) == 0 && i0 + ms < nl;
                 const uint32_t j = writer ? list[i0 + ms] : 0u;
                 if constexpr (MODE == kTotal) {
