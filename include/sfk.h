@@ -409,6 +409,12 @@ int sfk_eig_round_run(sfk_ctx* ctx, sfk_eig_round* r, const float* a, const floa
  * emission + stable tile argsort; the same isect lists either way.  Drops a
  * captured graph. */
 int sfk_eig_round_set_binned(sfk_eig_round* r, int on);
+/* Binned only: the raster takes each tile's isects straight from its
+ * supertile's depth-ordered (splat, bbox) pairs (default), or from per-tile
+ * lists placed by the binned emission; the same evidence either way.  No
+ * counterpart in the reference (its rounds render views one at a time).
+ * Drops a captured graph. */
+int sfk_eig_round_set_pair_raster(sfk_eig_round* r, int on);
 /* Device pointer of one view's u32 responsibility words and its scale. */
 int sfk_eig_round_info(const sfk_eig_round* r, uint32_t view, const uint32_t** resp,
                        float* scale, uint32_t* capacity, uint32_t* total_tiles);

@@ -117,6 +117,7 @@ SIGNATURES = {
     "sfk_eig_round_destroy": (None, [P]),
     "sfk_eig_round_run": (i32, [P, P, P, P, i32, P, P]),
     "sfk_eig_round_set_binned": (i32, [P, i32]),
+    "sfk_eig_round_set_pair_raster": (i32, [P, i32]),
     "sfk_eig_round_info": (i32, [P, u32, PP, P, P, P]),
     "sfk_eig_round_counters": (i32, [P, P, P]),
     "sfk_eig_round_plain": (i32, [P, P, P, P, P, P, u32, P, P, P, P, P]),

@@ -96,16 +96,6 @@ struct MemberSum<4> {
     static constexpr uint32_t kShift = 4;  // lane l holds member l >> 4
 };
 
-// Workgroup barrier ordering LDS only: unlike __syncthreads it does not wait
-// for this wave's outstanding global loads (a prefetch stays in flight).  For
-// code where no thread reads global memory another thread of the workgroup
-// wrote before the barrier.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // M = members per group: the gradients of M splats are summed over the wave
 // in one transposed pass; M = 4 halves the registers held across a group.
 template <int TW, int M>

@@ -221,6 +221,10 @@ struct sfk_eig_round {
     // Binned emission (compact grids): per-view supertile grids, supertile
     // ranges, chunk map and counts, per-rank isect offsets.
     bool binned = true;
+    // Binned: the raster takes each tile's isects straight from its
+    // supertile's sorted pairs (no per-tile lists are placed).
+    bool pair_raster = true;
+    const uint2* pairs = nullptr;
     uint32_t supers = 0, bin_shift = 1;  // supertile ids, bits of the local ones
     BinView* d_binviews = nullptr;
     uint32_t* offsets = nullptr;  // isect offsets of truncation candidates, by view-major splat
@@ -327,6 +331,8 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
             j.counters = r->counters;
             j.status = r->emit_status;
             j.status_zeroed = true;
+            j.pairs_only = r->pair_raster;
+            j.pairs_out = &r->pairs;
             rc = bin_emit(c, j);
             if (rc) return rc;
             r->sorted_tiles = nullptr;
@@ -382,6 +388,13 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
         ra.seg_len = n;
         // (slot order of the binned emission; the sorted path writes none)
         ra.tile_order = r->sorted_tiles == nullptr ? r->tile_order : nullptr;
+        if (binned && r->pair_raster) {
+            ra.tile_order = nullptr;
+            ra.pairs = r->pairs;
+            ra.pair_starts = r->bin_sstart;
+            ra.offs_by_splat = r->offsets;
+            ra.cap = r->capacity;
+        }
         SFK_HIP(c, launch_rasterize_batch(c->stream, ra));
     }
     {
@@ -594,6 +607,18 @@ extern "C" int sfk_eig_round_set_binned(sfk_eig_round* r, int on) {
     if (r->binned == (on != 0)) return SFK_OK;
     r->binned = on != 0;
     // A captured round holds the other emission: capture again on next use.
+    if (r->ctx) hipStreamSynchronize(r->ctx->stream);
+    if (r->exec) hipGraphExecDestroy(r->exec);
+    if (r->graph) hipGraphDestroy(r->graph);
+    r->exec = nullptr;
+    r->graph = nullptr;
+    return SFK_OK;
+}
+
+extern "C" int sfk_eig_round_set_pair_raster(sfk_eig_round* r, int on) {
+    if (!r) return SFK_E_ARG;
+    if (r->pair_raster == (on != 0)) return SFK_OK;
+    r->pair_raster = on != 0;
     if (r->ctx) hipStreamSynchronize(r->ctx->stream);
     if (r->exec) hipGraphExecDestroy(r->exec);
     if (r->graph) hipGraphDestroy(r->graph);

@@ -789,8 +789,8 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     static_assert(kBinMaxCap < kSat, "a saturated isect total must exceed every capacity");
     if (j.n == 0 || j.cap < 2 || j.cap > kBinMaxCap || j.supers == 0 || j.supers >= (1u << 24) ||
         j.nviews == 0 || j.nviews > 255 || j.view_shift == 0 || j.view_shift > 10 ||
-        j.supers % j.nviews ||
-        !j.gids || !j.tile_starts || !j.counters || !j.status)
+        j.supers % j.nviews || !j.counters || !j.status || !j.sstart ||
+        (j.pairs_only ? !j.pairs_out : (!j.gids || !j.tile_starts)))
         return fail(c, SFK_E_ARG, "binned emission: bad job");
     uint32_t* pair_keys = nullptr;
     uint2* pair_vals = nullptr;
@@ -831,6 +831,13 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     // One view of at most 256 supertiles sorted in one pass: the pass's digit
     // totals are the supertiles' pair counts, and the count pass derives the
     // supertile ranges and chunk slots from them (no ranges launch).
+    if (j.pairs_only) {
+        SFK_PROF(c, "bin_ranges");
+        SFK_HIP(c, launch_tile_starts(st, sk, j.cap, j.counters + 1, j.supers, j.sstart,
+                                      j.nviews > 1 ? j.nviews : 0u, j.view_shift));
+        *j.pairs_out = sv;
+        return SFK_OK;
+    }
     uint32_t db = 0;
     const uint32_t* totals = radix_scratch_totals(j.sort, &db);
     const bool ranges_in_count = j.nviews == 1 && j.view_shift <= db && j.supers <= 256;

@@ -6,6 +6,16 @@
 
 namespace sfk {
 
+// Workgroup barrier ordering LDS only: unlike __syncthreads it does not wait
+// for this wave's outstanding global loads (a prefetch stays in flight).  For
+// code where no thread reads global memory another thread of the workgroup
+// wrote before the barrier.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }

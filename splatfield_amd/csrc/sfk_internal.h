@@ -283,6 +283,11 @@ struct BinJob {
     uint32_t clear_n;
     uint32_t* host_total;   // optional device view of mapped pinned host memory:
                             // receives the raw isect count as well
+    // Pairs only: stop after the pair sort and the supertile ranges (sstart);
+    // *pairs_out receives the sorted pair values.  chunk_map, chunk_counts,
+    // scan_sums, tile_starts, tile_order, gids and tile_ids are unused.
+    bool pairs_only;
+    const uint2** pairs_out;
 };
 int bin_emit(sfk_ctx* c, const BinJob& j);
 
@@ -323,6 +328,17 @@ struct RasterArgs {
     uint32_t nviews, batch_tiles, seg_len;
     const float* proj_c;
     const float* opac;
+    // Batched, pair source (optional): no isect lists; each tile takes its
+    // isects straight from its supertile's sorted (splat | may-truncate << 31,
+    // compact bbox) pairs, pairs[pair_starts[s * nviews + view] ..
+    // pair_starts[s * nviews + view + 1]) for local supertile s, keeping the
+    // pairs whose bbox covers the tile (and, for a flagged pair, whose
+    // emission slot offs_by_splat[view-major splat] + index in bbox is below
+    // cap).
+    const uint2* pairs;
+    const uint32_t* pair_starts;
+    const uint32_t* offs_by_splat;
+    uint32_t cap;
 };
 hipError_t launch_rasterize(hipStream_t s, int mode, const RasterArgs& a);
 hipError_t launch_rasterize_batch(hipStream_t s, const RasterArgs& a);

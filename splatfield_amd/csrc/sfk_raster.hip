@@ -108,7 +108,121 @@ struct TileJob {
     uint32_t* resp;
     uint32_t* fg;
     uint32_t* bg;
+    // Pair source: the tile's supertile pairs [p0, p1), view-major splat
+    // slot = in-view splat + gid_add.
+    uint32_t p0, p1, gid_add;
 };
+
+// Pairs scanned per thread and step of stage_pairs: one (round, wave) hit
+// mask per lane of the prefix scan, at most 16 rounds (32 VGPRs of pairs).
+#ifndef SFK_PAIR_ROUNDS
+#define SFK_PAIR_ROUNDS 4
+#endif
+template <int NT>
+constexpr uint32_t pair_rounds() {
+    return 64 / (NT / 64) < SFK_PAIR_ROUNDS ? 64 / (NT / 64) : SFK_PAIR_ROUNDS;
+}
+
+// Pair source of the batched responsibility raster: stages the tile's next
+// isects, at most NT, into s_gid in pair order (= depth-rank order: the
+// sorted pairs of one supertile and view keep the emission's rank order),
+// from its supertile's pairs [p, J.p1); advances p past the last one taken
+// and returns their count (0 once the pairs are exhausted).  A pair is an
+// isect of tile (tx, ty) iff its compact bbox covers the tile and, for a
+// pair flagged as possibly truncated, its emission slot off + (ty - y0) *
+// (x1 - x0) + (tx - x0) is below the capacity (raster.rs:12-48 drops the
+// others) -- exactly the isects that map_isects + the stable tile sort put
+// in the tile's range, in the same order.  A step scans pair_rounds * NT
+// pairs (pair p + k * NT + t in round k); one barrier per step publishes the
+// waves' hit masks, from which every lane derives its hits' positions.
+// Workgroup-uniform; ends with a barrier after the last write to s_gid.
+template <int NT>
+__device__ __forceinline__ uint32_t stage_pairs(const TileJob& J, const RasterArgs& a, uint32_t& p,
+                                                uint32_t* s_gid, uint64_t* s_mask,
+                                                uint32_t* s_pnext) {
+    constexpr uint32_t NW = NT / 64, R = pair_rounds<NT>();
+    static_assert(R * NW <= 64, "one lane per (round, wave) mask");
+    const uint32_t t = threadIdx.x, w = t / 64, lane = lane_id();
+    const uint32_t pend = J.p1;
+    // Every lane issues all R loads (indices past the end clamped to the last
+    // pair, their bboxes cleared after the load), so that the loads in flight
+    // are the same on every path and a prefetch is not drained early.
+    auto load = [&](uint32_t q, uint2 (&pv)[R]) {
+#pragma unroll
+        for (uint32_t k = 0; k < R; ++k) {
+            const uint32_t i = q + k * NT + t;
+            pv[k] = a.pairs[i < pend ? i : pend - 1u];
+        }
+    };
+    uint32_t staged = 0, buf = 0;
+    // One step over cur[] (loaded), issuing the next step's loads into nxt[]
+    // while it ranks (that step starts at p + R * NT unless this one fills
+    // the chunk, which ends the staging).  Returns whether to go on.
+    auto step = [&](uint2 (&cur)[R], uint2 (&nxt)[R]) -> bool {
+        bool hit[R];
+#pragma unroll
+        for (uint32_t k = 0; k < R; ++k) {
+            if (p + k * NT + t >= pend) cur[k].y = 0u;  // bbox 0 covers nothing
+            const uint32_t cb = cur[k].y;
+            const uint32_t x0 = cb & 0xffu, y0 = (cb >> 8) & 0xffu, x1 = (cb >> 16) & 0xffu,
+                           y1 = cb >> 24;
+            // x0 <= tx < x1 and y0 <= ty < y1 (x1 >= x0, y1 >= y0)
+            bool h = J.tx - x0 < x1 - x0 && J.ty - y0 < y1 - y0;
+            if (h && (cur[k].x >> 31)) {
+                const uint32_t off = a.offs_by_splat[J.gid_add + (cur[k].x & 0x7fffffffu)];
+                const uint32_t idx = (J.ty - y0) * (x1 - x0) + (J.tx - x0);
+                h = off < a.cap && idx < a.cap - off;
+            }
+            hit[k] = h;
+        }
+        const uint32_t pn = p + R * NT;
+        load(pn < pend ? pn : p, nxt);
+        uint64_t m[R];
+#pragma unroll
+        for (uint32_t k = 0; k < R; ++k) m[k] = __ballot(hit[k]);
+        uint64_t* sm = s_mask + buf * (R * NW);
+        buf ^= 1u;
+        if (lane == 0) {
+#pragma unroll
+            for (uint32_t k = 0; k < R; ++k) sm[k * NW + w] = m[k];
+        }
+        lds_barrier();
+        // Lane j < R * NW of every wave: exclusive prefix of the hit counts
+        // in (round, wave) order, i.e. in pair order.
+        const uint32_t c = lane < R * NW ? (uint32_t)__popcll(sm[lane]) : 0u;
+        const uint32_t inc = wave_incl_scan_u32(c);
+        const uint32_t total = __builtin_amdgcn_readlane(inc, 63);
+        const uint32_t run = staged + total;
+#pragma unroll
+        for (uint32_t k = 0; k < R; ++k) {
+            const uint32_t j = k * NW + w;
+            const uint32_t base = staged + __shfl(inc - c, j, 64);
+            if ((m[k] >> lane) & 1u) {
+                const uint32_t pos = base + mask_rank(m[k]);
+                if (pos < (uint32_t)NT) s_gid[pos] = (cur[k].x & 0x7fffffffu) + J.gid_add;
+                if (pos == (uint32_t)NT - 1u && run > (uint32_t)NT) *s_pnext = p + k * NT + t + 1u;
+            }
+        }
+        if (run > (uint32_t)NT) {
+            // The last pair taken is the one at position NT - 1: resume after it.
+            lds_barrier();
+            p = *s_pnext;
+            staged = NT;
+            return false;
+        }
+        p = pn;
+        staged = run;
+        return staged < (uint32_t)NT && p < pend;
+    };
+    uint2 ping[R], pong[R];
+    if (p < pend) {
+        load(p, ping);
+        while (step(ping, pong) && step(pong, ping)) {
+        }
+    }
+    lds_barrier();
+    return staged;
+}
 
 // rasterize_kernel (raster.rs:87-230) for one tile: TW x TW threads, one per
 // pixel; splat rows staged through LDS in chunks of TW*TW.  Each wavefront
@@ -122,8 +236,9 @@ struct TileJob {
 // overlaps their LDS latency at the cost of registers: it pays when the
 // launch has too few waves to hide latency (one view's responsibility render)
 // and costs occupancy otherwise.
-template <int MODE, int TW, bool PF>
+template <int MODE, int TW, bool PF, bool PAIRS = false>
 __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& a) {
+    static_assert(!PAIRS || MODE == kTotal, "the pair source serves the batched total mode");
     constexpr int NT = TW * TW;
     constexpr int NW = NT / 64;
     __shared__ float4 s_geo[NT];  // mx, my, conic a, conic b
@@ -135,6 +250,8 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
     __shared__ uint32_t s_gid[MODE == kRgb ? 1 : NT];
     __shared__ uint32_t s_q0[MODE == kRgb ? 1 : NT];
     __shared__ uint32_t s_q1[MODE == kEvidence ? NT : 1];
+    __shared__ uint64_t s_mask[PAIRS ? 2 * pair_rounds<NT>() * NW : 1];
+    __shared__ uint32_t s_pnext;
 
     const uint32_t t = threadIdx.x;
     uint32_t lx, ly;
@@ -155,11 +272,22 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
     bool done = !in_bounds;
     uint32_t last = start;
 
-    for (uint32_t chunk = start; chunk < end; chunk += NT) {
-        __syncthreads();  // previous chunk fully consumed
+    uint32_t pnext = J.p0;
+    for (uint32_t chunk = start; PAIRS || chunk < end; chunk += NT) {
+        uint32_t cnt;
+        if constexpr (PAIRS) {
+            // previous chunk fully consumed; whole-tile early exit before
+            // scanning further pairs
+            if (__syncthreads_count(!done) == 0) break;
+            cnt = stage_pairs<NT>(J, a, pnext, s_gid, s_mask, &s_pnext);
+            if (cnt == 0) break;
+        } else {
+            __syncthreads();  // previous chunk fully consumed
+            cnt = (end - chunk) < (uint32_t)NT ? (end - chunk) : (uint32_t)NT;
+        }
         const uint32_t idx = chunk + t;
-        if (idx < end) {
-            uint32_t g = a.gids[idx];
+        if (PAIRS ? t < cnt : idx < end) {
+            uint32_t g = PAIRS ? s_gid[t] : a.gids[idx];
             const float4* row = J.proj + (size_t)g * J.rows4;
             float4 geo, aux;
             if (J.rows4 == 1) {
@@ -185,7 +313,6 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
         }
         // Staging fence and whole-tile early exit (raster.rs:166-171).
         if (__syncthreads_count(!done) == 0) break;
-        const uint32_t cnt = (end - chunk) < (uint32_t)NT ? (end - chunk) : (uint32_t)NT;
         constexpr int CH = MODE == kRgb ? 2 : 1;
         const uint32_t nl =
             __ballot(!done) ? wave_reach_list<CH>(s_box, s_tau, s_geo, s_aux, cnt, bx0, by0, list)
@@ -353,8 +480,9 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
     raster_tile<MODE, TW, MODE != kRgb>(J, a);
 }
 
-// Batched responsibility: one launch over the tiles of every view.
-template <int TW>
+// Batched responsibility: one launch over the tiles of every view, from the
+// sorted isect lists or (PAIRS) straight from the supertile pairs.
+template <int TW, bool PAIRS>
 __global__ __launch_bounds__(TW * TW) void k_rasterize_batch(RasterArgs a) {
     const uint32_t total = a.batch_tiles;
     uint32_t lo, local;
@@ -384,9 +512,21 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize_batch(RasterArgs a) {
     J.ty = local / bv.tiles_x;
     // Sorted by (local tile, view): list (view lo, tile local) is entry
     // local * nviews + lo of the starts table; ids are view-major slots.
-    const uint32_t slot = local * a.nviews + lo;
-    J.start = a.tile_starts[slot];
-    J.end = a.tile_starts[slot + 1];
+    if constexpr (PAIRS) {
+        const uint32_t ls =
+            (J.ty / kBinSuper) * ((bv.tiles_x + kBinSuper - 1) / kBinSuper) + J.tx / kBinSuper;
+        const uint32_t ps = ls * a.nviews + lo;
+        J.p0 = a.pair_starts[ps];
+        J.p1 = a.pair_starts[ps + 1];
+        J.start = 0;
+        J.end = 0;
+    } else {
+        const uint32_t slot = local * a.nviews + lo;
+        J.start = a.tile_starts[slot];
+        J.end = a.tile_starts[slot + 1];
+        J.p0 = J.p1 = 0;
+    }
+    J.gid_add = lo * a.seg_len;
     J.row_stride = 0;
     J.scale = bv.scale;
     J.proj = reinterpret_cast<const float4*>(a.proj);
@@ -399,7 +539,7 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize_batch(RasterArgs a) {
     RasterArgs b = a;
     b.final_t = nullptr;
     b.n_contrib = nullptr;
-    raster_tile<kTotal, TW, false>(J, b);
+    raster_tile<kTotal, TW, false, PAIRS>(J, b);
 }
 
 template <int MODE>
@@ -449,11 +589,20 @@ hipError_t launch_rasterize(hipStream_t s, int mode, const RasterArgs& a) {
 
 hipError_t launch_rasterize_batch(hipStream_t s, const RasterArgs& a) {
     if (a.batch_tiles == 0) return hipSuccess;
-    switch (a.tile_w) {
-        case 8: k_rasterize_batch<8><<<a.batch_tiles, 64, 0, s>>>(a); break;
-        case 16: k_rasterize_batch<16><<<a.batch_tiles, 256, 0, s>>>(a); break;
-        case 32: k_rasterize_batch<32><<<a.batch_tiles, 1024, 0, s>>>(a); break;
-        default: return hipErrorInvalidValue;
+    if (a.pairs) {
+        switch (a.tile_w) {
+            case 8: k_rasterize_batch<8, true><<<a.batch_tiles, 64, 0, s>>>(a); break;
+            case 16: k_rasterize_batch<16, true><<<a.batch_tiles, 256, 0, s>>>(a); break;
+            case 32: k_rasterize_batch<32, true><<<a.batch_tiles, 1024, 0, s>>>(a); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (a.tile_w) {
+            case 8: k_rasterize_batch<8, false><<<a.batch_tiles, 64, 0, s>>>(a); break;
+            case 16: k_rasterize_batch<16, false><<<a.batch_tiles, 256, 0, s>>>(a); break;
+            case 32: k_rasterize_batch<32, false><<<a.batch_tiles, 1024, 0, s>>>(a); break;
+            default: return hipErrorInvalidValue;
+        }
     }
     return hipGetLastError();
 }

@@ -174,6 +174,12 @@ class EigRound:
         """Binned isect emission (default) or emission + stable tile argsort."""
         self.ctx.check(lib().sfk_eig_round_set_binned(self.ptr, 1 if on else 0), "binned")
 
+    def set_pair_raster(self, on):
+        """Binned only: raster straight from the supertile pairs (default) or
+        from per-tile lists placed by the binned emission."""
+        self.ctx.check(lib().sfk_eig_round_set_pair_raster(self.ptr, 1 if on else 0),
+                       "pair raster")
+
     def run(self, state, use_graph=True, read=True):
         """Returns (eig per view, best view) when read is set."""
         eig = np.zeros(self.nviews, np.float64)
@@ -191,6 +197,13 @@ class EigRound:
         lib().sfk_eig_round_info(self.ptr, view, C.byref(p), C.byref(sc), None, None)
         t = GpuTensor(self.ctx, p.value, (self.splats.n,), np.uint32, False, self)
         return t.read(), sc.value
+
+    def capacity(self):
+        """Isect capacity of the round's buffers (refit on first graph use)."""
+        cap = C.c_uint32()
+        self.ctx.check(lib().sfk_eig_round_info(self.ptr, 0, None, None, C.byref(cap), None),
+                       "eig round info")
+        return cap.value
 
     def counters(self):
         out = np.zeros(2 * self.nviews + 1, np.uint32)

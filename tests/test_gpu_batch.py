@@ -115,10 +115,12 @@ def test_batch_c4_full_mixed_resolution(ctx):
     del rnd
 
 
-@pytest.mark.parametrize("tile_w", [16, 8])
-def test_batch_binned_matches_sorted(ctx, tile_w):
+@pytest.mark.parametrize("pair_raster", [True, False])
+@pytest.mark.parametrize("tile_w", [16, 8, 32])
+def test_batch_binned_matches_sorted(ctx, tile_w, pair_raster):
     """Batched binned emission (per-view supertile grids, ranges indexed
-    local tile * views + view) against the batched emission + stable tile
+    local tile * views + view), rasterized straight from the supertile pairs
+    or from placed per-tile lists, against the batched emission + stable tile
     argsort: every view's evidence words equal, eager and graph-replayed."""
     n = 32768
     attrs, sh, k = S.synthetic_scene(n, 0, seed=6)
@@ -134,6 +136,7 @@ def test_batch_binned_matches_sorted(ctx, tile_w):
     eig_s, best_s = ref.run(state, use_graph=False)
     words = [ref.resp(v)[0] for v in range(len(cams))]
     rnd = G.EigRound(splats, cams, sizes, tile_w=tile_w)
+    rnd.set_pair_raster(pair_raster)
     for graph in (False, True, True):
         eig_b, best_b = rnd.run(state, use_graph=graph)
         assert best_b == best_s
@@ -142,3 +145,32 @@ def test_batch_binned_matches_sorted(ctx, tile_w):
             np.testing.assert_array_equal(rnd.resp(v)[0], words[v], f"view {v}")
     c_b, c_s = rnd.counters(), ref.counters()
     np.testing.assert_array_equal(c_b, c_s)
+
+
+def test_batch_truncated_round(ctx):
+    """A round whose isects exceed its capacity (an eager run before any
+    capacity fit): the raster straight from the pairs drops exactly the
+    isects whose emission slot reaches the capacity, as the placed lists and
+    the emission + tile argsort do (raster.rs:12-48)."""
+    n = 1 << 16
+    attrs, sh, k = S.synthetic_scene(n, 0, seed=8)
+    attrs = attrs.reshape(S.ATTR_PLANES, n)
+    attrs[7:10] += np.float32(2.0)  # large splats: ~10^8 isects at 8 px tiles
+    attrs = attrs.reshape(-1)
+    splats = R.Splats(ctx, attrs, sh)
+    cams = G.sphere_cameras(2)
+    sizes = [(1024, 1024), (512, 512)]
+    state, a, b = make_state(ctx, n, seed=4)
+    words = {}
+    for mode in ("sorted", "placed", "pairs"):
+        rnd = G.EigRound(splats, cams, sizes, tile_w=8)
+        rnd.set_binned(mode != "sorted")
+        rnd.set_pair_raster(mode == "pairs")
+        rnd.run(state, use_graph=False)
+        c = rnd.counters()
+        assert c[-1] > rnd.capacity(), "the round must truncate"
+        words[mode] = [rnd.resp(v)[0] for v in range(len(cams))]
+        del rnd
+    for mode in ("placed", "pairs"):
+        for v in range(len(cams)):
+            np.testing.assert_array_equal(words[mode][v], words["sorted"][v], f"{mode} view {v}")
