@@ -495,6 +495,7 @@ unsafe extern "C" {
         best_out: *mut u32,
     ) -> c_int;
     pub fn sfk_eig_round_set_binned(r: *mut sfk_eig_round, on: c_int) -> c_int;
+    pub fn sfk_eig_round_set_pair_raster(r: *mut sfk_eig_round, on: c_int) -> c_int;
     pub fn sfk_eig_round_info(
         r: *const sfk_eig_round,
         view: u32,

@@ -214,7 +214,10 @@ int sfk_prepare_isects_view(sfk_ctx* ctx, const sfk_splats* splats,
 /* SYNC: [isects_raw, visible, isects]. */
 int sfk_tile_isects_counts(sfk_ctx* ctx, const sfk_tile_isects* isects, uint32_t counts[3]);
 
-/* Finalize (src/render.rs:220-239, 382-460) in its three modes. */
+/* Finalize (src/render.rs:220-239, 382-460) in its three modes.  The
+ * responsibility modes take the accumulators' fixed-point scale, 2^k with
+ * k = fractional_bits(w, h) in [0, 32] (src/seg/mod.rs:31-34); any other
+ * scale is rejected (SFK_E_ARG). */
 int sfk_finalize_rgb(sfk_ctx* ctx, sfk_render_scratch* scratch, const sfk_tile_isects* isects);
 int sfk_finalize_total(sfk_ctx* ctx, sfk_render_scratch* scratch, const sfk_tile_isects* isects,
                        float scale, uint32_t* resp);

@@ -229,13 +229,9 @@ __global__ __launch_bounds__(TW * TW) void k_raster_bwd(BwdArgs a) {
                 // is below 1/255 anyway.
                 const float power = fmaf(dx, fmaf(geo.z, dx, geo.w * dy), aux.x * dy * dy);
                 const bool in = member && j < mine && !(power > kPowerSkip);
-                if (!__ballot(in)) {
-#pragma unroll
-                    for (int k = 0; k < 9; ++k) g[k][s] = 0.0f;
-                    continue;
-                }
-                // Straight-line evaluation for the whole wave; lanes that do
-                // not contribute keep their state through selects.
+                // Straight-line evaluation for the whole wave, every member
+                // (a wave-wide skip test costs more than it saves); lanes
+                // that do not contribute keep their state through selects.
                 // exp(-power) by the hardware exp2: within an ulp or two of the
                 // forward's exp, so alpha and T agree with the forward to
                 // rounding (a contributor test can only differ where alpha is

@@ -35,7 +35,9 @@ constexpr uint32_t kMaxViews = 64;
 // One thread per splat, looping over the views: the 44-byte attribute row is
 // read once for all views.  The projected rows are split: per (view, splat)
 // proj = (mean x, mean y, conic a, conic b) and proj_c = conic c; the
-// opacity, the same in every view, once per splat (opac).
+// opacity, the same in every view, once per splat (opac).  It also clears
+// the round's responsibility words (resp, per view and splat), which the
+// raster then accumulates into.
 __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restrict__ views,
                                                        uint32_t nviews,
                                                        const float* __restrict__ A, uint32_t n,
@@ -47,7 +49,8 @@ __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restric
                                                        uint32_t* __restrict__ tile_counts,
                                                        uint2* __restrict__ bbox,
                                                        uint32_t* __restrict__ cbbox,
-                                                       uint32_t* __restrict__ counters) {
+                                                       uint32_t* __restrict__ counters,
+                                                       uint32_t* __restrict__ resp) {
     __shared__ sfk_view s_views[kMaxViews];
     __shared__ float2 s_lims[kMaxViews];
     for (uint32_t k = threadIdx.x; k < nviews * (sizeof(sfk_view) / 4); k += 256)
@@ -84,6 +87,7 @@ __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restric
                 if (!cbbox) bbox[gi] = make_uint2(o.bb_lo, o.bb_hi);
             }
             depth_keys[gi] = key;
+            resp[gi] = 0u;
             // Compact grids: the bbox word is all the emission needs.
             if (cbbox) cbbox[gi] = vis ? compact_bbox(o) : 0u;
             else tile_counts[gi] = cnt;
@@ -290,7 +294,7 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
         k_project_batch<<<div_up(n, 256), 256, 0, c->stream>>>(
             r->d_views, V, attrs, n, 1.0f / (float)r->tile_w, r->keys, r->proj, r->proj_c,
             r->opac, r->counts,
-            r->bbox, r->compact ? r->cbbox : nullptr, r->counters);
+            r->bbox, r->compact ? r->cbbox : nullptr, r->counters, r->resp);
         SFK_LAUNCHED(c, "batch project");
     }
     EmitSource src;
@@ -370,7 +374,6 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
                                           r->max_tiles * V, r->starts, V, r->tile_bits));
         }
     }
-    SFK_HIP(c, launch_fill_u32(c->stream, r->resp, 0u, vn));
     {
         SFK_PROF(c, "batch_raster_total");
         RasterArgs ra;

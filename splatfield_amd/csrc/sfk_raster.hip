@@ -268,7 +268,13 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
     if (MODE == kEvidence && in_bounds)
         inside = (a.mask[px / 32 + py * a.words_per_row] >> (px % 32)) & 1u;
 
-    float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    // The responsibility modes carry T scaled by their fixed-point scale (a
+    // power of two, so the scaling is exact and commutes with every rounding
+    // below): the weight `(alpha T) scale` is then one product alpha Ts, and
+    // the saturation test T < 1/255 reads Ts < scale / 255.
+    const float tscale = MODE == kRgb ? 1.0f : J.scale;
+    const float tdone = kInv255 * tscale;
+    float T = tscale, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     bool done = !in_bounds;
     uint32_t last = start;
 
@@ -298,14 +304,12 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
                 aux = row[1];
             }
             s_box[t] = reach_box(geo, aux, &s_tau[t]);
+            // Conic a and c staged halved, as the adjoint stages them.
+            s_geo[t] = make_float4(geo.x, geo.y, 0.5f * geo.z, geo.w);
+            s_aux[t] = make_float4(0.5f * aux.x, aux.y, aux.z, aux.w);
             if constexpr (MODE == kRgb) {
-                // Conic a and c staged halved, as the adjoint stages them.
-                s_geo[t] = make_float4(geo.x, geo.y, 0.5f * geo.z, geo.w);
-                s_aux[t] = make_float4(0.5f * aux.x, aux.y, aux.z, aux.w);
                 s_blue[t] = row[2].x;
             } else {
-                s_geo[t] = geo;
-                s_aux[t] = aux;
                 s_gid[t] = g;
                 s_q0[t] = 0;
                 if constexpr (MODE == kEvidence) s_q1[t] = 0;
@@ -313,9 +317,8 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
         }
         // Staging fence and whole-tile early exit (raster.rs:166-171).
         if (__syncthreads_count(!done) == 0) break;
-        constexpr int CH = MODE == kRgb ? 2 : 1;
         const uint32_t nl =
-            __ballot(!done) ? wave_reach_list<CH>(s_box, s_tau, s_geo, s_aux, cnt, bx0, by0, list)
+            __ballot(!done) ? wave_reach_list<2>(s_box, s_tau, s_geo, s_aux, cnt, bx0, by0, list)
                             : 0u;
         // Listed splats in groups of 8 (entries past the list end repeat the
         // last entry and are masked off).  Each member is evaluated straight
@@ -370,7 +373,10 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
                     // evaluation, not to its bits.
                     power = fmaf(dx, fmaf(g4.z, dx, g4.w * dy), x4.x * dy * dy);
                 } else {
-                    power = 0.5f * (g4.z * dx * dx + x4.x * dy * dy) + g4.w * dx * dy;
+                    // 0.5 (a dx dx + c dy dy) + b dx dy (raster.rs:183-186) with
+                    // a and c staged halved: the halving is exact and
+                    // commutes with the roundings, so the bits are the same.
+                    power = (g4.z * dx * dx + x4.x * dy * dy) + g4.w * dx * dy;
                 }
                 // Every listed splat is in reach of some pixel of the block, so
                 // only lanes that are done can leave a member unused: each
@@ -392,15 +398,15 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
                     cg = con ? fmaf(x4.w, vis, cg) : cg;
                     cb = con ? fmaf(bl, vis, cb) : cb;
                 } else {
-                    // `f32 as u32` of the weight: on a contributing lane
-                    // 0 < vis <= 0.999 and scale <= 2^32, so the product lies
-                    // in (0, 2^32) and truncates exactly as the saturating
-                    // conversion would.
-                    q[s] = con ? (uint32_t)(vis * J.scale) : 0u;
+                    // `f32 as u32` of the weight (alpha T) scale = alpha Ts: on
+                    // a contributing lane it lies in (0, 0.999 * 2^32], where
+                    // the conversion truncates exactly as the saturating one
+                    // would.
+                    q[s] = con ? (uint32_t)vis : 0u;
                 }
                 T = con ? T * (1.0f - alpha) : T;
                 last = con ? chunk + jj[s] + 1 : last;
-                done = done || (con && T < kInv255);
+                done = done || (con && T < tdone);
             }
             if constexpr (MODE != kRgb) {
                 // The u32 weights of the 8 members are summed over the wave
@@ -444,7 +450,7 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
 
     if (in_bounds) {
         const size_t p = (size_t)py * J.w + px;
-        if (a.final_t) a.final_t[p] = T;
+        if (a.final_t) a.final_t[p] = MODE == kRgb ? T : T * (1.0f / tscale);
         if (a.n_contrib) a.n_contrib[p] = last;
         if constexpr (MODE == kRgb) {
             if (a.rgba) reinterpret_cast<float4*>(a.rgba)[p] = make_float4(cr, cg, cb, 1.0f - T);

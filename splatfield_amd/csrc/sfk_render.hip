@@ -683,11 +683,21 @@ extern "C" int sfk_finalize_rgb(sfk_ctx* c, sfk_render_scratch* s, const sfk_til
     return SFK_OK;
 }
 
+// The accumulators' fixed-point scale: 2^k for k = fractional_bits(w, h) in
+// [0, 32] (seg/mod.rs:31-34, 178).  The rasterizers rely on it being a power
+// of two (they carry T pre-scaled, exactly).
+static bool fixed_point_scale(float scale) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, scale);
+    return (u & 0x807fffffu) == 0 && scale >= 1.0f && scale <= 4294967296.0f;
+}
+
 extern "C" int sfk_finalize_total(sfk_ctx* c, sfk_render_scratch* s, const sfk_tile_isects* t,
                                   float scale, uint32_t* resp) {
     int rc = check_isects(c, s, t);
     if (rc) return rc;
     if (!resp) return fail(c, SFK_E_ARG, "finalize_total: null resp");
+    if (!fixed_point_scale(scale))
+        return fail(c, SFK_E_ARG, "finalize_total: scale %g is not 2^k, k in [0, 32]", scale);
     RasterArgs a = raster_args(s, t);
     a.scale = scale;
     a.resp = resp;
@@ -703,6 +713,8 @@ extern "C" int sfk_finalize_evidence(sfk_ctx* c, sfk_render_scratch* s, const sf
     if (rc) return rc;
     if (!mask || !fg || !bg || words_per_row < (t->w + 31) / 32)
         return fail(c, SFK_E_ARG, "finalize_evidence: bad mask/fg/bg");
+    if (!fixed_point_scale(scale))
+        return fail(c, SFK_E_ARG, "finalize_evidence: scale %g is not 2^k, k in [0, 32]", scale);
     RasterArgs a = raster_args(s, t);
     a.scale = scale;
     a.mask = mask;
