@@ -229,6 +229,10 @@ struct sfk_eig_round {
     // supertile's sorted pairs (no per-tile lists are placed).
     bool pair_raster = true;
     const uint2* pairs = nullptr;
+    // Scratch of the direct pair emission (views of at most kPairMaxDigits
+    // supertiles).
+    uint32_t* pair_table = nullptr;
+    uint32_t* pair_table_sums = nullptr;
     uint32_t supers = 0, bin_shift = 1;  // supertile ids, bits of the local ones
     BinView* d_binviews = nullptr;
     uint32_t* offsets = nullptr;  // isect offsets of truncation candidates, by view-major splat
@@ -337,6 +341,8 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
             j.status_zeroed = true;
             j.pairs_only = r->pair_raster;
             j.pairs_out = &r->pairs;
+            j.pair_table = r->pair_table;
+            j.pair_table_sums = r->pair_table_sums;
             rc = bin_emit(c, j);
             if (rc) return rc;
             r->sorted_tiles = nullptr;
@@ -523,6 +529,10 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
         RALLOC(r->scan_sums, (size_t)div_up((uint64_t)r->max_tiles * nviews, kScanBlock) * 4);
         if ((uint64_t)r->max_tiles * nviews <= kTileOrderMax)
             RALLOC(r->tile_order, (size_t)r->max_tiles * nviews * 4);
+        if (max_supers <= kPairMaxDigits) {
+            RALLOC(r->pair_table, (size_t)pair_table_words(n, nviews, max_supers) * 4);
+            RALLOC(r->pair_table_sums, (size_t)pair_table_sum_words(n, nviews, max_supers) * 4);
+        }
     } else {
         RALLOC(r->counts, vn * 4);
         RALLOC(r->bbox, vn * 8);
@@ -568,7 +578,7 @@ extern "C" void sfk_eig_round_destroy(sfk_eig_round* r) {
     void* bufs[] = {r->d_views, r->d_bviews, r->proj, r->proj_c, r->opac, r->keys, r->counts, r->bbox, r->cbbox,
                     r->counters, r->starts, r->resp, r->h0, r->partials, r->eig, r->best,
                     r->emit_status, r->d_binviews, r->offsets, r->bin_sstart, r->scan_sums,
-                    r->d_rebase, r->tile_order};
+                    r->d_rebase, r->tile_order, r->pair_table, r->pair_table_sums};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (r->host_out) hipHostFree(r->host_out);

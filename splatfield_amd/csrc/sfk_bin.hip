@@ -798,6 +798,8 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
         return fail(c, SFK_E_ARG, "binned emission: pair sort scratch");
     if ((j.seg_len ? j.seg_len : j.n) > 0x7fffffffu || !j.offs_by_splat)
         return fail(c, SFK_E_ARG, "binned emission: splat ids need 31 bits");
+    if (j.pairs_only && j.pair_table && j.supers / j.nviews <= kPairMaxDigits)
+        return bin_pairs_direct(c, j, j.supers / j.nviews);
     if (!j.status_zeroed)
         SFK_HIP(c, launch_fill_u32(st, j.status, 0u, scan_emit_status_words(j.n)));
     {

@@ -288,8 +288,18 @@ struct BinJob {
     // scan_sums, tile_starts, tile_order, gids and tile_ids are unused.
     bool pairs_only;
     const uint2** pairs_out;
+    // Pairs only, optional: with at most kPairMaxDigits local supertiles per
+    // view, the pairs are written straight into supertile order (no look-back,
+    // pair sort or ranges pass; sfk_bin_pairs.hip) using these
+    // pair_table_words / pair_table_sum_words words of scratch.
+    uint32_t* pair_table;
+    uint32_t* pair_table_sums;
 };
 int bin_emit(sfk_ctx* c, const BinJob& j);
+constexpr uint32_t kPairMaxDigits = 256;
+uint32_t pair_table_words(uint32_t n_per_view, uint32_t views, uint32_t digits);
+uint32_t pair_table_sum_words(uint32_t n_per_view, uint32_t views, uint32_t digits);
+int bin_pairs_direct(sfk_ctx* c, const BinJob& j, uint32_t digits);
 
 // Raster stage.
 hipError_t launch_map_isects(hipStream_t s, const uint32_t* order, const uint32_t* tile_counts,

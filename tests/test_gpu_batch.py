@@ -147,12 +147,14 @@ def test_batch_binned_matches_sorted(ctx, tile_w, pair_raster):
     np.testing.assert_array_equal(c_b, c_s)
 
 
-def test_batch_truncated_round(ctx):
+@pytest.mark.parametrize("n", [1 << 16, 1 << 20])
+def test_batch_truncated_round(ctx, n):
     """A round whose isects exceed its capacity (an eager run before any
     capacity fit): the raster straight from the pairs drops exactly the
     isects whose emission slot reaches the capacity, as the placed lists and
-    the emission + tile argsort do (raster.rs:12-48)."""
-    n = 1 << 16
+    the emission + tile argsort do (raster.rs:12-48).  At 2^20 splats the
+    (supertile, splat) pairs exceed the capacity too, and only those at
+    rank-order slots below it are kept."""
     attrs, sh, k = S.synthetic_scene(n, 0, seed=8)
     attrs = attrs.reshape(S.ATTR_PLANES, n)
     attrs[7:10] += np.float32(2.0)  # large splats: ~10^8 isects at 8 px tiles
@@ -169,6 +171,20 @@ def test_batch_truncated_round(ctx):
         rnd.run(state, use_graph=False)
         c = rnd.counters()
         assert c[-1] > rnd.capacity(), "the round must truncate"
+        if n == 1 << 20 and mode == "pairs":
+            # (supertile, splat) pairs of the two views, from the oracle's
+            # tile bboxes: more than the capacity holds.
+            pairs = 0
+            for cam, (w, h) in zip(cams, sizes):
+                view = O.View.from_buffer_copy(bytes(cam.view(w, h, 8)))
+                bb = O.project(view, attrs, sh, n, k, 8, nthreads=8)["bbox"].reshape(-1, 2)
+                x0, y0 = bb[:, 0] & 0xffff, bb[:, 0] >> 16
+                x1, y1 = bb[:, 1] & 0xffff, bb[:, 1] >> 16
+                live = (x1 > x0) & (y1 > y0)
+                sx = (x1[live] - 1) // 8 - x0[live] // 8 + 1
+                sy = (y1[live] - 1) // 8 - y0[live] // 8 + 1
+                pairs += int((sx.astype(np.int64) * sy).sum())
+            assert pairs > rnd.capacity(), "the pairs must exceed the capacity"
         words[mode] = [rnd.resp(v)[0] for v in range(len(cams))]
         del rnd
     for mode in ("placed", "pairs"):
