@@ -286,11 +286,16 @@ __global__ __launch_bounds__(256) void k_pair_starts(PairArgs a) {
 template <uint32_t DB>
 __global__ __launch_bounds__(kThreads) void k_pair_write(PairArgs a) {
     constexpr uint32_t kOwnWin = 512;  // pair slots per owner window of a wave
+    // LDS kept below 80 KB (two workgroups per CU): the per-wave supertile
+    // positions sized for this instantiation's 2^DB supertiles, the spans in
+    // 16 bits with the reciprocals of their widths in a table.
+    constexpr uint32_t DS = 1u << DB;
     __shared__ uint32_t s_incl[kRanks];  // inclusive kept-pair prefix within the wave's ranks
-    __shared__ uint32_t s_span[kRanks];  // d0 | swx << 8 | span_inv(swx) << 14
+    __shared__ uint16_t s_span[kRanks];  // d0 | swx << 8
     __shared__ uint2 s_val[kRanks];      // pair value: (splat | may-truncate << 31, bbox)
     __shared__ uint8_t s_own[kWaves][kOwnWin];
-    __shared__ uint32_t s_wc[kWaves][kPairMaxDigits];  // per wave and supertile: next position
+    __shared__ uint32_t s_wc[kWaves][DS];  // per wave and supertile: next position
+    __shared__ uint32_t s_inv[33];         // span_inv(swx), swx in [1, 32]
     __shared__ uint32_t s_tmp[kWaves + 1];
     const uint32_t t = threadIdx.x, lane = lane_id(), wv = t / 64, w = blockIdx.x;
     uint32_t v, r0, r1;
@@ -301,7 +306,8 @@ __global__ __launch_bounds__(kThreads) void k_pair_write(PairArgs a) {
 #pragma unroll
     for (uint32_t k = 0; k < kRpt; ++k)
         oc[k] = rb + k < r1 ? a.ordered[rb + k] : make_uint2(0u, 0u);
-    for (uint32_t i = t; i < kWaves * kPairMaxDigits; i += kThreads) (&s_wc[0][0])[i] = 0;
+    for (uint32_t i = t; i < kWaves * DS; i += kThreads) (&s_wc[0][0])[i] = 0;
+    if (t >= 1 && t <= 32) s_inv[t] = span_inv(t);
     if (a.order_out) {
 #pragma unroll
         for (uint32_t k = 0; k < kRpt; ++k)
@@ -333,7 +339,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_write(PairArgs a) {
         kept[k] = slot < a.cap ? min(s[k].sc, a.cap - slot) : 0u;
         ck += kept[k];
         s_val[t * kRpt + k] = make_uint2(oc[k].x | (cut ? 0x80000000u : 0u), oc[k].y);
-        s_span[t * kRpt + k] = s[k].d0 | (s[k].swx << 8) | (span_inv(s[k].swx) << 14);
+        s_span[t * kRpt + k] = (uint16_t)(s[k].d0 | (s[k].swx << 8));
         irun += s[k].cnt;
         prun += s[k].sc;
     }
@@ -383,7 +389,8 @@ __global__ __launch_bounds__(kThreads) void k_pair_write(PairArgs a) {
                 own = s_own[wv][e - w0];
                 const uint32_t j = e - (own ? wincl[own - 1] : 0u);
                 const uint32_t sp = s_span[first + own];
-                d = pair_digit(sp & 0xffu, (sp >> 8) & 63u, sp >> 14, vw.supers_x, j);
+                const uint32_t swx = sp >> 8;
+                d = pair_digit(sp & 0xffu, swx, s_inv[swx], vw.supers_x, j);
             }
             const uint64_t peers = match_peers<DB>(d, live);
             const uint32_t base = live ? wc[d] : 0u;
