@@ -441,8 +441,11 @@ def main():
 
     splats = R.Splats(ctx, attrs, sh)
     scratch = R.RenderScratch(ctx, n, w, h)
-    # The frame reads tile ranges and splat ids only: no per-isect tile ids.
+    # The frame never reads the isect arrays: no per-isect tile ids, and the
+    # rasterizers take each tile's isects from the sorted supertile pairs,
+    # placing only the ones they consume (the backward reads those).
     scratch.set_tile_ids(False)
+    scratch.set_pair_raster(True)
     loss = GpuTensor.empty(ctx, 1, np.float64)
     g_attrs = GpuTensor.empty(ctx, n * 11, np.float32)
     g_sh = GpuTensor.empty(ctx, n * 3 * k, np.float32)
@@ -514,8 +517,9 @@ def main():
     b_bwd_ref = 40 * ni + 32 * px + 36 * vis + 2 * (n * (44 + 4 * 3 * k)) + 36 * vis
     frame_gbs = (b_fwd_ref + b_bwd_ref) / (ms_per_step * 1e-3) / 1e9
     # The same for the reads and writes this pipeline must make.
+    # (No placement pass: the rasterizer reads the sorted pairs instead.)
     b_fwd = (kb["project"] + depth_passes * 28 * n + 28 * pairs + kb["bin_scan"] +
-             kb["bin_count"] + kb["bin_place"] + kb["raster_rgb"])
+             kb["bin_count"] + 8 * pairs + kb["raster_rgb"])
     b_bwd = 40 * ni + 32 * px + 36 * vis + n * (44 + 4 * 3 * k) + 36 * vis + n * (44 + 4 * 3 * k)
     step_gbs = (b_fwd + b_bwd) / (ms_per_step * 1e-3) / 1e9
     stage("copy ceiling: 256 MiB device copy")

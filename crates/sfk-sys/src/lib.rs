@@ -270,6 +270,7 @@ unsafe extern "C" {
     pub fn sfk_render_scratch_set_digit_bits(s: *mut sfk_render_scratch, digit_bits: u32) -> c_int;
     pub fn sfk_render_scratch_set_binned(s: *mut sfk_render_scratch, on: c_int) -> c_int;
     pub fn sfk_render_scratch_set_tile_ids(s: *mut sfk_render_scratch, on: c_int) -> c_int;
+    pub fn sfk_render_scratch_set_pair_raster(s: *mut sfk_render_scratch, on: c_int) -> c_int;
     pub fn sfk_prepare_isects(
         ctx: *mut sfk_ctx,
         splats: *const sfk_splats,

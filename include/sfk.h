@@ -190,6 +190,17 @@ int sfk_render_scratch_set_binned(sfk_render_scratch* s, int on);
  * The sorting emission always has them (they are its sort keys).  No
  * counterpart in the reference, whose emission sorts by tile id. */
 int sfk_render_scratch_set_tile_ids(sfk_render_scratch* s, int on);
+/* Pair raster (off by default; binned emission only).  The emission stops
+ * after the tile counts, tile_starts and the tile order, without placing
+ * the isects; each rasterizer (finalize in any mode) takes a tile's isects
+ * straight from its supertile's depth-ordered (splat, bbox) pairs, and
+ * writes the ones it consumes -- every isect up to the tile's last
+ * contributor -- into gaussian_ids at their slots.  Images, final T,
+ * n_contrib, evidence words and the backward pass are unchanged;
+ * sfk_tile_isects.gaussian_ids then holds only those consumed isects and
+ * tile_ids is NULL.  For a caller that never reads the isect arrays (a
+ * training loop).  No counterpart in the reference. */
+int sfk_render_scratch_set_pair_raster(sfk_render_scratch* s, int on);
 
 /* TileIsects (src/render.rs:463-471) plus the device-side counters. */
 typedef struct {

@@ -96,6 +96,7 @@ SIGNATURES = {
     "sfk_render_scratch_set_digit_bits": (i32, [P, u32]),
     "sfk_render_scratch_set_binned": (i32, [P, i32]),
     "sfk_render_scratch_set_tile_ids": (i32, [P, i32]),
+    "sfk_render_scratch_set_pair_raster": (i32, [P, i32]),
     "sfk_prepare_isects": (i32, [P, P, P, C.POINTER(Camera), u32, u32, C.POINTER(TileIsects)]),
     "sfk_prepare_isects_view": (i32, [P, P, P, C.POINTER(View), u32, u32,
                                       C.POINTER(TileIsects)]),

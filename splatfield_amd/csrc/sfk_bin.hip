@@ -476,21 +476,22 @@ __device__ __forceinline__ uint64_t class_peers(uint32_t k, bool valid) {
 // prefixes, total at [n]) heaviest work class first, by a counting sort on
 // work_class of their isect counts (order inside a class unspecified).  One
 // 256-thread workgroup; hist holds 65 words of LDS.
+template <uint32_t NT = 256>
 __device__ void tile_order_block(const uint32_t* __restrict__ starts, uint32_t n,
                                  uint32_t* __restrict__ order, uint32_t* hist) {
-    constexpr uint32_t U = 8;  // tiles per thread whose counts load together
-    for (uint32_t i = threadIdx.x; i < 65; i += 256) hist[i] = 0;
+    constexpr uint32_t U = NT >= 1024 ? 2 : 8;  // tiles per thread whose counts load together
+    for (uint32_t i = threadIdx.x; i < 65; i += NT) hist[i] = 0;
     __syncthreads();
-    for (uint32_t t0 = 0; t0 < n; t0 += 256 * U) {
+    for (uint32_t t0 = 0; t0 < n; t0 += NT * U) {
         uint32_t k[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
-            const uint32_t t = t0 + u * 256 + threadIdx.x;
+            const uint32_t t = t0 + u * NT + threadIdx.x;
             k[u] = t < n ? starts[t + 1] - starts[t] : 0u;
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
-            const uint32_t t = t0 + u * 256 + threadIdx.x;
+            const uint32_t t = t0 + u * NT + threadIdx.x;
             const bool valid = t < n;
             const uint32_t kc = work_class(k[u]);
             const uint64_t peers = class_peers(kc, valid);
@@ -507,16 +508,16 @@ __device__ void tile_order_block(const uint32_t* __restrict__ starts, uint32_t n
         }
     }
     __syncthreads();
-    for (uint32_t t0 = 0; t0 < n; t0 += 256 * U) {
+    for (uint32_t t0 = 0; t0 < n; t0 += NT * U) {
         uint32_t k[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
-            const uint32_t t = t0 + u * 256 + threadIdx.x;
+            const uint32_t t = t0 + u * NT + threadIdx.x;
             k[u] = t < n ? starts[t + 1] - starts[t] : 0u;
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
-            const uint32_t t = t0 + u * 256 + threadIdx.x;
+            const uint32_t t = t0 + u * NT + threadIdx.x;
             const bool valid = t < n;
             const uint32_t kc = work_class(k[u]);
             const uint64_t peers = class_peers(kc, valid);
@@ -732,11 +733,17 @@ __device__ void block_scan_small(uint32_t* __restrict__ buf, uint32_t n) {
 // chunk counts over the supertile's chunks (in place), and the tile total
 // into tile_starts.  The four waves split the chunks; their partial sums meet
 // in LDS.
+// With class_count (65 words, zero on entry) and tile_cls (per tile), it
+// also counts its tiles' work classes, one device atomic per distinct class
+// of the supertile, and gives each tile its class and index within the
+// class (class << 24 | index; the order inside a class is the atomics').
 __global__ __launch_bounds__(256) void k_bin_chunk_scan(const uint32_t* __restrict__ sstart,
                                                         const BinView* __restrict__ views,
                                                         BinView view0, uint32_t nviews,
                                                         uint32_t* __restrict__ counts,
-                                                        uint32_t* __restrict__ tile_starts) {
+                                                        uint32_t* __restrict__ tile_starts,
+                                                        uint32_t* __restrict__ class_count,
+                                                        uint32_t* __restrict__ tile_cls) {
     __shared__ uint32_t s_part[4][64];
     const uint32_t s = blockIdx.x;
     const uint32_t lane = threadIdx.x & 63u, g = threadIdx.x >> 6;
@@ -769,8 +776,18 @@ __global__ __launch_bounds__(256) void k_bin_chunk_scan(const uint32_t* __restri
         const uint32_t ls = s / nviews;
         const uint32_t sy = ls / vw.supers_x, sx = ls - sy * vw.supers_x;
         const uint32_t tx = sx * kBinSuper + (lane & 7u), ty = sy * kBinSuper + (lane >> 3);
-        if (tx < vw.tiles_x && ty < vw.tiles_y)
-            tile_starts[(ty * vw.tiles_x + tx) * vw.out_mul + vw.out_add] = tot;
+        const bool live = tx < vw.tiles_x && ty < vw.tiles_y;
+        const uint32_t slot = (ty * vw.tiles_x + tx) * vw.out_mul + vw.out_add;
+        if (live) tile_starts[slot] = tot;
+        if (class_count) {
+            const uint32_t kc = work_class(tot);
+            const uint64_t peers = class_peers(kc, live);
+            const uint32_t rank = mask_rank(peers);
+            uint32_t base = 0;
+            if (live && rank == 0) base = atomicAdd(&class_count[kc], (uint32_t)__popcll(peers));
+            base = __shfl(base, live ? __ffsll((unsigned long long)peers) - 1 : 0, 64);
+            if (live) tile_cls[slot] = (kc << 24) | (base + rank);
+        }
     }
 }
 
@@ -782,6 +799,32 @@ __global__ __launch_bounds__(kSmallScanThreads) void k_scan_small(uint32_t* __re
     block_scan_small<kSmallScanThreads>(buf, n);
 }
 
+// The same scan of the tile counts into tile_starts, then the rasterizers'
+// heaviest-first tile order (when no placement pass writes it) from the
+// classes the chunk scan counted: class starts heaviest first (one wave),
+// then every tile to its class start + its index within the class.  The
+// class counts are left zero for the next prepare.
+__global__ __launch_bounds__(kSmallScanThreads) void k_scan_small_order(
+    uint32_t* __restrict__ buf, uint32_t n, uint32_t* __restrict__ order,
+    uint32_t* __restrict__ class_count, const uint32_t* __restrict__ tile_cls) {
+    __shared__ uint32_t s_base[65];
+    if (threadIdx.x < 64) {
+        // lane l: class 64 - l; class 0 (empty tiles) last
+        const uint32_t l = threadIdx.x;
+        const uint32_t c = class_count[64 - l];
+        const uint32_t inc = wave_incl_scan_u32(c);
+        s_base[64 - l] = inc - c;
+        if (l == 63) s_base[0] = inc;
+        class_count[64 - l] = 0;
+        if (l == 63) class_count[0] = 0;
+    }
+    block_scan_small<kSmallScanThreads>(buf, n);  // (its barriers order s_base)
+    for (uint32_t i = threadIdx.x; i < n; i += kSmallScanThreads) {
+        const uint32_t v = tile_cls[i];
+        order[s_base[v >> 24] + (v & 0xffffffu)] = i;
+    }
+}
+
 }  // namespace
 
 int bin_emit(sfk_ctx* c, const BinJob& j) {
@@ -790,7 +833,8 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     if (j.n == 0 || j.cap < 2 || j.cap > kBinMaxCap || j.supers == 0 || j.supers >= (1u << 24) ||
         j.nviews == 0 || j.nviews > 255 || j.view_shift == 0 || j.view_shift > 10 ||
         j.supers % j.nviews || !j.counters || !j.status || !j.sstart ||
-        (j.pairs_only ? !j.pairs_out : (!j.gids || !j.tile_starts)))
+        (j.pairs_only ? !j.pairs_out : (!j.gids || !j.tile_starts)) ||
+        (j.skip_place && !j.pairs_out))
         return fail(c, SFK_E_ARG, "binned emission: bad job");
     uint32_t* pair_keys = nullptr;
     uint2* pair_vals = nullptr;
@@ -890,16 +934,33 @@ int bin_emit(sfk_ctx* c, const BinJob& j) {
     {
         SFK_PROF(c, "bin_tile_scan");
         if (j.zero_starts) SFK_HIP(c, launch_fill_u32(st, j.tile_starts, 0u, j.out_tiles));
+        // Without a placement pass, the chunk scan counts the tile classes
+        // and the tile scan writes the heaviest-first order.
+        const bool order_here = j.skip_place && j.tile_order && j.class_count &&
+                                j.out_tiles <= kSmallScanMax && j.out_tiles <= kTileOrderMax;
         k_bin_chunk_scan<<<j.supers, 256, 0, st>>>(j.sstart, j.views, j.view0, j.nviews,
-                                                    j.chunk_counts, j.tile_starts);
+                                                    j.chunk_counts, j.tile_starts,
+                                                    order_here ? j.class_count : nullptr,
+                                                    j.tile_cls);
         SFK_LAUNCHED(c, "bin chunk scan");
-        if (j.out_tiles <= kSmallScanMax) {
+        if (order_here) {
+            k_scan_small_order<<<1, kSmallScanThreads, 0, st>>>(j.tile_starts, j.out_tiles,
+                                                                 j.tile_order, j.class_count,
+                                                                 j.tile_cls);
+            SFK_LAUNCHED(c, "bin tile scan");
+        } else if (j.out_tiles <= kSmallScanMax) {
             k_scan_small<<<1, kSmallScanThreads, 0, st>>>(j.tile_starts, j.out_tiles);
             SFK_LAUNCHED(c, "bin tile scan");
         } else {
             SFK_HIP(c, launch_scan(st, nullptr, j.tile_starts, j.tile_starts, j.out_tiles,
                                    j.scan_sums, 0, j.tile_starts + j.out_tiles));
         }
+    }
+    if (j.skip_place) {
+        // The rasterizers take each tile's isects from the sorted pairs
+        // themselves (and write back the ones they consume).
+        *j.pairs_out = sv;
+        return SFK_OK;
     }
     {
         SFK_PROF(c, "bin_place");

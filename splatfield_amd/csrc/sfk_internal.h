@@ -294,6 +294,14 @@ struct BinJob {
     // pair_table_words / pair_table_sum_words words of scratch.
     uint32_t* pair_table;
     uint32_t* pair_table_sums;
+    // Skip the placement pass: counts, tile starts and the tile order as
+    // usual, *pairs_out the sorted pairs; the rasterizers place what they
+    // consume (RasterArgs::gids_out).
+    bool skip_place;
+    // Skip place with a tile order: 65 words of class counts (zero before the
+    // first prepare; each emission leaves them zero) and one word per tile.
+    uint32_t* class_count;
+    uint32_t* tile_cls;
 };
 int bin_emit(sfk_ctx* c, const BinJob& j);
 constexpr uint32_t kPairMaxDigits = 256;
@@ -349,6 +357,10 @@ struct RasterArgs {
     const uint32_t* pair_starts;
     const uint32_t* offs_by_splat;
     uint32_t cap;
+    // One view, pair source: the rasterizers write the isects they consume
+    // to gids_out at their slots of the tile ranges (tile_starts), so that a
+    // backward pass finds every isect up to each pixel's last contributor.
+    uint32_t* gids_out;
 };
 hipError_t launch_rasterize(hipStream_t s, int mode, const RasterArgs& a);
 hipError_t launch_rasterize_batch(hipStream_t s, const RasterArgs& a);

@@ -238,7 +238,7 @@ __device__ __forceinline__ uint32_t stage_pairs(const TileJob& J, const RasterAr
 // and costs occupancy otherwise.
 template <int MODE, int TW, bool PF, bool PAIRS = false>
 __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& a) {
-    static_assert(!PAIRS || MODE == kTotal, "the pair source serves the batched total mode");
+
     constexpr int NT = TW * TW;
     constexpr int NW = NT / 64;
     __shared__ float4 s_geo[NT];  // mx, my, conic a, conic b
@@ -247,7 +247,7 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
     __shared__ float s_tau[NT];   // power bound of the ellipse test
     __shared__ uint16_t s_list[NW][NT];
     __shared__ float s_blue[MODE == kRgb ? NT : 1];
-    __shared__ uint32_t s_gid[MODE == kRgb ? 1 : NT];
+    __shared__ uint32_t s_gid[MODE == kRgb && !PAIRS ? 1 : NT];
     __shared__ uint32_t s_q0[MODE == kRgb ? 1 : NT];
     __shared__ uint32_t s_q1[MODE == kEvidence ? NT : 1];
     __shared__ uint64_t s_mask[PAIRS ? 2 * pair_rounds<NT>() * NW : 1];
@@ -287,6 +287,9 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
             if (__syncthreads_count(!done) == 0) break;
             cnt = stage_pairs<NT>(J, a, pnext, s_gid, s_mask, &s_pnext);
             if (cnt == 0) break;
+            // Every chunk but the last holds NT isects, so `chunk` is the
+            // slot of this one's first in the tile's range.
+            if (a.gids_out && t < cnt) a.gids_out[chunk + t] = s_gid[t];
         } else {
             __syncthreads();  // previous chunk fully consumed
             cnt = (end - chunk) < (uint32_t)NT ? (end - chunk) : (uint32_t)NT;
@@ -465,7 +468,7 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
     }
 }
 
-template <int MODE, int TW>
+template <int MODE, int TW, bool PAIRS>
 __global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
     const uint32_t ntiles = a.tiles_x * a.tiles_y;
     const uint32_t tile = a.tile_order ? a.tile_order[blockIdx.x] : xcd_remap(blockIdx.x, ntiles);
@@ -483,7 +486,15 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize(RasterArgs a) {
     J.opac = nullptr;
     J.obase = 0;
     J.resp = a.resp; J.fg = a.fg; J.bg = a.bg;
-    raster_tile<MODE, TW, MODE != kRgb>(J, a);
+    J.p0 = J.p1 = 0;
+    J.gid_add = 0;
+    if constexpr (PAIRS) {
+        const uint32_t ls =
+            (J.ty / kBinSuper) * ((a.tiles_x + kBinSuper - 1) / kBinSuper) + J.tx / kBinSuper;
+        J.p0 = a.pair_starts[ls];
+        J.p1 = a.pair_starts[ls + 1];
+    }
+    raster_tile<MODE, TW, MODE != kRgb, PAIRS>(J, a);
 }
 
 // Batched responsibility: one launch over the tiles of every view, from the
@@ -548,17 +559,22 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize_batch(RasterArgs a) {
     raster_tile<kTotal, TW, false, PAIRS>(J, b);
 }
 
-template <int MODE>
+template <int MODE, bool PAIRS>
 hipError_t raster_tw(hipStream_t s, const RasterArgs& a) {
     uint32_t nt = a.tiles_x * a.tiles_y;
     if (nt == 0) return hipSuccess;
     switch (a.tile_w) {
-        case 8: k_rasterize<MODE, 8><<<nt, 64, 0, s>>>(a); break;
-        case 16: k_rasterize<MODE, 16><<<nt, 256, 0, s>>>(a); break;
-        case 32: k_rasterize<MODE, 32><<<nt, 1024, 0, s>>>(a); break;
+        case 8: k_rasterize<MODE, 8, PAIRS><<<nt, 64, 0, s>>>(a); break;
+        case 16: k_rasterize<MODE, 16, PAIRS><<<nt, 256, 0, s>>>(a); break;
+        case 32: k_rasterize<MODE, 32, PAIRS><<<nt, 1024, 0, s>>>(a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t raster_src(hipStream_t s, const RasterArgs& a) {
+    return a.pairs ? raster_tw<MODE, true>(s, a) : raster_tw<MODE, false>(s, a);
 }
 
 }  // namespace
@@ -586,9 +602,9 @@ hipError_t launch_tile_starts(hipStream_t s, const uint32_t* tile_ids, uint32_t 
 
 hipError_t launch_rasterize(hipStream_t s, int mode, const RasterArgs& a) {
     switch (mode) {
-        case kRgb: return raster_tw<kRgb>(s, a);
-        case kTotal: return raster_tw<kTotal>(s, a);
-        case kEvidence: return raster_tw<kEvidence>(s, a);
+        case kRgb: return raster_src<kRgb>(s, a);
+        case kTotal: return raster_src<kTotal>(s, a);
+        case kEvidence: return raster_src<kEvidence>(s, a);
         default: return hipErrorInvalidValue;
     }
 }

@@ -233,10 +233,21 @@ struct sfk_render_scratch {
     // tile counts (the last two sized by the isect capacity).
     bool binned = true;
     bool want_tile_ids = true;  // the binned emission also writes tile_ids
+    // Pair raster: the binned emission stops before placing the isects; the
+    // rasterizers take each tile's isects from its supertile's sorted pairs
+    // and write back the ones they consume (pairs_ready while the last
+    // prepare left its pairs for them).
+    bool pair_raster = false;
+    bool pairs_ready = false;
+    const uint2* pairs = nullptr;
     uint32_t* bin_sstart = nullptr;
     uint32_t* bin_map = nullptr;
     uint32_t* bin_counts = nullptr;
     uint32_t* bin_offs = nullptr;  // isect offsets of truncation candidates, by splat
+    // Pair raster: work-class counts (65 words, kept zero between prepares)
+    // and per-tile class + index, from which the tile order is written.
+    uint32_t* class_count = nullptr;
+    uint32_t* tile_cls = nullptr;
     // d colour / d view direction per splat (9 planes), kept by the RGB
     // projection once a backward has run, so the next adjoint reads 9 floats
     // per splat instead of the SH coefficients.
@@ -254,7 +265,7 @@ static void scratch_free_frame(sfk_render_scratch* s) {
                     s->counters, s->tile_starts, s->bitmap, s->rgba, s->final_t, s->n_contrib,
                     s->tile_ids, s->gaussian_ids, s->grad2d, s->emit_status, s->cbbox,
                     s->order_buf, s->bin_sstart, s->bin_map, s->bin_counts, s->bin_offs,
-                    s->dcol, s->tile_order};
+                    s->dcol, s->tile_order, s->class_count, s->tile_cls};
     for (void* p : bufs)
         if (p) hipFree(p);
     s->depth_keys = nullptr; s->projected = nullptr; s->tile_counts = nullptr; s->bbox = nullptr;
@@ -263,11 +274,13 @@ static void scratch_free_frame(sfk_render_scratch* s) {
     s->gaussian_ids = nullptr;
     s->tile_order = nullptr;
     s->order_valid = false;
+    s->pairs_ready = false;
     s->grad2d = nullptr;
     s->grad2d_dirty = true;
     s->emit_status = nullptr;
     s->cbbox = s->order_buf = nullptr;
     s->bin_sstart = s->bin_map = s->bin_counts = s->bin_offs = nullptr;
+    s->class_count = s->tile_cls = nullptr;
     s->dcol = nullptr;
     s->has_dcol = false;
     s->has_view = false;
@@ -339,6 +352,9 @@ static int scratch_build(sfk_render_scratch* s, uint32_t n, uint32_t w, uint32_t
         SFK_HIP(c, hipMalloc(&s->bin_sstart,
                              ((size_t)bin_supers(s->tiles_x, s->tiles_y) + 1) * 4));
         SFK_HIP(c, hipMalloc(&s->bin_offs, nn * 4));
+        SFK_HIP(c, hipMalloc(&s->class_count, 65 * 4));
+        SFK_HIP(c, hipMemsetAsync(s->class_count, 0, 65 * 4, c->stream));
+        SFK_HIP(c, hipMalloc(&s->tile_cls, (size_t)tiles * 4));
     }
     uint64_t cap = std::min<uint64_t>(tiles * n, kInitialIsectsCap);
     return alloc_isects(s, next_pow2(cap ? cap : 1));
@@ -396,6 +412,12 @@ extern "C" int sfk_render_scratch_set_binned(sfk_render_scratch* s, int on) {
 extern "C" int sfk_render_scratch_set_tile_ids(sfk_render_scratch* s, int on) {
     if (!s) return SFK_E_ARG;
     s->want_tile_ids = on != 0;
+    return SFK_OK;
+}
+
+extern "C" int sfk_render_scratch_set_pair_raster(sfk_render_scratch* s, int on) {
+    if (!s) return SFK_E_ARG;
+    s->pair_raster = on != 0;
     return SFK_OK;
 }
 
@@ -499,6 +521,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
     const uint32_t tiles = s->tiles_x * s->tiles_y;
     const bool compact = compact_grid(s);
     s->order_valid = false;
+    s->pairs_ready = false;
 
     // Counters (and, for the binned emission, its look-back words) start at
     // 0: cleared here, or left clear by the last binned prepare.
@@ -562,7 +585,11 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
             j.tile_starts = s->tile_starts;
             j.tile_order = s->tile_order;
             j.gids = s->gaussian_ids;
-            j.tile_ids = s->want_tile_ids ? s->tile_ids : nullptr;
+            j.tile_ids = s->want_tile_ids && !s->pair_raster ? s->tile_ids : nullptr;
+            j.skip_place = s->pair_raster;
+            j.pairs_out = &s->pairs;
+            j.class_count = s->class_count;
+            j.tile_cls = s->tile_cls;
             j.counters = ctr;
             j.status = s->emit_status;
             j.status_zeroed = true;
@@ -572,6 +599,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
             j.host_total = s->host_counters_dev;  // the raw count, for next frame's capacity
             rc = bin_emit(c, j);
             if (rc) return rc;
+            s->pairs_ready = s->pair_raster;
             s->bank = bank ^ 1u;
             s->emit_clean = true;
             s->order_valid = s->tile_order != nullptr;
@@ -658,6 +686,13 @@ static RasterArgs raster_args(sfk_render_scratch* s, const sfk_tile_isects* t) {
     a.tile_order = s->order_valid ? s->tile_order : nullptr;
     a.gids = t->gaussian_ids;
     a.proj = s->projected;
+    if (s->pairs_ready) {
+        a.pairs = s->pairs;
+        a.pair_starts = s->bin_sstart;
+        a.offs_by_splat = s->bin_offs;
+        a.cap = s->capacity;
+        a.gids_out = s->gaussian_ids;
+    }
     return a;
 }
 

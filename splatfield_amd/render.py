@@ -288,6 +288,13 @@ class RenderScratch:
         self.ctx.check(lib().sfk_render_scratch_set_tile_ids(self.ptr, 1 if on else 0),
                        "tile ids")
 
+    def set_pair_raster(self, on):
+        """Pair raster (default off): the rasterizers take each tile's isects
+        from its supertile's sorted pairs and write back only the ones they
+        consume; images, evidence and gradients are unchanged."""
+        self.ctx.check(lib().sfk_render_scratch_set_pair_raster(self.ptr, 1 if on else 0),
+                       "pair raster")
+
     def reserve_isects(self, n):
         self.ctx.check(lib().sfk_render_scratch_reserve_isects(self.ptr, n), "reserve")
 
