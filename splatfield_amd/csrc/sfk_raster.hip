@@ -136,24 +136,32 @@ constexpr uint32_t pair_rounds() {
 // pairs (pair p + k * NT + t in round k); one barrier per step publishes the
 // waves' hit masks, from which every lane derives its hits' positions.
 // Workgroup-uniform; ends with a barrier after the last write to s_gid.
+// The pairs of one step of stage_pairs starting at q < pend.  Every lane
+// issues all R loads (indices past the end clamped to the last pair, their
+// bboxes cleared when used), so that the loads in flight are the same on
+// every path and a prefetch is not drained early.
+template <int NT>
+__device__ __forceinline__ void pair_load(const RasterArgs& a, uint32_t pend, uint32_t q,
+                                          uint2 (&pv)[pair_rounds<NT>()]) {
+#pragma unroll
+    for (uint32_t k = 0; k < pair_rounds<NT>(); ++k) {
+        const uint32_t i = q + k * NT + threadIdx.x;
+        pv[k] = a.pairs[i < pend ? i : pend - 1u];
+    }
+}
+
+// `first` holds the pairs of the first step (pair_load at p, when p < J.p1),
+// loaded by the caller ahead of its barrier.
 template <int NT>
 __device__ __forceinline__ uint32_t stage_pairs(const TileJob& J, const RasterArgs& a, uint32_t& p,
+                                                uint2 (&first)[pair_rounds<NT>()],
                                                 uint32_t* s_gid, uint64_t* s_mask,
                                                 uint32_t* s_pnext) {
     constexpr uint32_t NW = NT / 64, R = pair_rounds<NT>();
     static_assert(R * NW <= 64, "one lane per (round, wave) mask");
     const uint32_t t = threadIdx.x, w = t / 64, lane = lane_id();
     const uint32_t pend = J.p1;
-    // Every lane issues all R loads (indices past the end clamped to the last
-    // pair, their bboxes cleared after the load), so that the loads in flight
-    // are the same on every path and a prefetch is not drained early.
-    auto load = [&](uint32_t q, uint2 (&pv)[R]) {
-#pragma unroll
-        for (uint32_t k = 0; k < R; ++k) {
-            const uint32_t i = q + k * NT + t;
-            pv[k] = a.pairs[i < pend ? i : pend - 1u];
-        }
-    };
+    auto load = [&](uint32_t q, uint2 (&pv)[R]) { pair_load<NT>(a, pend, q, pv); };
     uint32_t staged = 0, buf = 0;
     // One step over cur[] (loaded), issuing the next step's loads into nxt[]
     // while it ranks (that step starts at p + R * NT unless this one fills
@@ -214,10 +222,9 @@ __device__ __forceinline__ uint32_t stage_pairs(const TileJob& J, const RasterAr
         staged = run;
         return staged < (uint32_t)NT && p < pend;
     };
-    uint2 ping[R], pong[R];
+    uint2 pong[R];
     if (p < pend) {
-        load(p, ping);
-        while (step(ping, pong) && step(pong, ping)) {
+        while (step(first, pong) && step(pong, first)) {
         }
     }
     lds_barrier();
@@ -282,10 +289,12 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
     for (uint32_t chunk = start; PAIRS || chunk < end; chunk += NT) {
         uint32_t cnt;
         if constexpr (PAIRS) {
-            // previous chunk fully consumed; whole-tile early exit before
-            // scanning further pairs
+            // The first step's pairs load across the barrier: previous chunk
+            // fully consumed; whole-tile early exit before scanning further.
+            uint2 first[pair_rounds<NT>()];
+            if (pnext < J.p1) pair_load<NT>(a, J.p1, pnext, first);
             if (__syncthreads_count(!done) == 0) break;
-            cnt = stage_pairs<NT>(J, a, pnext, s_gid, s_mask, &s_pnext);
+            cnt = stage_pairs<NT>(J, a, pnext, first, s_gid, s_mask, &s_pnext);
             if (cnt == 0) break;
             // Every chunk but the last holds NT isects, so `chunk` is the
             // slot of this one's first in the tile's range.
