@@ -53,8 +53,13 @@ __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restric
                                                        uint32_t* __restrict__ resp) {
     __shared__ sfk_view s_views[kMaxViews];
     __shared__ float2 s_lims[kMaxViews];
+    // Informational per-view counts (isects, visible): each wave adds its
+    // sums here, the workgroup adds them to one of kCountSlots slot pairs
+    // per view once, after the loop (no barrier per view).
+    __shared__ uint32_t s_cnt[kMaxViews][2];
     for (uint32_t k = threadIdx.x; k < nviews * (sizeof(sfk_view) / 4); k += 256)
         reinterpret_cast<float*>(s_views)[k] = reinterpret_cast<const float*>(views)[k];
+    for (uint32_t k = threadIdx.x; k < 2 * nviews; k += 256) (&s_cnt[0][0])[k] = 0u;
     __syncthreads();
     // Per view once, not per splat and view: the clamp limits.
     for (uint32_t k = threadIdx.x; k < nviews; k += 256) s_lims[k] = view_lims(s_views[k]);
@@ -92,7 +97,18 @@ __global__ __launch_bounds__(256) void k_project_batch(const sfk_view* __restric
             if (cbbox) cbbox[gi] = vis ? compact_bbox(o) : 0u;
             else tile_counts[gi] = cnt;
         }
-        block_count_pair(cnt, vis, counters + 2 + (size_t)v * 2 * kCountSlots);
+        const uint32_t wc = wave_sum_last_u32(cnt);  // valid in lane 63
+        const uint32_t wvis = (uint32_t)__popcll(__ballot(vis));
+        if (lane_id() == 63) {
+            if (wc) atomicAdd(&s_cnt[v][0], wc);
+            if (wvis) atomicAdd(&s_cnt[v][1], wvis);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < 2 * nviews; k += 256) {
+        const uint32_t c = (&s_cnt[0][0])[k];
+        if (c) atomicAdd(&counters[2 + ((size_t)(k / 2) * kCountSlots + blockIdx.x % kCountSlots) * 2 +
+                                   (k & 1u)], c);
     }
 }
 
