@@ -251,6 +251,10 @@ struct sfk_eig_round {
     uint32_t* pair_table_sums = nullptr;
     uint32_t supers = 0, bin_shift = 1;  // supertile ids, bits of the local ones
     BinView* d_binviews = nullptr;
+    // Views of the direct pair emission: supertiles of 2^pair_shift tiles a
+    // side, at most pair_supers per view.
+    BinView* d_pairviews = nullptr;
+    uint32_t pair_shift = kBinSuperShift, pair_supers = 0;
     uint32_t* offsets = nullptr;  // isect offsets of truncation candidates, by view-major splat
     uint32_t* bin_sstart = nullptr;
     uint32_t* bin_map = nullptr;
@@ -359,6 +363,11 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
             j.pairs_out = &r->pairs;
             j.pair_table = r->pair_table;
             j.pair_table_sums = r->pair_table_sums;
+            if (r->pair_raster && r->pair_table) {
+                // The direct pair emission, on its own supertile grids.
+                j.views = r->d_pairviews;
+                j.supers = r->pair_supers * V;
+            }
             rc = bin_emit(c, j);
             if (rc) return rc;
             r->sorted_tiles = nullptr;
@@ -419,6 +428,7 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
             ra.pair_starts = r->bin_sstart;
             ra.offs_by_splat = r->offsets;
             ra.cap = r->capacity;
+            ra.super_shift = r->pair_table ? r->pair_shift : kBinSuperShift;
         }
         SFK_HIP(c, launch_rasterize_batch(c->stream, ra));
     }
@@ -504,6 +514,22 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
         b.out_mul = nviews;
         b.out_add = v;
         b.gid_add = v * n;
+        b.super_shift = kBinSuperShift;
+    }
+    // The direct pair emission's supertiles: kPairSuperPx pixels a side, at
+    // least 2 x 2 tiles.  Smaller supertiles mean more pairs per splat but a
+    // shorter scan of its supertile's pairs for every tile.
+    // The pair write ranks spans of at most kPairMaxSpan supertiles a row.
+    r->pair_shift = 1;
+    while ((r->tile_w << r->pair_shift) < kPairSuperPx) ++r->pair_shift;
+    for (uint32_t v = 0; v < nviews; ++v)
+        while (div_up(r->bviews[v].tiles_x, 1u << r->pair_shift) > kPairMaxSpan) ++r->pair_shift;
+    std::vector<BinView> pairviews(binviews);
+    for (uint32_t v = 0; v < nviews; ++v) {
+        BinView& b = pairviews[v];
+        b.supers_x = div_up(b.tiles_x, 1u << r->pair_shift);
+        b.super_shift = r->pair_shift;
+        r->pair_supers = std::max(r->pair_supers, b.supers_x * div_up(b.tiles_y, 1u << r->pair_shift));
     }
     if (r->tile_bits + 6 > 32 || (uint64_t)r->max_tiles * nviews >= 0xffffffffull) {
         delete r;
@@ -541,13 +567,16 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
         }
         RALLOC(r->d_binviews, nviews * sizeof(BinView));
         RALLOC(r->offsets, vn * 4);
-        RALLOC(r->bin_sstart, ((size_t)r->supers + 1) * 4);
+        RALLOC(r->bin_sstart,
+               ((size_t)std::max(r->supers, r->pair_supers * nviews) + 1) * 4);
         RALLOC(r->scan_sums, (size_t)div_up((uint64_t)r->max_tiles * nviews, kScanBlock) * 4);
         if ((uint64_t)r->max_tiles * nviews <= kTileOrderMax)
             RALLOC(r->tile_order, (size_t)r->max_tiles * nviews * 4);
-        if (max_supers <= kPairMaxDigits) {
-            RALLOC(r->pair_table, (size_t)pair_table_words(n, nviews, max_supers) * 4);
-            RALLOC(r->pair_table_sums, (size_t)pair_table_sum_words(n, nviews, max_supers) * 4);
+        if (r->pair_supers <= kPairMaxDigits) {
+            RALLOC(r->d_pairviews, nviews * sizeof(BinView));
+            RALLOC(r->pair_table, (size_t)pair_table_words(n, nviews, r->pair_supers) * 4);
+            RALLOC(r->pair_table_sums,
+                   (size_t)pair_table_sum_words(n, nviews, r->pair_supers) * 4);
         }
     } else {
         RALLOC(r->counts, vn * 4);
@@ -575,6 +604,9 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     if (e == hipSuccess && r->d_binviews)
         e = hipMemcpyAsync(r->d_binviews, binviews.data(), nviews * sizeof(BinView),
                            hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && r->d_pairviews)
+        e = hipMemcpyAsync(r->d_pairviews, pairviews.data(), nviews * sizeof(BinView),
+                           hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail_out(hip_fail(c, e, "eig round upload"));
     int rc = radix_scratch_create_ex(c, (uint32_t)vn, 4, r->compact ? 8 : 4, &r->sort_depth);
@@ -594,7 +626,8 @@ extern "C" void sfk_eig_round_destroy(sfk_eig_round* r) {
     void* bufs[] = {r->d_views, r->d_bviews, r->proj, r->proj_c, r->opac, r->keys, r->counts, r->bbox, r->cbbox,
                     r->counters, r->starts, r->resp, r->h0, r->partials, r->eig, r->best,
                     r->emit_status, r->d_binviews, r->offsets, r->bin_sstart, r->scan_sums,
-                    r->d_rebase, r->tile_order, r->pair_table, r->pair_table_sums};
+                    r->d_rebase, r->tile_order, r->pair_table, r->pair_table_sums,
+                    r->d_pairviews};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (r->host_out) hipHostFree(r->host_out);

@@ -80,7 +80,7 @@ struct Span {
     uint32_t sc;   // pairs (supertiles of the bbox)
     uint32_t d0, swx;
 };
-__device__ __forceinline__ Span rank_span(uint32_t cb, uint32_t supers_x) {
+__device__ __forceinline__ Span rank_span(uint32_t cb, uint32_t supers_x, uint32_t sh) {
     const uint32_t x0 = cb & 0xffu, y0 = (cb >> 8) & 0xffu, x1 = (cb >> 16) & 0xffu,
                    y1 = cb >> 24;
     Span s;
@@ -89,9 +89,9 @@ __device__ __forceinline__ Span rank_span(uint32_t cb, uint32_t supers_x) {
     s.d0 = 0;
     s.swx = 1;
     if (s.cnt) {
-        const uint32_t sx0 = x0 / kBinSuper, sy0 = y0 / kBinSuper;
-        s.swx = (x1 - 1) / kBinSuper - sx0 + 1;
-        s.sc = s.swx * ((y1 - 1) / kBinSuper - sy0 + 1);
+        const uint32_t sx0 = x0 >> sh, sy0 = y0 >> sh;
+        s.swx = ((x1 - 1) >> sh) - sx0 + 1;
+        s.sc = s.swx * (((y1 - 1) >> sh) - sy0 + 1);
         s.d0 = sy0 * supers_x + sx0;
     }
     return s;
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_count(PairArgs a) {
     uint32_t ci = 0, cp = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kRpt; ++k) {
-        const Span s = rank_span(cb[k], vw.supers_x);
+        const Span s = rank_span(cb[k], vw.supers_x, vw.super_shift);
         ci += s.cnt;
         cp += s.sc;
         hist_pairs(s, s.sc, vw.supers_x, s_hist);
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_bases(PairArgs a) {
 #pragma unroll
     for (uint32_t k = 0; k < kRpt; ++k) {
         const uint32_t r = r0 + t * kRpt + k;  // consecutive: rank order
-        s[k] = rank_span(r < r1 ? a.ordered[r].y : 0u, vw.supers_x);
+        s[k] = rank_span(r < r1 ? a.ordered[r].y : 0u, vw.supers_x, vw.super_shift);
         cp += s[k].sc;
     }
     uint32_t total;
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_write(PairArgs a) {
     uint32_t ci = 0, cp = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kRpt; ++k) {
-        s[k] = rank_span(oc[k].y, vw.supers_x);
+        s[k] = rank_span(oc[k].y, vw.supers_x, vw.super_shift);
         ci += s[k].cnt;
         cp += s[k].sc;
     }

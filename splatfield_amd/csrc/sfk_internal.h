@@ -221,14 +221,17 @@ struct BatchView {
 // order, as emission + stable tile argsort + tile ranges produce them,
 // without sorting the isects (sfk_bin.hip).
 constexpr uint32_t kBinSuper = 8;     // tiles per supertile side
+constexpr uint32_t kBinSuperShift = 3;
 constexpr uint32_t kBinChunk = 512;   // sorted pairs per wavefront chunk
 struct BinView {
     uint32_t tiles_x, tiles_y;  // this view's tile grid
-    uint32_t supers_x;          // supertiles per row, ceil(tiles_x / 8)
+    uint32_t supers_x;          // supertiles per row, ceil(tiles_x >> super_shift)
     uint32_t key_base;          // view << view_shift: pair keys are key_base + local supertile
     uint32_t out_mul, out_add;  // tile_starts index = local tile * out_mul + out_add
     uint32_t gid_add;           // written splat id = in-view splat + gid_add
-    uint32_t pad;
+    // log2 of the supertile side in tiles: kBinSuperShift, except for the
+    // direct pair emission (sfk_bin_pairs.hip), which takes other sizes.
+    uint32_t super_shift;
 };
 // Largest isect capacity of the binned emission.  Its prefixes saturate at
 // 2^31 - 1, so a saturated total must still read as above the capacity
@@ -305,6 +308,15 @@ struct BinJob {
 };
 int bin_emit(sfk_ctx* c, const BinJob& j);
 constexpr uint32_t kPairMaxDigits = 256;
+// Smallest supertile side, in pixels, of the batched round's direct pair
+// emission.
+#ifndef SFK_PAIR_SUPER_PX
+#define SFK_PAIR_SUPER_PX 64
+#endif
+constexpr uint32_t kPairSuperPx = SFK_PAIR_SUPER_PX;
+// Most supertiles a row of the direct pair emission's views (its pair write
+// ranks a bbox's supertiles by a reciprocal table of the row spans).
+constexpr uint32_t kPairMaxSpan = 32;
 uint32_t pair_table_words(uint32_t n_per_view, uint32_t views, uint32_t digits);
 uint32_t pair_table_sum_words(uint32_t n_per_view, uint32_t views, uint32_t digits);
 int bin_pairs_direct(sfk_ctx* c, const BinJob& j, uint32_t digits);
@@ -357,6 +369,9 @@ struct RasterArgs {
     const uint32_t* pair_starts;
     const uint32_t* offs_by_splat;
     uint32_t cap;
+    // Batched pair source: log2 of the supertile side in tiles (the pair
+    // emission's BinView::super_shift).
+    uint32_t super_shift;
     // One view, pair source: the rasterizers write the isects they consume
     // to gids_out at their slots of the tile ranges (tile_starts), so that a
     // backward pass finds every isect up to each pixel's last contributor.

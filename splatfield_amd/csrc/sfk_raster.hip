@@ -175,8 +175,8 @@ __device__ __forceinline__ uint32_t stage_pairs(const TileJob& J, const RasterAr
             const uint32_t x0 = cb & 0xffu, y0 = (cb >> 8) & 0xffu, x1 = (cb >> 16) & 0xffu,
                            y1 = cb >> 24;
             // x0 <= tx < x1 and y0 <= ty < y1 (x1 >= x0, y1 >= y0)
-            bool h = J.tx - x0 < x1 - x0 && J.ty - y0 < y1 - y0;
-            if (h && (cur[k].x >> 31)) {
+            bool h = (J.tx - x0 < x1 - x0) & (J.ty - y0 < y1 - y0);
+            if (h & ((cur[k].x >> 31) != 0u)) {
                 const uint32_t off = a.offs_by_splat[J.gid_add + (cur[k].x & 0x7fffffffu)];
                 const uint32_t idx = (J.ty - y0) * (x1 - x0) + (J.tx - x0);
                 h = off < a.cap && idx < a.cap - off;
@@ -188,6 +188,35 @@ __device__ __forceinline__ uint32_t stage_pairs(const TileJob& J, const RasterAr
         uint64_t m[R];
 #pragma unroll
         for (uint32_t k = 0; k < R; ++k) m[k] = __ballot(hit[k]);
+        if constexpr (NW == 1) {
+            // One wavefront: the hit masks are scalar and the positions are
+            // running popcounts, with no exchange through LDS.
+            uint32_t run = staged;
+#pragma unroll
+            for (uint32_t k = 0; k < R; ++k) run += (uint32_t)__popcll(m[k]);
+            uint32_t base = staged, resume = 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < R; ++k) {
+                const uint32_t pos = base + mask_rank(m[k]);
+                const bool take = hit[k];
+                if (take && pos < (uint32_t)NT)
+                    s_gid[pos] = (cur[k].x & 0x7fffffffu) + J.gid_add;
+                if (run > (uint32_t)NT) {
+                    // The last pair taken is the one at position NT - 1.
+                    const uint64_t at = __ballot(take && pos == (uint32_t)NT - 1u);
+                    if (at) resume = p + k * NT + (uint32_t)__builtin_ctzll(at) + 1u;
+                }
+                base += (uint32_t)__popcll(m[k]);
+            }
+            if (run > (uint32_t)NT) {
+                p = resume;
+                staged = NT;
+                return false;
+            }
+            p = pn;
+            staged = run;
+            return staged < (uint32_t)NT && p < pend;
+        }
         uint64_t* sm = s_mask + buf * (R * NW);
         buf ^= 1u;
         if (lane == 0) {
@@ -243,6 +272,18 @@ __device__ __forceinline__ uint32_t stage_pairs(const TileJob& J, const RasterAr
 // overlaps their LDS latency at the cost of registers: it pays when the
 // launch has too few waves to hide latency (one view's responsibility render)
 // and costs occupancy otherwise.
+// Barrier, then whether pred holds on any thread of the workgroup (a ballot
+// when the workgroup is one wavefront).
+template <int NW>
+__device__ __forceinline__ bool block_any(bool pred) {
+    if constexpr (NW == 1) {
+        __syncthreads();
+        return __ballot(pred) != 0u;
+    } else {
+        return __syncthreads_count(pred) != 0;
+    }
+}
+
 template <int MODE, int TW, bool PF, bool PAIRS = false>
 __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& a) {
 
@@ -293,7 +334,7 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
             // fully consumed; whole-tile early exit before scanning further.
             uint2 first[pair_rounds<NT>()];
             if (pnext < J.p1) pair_load<NT>(a, J.p1, pnext, first);
-            if (__syncthreads_count(!done) == 0) break;
+            if (!block_any<NW>(!done)) break;
             cnt = stage_pairs<NT>(J, a, pnext, first, s_gid, s_mask, &s_pnext);
             if (cnt == 0) break;
             // Every chunk but the last holds NT isects, so `chunk` is the
@@ -328,7 +369,7 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
             }
         }
         // Staging fence and whole-tile early exit (raster.rs:166-171).
-        if (__syncthreads_count(!done) == 0) break;
+        if (!block_any<NW>(!done)) break;
         const uint32_t nl =
             __ballot(!done) ? wave_reach_list<2>(s_box, s_tau, s_geo, s_aux, cnt, bx0, by0, list)
                             : 0u;
@@ -539,8 +580,8 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize_batch(RasterArgs a) {
     // Sorted by (local tile, view): list (view lo, tile local) is entry
     // local * nviews + lo of the starts table; ids are view-major slots.
     if constexpr (PAIRS) {
-        const uint32_t ls =
-            (J.ty / kBinSuper) * ((bv.tiles_x + kBinSuper - 1) / kBinSuper) + J.tx / kBinSuper;
+        const uint32_t sh = a.super_shift;
+        const uint32_t ls = (J.ty >> sh) * ((bv.tiles_x + (1u << sh) - 1u) >> sh) + (J.tx >> sh);
         const uint32_t ps = ls * a.nviews + lo;
         J.p0 = a.pair_starts[ps];
         J.p1 = a.pair_starts[ps + 1];

@@ -569,6 +569,7 @@ static int prepare_impl(sfk_ctx* c, const sfk_splats* sp, sfk_render_scratch* s,
             j.view0.tiles_x = s->tiles_x;
             j.view0.tiles_y = s->tiles_y;
             j.view0.supers_x = div_up(s->tiles_x, kBinSuper);
+            j.view0.super_shift = kBinSuperShift;
             j.view0.out_mul = 1;
             j.nviews = 1;
             j.supers = bin_supers(s->tiles_x, s->tiles_y);
