@@ -284,7 +284,10 @@ __device__ __forceinline__ bool block_any(bool pred) {
     }
 }
 
-template <int MODE, int TW, bool PF, bool PAIRS = false>
+// PIX: the per-pixel outputs (final T, n_contrib, colour) may be written;
+// without them (the batched responsibility raster) no pixel tracks its last
+// contributor.
+template <int MODE, int TW, bool PF, bool PAIRS = false, bool PIX = true>
 __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& a) {
 
     constexpr int NT = TW * TW;
@@ -458,7 +461,7 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
                     q[s] = con ? (uint32_t)vis : 0u;
                 }
                 T = con ? T * (1.0f - alpha) : T;
-                last = con ? chunk + jj[s] + 1 : last;
+                if constexpr (PIX) last = con ? chunk + jj[s] + 1 : last;
                 done = done || (con && T < tdone);
             }
             if constexpr (MODE != kRgb) {
@@ -501,7 +504,7 @@ __device__ __forceinline__ void raster_tile(const TileJob& J, const RasterArgs& 
         }
     }
 
-    if (in_bounds) {
+    if (PIX && in_bounds) {
         const size_t p = (size_t)py * J.w + px;
         if (a.final_t) a.final_t[p] = MODE == kRgb ? T : T * (1.0f / tscale);
         if (a.n_contrib) a.n_contrib[p] = last;
@@ -606,7 +609,7 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize_batch(RasterArgs a) {
     RasterArgs b = a;
     b.final_t = nullptr;
     b.n_contrib = nullptr;
-    raster_tile<kTotal, TW, false, PAIRS>(J, b);
+    raster_tile<kTotal, TW, false, PAIRS, false>(J, b);
 }
 
 template <int MODE, bool PAIRS>
