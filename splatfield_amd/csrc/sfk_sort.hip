@@ -214,6 +214,49 @@ __global__ __launch_bounds__(NT) void k_digscan(SortArgs a) {
     if (threadIdx.x == 0) a.totals[(size_t)s * RADIX + d] = carry;
 }
 
+// The same scans with a wavefront per (segment, digit) row, for rows of at
+// most kRowScanMax blocks: coalesced loads of 64 consecutive counts, a DPP
+// scan each, no barrier.  (A workgroup per row of a few hundred counts is
+// mostly idle threads and two barriers.)
+constexpr uint32_t kRowScanMax = 512;
+__device__ __forceinline__ uint32_t scan_blocks(const SortArgs& a) {
+    if (a.nseg <= 1) {
+        uint32_t n = a.n_max;
+        if (a.d_n) {
+            const uint32_t m = *a.d_n;
+            n = m < n ? m : n;
+        }
+        return (n + a.kpb - 1) / a.kpb;
+    }
+    return (a.seg_len + a.kpb - 1) / a.kpb;
+}
+template <int RADIX>
+__global__ __launch_bounds__(NT) void k_digscan_rows(SortArgs a, uint32_t rows) {
+    const uint32_t q = blockIdx.x * NW + threadIdx.x / 64;
+    if (q >= rows) return;
+    // row q = (segment, digit) of the pass's 2^width digits
+    const size_t r = (size_t)(q >> a.width) * RADIX + (q & ((1u << a.width) - 1u));
+    const uint32_t lane = lane_id();
+    const uint32_t nb = scan_blocks(a);
+    uint32_t* row = a.hist + r * a.stride;
+    constexpr uint32_t C = kRowScanMax / 64;
+    uint32_t v[C];
+#pragma unroll
+    for (uint32_t e = 0; e < C; ++e) {
+        const uint32_t idx = e * 64 + lane;
+        v[e] = idx < nb ? row[idx] : 0u;
+    }
+    uint32_t carry = 0;
+#pragma unroll
+    for (uint32_t e = 0; e < C; ++e) {
+        const uint32_t idx = e * 64 + lane;
+        const uint32_t inc = wave_incl_scan_u32(v[e]);
+        if (idx < nb) row[idx] = carry + inc - v[e];
+        carry += __builtin_amdgcn_readlane(inc, 63);
+    }
+    if (lane == 0) a.totals[r] = carry;
+}
+
 template <typename V>
 __device__ __forceinline__ V identity_val(uint32_t li, const uint32_t* payload, size_t gi);
 template <>
@@ -392,13 +435,23 @@ hipError_t pass_impl(hipStream_t st, const SortArgs& a) {
     constexpr uint32_t KPB = NT * EPT;
     uint32_t grid = a.nseg <= 1 ? div_up(a.n_max, KPB) : a.nseg * div_up(a.seg_len, KPB);
     if (grid == 0) return hipSuccess;
+    // Rows of the digit scan: at most `grid` blocks each (fewer when the
+    // count read from device memory is smaller).
+    const uint32_t rows = (a.nseg <= 1 ? 1u : a.nseg) << a.width;
+    const uint32_t row_blocks = a.nseg <= 1 ? grid : div_up(a.seg_len, KPB);
+    auto digscan = [&]() {
+        if (row_blocks <= kRowScanMax)
+            k_digscan_rows<RADIX><<<div_up(rows, NW), NT, 0, st>>>(a, rows);
+        else
+            k_digscan<RADIX><<<rows, NT, 0, st>>>(a);
+    };
     if (a.nseg <= 1 && a.d_n && grid > kMaxLiveGrid) {
         k_upsweep<K, RADIX, EPT, true><<<kMaxLiveGrid, NT, 0, st>>>(a);
-        k_digscan<RADIX><<<1u << a.width, NT, 0, st>>>(a);
+        digscan();
         k_scatter<K, V, RADIX, EPT, true><<<kMaxLiveGrid, NT, 0, st>>>(a);
     } else {
         k_upsweep<K, RADIX, EPT, false><<<grid, NT, 0, st>>>(a);
-        k_digscan<RADIX><<<(a.nseg <= 1 ? 1u : a.nseg) << a.width, NT, 0, st>>>(a);
+        digscan();
         k_scatter<K, V, RADIX, EPT, false><<<grid, NT, 0, st>>>(a);
     }
     return hipGetLastError();
