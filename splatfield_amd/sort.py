@@ -49,6 +49,20 @@ class RadixScratch:
         return (GpuTensor(self.ctx, ok.value, (n,), kdt, owned=False, owner=self),
                 GpuTensor(self.ctx, ov.value, (n,), np.uint32, owned=False, owner=self))
 
+    def argsort_segmented(self, keys, nseg, seg_len, bits, write_keys=True):
+        """nseg independent stable argsorts of seg_len keys each, back to back
+        (the batched multi-view depth sort; no reference counterpart).  Values
+        are indices within each segment.  Returns (keys, vals) GpuTensors that
+        alias the scratch."""
+        ok, ov = C.c_void_p(), C.c_void_p()
+        self.ctx.check(lib().sfk_argsort_u32_segmented(self.ctx.ptr, self.ptr, keys.ptr, None,
+                                                       nseg, seg_len, bits, int(write_keys),
+                                                       C.byref(ok), C.byref(ov)),
+                       "argsort segmented")
+        n = nseg * seg_len
+        return (GpuTensor(self.ctx, ok.value, (n,), np.uint32, owned=False, owner=self),
+                GpuTensor(self.ctx, ov.value, (n,), np.uint32, owned=False, owner=self))
+
     def __del__(self):
         try:
             if self.ctx.ptr:  # the context must outlive the objects it made

@@ -32,6 +32,29 @@ def test_sorting_stable(ctx, n, digit_bits):
     np.testing.assert_array_equal(ok, np.sort(keys))
 
 
+@pytest.mark.parametrize("nseg,seg_len,digit_bits", [
+    (3, 100003, 8),      # one workgroup per block, ragged last block
+    (8, 1 << 20, 8),     # 2048 blocks: two blocks per upsweep workgroup
+    (5, 1 << 20, 4),     # 1280 blocks, 4-bit digits
+    (9, 700001, 8),      # odd block count: the last workgroup takes one block
+])
+def test_sorting_segmented(ctx, nseg, seg_len, digit_bits):
+    """The batched depth sort's segmented argsort: each segment stably sorted
+    on 24-bit keys with ties, values the indices within the segment, against
+    numpy's stable argsort per segment."""
+    rng = np.random.default_rng(seg_len + nseg)
+    keys = rng.integers(0, 1 << 24, nseg * seg_len, dtype=np.uint32)
+    keys[rng.random(keys.size) < 0.1] = 12345  # ties
+    s = SO.RadixScratch(ctx, nseg * seg_len, 4, digit_bits)
+    ok, ov = s.argsort_segmented(GpuTensor.from_numpy(ctx, keys), nseg, seg_len, 24)
+    ok, ov = ok.read(), ov.read()
+    for g in range(nseg):
+        seg = keys[g * seg_len:(g + 1) * seg_len]
+        np.testing.assert_array_equal(ov[g * seg_len:(g + 1) * seg_len],
+                                      np.argsort(seg, kind="stable"), f"segment {g}")
+        np.testing.assert_array_equal(ok[g * seg_len:(g + 1) * seg_len], np.sort(seg))
+
+
 def test_sort_values_only(ctx):
     """sort.rs:455-473: write_keys = false still yields the stable permutation;
     NULL values give the identity argsort."""
