@@ -255,6 +255,12 @@ struct sfk_eig_round {
     // side, at most pair_supers per view.
     BinView* d_pairviews = nullptr;
     uint32_t pair_shift = kBinSuperShift, pair_supers = 0;
+    // Launch order of the pair raster (start-table slots local * views +
+    // view, ~0u for an idle workgroup): workgroup b runs on XCD b % 8, and
+    // every supertile row of every view goes whole to one XCD, dealt on a
+    // diagonal so that each XCD gets a like share of every view.
+    uint32_t* xcd_order = nullptr;
+    uint32_t xcd_order_len = 0;
     uint32_t* offsets = nullptr;  // isect offsets of truncation candidates, by view-major splat
     uint32_t* bin_sstart = nullptr;
     uint32_t* bin_map = nullptr;
@@ -429,6 +435,10 @@ static int round_enqueue(sfk_eig_round* r, const float* a, const float* b) {
             ra.offs_by_splat = r->offsets;
             ra.cap = r->capacity;
             ra.super_shift = r->pair_table ? r->pair_shift : kBinSuperShift;
+            if (r->pair_table && r->xcd_order) {
+                ra.tile_order = r->xcd_order;
+                ra.batch_tiles = r->xcd_order_len;
+            }
         }
         SFK_HIP(c, launch_rasterize_batch(c->stream, ra));
     }
@@ -525,11 +535,39 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     for (uint32_t v = 0; v < nviews; ++v)
         while (div_up(r->bviews[v].tiles_x, 1u << r->pair_shift) > kPairMaxSpan) ++r->pair_shift;
     std::vector<BinView> pairviews(binviews);
+    std::vector<uint32_t> xcd_order;
     for (uint32_t v = 0; v < nviews; ++v) {
         BinView& b = pairviews[v];
         b.supers_x = div_up(b.tiles_x, 1u << r->pair_shift);
         b.super_shift = r->pair_shift;
         r->pair_supers = std::max(r->pair_supers, b.supers_x * div_up(b.tiles_y, 1u << r->pair_shift));
+    }
+    {
+        // Bands of 2^pair_shift tile rows (one supertile row), band k of
+        // view v to XCD (global band index + v) % 8; each XCD's list holds
+        // its bands in order, a supertile's tiles together, and the eight
+        // lists interleave into the launch order (idle slots pad the short
+        // ones).  With a contiguous range of tiles per XCD, the XCDs holding
+        // the costlier views finished up to 1.4x later.
+        std::vector<uint32_t> per[8];
+        uint32_t gb = 0;
+        const uint32_t side = 1u << r->pair_shift;
+        for (uint32_t v = 0; v < nviews; ++v) {
+            const uint32_t tx = r->bviews[v].tiles_x, ty = r->bviews[v].tiles_y;
+            const uint32_t sx_n = div_up(tx, side);
+            for (uint32_t y0 = 0; y0 < ty; y0 += side, ++gb) {
+                const uint32_t x8 = (gb + v) % 8;
+                for (uint32_t sx = 0; sx < sx_n; ++sx)
+                    for (uint32_t y = y0; y < std::min(ty, y0 + side); ++y)
+                        for (uint32_t x = sx * side; x < std::min(tx, (sx + 1) * side); ++x)
+                            per[x8].push_back((y * tx + x) * nviews + v);
+            }
+        }
+        size_t longest = 0;
+        for (uint32_t x = 0; x < 8; ++x) longest = std::max(longest, per[x].size());
+        xcd_order.assign(longest * 8, ~0u);
+        for (uint32_t x = 0; x < 8; ++x)
+            for (size_t q = 0; q < per[x].size(); ++q) xcd_order[q * 8 + x] = per[x][q];
     }
     if (r->tile_bits + 6 > 32 || (uint64_t)r->max_tiles * nviews >= 0xffffffffull) {
         delete r;
@@ -574,6 +612,10 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
             RALLOC(r->tile_order, (size_t)r->max_tiles * nviews * 4);
         if (r->pair_supers <= kPairMaxDigits) {
             RALLOC(r->d_pairviews, nviews * sizeof(BinView));
+            if (xcd_order.size() < 0xffffffffull) {
+                RALLOC(r->xcd_order, xcd_order.size() * 4);
+                r->xcd_order_len = (uint32_t)xcd_order.size();
+            }
             RALLOC(r->pair_table, (size_t)pair_table_words(n, nviews, r->pair_supers) * 4);
             RALLOC(r->pair_table_sums,
                    (size_t)pair_table_sum_words(n, nviews, r->pair_supers) * 4);
@@ -607,6 +649,9 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     if (e == hipSuccess && r->d_pairviews)
         e = hipMemcpyAsync(r->d_pairviews, pairviews.data(), nviews * sizeof(BinView),
                            hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && r->xcd_order)
+        e = hipMemcpyAsync(r->xcd_order, xcd_order.data(), xcd_order.size() * 4,
+                           hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail_out(hip_fail(c, e, "eig round upload"));
     int rc = radix_scratch_create_ex(c, (uint32_t)vn, 4, r->compact ? 8 : 4, &r->sort_depth);
@@ -627,7 +672,7 @@ extern "C" void sfk_eig_round_destroy(sfk_eig_round* r) {
                     r->counters, r->starts, r->resp, r->h0, r->partials, r->eig, r->best,
                     r->emit_status, r->d_binviews, r->offsets, r->bin_sstart, r->scan_sums,
                     r->d_rebase, r->tile_order, r->pair_table, r->pair_table_sums,
-                    r->d_pairviews};
+                    r->d_pairviews, r->xcd_order};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (r->host_out) hipHostFree(r->host_out);

@@ -557,8 +557,9 @@ __global__ __launch_bounds__(TW * TW) void k_rasterize_batch(RasterArgs a) {
     const uint32_t total = a.batch_tiles;
     uint32_t lo, local;
     if (a.tile_order) {
-        // A start-table slot local * nviews + view, heaviest first; slots
-        // past a smaller view's grid have empty ranges and are skipped.
+        // A start-table slot local * nviews + view in launch order
+        // (heaviest first, or dealt to the XCDs); slots past a view's grid
+        // (smaller views, idle workgroups) are skipped.
         const uint32_t slot = a.tile_order[blockIdx.x];
         lo = slot % a.nviews;
         local = slot / a.nviews;
