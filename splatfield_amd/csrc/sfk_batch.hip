@@ -529,11 +529,18 @@ extern "C" int sfk_eig_round_create(sfk_ctx* c, const sfk_splats* sp, const sfk_
     // The direct pair emission's supertiles: kPairSuperPx pixels a side, at
     // least 2 x 2 tiles.  Smaller supertiles mean more pairs per splat but a
     // shorter scan of its supertile's pairs for every tile.
-    // The pair write ranks spans of at most kPairMaxSpan supertiles a row.
+    // The pair write ranks spans of at most kPairMaxSpan supertiles a row,
+    // and the direct emission takes at most kPairMaxDigits a view: larger
+    // views get larger supertiles.
     r->pair_shift = 1;
     while ((r->tile_w << r->pair_shift) < kPairSuperPx) ++r->pair_shift;
-    for (uint32_t v = 0; v < nviews; ++v)
-        while (div_up(r->bviews[v].tiles_x, 1u << r->pair_shift) > kPairMaxSpan) ++r->pair_shift;
+    for (uint32_t v = 0; v < nviews; ++v) {
+        const uint32_t tx = r->bviews[v].tiles_x, ty = r->bviews[v].tiles_y;
+        while (r->pair_shift < 8 &&
+               (div_up(tx, 1u << r->pair_shift) > kPairMaxSpan ||
+                div_up(tx, 1u << r->pair_shift) * div_up(ty, 1u << r->pair_shift) > kPairMaxDigits))
+            ++r->pair_shift;
+    }
     std::vector<BinView> pairviews(binviews);
     std::vector<uint32_t> xcd_order;
     for (uint32_t v = 0; v < nviews; ++v) {

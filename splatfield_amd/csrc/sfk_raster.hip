@@ -123,7 +123,7 @@ constexpr uint32_t pair_rounds() {
     return 64 / (NT / 64) < SFK_PAIR_ROUNDS ? 64 / (NT / 64) : SFK_PAIR_ROUNDS;
 }
 
-// Pair source of the batched responsibility raster: stages the tile's next
+// Pair source of the rasterizers (one view or a batch): stages the tile's next
 // isects, at most NT, into s_gid in pair order (= depth-rank order: the
 // sorted pairs of one supertile and view keep the emission's rank order),
 // from its supertile's pairs [p, J.p1); advances p past the last one taken
@@ -134,7 +134,8 @@ constexpr uint32_t pair_rounds() {
 // others) -- exactly the isects that map_isects + the stable tile sort put
 // in the tile's range, in the same order.  A step scans pair_rounds * NT
 // pairs (pair p + k * NT + t in round k); one barrier per step publishes the
-// waves' hit masks, from which every lane derives its hits' positions.
+// waves' hit masks, from which every lane derives its hits' positions (a
+// one-wavefront tile ranks them by running popcounts, with no barrier).
 // Workgroup-uniform; ends with a barrier after the last write to s_gid.
 // The pairs of one step of stage_pairs starting at q < pend.  Every lane
 // issues all R loads (indices past the end clamped to the last pair, their

@@ -147,6 +147,34 @@ def test_batch_binned_matches_sorted(ctx, tile_w, pair_raster):
     np.testing.assert_array_equal(c_b, c_s)
 
 
+def test_batch_large_views_pair_supertiles(ctx):
+    """A 2040 x 2040 view at 8 px tiles (255 x 255 tiles) holds more than
+    256 of the pair emission's 64-pixel supertiles, so the round takes 128-
+    pixel ones for it and stays on the direct pair emission; a wide view
+    tests the row limit.  Evidence words and EIG equal the batched emission +
+    stable tile argsort's, eager and graph-replayed."""
+    n = 32768
+    attrs, sh, k = S.synthetic_scene(n, 0, seed=9)
+    attrs = attrs.reshape(S.ATTR_PLANES, n)
+    attrs[7:10] += np.float32(1.0)
+    attrs = attrs.reshape(-1)
+    splats = R.Splats(ctx, attrs, sh)
+    cams = G.sphere_cameras(3)
+    sizes = [(2040, 2040), (2040, 64), (64, 64)]
+    state, a, b = make_state(ctx, n, seed=5)
+    ref = G.EigRound(splats, cams, sizes, tile_w=8)
+    ref.set_binned(False)
+    eig_s, best_s = ref.run(state, use_graph=True)
+    words = [ref.resp(v)[0] for v in range(len(cams))]
+    rnd = G.EigRound(splats, cams, sizes, tile_w=8)
+    for graph in (True, True):
+        eig_b, best_b = rnd.run(state, use_graph=graph)
+        assert best_b == best_s
+        np.testing.assert_array_equal(eig_b, eig_s)
+        for v in range(len(cams)):
+            np.testing.assert_array_equal(rnd.resp(v)[0], words[v], f"view {v}")
+
+
 @pytest.mark.parametrize("n", [1 << 16, 1 << 20])
 def test_batch_truncated_round(ctx, n):
     """A round whose isects exceed its capacity (an eager run before any
